@@ -1,0 +1,294 @@
+"""Synthetic system builder: SPC/E water droplets carrying 29-site M(D'E4OH)M
+surfactants above a two-layer graphite sheet, in the reference's reduced units.
+
+Site layout (the order the reference reads its configuration in,
+``src/read_config.f90:19-34``): all waters first, each as (O, H1, H2), then
+all surfactants, 29 consecutive sites each (site 14 is the oxygen the
+reference files the molecule under, ``src/read_config.f90:145-175``).
+
+Velocities use the reference's storage convention: v*dt (displacement per
+step); accelerations are stored as F*dt^2/(2m) (``src/init_accel.f90:21``).
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import forcefield as ff
+
+WATER_DENSITY_A3 = 0.0334  # molecules per cubic Angstrom
+
+
+@dataclass
+class System:
+    n_water: int
+    n_surf: int
+    pos: np.ndarray            # (N, 3) float64, sigma units
+    vel: np.ndarray            # (N, 3) v*dt
+    acc: np.ndarray = None     # (N, 3) F*dt^2/(2m)
+    graphite: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+
+    @property
+    def n_sites(self):
+        return 3 * self.n_water + 29 * self.n_surf
+
+    def types(self):
+        """Site type in the 32-type space (0..2 water, 3..31 surfactant)."""
+        t = np.empty(self.n_sites, dtype=np.int32)
+        t[:3 * self.n_water] = np.tile(np.arange(3, dtype=np.int32), self.n_water)
+        t[3 * self.n_water:] = np.tile(np.arange(3, 32, dtype=np.int32), self.n_surf)
+        return t
+
+    def masses(self):
+        return ff.type_tables()['mass'][self.types()]
+
+    def copy(self):
+        return System(self.n_water, self.n_surf, self.pos.copy(), self.vel.copy(),
+                      None if self.acc is None else self.acc.copy(),
+                      self.graphite.copy())
+
+
+def _rot_random(rng):
+    """Uniform random rotation matrix (from a random unit quaternion)."""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([
+        [a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+        [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+        [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def water_template():
+    """Rigid water with |OH| = 1 A and HOH = 109.47 deg, the constraint
+    geometry of ``src/init_arrays.f90:566-574``."""
+    half = 0.5 * 109.47 * math.pi / 180.0
+    r = 1.0 / ff.SIGMA
+    return np.array([[0.0, 0.0, 0.0],
+                     [r * math.sin(half), r * math.cos(half), 0.0],
+                     [-r * math.sin(half), r * math.cos(half), 0.0]])
+
+
+def _place(a, b, c, bond, angle, dih):
+    """Position of a new site bonded to a, with angle (new, a, b) and dihedral
+    (new, a, b, c) -- the usual z-matrix construction."""
+    bc = b - c
+    ab = a - b
+    n = np.cross(bc, ab)
+    if np.linalg.norm(n) < 1e-12:
+        n = np.cross(ab, [1.0, 0.3, 0.1])
+    n /= np.linalg.norm(n)
+    u = ab / np.linalg.norm(ab)
+    m = np.cross(n, u)
+    d2 = np.array([-bond * math.cos(angle),
+                   bond * math.sin(angle) * math.cos(dih),
+                   bond * math.sin(angle) * math.sin(dih)])
+    return a + d2[0] * u + d2[1] * m + d2[2] * n
+
+
+def _bond_graph_distance():
+    adj = {i: set() for i in range(1, 30)}
+    for b in range(28):
+        adj[ff.IBOND[b]].add(ff.JBOND[b])
+        adj[ff.JBOND[b]].add(ff.IBOND[b])
+    dist = np.full((30, 30), 99, dtype=int)
+    for s in range(1, 30):
+        dist[s, s] = 0
+        front = [s]
+        while front:
+            nxt = []
+            for a in front:
+                for c in adj[a]:
+                    if dist[s, c] > dist[s, a] + 1:
+                        dist[s, c] = dist[s, a] + 1
+                        nxt.append(c)
+            front = nxt
+    return dist[1:, 1:]
+
+
+def surfactant_template():
+    """One extended M(D'E4OH)M conformer whose 28 bond lengths equal the
+    constraint lengths exactly and whose angles sit at their equilibria; the
+    siloxane-head dihedrals are picked on a small grid to maximise the closest
+    approach of sites three or more bonds apart."""
+    blen = {}
+    for b in range(28):
+        i, j = ff.IBOND[b], ff.JBOND[b]
+        blen[(i, j)] = blen[(j, i)] = math.sqrt(ff.DSQS[b])
+    ang = {}
+    for a in range(36):
+        i, j, k = ff.ANG_I[a], ff.ANG_J[a], ff.ANG_K[a]
+        ang[(i, j, k)] = ang[(k, j, i)] = ff.TH[a]
+    tet = 109.5 * math.pi / 180.0
+
+    def A(i, j, k):
+        return ang.get((i, j, k), tet)
+
+    x = {}
+    x[1] = np.zeros(3)
+    x[2] = np.array([blen[(1, 2)], 0.0, 0.0])
+    th = A(1, 2, 3)
+    x[3] = x[2] + blen[(2, 3)] * np.array([-math.cos(th), math.sin(th), 0.0])
+    for s in range(4, 19):  # extended chain, all trans
+        x[s] = _place(x[s - 1], x[s - 2], x[s - 3], blen[(s - 1, s)],
+                      A(s, s - 1, s - 2), math.pi)
+    far = _bond_graph_distance() >= 3
+    third = 2 * math.pi / 3
+
+    def head(d0, d1, d2):
+        y = dict(x)
+        y[19] = _place(y[18], y[17], y[16], blen[(18, 19)], A(19, 18, 17), d0)
+        y[20] = _place(y[18], y[17], y[16], blen[(18, 20)], A(20, 18, 17), d0 + third)
+        y[25] = _place(y[18], y[17], y[16], blen[(18, 25)], A(25, 18, 17), d0 - third)
+        y[21] = _place(y[20], y[18], y[17], blen[(20, 21)], A(21, 20, 18), d1)
+        for s, d in ((22, 0.0), (23, third), (24, -third)):
+            y[s] = _place(y[21], y[20], y[18], blen[(21, s)], A(s, 21, 20), math.pi + d)
+        y[26] = _place(y[25], y[18], y[17], blen[(25, 26)], A(26, 25, 18), d2)
+        for s, d in ((27, 0.0), (28, third), (29, -third)):
+            y[s] = _place(y[26], y[25], y[18], blen[(26, s)], A(s, 26, 25), math.pi + d)
+        X = np.array([y[s] for s in range(1, 30)])
+        D = np.sqrt(((X[:, None] - X[None]) ** 2).sum(-1))
+        return D[far].min(), X
+
+    best = None
+    grid = np.linspace(-math.pi, math.pi, 12, endpoint=False)
+    for d0 in grid:
+        for d1 in grid:
+            for d2 in grid:
+                dmin, X = head(d0, d1, d2)
+                if best is None or dmin > best[0]:
+                    best = (dmin, X)
+    return best[1]
+
+
+def _align(v_from, v_to):
+    """Rotation matrix taking unit vector v_from onto unit vector v_to."""
+    a = v_from / np.linalg.norm(v_from)
+    b = v_to / np.linalg.norm(v_to)
+    v = np.cross(a, b)
+    c = float(np.dot(a, b))
+    if c < -0.999999:
+        p = np.cross(a, [1.0, 0.0, 0.0])
+        if np.linalg.norm(p) < 1e-6:
+            p = np.cross(a, [0.0, 1.0, 0.0])
+        p /= np.linalg.norm(p)
+        return 2.0 * np.outer(p, p) - np.eye(3)
+    vx = np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
+    return np.eye(3) + vx + vx @ vx / (1.0 + c)
+
+
+def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6):
+    """Positions of one droplet: waters on a simple-cubic lattice at liquid
+    density with random orientations, cut to the n_water lattice sites closest
+    to the centre, and n_surf surfactants spread over the droplet surface with
+    the EO chain (site 1 end) pointing inward and the trisiloxane outward.
+
+    Returns (water_pos (n_water,3,3), surf_pos (n_surf,29,3)) in sigma units.
+    """
+    rng = np.random.default_rng(seed)
+    a = (1.0 / WATER_DENSITY_A3) ** (1.0 / 3.0) / ff.SIGMA
+    c = np.asarray(center_A, dtype=float) / ff.SIGMA
+    r_drop = (3.0 * n_water / (4.0 * math.pi * WATER_DENSITY_A3)) ** (1.0 / 3.0) / ff.SIGMA
+    tmpl = surfactant_template()
+    axis = tmpl[17] - tmpl[0]
+    length = np.linalg.norm(axis)
+    surf = np.zeros((n_surf, 29, 3))
+    for s in range(n_surf):
+        # Fibonacci sphere for evenly spread anchor directions
+        zz = 1.0 - 2.0 * (s + 0.5) / max(n_surf, 1)
+        phi = s * math.pi * (3.0 - math.sqrt(5.0))
+        d = np.array([math.sqrt(max(0.0, 1 - zz * zz)) * math.cos(phi),
+                      math.sqrt(max(0.0, 1 - zz * zz)) * math.sin(phi), zz])
+        R = _align(axis, d)
+        spin = _align(d, d)
+        ang = rng.uniform(0, 2 * math.pi)
+        K = np.array([[0, -d[2], d[1]], [d[2], 0, -d[0]], [-d[1], d[0], 0]])
+        spin = np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * K @ K
+        body = (spin @ (R @ (tmpl - tmpl[0]).T)).T
+        # site 1 a little inside the surface, siloxane head outside
+        start = c + d * max(r_drop + 0.35 * length - length, 0.75 * r_drop)
+        surf[s] = body + start
+    # lattice sites inside a generous sphere, closest to the centre first
+    nmax = int(math.ceil((r_drop * 1.6) / a)) + 2
+    g = np.arange(-nmax, nmax + 1) * a
+    X, Y, Z = np.meshgrid(g, g, g, indexing='ij')
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    pts += rng.uniform(-0.02, 0.02, size=3)
+    d2 = np.sum(pts * pts, axis=1)
+    pts = pts[np.argsort(d2, kind='stable')] + c
+    pts = pts[:int(n_water * 1.5) + 64]
+    if n_surf:
+        from scipy.spatial import cKDTree
+        near = cKDTree(surf.reshape(-1, 3)).query(pts, k=1)[0]
+        pts = pts[near > min_sep_A / ff.SIGMA]
+    if len(pts) < n_water:
+        raise ValueError("lattice too small for the requested water count")
+    pts = pts[:n_water]
+    w = water_template()
+    q = rng.normal(size=(n_water, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    qa, qb, qc, qd = q.T
+    R = np.stack([
+        np.stack([qa * qa + qb * qb - qc * qc - qd * qd, 2 * (qb * qc - qa * qd),
+                  2 * (qb * qd + qa * qc)], axis=1),
+        np.stack([2 * (qb * qc + qa * qd), qa * qa - qb * qb + qc * qc - qd * qd,
+                  2 * (qc * qd - qa * qb)], axis=1),
+        np.stack([2 * (qb * qd - qa * qc), 2 * (qc * qd + qa * qb),
+                  qa * qa - qb * qb - qc * qc + qd * qd], axis=1)], axis=1)
+    waters = np.einsum('kij,sj->ksi', R, w) + pts[:, None, :]
+    return waters, surf
+
+
+def graphite_sheet(half_side_A=60.0, z_top=-16.0, layer_gap_A=3.41, ccbond_A=1.421):
+    """Two AB-stacked graphene layers of C atoms (sigma units), the top layer at
+    z = z_top sigma and the second 3.41 A below, like the sheet the reference's
+    own generator writes (``src/make_graphite.f90:14-80``).  Coordinates are
+    rounded to 1e-4 sigma, the precision of the reference's graphite file
+    (``src/read_graphite.f90:216``)."""
+    a = ccbond_A / ff.SIGMA
+    h = half_side_A / ff.SIGMA
+    ax = np.array([1.5 * a, math.sqrt(3) / 2 * a])
+    ay = np.array([0.0, math.sqrt(3) * a])
+    n = int(h / a) + 3
+    out = []
+    for layer in range(2):
+        zl = z_top - layer * layer_gap_A / ff.SIGMA
+        shift = np.array([a, 0.0]) * layer
+        for i in range(-n, n + 1):
+            for j in range(-n, n + 1):
+                base = i * ax + j * ay + shift
+                for b in (np.zeros(2), np.array([a, 0.0])):
+                    p = base + b
+                    if abs(p[0]) <= h and abs(p[1]) <= h:
+                        out.append((p[0], p[1], zl))
+    g = np.round(np.array(out), 4)
+    return g
+
+
+def maxwell_velocities(masses, tr, seed):
+    """Gaussian velocities at reduced temperature tr in the v*dt convention
+    (per degree of freedom m <v^2> = tr dt^2, cf. ``src/compute_temperature.f90:242``)."""
+    rng = np.random.default_rng(seed)
+    sd = np.sqrt(tr / masses)[:, None] * ff.DT
+    return rng.normal(size=(len(masses), 3)) * sd
+
+
+def assemble(droplets, graphite=None, tr=ff.TR_DEFAULT, seed=0):
+    """Concatenate droplets (lists of (waters, surfs)) into one System with all
+    waters first, then all surfactants."""
+    ws = [d[0] for d in droplets if len(d[0])]
+    ss = [d[1] for d in droplets if len(d[1])]
+    W = np.concatenate(ws, axis=0) if ws else np.zeros((0, 3, 3))
+    S = np.concatenate(ss, axis=0) if ss else np.zeros((0, 29, 3))
+    pos = np.concatenate([W.reshape(-1, 3), S.reshape(-1, 3)], axis=0)
+    sys_ = System(len(W), len(S), pos, np.zeros_like(pos),
+                  graphite=np.zeros((0, 3)) if graphite is None else graphite)
+    sys_.vel = maxwell_velocities(sys_.masses(), tr, seed + 7)
+    return sys_
+
+
+def bench_system(n_water=20000, n_surf=200, seed=20261204):
+    """The throughput workload: one droplet of n_water SPC/E waters carrying
+    n_surf surfactants, centred in the reference's 144 A cube above the sheet."""
+    d = droplet(n_water, n_surf, center_A=(0.0, 0.0, 4.0), seed=seed)
+    return assemble([d], graphite=graphite_sheet(), seed=seed)
