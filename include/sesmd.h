@@ -1,0 +1,145 @@
+/* sesmd -- MI355X (gfx950) per-timestep force and integration path of the
+ * surfactant-droplet MD code jhalverson/surfactant_enhanced_spreading.
+ *
+ * Plain C ABI: opaque context, host pointers and sizes, int status codes
+ * (0 = success, nonzero = error; the text is in sesmd_last_error()).  All
+ * quantities are in the reference's reduced units (sigma = 3.166 A,
+ * epsilon = 650.2 J/mol, m_O); host arrays are (N, 3) row-major doubles with
+ * all waters (O, H1, H2) first, then 29-site surfactants -- the order
+ * src/read_config.f90:19-34 reads them in.  Velocities are stored as v*dt and
+ * accelerations as F dt^2 / (2m), as in src/init_accel.f90:21.
+ *
+ * State stays resident on the GPU between calls; the host arrays are only
+ * read or written by the set_/get_ calls.  A context is not thread-safe: call
+ * it from one thread, like the reference calls its routines from the rank's
+ * main thread.
+ */
+#ifndef SESMD_H
+#define SESMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sesmd_ctx sesmd_ctx;
+
+/* Force field in a 32-type space (0..2 water O/H1/H2, 3..31 surfactant sites
+ * 1..29).  The reference holds the same numbers in COMMON /cmn29/, /cmn30/,
+ * /cmn100/, /cmn31/, /cmn48/ (src/mybox.cmns:192-201,236-239,256) plus the
+ * site lists hard-coded in src/eval_valence.f90:13-95 and
+ * src/eval_dihedral.f90:23-100. */
+typedef struct sesmd_forcefield {
+    double q[32], mass[32];              /* charge (e), mass / m_O            */
+    double lj_c[32 * 32], lj_s6[32 * 32];/* (48 eps) s6 and s6 = sigma^6       */
+    double gr_c[32], gr_s6[32];          /* the same against graphite carbon  */
+    double dsqw[3], dsqs[28];            /* squared constraint lengths        */
+    int ibond[28], jbond[28];            /* surfactant bonds (1-based sites)  */
+    double th[36], rkth[36];             /* angle equilibria / force consts   */
+    int ang_i[36], ang_j[36], ang_k[36];
+    double dva[30], dvb[30], dvc[30];    /* dihedral coefficients             */
+    int dih_i[30], dih_j[30], dih_k[30], dih_l[30];
+    int n_internal;                      /* intramolecular LJ+Coulomb pairs   */
+    int int_j[320], int_k[320];
+    double rifact[33];                   /* 0! .. 32!                         */
+    double xpi, rc, rs, dt, dtsq2;       /* Coulomb const, cutoff, box, dt    */
+    double clamp_opls, clamp_be;         /* dihedral cos clamps               */
+} sesmd_forcefield;
+
+/* precision of the pair forces */
+enum { SESMD_FP64 = 0, SESMD_MIXED = 1 };
+
+/* force components, bits of `what` in sesmd_forces (one per reference
+ * routine family) */
+enum {
+    SESMD_NEAR = 1,          /* the 39 eval[wS]* box-pair kernels             */
+    SESMD_FAR = 2,           /* evalLjk4 (L2P of the frozen local expansions) */
+    SESMD_INTERNAL = 4,      /* eval_surfactant_internal                      */
+    SESMD_SUBTRACT = 8,      /* eval_subtract_fma                             */
+    SESMD_VALENCE = 16,      /* eval_valence                                  */
+    SESMD_DIHEDRAL = 32,     /* eval_dihedral                                 */
+    SESMD_DIHEDRAL_BE = 64,  /* eval_dihedral_berendsen                       */
+    SESMD_WALL = 128,        /* eval_wall                                     */
+    SESMD_GRAPHITE = 256,    /* eval_graphite                                 */
+    SESMD_ALL = 511
+};
+
+/* Create a context for n_water waters and n_surf surfactants in a cube of
+ * (4 nb2)^3 L4 boxes of side ff->rs (the reference: nb2 = 3, i.e. its 27
+ * ranks' L2 boxes, src/mybox.cmns:55-57), multipole order p (reference 4,
+ * src/mybox.cmns:49).  Replaces the reference's start-up of
+ * init_box_dimensions/get_box_center/init_arrays (src/mybox.f90:22,54-55). */
+int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, int p,
+                 const sesmd_forcefield *ff, int precision);
+int sesmd_destroy(sesmd_ctx *ctx);
+const char *sesmd_last_error(void);
+
+/* positions, velocities (v dt), accelerations (F dt^2/2m) and the Nose-Hoover
+ * friction eta; NULL skips an array */
+int sesmd_set_state(sesmd_ctx *ctx, const double *r, const double *v, const double *a,
+                    double eta);
+int sesmd_get_state(sesmd_ctx *ctx, double *r, double *v, double *a, double *eta);
+/* Nose-Hoover target tr and the constants pnh1, pnh2 of src/mybox.f90:44-51 */
+int sesmd_set_nh(sesmd_ctx *ctx, double tr, double pnh1, double pnh2);
+/* frozen graphite carbons (reduced units); the per-column selection of
+ * src/read_graphite.f90:203-229 is made here */
+int sesmd_set_graphite(sesmd_ctx *ctx, int n, const double *xyz);
+
+/* Every-ifar-steps work of src/mybox.f90:142-175 on the GPU: molecule classes
+ * and storage boxes (read_config/delete_*, ibox*), the per-box site sort that
+ * replaces the build* lists, the intramolecular subtraction lists
+ * (buildinternal) -- and, in sesmd_fmm, the far-field chain computeOnm4 ->
+ * computeOnm3 -> computeOnm2 -> computeLjk2 -> shiftLjk2 -> computeLjk3 ->
+ * shiftLjk3 -> computeLjk4. */
+int sesmd_rebuild(sesmd_ctx *ctx);
+int sesmd_fmm(sesmd_ctx *ctx);
+
+/* Forces at the current positions into the device force array (overwritten).
+ * near_mask selects (receiver kind, source kind) pairs of the near field, bit
+ * rk*9+sk; ~0 = all.  Receiver kinds 0..5: W1@0 W2@0 W2@1 S@0 S@1 S@2, source
+ * kinds 0..8: W1@0 W1@1 W2@0 W2@1 W2@2 S@0 S@1 S@2 S@3 (class @ storage ring);
+ * each of the 39 eval[wS]* kernels is a fixed subset. */
+int sesmd_forces(sesmd_ctx *ctx, unsigned what, uint64_t near_mask);
+int sesmd_get_forces(sesmd_ctx *ctx, double *f);
+int sesmd_set_forces(sesmd_ctx *ctx, const double *f);
+
+/* Integrator.  sesmd_move_a: part A for every species (move_surfactant_a,
+ * move_w1_a, move_w2_a of src/mybox.f90:136-138: predictor, SHAKE, the six
+ * Nose-Hoover kinetic sums, the eta update of src/move_w2_a.f90:154-157, force
+ * zeroing); sums6 (host, may be NULL) receives sv1 sv2 w1v1 w1v2 w2v1 w2v2.
+ * sesmd_move_b: part B (move_*_b, src/mybox.f90:225-227) from the device
+ * forces.  sesmd_init_accel: src/init_accel.f90. */
+int sesmd_move_a(sesmd_ctx *ctx, double *sums6);
+int sesmd_move_b(sesmd_ctx *ctx);
+int sesmd_init_accel(sesmd_ctx *ctx);
+
+/* nsteps full steps in the order of src/mybox.f90:130-227 (part A; rebuild +
+ * far field when kb % ifar == 0; forces; part B), all on the GPU, the regular
+ * steps replayed from a captured HIP graph.  *kb is the step counter. */
+int sesmd_steps(sesmd_ctx *ctx, int nsteps, int ifar, int *kb);
+
+/* Inspection: classes (per molecule 0 W1, 1 W2, 2 S), storage box (global L4
+ * index x + nb*(y + nb*z)), per-site central box, receiver kind (-1 none) and
+ * subtraction mask (bit k: site k+1 of the same surfactant); any may be NULL */
+int sesmd_get_classes(sesmd_ctx *ctx, int *cls, int *anc, int *cbox, int *rkind,
+                      unsigned *isub);
+/* expansions as complex (re, im) pairs: O4/L4 nb^3*lp, O3/L3 (nb/2)^3*lp, O2/L2 nb2^3*lp */
+int sesmd_get_expansions(sesmd_ctx *ctx, double *O4, double *L4, double *O3, double *L3,
+                         double *O2, double *L2);
+/* counters: SHAKE/RATTLE molecules that hit the iteration limit, cumulative */
+int sesmd_get_counters(sesmd_ctx *ctx, int *shake_fail);
+/* device pointers of the resident SoA arrays (x,y,z blocks of N doubles) */
+int sesmd_device_arrays(sesmd_ctx *ctx, void **r, void **v, void **a, void **f);
+/* the HIP stream every kernel of this context is launched on */
+void *sesmd_stream(sesmd_ctx *ctx);
+/* Launch the near-field kernel alone (for timing) on the current sorted data. */
+int sesmd_near_only(sesmd_ctx *ctx);
+/* pair count of the current near field (unordered intermolecular pairs in the
+ * 3^3 blocks, and those within rc with an LJ term), for roofline accounting */
+int sesmd_pair_counts(sesmd_ctx *ctx, double *pairs, double *lj_pairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,40 @@
+"""Build libsesmd.so (the HIP engine) in-tree for gfx950.
+
+    python -m surfactant_enhanced_spreading_amd.csrc.build
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+ROOT = os.path.dirname(PKG)
+OUT = os.path.join(PKG, 'libsesmd.so')
+ARCH = os.environ.get('SESMD_ARCH', 'gfx950')
+
+# -packed-fp32-ops off: no v_pk_*_f32 generated from scalar/float2 code.
+FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', '--offload-arch=' + ARCH,
+         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+         '-Wno-unused-result', '-I' + os.path.join(ROOT, 'include')]
+
+
+def build(verbose=False):
+    hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+    src = os.path.join(HERE, 'sesmd.hip')
+    tmp = OUT + '.tmp'
+    cmd = [hipcc] + FLAGS + ['-o', tmp, src]
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    err = '\n'.join(l for l in r.stderr.splitlines()
+                    if 'packed-fp32-ops' not in l)
+    if r.returncode != 0:
+        raise RuntimeError('hipcc failed:\n' + err)
+    if verbose and err.strip():
+        print(err)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == '__main__':
+    print(build(verbose=True))

@@ -1,0 +1,2198 @@
+// sesmd: MI355X (gfx950) per-timestep force and integration path of the
+// surfactant-droplet MD code jhalverson/surfactant_enhanced_spreading.
+//
+// One translation unit: context + C ABI (include/sesmd.h), host-side
+// precomputation of the FMM translation operators, and the device kernels.
+//
+// Device layout (all resident in HBM, sized once at create):
+//   site state      SoA doubles rx ry rz | vx vy vz | ax ay az | fx fy fz (N each)
+//   site statics    type[N], mol[N]; molecule first-site table first[nmol+1]
+//   rebuild state   cls[nmol], anc[nmol] (storage box), cbox[N], rkind[N], isub[N]
+//   box sort        perm[N] (sites sorted by cbox, stable), bstart[B+1]
+//   near field      spos[N] (double4 x y z q, sorted), smeta[N] (int4), work list
+//   far field       O4/L4 [B][lp], O3/L3, O2/L2 as double2; translation operators
+//
+// Every kernel launches on the context's one stream; nothing synchronises the
+// host inside a step, so sesmd_steps replays the regular step from a HIP graph.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sesmd.h"
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_err;
+
+static int fail(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return 1;
+}
+
+#define HC(x)                                                                         \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess)                                                         \
+            return fail("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *sesmd_last_error(void) { return g_err.c_str(); }
+
+// ------------------------------------------------------------- constants
+
+constexpr int MAXP = 8;                       // highest supported expansion order
+constexpr int MAXLP = (MAXP + 1) * (MAXP + 1);
+constexpr int MAXLK = (MAXP + 1) * (MAXP + 2) / 2;  // rows with k >= 0
+constexpr int NEAR_TILE = 64;                 // receivers per near-field work item
+constexpr int NH_VALS = 18;                   // kinetic partial sums per block
+
+struct Geo {
+    int n, nw, ns, nmol, nb2, nb, nb3, n3, n33, n2, p, lp, lk;
+    double rs, rsd, rhs, rc, rc2, xpi, dt, dtsq2;
+};
+
+// per-site term lists of the surfactant topology (built on the host)
+struct Topo {
+    int npair[29];
+    int pair[29][32];   // partner sites (0-based) of the internal LJ+Coulomb pairs
+    int nang[29];
+    int ang[29][12];    // angle index * 3 + role (0 = i, 1 = j, 2 = k)
+    int ndih[29];
+    int dih[29][16];    // dihedral index * 4 + role
+};
+
+struct sesmd_ctx {
+    Geo g;
+    int precision;
+    sesmd_forcefield ff;
+    hipStream_t stream;
+    // device state
+    double *r, *v, *a, *f;  // each 3*N: x block, y block, z block
+    int *type, *mol, *first;
+    int *cls, *anc, *cbox, *rkind;
+    unsigned *isub;
+    int *key_in, *key_out, *val_in, *perm;
+    void *cub_tmp;
+    size_t cub_bytes;
+    int *bstart, *nvalid;
+    int2 *work;
+    int *nwork;
+    int max_work;
+    double4 *spos;
+    int4 *smeta;
+    // far field
+    double2 *O4, *L4, *O3, *L3, *O2, *L2;
+    int *ne4, *ne3, *ne2;         // nonempty flags per box
+    double2 *T4, *T3, *T2;        // M2L operators per offset
+    double2 *M43, *M32, *S23, *S34; // M2M / L2L operators per child
+    double *c4;                   // L4 centres (3 per box)
+    double *lfac;                 // rfctr(j,|k|) table
+    // graphite
+    int ngr;
+    double *gr;
+    int *gstart, *gidx;
+    // forcefield + topology on the device
+    sesmd_forcefield *dff;
+    Topo *dtopo;
+    // integrator
+    double *partials;  // [nblocks][NH_VALS]
+    double *nh;        // [0] eta, [1..6] sums, [7] tr, [8] pnh1, [9] pnh2
+    int *counters;     // [0] shake failures
+    int nblk_w, nblk_s;
+    // graph of the regular step
+    hipGraphExec_t graph;
+    bool graph_ok;
+    bool built;
+};
+
+// ------------------------------------------------------------ host math
+
+namespace hostm {
+
+// rLegendreP, src/rLegendreP.f90:3-48
+static double legendre(int l, int m, double xx)
+{
+    if (l == -1 && m == 0) return 1.0;
+    if (m > l) return 0.0;
+    double pmm = 1.0;
+    if (m > 0) {
+        double somx2 = std::sqrt((1.0 - xx) * (1.0 + xx));
+        double fact = 1.0;
+        for (int i = 1; i <= m; i++) {
+            pmm = -pmm * fact * somx2;
+            fact = fact + 2.0;
+        }
+    }
+    if (l == m) return pmm;
+    double pmmp1 = xx * (2 * m + 1) * pmm;
+    if (l == m + 1) return pmmp1;
+    double pll = 0.0;
+    for (int ll = m + 2; ll <= l; ll++) {
+        pll = (xx * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pmm) / (ll - m);
+        pmm = pmmp1;
+        pmmp1 = pll;
+    }
+    return pll;
+}
+
+typedef std::complex<double> C;
+
+// Ylm, src/ifma.f90:12-26: the phase factor goes through a default-kind
+// (single precision) cmplx(), so it is rounded to float.
+static C ylm(const double *fact, int n, int m, double a, double b)
+{
+    int am = std::abs(m);
+    double s = std::sqrt(fact[n - am] / fact[n + am]) * legendre(n, am, std::cos(a));
+    float cr = (float)std::cos(m * b), ci = (float)std::sin(m * b);
+    return C(s * (double)cr, s * (double)ci);
+}
+
+static double anm(const double *fact, int n, int m)
+{
+    return (double)((n & 1) ? -1 : 1) / std::sqrt(fact[n - m] * fact[n + m]);
+}
+static int sgnpow(int e) { return (e & 1) ? -1 : 1; }
+static int ikm1(int a, int b) { return sgnpow((std::abs(a) - std::abs(b) - std::abs(a - b)) / 2); }
+static int ikm2(int a, int b) { return sgnpow((std::abs(a - b) - std::abs(a) - std::abs(b)) / 2); }
+static int ikm3(int a, int b) { return sgnpow((std::abs(b) - std::abs(b - a) - std::abs(a)) / 2); }
+static int idx(int n, int m) { return n * (n + 1) + m; }
+static int rowk(int j, int k) { return j * (j + 1) / 2 + k; }  // k >= 0 rows
+static double ipow(double a, int b)
+{
+    int recip = b < 0;
+    if (recip) b = -b;
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+static void angles(double x, double y, double z, double &rho, double &al, double &be)
+{
+    rho = std::sqrt(x * x + y * y + z * z);
+    al = std::acos(z / rho);
+    be = std::atan2(y, x);
+}
+
+// M2L operator, src/computeLjk4.f90:134-160: L_jk += sum_nm T[jk][nm] O_nm
+// for the source-minus-target vector (x, y, z); rows k >= 0 only.
+static void m2l_op(const double *fact, int p, double x, double y, double z, C *T)
+{
+    int lp = (p + 1) * (p + 1);
+    double rho, al, be;
+    angles(x, y, z, rho, al, be);
+    for (int j = 0; j <= p; j++)
+        for (int k = 0; k <= j; k++) {
+            double ajk = anm(fact, j, k);
+            for (int n = 0; n <= p; n++)
+                for (int m = -n; m <= n; m++) {
+                    double rtmp = (double)ikm2(k, m) * ajk * anm(fact, n, m) /
+                                  ((double)sgnpow(n) * anm(fact, j + n, m - k) * ipow(rho, j + n + 1));
+                    T[rowk(j, k) * lp + idx(n, m)] = rtmp * ylm(fact, j + n, m - k, al, be);
+                }
+        }
+}
+
+// M2M operator, src/computeOnm3.f90:236-258: parent_jk += sum M[jk][.] child,
+// vector = child centre - parent centre
+static void m2m_op(const double *fact, int p, double x, double y, double z, C *M)
+{
+    int lp = (p + 1) * (p + 1);
+    double rho, al, be;
+    angles(x, y, z, rho, al, be);
+    for (int i = 0; i < (p + 1) * (p + 2) / 2 * lp; i++) M[i] = 0.0;
+    for (int j = 0; j <= p; j++)
+        for (int k = 0; k <= j; k++) {
+            double ajk = anm(fact, j, k);
+            for (int n = 0; n <= j; n++)
+                for (int m = -n; m <= n; m++) {
+                    if (j - n - std::abs(k - m) < 0) continue;
+                    double rtmp = (double)ikm1(k, m) * anm(fact, n, m) * anm(fact, j - n, k - m) *
+                                  ipow(rho, n) / ajk;
+                    M[rowk(j, k) * lp + idx(j - n, k - m)] = rtmp * ylm(fact, n, -m, al, be);
+                }
+        }
+}
+
+// L2L operator, src/shiftLjk3.f90:118-146: child_jk += sum S[jk][nm] parent_nm,
+// vector = parent centre - child centre
+static void l2l_op(const double *fact, int p, double x, double y, double z, C *S)
+{
+    int lp = (p + 1) * (p + 1);
+    double rho, al, be;
+    angles(x, y, z, rho, al, be);
+    for (int i = 0; i < (p + 1) * (p + 2) / 2 * lp; i++) S[i] = 0.0;
+    for (int j = 0; j <= p; j++)
+        for (int k = 0; k <= j; k++) {
+            double ajk = anm(fact, j, k);
+            for (int n = 0; n <= p; n++)
+                for (int m = -n; m <= n; m++) {
+                    if (n - j - std::abs(m - k) < 0) continue;
+                    double rtmp = (double)ikm3(k, m) * anm(fact, n - j, m - k) * ajk *
+                                  ipow(rho, n - j) / ((double)sgnpow(n + j) * anm(fact, n, m));
+                    S[rowk(j, k) * lp + idx(n, m)] = rtmp * ylm(fact, n - j, m - k, al, be);
+                }
+        }
+}
+
+}  // namespace hostm
+
+// L2 centre along one axis, src/init_box_dimensions.f90:40-42
+static double rcen2(const Geo &g, int ix)
+{
+    return g.rsd * (ix + 1) / (double)g.nb2 - g.rhs - g.rsd / (double)g.nb2 / 2.0;
+}
+
+// ------------------------------------------------------------ device helpers
+
+__device__ __forceinline__ double rsqrt64(double x)
+{
+    // v_rsq_f64 estimate refined by two Newton steps (~1 ulp)
+    double y = __builtin_amdgcn_rsq(x);
+    double hx = 0.5 * x;
+    y = y * fma(-hx * y, y, 1.5);
+    y = y * fma(-hx * y, y, 1.5);
+    return y;
+}
+
+__device__ __forceinline__ int iabs_d(int a) { return a < 0 ? -a : a; }
+
+__device__ __forceinline__ int src_kind(int cls, int ring)
+{
+    if (cls == 0) return ring <= 1 ? ring : -1;
+    if (cls == 1) return ring <= 2 ? 2 + ring : -1;
+    return ring <= 3 ? 5 + ring : -1;
+}
+__device__ __forceinline__ int recv_kind(int cls, int ring)
+{
+    if (cls == 0) return ring == 0 ? 0 : -1;
+    if (cls == 1) return ring <= 1 ? 1 + ring : -1;
+    return ring <= 2 ? 3 + ring : -1;
+}
+__device__ __forceinline__ int cheb(int b1, int b2, int nb)
+{
+    int dx = iabs_d(b1 % nb - b2 % nb), dy = iabs_d((b1 / nb) % nb - (b2 / nb) % nb),
+        dz = iabs_d(b1 / (nb * nb) - b2 / (nb * nb));
+    return max(dx, max(dy, dz));
+}
+__device__ __forceinline__ double d_rcen2(const Geo &g, int ix)
+{
+    return g.rsd * (ix + 1) / (double)g.nb2 - g.rhs - g.rsd / (double)g.nb2 / 2.0;
+}
+// L4 centre along one axis for global box coordinate gx (rcen4 of the rank)
+__device__ __forceinline__ double d_cen4(const Geo &g, int gx)
+{
+    int i = gx % 4 + 1;
+    return 4.0 * g.rs * i / 4.0 - 2.0 * g.rs - 4.0 * g.rs / 8.0 + d_rcen2(g, gx / 4);
+}
+
+// ------------------------------------------------------------ fill kernels
+
+__global__ void k_fill_d(double *p, size_t n, double v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_i(int *p, size_t n, int v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------ rebuild
+
+// Molecule classes and storage boxes: read_config / delete_w1 logic,
+// src/read_config.f90:44-175 (ibox2 + ibox64, int() truncating toward 0).
+__global__ void k_classify(Geo g, const double *__restrict__ r, const int *__restrict__ first,
+                           int *__restrict__ cls, int *__restrict__ anc)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.nmol) return;
+    const double *rx = r, *ry = r + g.n, *rz = r + 2 * g.n;
+    int nb2 = g.nb2;
+    auto ib2 = [&](int s) {
+        return (int)((rx[s] + g.rhs) * nb2 / g.rsd) + (int)((ry[s] + g.rhs) * nb2 / g.rsd) * nb2 +
+               (int)((rz[s] + g.rhs) * nb2 / g.rsd) * nb2 * nb2;
+    };
+    auto ib64 = [&](int s, int rk, int *l) {
+        int qx = rk % nb2, qy = (rk / nb2) % nb2, qz = rk / (nb2 * nb2);
+        l[0] = (int)((rx[s] - d_rcen2(g, qx) + 2.0 * g.rs) / g.rs);
+        l[1] = (int)((ry[s] - d_rcen2(g, qy) + 2.0 * g.rs) / g.rs);
+        l[2] = (int)((rz[s] - d_rcen2(g, qz) + 2.0 * g.rs) / g.rs);
+        return l[0] + 4 * l[1] + 16 * l[2];
+    };
+    auto glob = [&](int rk, const int *l) {
+        int qx = rk % nb2, qy = (rk / nb2) % nb2, qz = rk / (nb2 * nb2);
+        int gx = min(max(4 * qx + l[0], 0), g.nb - 1), gy = min(max(4 * qy + l[1], 0), g.nb - 1),
+            gz = min(max(4 * qz + l[2], 0), g.nb - 1);
+        return gx + g.nb * (gy + g.nb * gz);
+    };
+    int f = first[m];
+    if (m < g.nw) {
+        int ibO = ib2(f), ibG = ib2(f + 1), ibH = ib2(f + 2);
+        int lO[3], lG[3], lH[3];
+        int bO = ib64(f, ibO, lO);
+        bool w1 = false;
+        if (ibO == ibG && ibO == ibH) w1 = (bO == ib64(f + 1, ibO, lG)) && (bO == ib64(f + 2, ibO, lH));
+        cls[m] = w1 ? 0 : 1;
+        anc[m] = glob(ibO, lO);
+    } else {
+        int s = f + 13;  // site 14 files the molecule
+        int rk = ib2(s), l[3];
+        ib64(s, rk, l);
+        cls[m] = 2;
+        anc[m] = glob(rk, l);
+    }
+}
+
+// Site boxes: W1 sites live in their molecule's box; W2 and surfactant sites in
+// the box whose half-open cube [c - rs/2, c + rs/2) holds them (isCentral,
+// src/ibox.f90:101-124, as the list builders use it).  Receiver kinds follow.
+__global__ void k_site_box(Geo g, const double *__restrict__ r, const int *__restrict__ mol,
+                           const int *__restrict__ cls, const int *__restrict__ anc,
+                           int *__restrict__ cbox, int *__restrict__ rkind, int *__restrict__ key,
+                           int *__restrict__ val)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    int m = mol[i];
+    int c = cls[m];
+    int b;
+    if (c == 0) {
+        b = anc[m];
+    } else {
+        int gc[3];
+        for (int a = 0; a < 3; a++) {
+            double x = r[a * g.n + i];
+            int q = (int)floor((x + g.rhs) / g.rs);
+            q = min(max(q, 0), g.nb - 1);
+            int best = q;
+            for (int d = -1; d <= 1; d++) {
+                int qq = q + d;
+                if (qq < 0 || qq >= g.nb) continue;
+                double cc = d_cen4(g, qq);
+                if (x < cc + g.rs / 2.0 && x >= cc - g.rs / 2.0) {
+                    best = qq;
+                    break;
+                }
+            }
+            gc[a] = best;
+        }
+        b = gc[0] + g.nb * (gc[1] + g.nb * gc[2]);
+    }
+    cbox[i] = b;
+    rkind[i] = recv_kind(c, cheb(anc[m], b, g.nb));
+    key[i] = b;
+    val[i] = i;
+}
+
+// buildinternal, src/buildinternal.f90:11-63: the same-molecule partners of a
+// surfactant site that lie outside the 3^3 block around the site's box (box by
+// ibox1000 of the molecule's rank, centre by rcen1000, test by isnearest).
+__global__ void k_isub(Geo g, const double *__restrict__ r, const int *__restrict__ anc,
+                       unsigned *__restrict__ isub)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 29 * g.ns) return;
+    int s = t / 29, j = t % 29;
+    int m = g.nw + s;
+    int base = 3 * g.nw + 29 * s;
+    int a = anc[m];
+    int q[3] = {(a % g.nb) / 4, ((a / g.nb) % g.nb) / 4, (a / (g.nb * g.nb)) / 4};
+    double cen[3];
+    for (int d = 0; d < 3; d++) {
+        double c2 = d_rcen2(g, q[d]);
+        double xm = r[d * g.n + base + j];
+        int li = (int)((xm - c2 + 5.0 * g.rs) / g.rs) + 1;
+        cen[d] = 10.0 * g.rs * li / 10.0 - 10.0 * g.rs / 2.0 - 10.0 * g.rs / 20.0 + c2;
+    }
+    double rs15 = 1.5 * g.rs;
+    unsigned mask = 0;
+    for (int k = 0; k < 29; k++) {
+        if (k == j) continue;
+        bool near = true;
+        for (int d = 0; d < 3; d++) {
+            double xn = r[d * g.n + base + k];
+            near = near && (xn < cen[d] + rs15 && xn >= cen[d] - rs15);
+        }
+        if (!near) mask |= 1u << k;
+    }
+    isub[base + j] = mask;
+}
+
+// first sorted position of every box (binary search in the sorted keys)
+__global__ void k_bstart(Geo g, const int *__restrict__ keys, int *__restrict__ bstart)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > g.nb3) return;
+    int lo = 0, hi = g.n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] < b) lo = mid + 1;
+        else hi = mid;
+    }
+    bstart[b] = lo;
+}
+
+// near-field work list: one item per NEAR_TILE receivers of a box; single block
+__global__ void k_worklist(Geo g, const int *__restrict__ bstart, int2 *__restrict__ work,
+                           int *__restrict__ nwork, int max_work)
+{
+    __shared__ int s_cnt[1024];
+    __shared__ int s_base;
+    if (threadIdx.x == 0) s_base = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < g.nb3; b0 += blockDim.x) {
+        int b = b0 + threadIdx.x;
+        int nch = 0;
+        if (b < g.nb3) nch = (bstart[b + 1] - bstart[b] + NEAR_TILE - 1) / NEAR_TILE;
+        s_cnt[threadIdx.x] = nch;
+        __syncthreads();
+        // inclusive scan (Hillis-Steele)
+        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+            int v = threadIdx.x >= (unsigned)off ? s_cnt[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_cnt[threadIdx.x] += v;
+            __syncthreads();
+        }
+        int excl = s_base + s_cnt[threadIdx.x] - nch;
+        for (int c = 0; c < nch; c++)
+            if (excl + c < max_work) work[excl + c] = make_int2(b, bstart[b] + c * NEAR_TILE);
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) s_base += s_cnt[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *nwork = min(s_base, max_work);
+}
+
+// static metadata of the sorted sites: (site, type | cls << 8, mol, anc)
+__global__ void k_gather_meta(Geo g, const int *__restrict__ perm, const int *__restrict__ type,
+                              const int *__restrict__ mol, const int *__restrict__ cls,
+                              const int *__restrict__ anc, int4 *__restrict__ smeta)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    int i = perm[k];
+    int m = mol[i];
+    smeta[k] = make_int4(i, type[i] | (cls[m] << 8), m, anc[m]);
+}
+
+// sorted positions + charge, every step
+__global__ void k_gather_pos(Geo g, const int *__restrict__ perm, const double *__restrict__ r,
+                             const int *__restrict__ type, const sesmd_forcefield *__restrict__ ff,
+                             double4 *__restrict__ spos)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    int i = perm[k];
+    spos[k] = make_double4(r[i], r[g.n + i], r[2 * g.n + i], ff->q[type[i]]);
+}
+
+// ------------------------------------------------------------ near field
+
+// The near field of the 39 eval[wS]* kernels (e.g. src/evalw1w1.f90:28-245,
+// src/evalSS.f90:13-125): each receiver (a site central in box b at the last
+// rebuild) gets, from every site of another molecule in the 3^3 block around b,
+// Coulomb xpi q_i q_j r/r^3 plus LJ c r^-6 (s6 r^-6 - 1/2) r/r^2 for r <= rc.
+// One 64-lane wavefront per NEAR_TILE receivers of a box; the 27 source boxes
+// are streamed through LDS tiles; each lane owns one receiver, so forces are
+// written without atomics.  MASKED keeps only the (receiver kind, source kind)
+// pairs of `mask` (one reference kernel or a subset of them).
+template <bool MASKED>
+__global__ __launch_bounds__(NEAR_TILE) void k_near(
+    Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
+    const int *__restrict__ bstart, const int *__restrict__ rkind, const int2 *__restrict__ work,
+    const int *__restrict__ nwork, const sesmd_forcefield *__restrict__ ff, unsigned long long mask,
+    double *__restrict__ f)
+{
+    __shared__ double s_ljc[32 * 32];
+    __shared__ double s_ljs6[32 * 32];
+    __shared__ double4 s_pos[NEAR_TILE];
+    __shared__ int s_mol[NEAR_TILE];
+    __shared__ int s_tk[NEAR_TILE];  // type | (source kind + 1) << 8
+
+    int item = blockIdx.x;
+    if (item >= *nwork) return;
+    int2 w = work[item];
+    int b = w.x;
+    for (int t = threadIdx.x; t < 1024; t += NEAR_TILE) {
+        s_ljc[t] = ff->lj_c[t];
+        s_ljs6[t] = ff->lj_s6[t];
+    }
+    int kend = bstart[b + 1];
+    int k = w.y + threadIdx.x;
+    bool active = k < kend;
+    double xi = 0, yi = 0, zi = 0, qi = 0;
+    int ti = 0, mi = -1, rk = -1, site = 0;
+    if (active) {
+        double4 pi = spos[k];
+        int4 me = smeta[k];
+        xi = pi.x;
+        yi = pi.y;
+        zi = pi.z;
+        qi = pi.w * g.xpi;
+        site = me.x;
+        ti = me.y & 255;
+        mi = me.z;
+        rk = rkind[site];
+        if (rk < 0) active = false;
+    }
+    int nb = g.nb;
+    int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+    double accx = 0.0, accy = 0.0, accz = 0.0;
+    const double rc2 = g.rc2;
+    int rowoff = ti * 32;
+    for (int dz = -1; dz <= 1; dz++) {
+        int z = bz + dz;
+        if (z < 0 || z >= nb) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            int y = by + dy;
+            if (y < 0 || y >= nb) continue;
+            int x0 = max(bx - 1, 0), x1 = min(bx + 1, nb - 1);
+            // the three x-neighbours are contiguous in the sorted order
+            int s0 = bstart[x0 + nb * (y + nb * z)];
+            int s1 = bstart[x1 + 1 + nb * (y + nb * z)];
+            for (int t0 = s0; t0 < s1; t0 += NEAR_TILE) {
+                __syncthreads();
+                int t = t0 + threadIdx.x;
+                if (t < s1) {
+                    int4 me = smeta[t];
+                    s_pos[threadIdx.x] = spos[t];
+                    s_mol[threadIdx.x] = me.z;
+                    int sk = src_kind(me.y >> 8, cheb(me.w, b, nb));
+                    s_tk[threadIdx.x] = (me.y & 255) | ((sk + 1) << 8);
+                }
+                __syncthreads();
+                int nt = min(NEAR_TILE, s1 - t0);
+                if (!active) continue;
+                for (int u = 0; u < nt; u++) {
+                    int tk = s_tk[u];
+                    int sk = (tk >> 8) - 1;
+                    bool ok = (s_mol[u] != mi) && (sk >= 0);
+                    if (MASKED) ok = ok && ((mask >> (rk * 9 + sk)) & 1ull);
+                    if (!ok) continue;
+                    double4 pj = s_pos[u];
+                    double dx = xi - pj.x, dy2 = yi - pj.y, dz2 = zi - pj.z;
+                    double r2 = dx * dx + dy2 * dy2 + dz2 * dz2;
+                    double rinv = rsqrt64(r2);
+                    double rinv2 = rinv * rinv;
+                    double ft = qi * pj.w * (rinv2 * rinv);
+                    double c = s_ljc[rowoff + (tk & 255)];
+                    if (c != 0.0 && r2 <= rc2) {
+                        double s6 = s_ljs6[rowoff + (tk & 255)];
+                        double r6 = rinv2 * rinv2 * rinv2;
+                        ft += c * r6 * (s6 * r6 - 0.5) * rinv2;
+                    }
+                    accx = fma(dx, ft, accx);
+                    accy = fma(dy2, ft, accy);
+                    accz = fma(dz2, ft, accz);
+                }
+            }
+        }
+    }
+    if (active) {
+        f[site] += accx;
+        f[g.n + site] += accy;
+        f[2 * g.n + site] += accz;
+    }
+}
+
+// pair counts for roofline accounting (unordered pairs, counted from both sides / 2)
+__global__ void k_pair_count(Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
+                             const int *__restrict__ bstart, const int *__restrict__ rkind,
+                             const sesmd_forcefield *__restrict__ ff,
+                             unsigned long long *__restrict__ cnt)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    int4 me = smeta[k];
+    if (rkind[me.x] < 0) return;
+    double4 pi = spos[k];
+    int b = me.x >= 0 ? 0 : 0;
+    // find the box of sorted position k
+    int lo = 0, hi = g.nb3;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (bstart[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    b = lo;
+    int nb = g.nb, bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+    unsigned long long np = 0, nl = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                int x = bx + dx, y = by + dy, z = bz + dz;
+                if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
+                int bb = x + nb * (y + nb * z);
+                for (int t = bstart[bb]; t < bstart[bb + 1]; t++) {
+                    int4 mj = smeta[t];
+                    if (mj.z == me.z) continue;
+                    if (src_kind(mj.y >> 8, cheb(mj.w, b, nb)) < 0) continue;
+                    np++;
+                    double4 pj = spos[t];
+                    double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
+                    double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (r2 <= g.rc2 && ff->lj_c[(me.y & 255) * 32 + (mj.y & 255)] != 0.0) nl++;
+                }
+            }
+    atomicAdd(&cnt[0], np);
+    atomicAdd(&cnt[1], nl);
+}
+
+// ------------------------------------------------------------ far field
+
+// P2M, src/computeOnm4.f90:142-162: O_n^m = sum q rho^n Y_n^{-m}(alpha, beta)
+// about the box centre over the box's receivers; one block per box, one
+// thread per site, deterministic tree reduction in LDS.
+__global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
+                                            const int *__restrict__ perm,
+                                            const int *__restrict__ bstart,
+                                            const int *__restrict__ rkind,
+                                            const int *__restrict__ type,
+                                            const sesmd_forcefield *__restrict__ ff,
+                                            double2 *__restrict__ O4, int *__restrict__ ne4)
+{
+    __shared__ double s_re[64][MAXLK], s_im[64][MAXLK];
+    int b = blockIdx.x;
+    int p = g.p, lk = g.lk, lp = g.lp;
+    int nb = g.nb;
+    double cx = d_cen4(g, b % nb), cy = d_cen4(g, (b / nb) % nb), cz = d_cen4(g, b / (nb * nb));
+    double are[MAXLK], aim[MAXLK];
+    for (int c = 0; c < lk; c++) are[c] = aim[c] = 0.0;
+    int s0 = bstart[b], s1 = bstart[b + 1];
+    int any = 0;
+    for (int s = s0 + threadIdx.x; s < s1; s += 64) {
+        int i = perm[s];
+        if (rkind[i] < 0) continue;
+        any = 1;
+        double dx = r[i] - cx, dy = r[g.n + i] - cy, dz = r[2 * g.n + i] - cz;
+        double rho = sqrt(dx * dx + dy * dy + dz * dz);
+        double al = acos(dz / rho), be = atan2(dy, dx);
+        double ca = cos(al);
+        double q = ff->q[type[i]];
+        // P_n^m(cos alpha) by the recurrence of rLegendreP, columns m = 0..p
+        double P[MAXP + 1][MAXP + 1];
+        double somx2 = sqrt((1.0 - ca) * (1.0 + ca));
+        for (int m = 0; m <= p; m++) {
+            double pmm = 1.0, fact = 1.0;
+            for (int i2 = 1; i2 <= m; i2++) {
+                pmm = -pmm * fact * somx2;
+                fact = fact + 2.0;
+            }
+            P[m][m] = pmm;
+            if (m + 1 <= p) {
+                double pmmp1 = ca * (2 * m + 1) * pmm;
+                P[m + 1][m] = pmmp1;
+                double pm2 = pmm;
+                for (int ll = m + 2; ll <= p; ll++) {
+                    double pll = (ca * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pm2) / (ll - m);
+                    pm2 = pmmp1;
+                    pmmp1 = pll;
+                    P[ll][m] = pll;
+                }
+            }
+        }
+        double rhon = 1.0;
+        for (int n = 0; n <= p; n++) {
+            for (int m = 0; m <= n; m++) {
+                // Y_n^{-m}: phase exp(-i m beta) rounded to single precision
+                float cr = (float)cos(-m * be), ci = (float)sin(-m * be);
+                double sc = sqrt(ff->rifact[n - m] / ff->rifact[n + m]) * P[n][m];
+                double w = q * rhon;
+                int c = n * (n + 1) / 2 + m;
+                are[c] += w * (sc * (double)cr);
+                aim[c] += w * (sc * (double)ci);
+            }
+            rhon *= rho;
+        }
+    }
+    for (int c = 0; c < lk; c++) {
+        s_re[threadIdx.x][c] = are[c];
+        s_im[threadIdx.x][c] = aim[c];
+    }
+    any = __syncthreads_or(any);
+    for (int off = 32; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int c = 0; c < lk; c++) {
+                s_re[threadIdx.x][c] += s_re[threadIdx.x + off][c];
+                s_im[threadIdx.x][c] += s_im[threadIdx.x + off][c];
+            }
+        __syncthreads();
+    }
+    for (int c = threadIdx.x; c < lp; c += 64) {
+        int n = (int)floor(sqrt((double)c) + 1e-9);
+        int m = c - n * (n + 1);
+        int am = m < 0 ? -m : m;
+        int rrow = n * (n + 1) / 2 + am;
+        double re = s_re[0][rrow], im = s_im[0][rrow];
+        O4[(size_t)b * lp + c] = make_double2(re, m < 0 ? -im : im);
+    }
+    if (threadIdx.x == 0) ne4[b] = any;
+}
+
+// complex (re, im) helpers
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Dense contraction out_jk (+)= sum_c Op[jk][c] in_c for k >= 0 rows, then the
+// k < 0 half by conjugation.  One block of 64 threads per target; lane jk
+// (< lk) accumulates; used by M2M (8 children), M2L (interaction list) and L2L.
+__device__ void contract_rows(const Geo &g, const double2 *__restrict__ op, const double2 *__restrict__ in,
+                              double2 &acc, int row)
+{
+    int lp = g.lp;
+    const double2 *o = op + (size_t)row * lp;
+    for (int c = 0; c < lp; c++) {
+        double2 t = cmul(o[c], in[c]);
+        acc.x += t.x;
+        acc.y += t.y;
+    }
+}
+
+__device__ __forceinline__ void store_rows(const Geo &g, double2 *out, int row, double2 acc)
+{
+    // row -> (j, k)
+    int j = 0;
+    while ((j + 1) * (j + 2) / 2 <= row) j++;
+    int k = row - j * (j + 1) / 2;
+    out[j * (j + 1) + k] = acc;
+    if (k > 0) out[j * (j + 1) - k] = make_double2(acc.x, -acc.y);
+}
+
+// M2M (computeOnm3 / computeOnm2): parent = sum over 8 children of M[child] O_child
+__global__ __launch_bounds__(64) void k_m2m(Geo g, int nd_parent, const double2 *__restrict__ Oc,
+                                            const int *__restrict__ nec, const double2 *__restrict__ M,
+                                            double2 *__restrict__ Op, int *__restrict__ nep)
+{
+    int P = blockIdx.x;
+    int row = threadIdx.x;
+    int px = P % nd_parent, py = (P / nd_parent) % nd_parent, pz = P / (nd_parent * nd_parent);
+    int ndc = 2 * nd_parent;
+    double2 acc = make_double2(0.0, 0.0);
+    int any = 0;
+    for (int kc = 0; kc < 8; kc++) {
+        int C = (2 * px + (kc & 1)) + ndc * ((2 * py + ((kc >> 1) & 1)) + ndc * (2 * pz + (kc >> 2)));
+        if (!nec[C]) continue;
+        any = 1;
+        if (row < g.lk) contract_rows(g, M + (size_t)kc * g.lk * g.lp, Oc + (size_t)C * g.lp, acc, row);
+    }
+    if (row < g.lk) store_rows(g, Op + (size_t)P * g.lp, row, acc);
+    if (row == 0) nep[P] = any;
+}
+
+// M2L over the interaction list (computeLjk4 / computeLjk3): sources are the
+// children of the parent's neighbours that are not neighbours of the target.
+// accumulate = the table already holds the L2L part (shiftLjk zeroes, then
+// computeLjk accumulates).
+__global__ __launch_bounds__(64) void k_m2l(Geo g, int nd, const double2 *__restrict__ O,
+                                            const int *__restrict__ ne, const double2 *__restrict__ T,
+                                            double2 *__restrict__ L)
+{
+    int Tb = blockIdx.x;
+    int row = threadIdx.x;
+    int tx = Tb % nd, ty = (Tb / nd) % nd, tz = Tb / (nd * nd);
+    int lo[3], hi[3], t[3] = {tx, ty, tz};
+    for (int a = 0; a < 3; a++) {
+        int par = t[a] / 2;
+        lo[a] = max(2 * (par - 1), 0);
+        hi[a] = min(2 * (par + 1) + 1, nd - 1);
+    }
+    if (row >= g.lk) return;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int sz = lo[2]; sz <= hi[2]; sz++)
+        for (int sy = lo[1]; sy <= hi[1]; sy++)
+            for (int sx = lo[0]; sx <= hi[0]; sx++) {
+                int dx = sx - tx, dy = sy - ty, dz = sz - tz;
+                if (iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1) continue;
+                int S = sx + nd * (sy + nd * sz);
+                if (!ne[S]) continue;
+                int off = (dx + 3) + 7 * ((dy + 3) + 7 * (dz + 3));
+                contract_rows(g, T + (size_t)off * g.lk * g.lp, O + (size_t)S * g.lp, acc, row);
+            }
+    // add onto the L2L part (row k >= 0), rebuild the k < 0 half
+    int j = 0;
+    while ((j + 1) * (j + 2) / 2 <= row) j++;
+    int k = row - j * (j + 1) / 2;
+    double2 *Lt = L + (size_t)Tb * g.lp;
+    double2 cur = Lt[j * (j + 1) + k];
+    cur.x += acc.x;
+    cur.y += acc.y;
+    Lt[j * (j + 1) + k] = cur;
+    if (k > 0) Lt[j * (j + 1) - k] = make_double2(cur.x, -cur.y);
+}
+
+// top level (computeLjk2): every L2 box that is not adjacent to the target
+__global__ __launch_bounds__(64) void k_m2l_top(Geo g, const double2 *__restrict__ O2,
+                                                const int *__restrict__ ne2,
+                                                const double2 *__restrict__ T2,
+                                                double2 *__restrict__ L2)
+{
+    int Tb = blockIdx.x, row = threadIdx.x;
+    int nd = g.nb2;
+    int tx = Tb % nd, ty = (Tb / nd) % nd, tz = Tb / (nd * nd);
+    if (row >= g.lk) return;
+    double2 acc = make_double2(0.0, 0.0);
+    int w = 2 * nd - 1;
+    for (int S = 0; S < g.n2; S++) {
+        int sx = S % nd, sy = (S / nd) % nd, sz = S / (nd * nd);
+        int dx = sx - tx, dy = sy - ty, dz = sz - tz;
+        if (iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1) continue;
+        if (!ne2[S]) continue;
+        int off = (dx + nd - 1) + w * ((dy + nd - 1) + w * (dz + nd - 1));
+        contract_rows(g, T2 + (size_t)off * g.lk * g.lp, O2 + (size_t)S * g.lp, acc, row);
+    }
+    store_rows(g, L2 + (size_t)Tb * g.lp, row, acc);
+}
+
+// L2L (shiftLjk2 / shiftLjk3): child table = S[child position] parent
+__global__ __launch_bounds__(64) void k_l2l(Geo g, int nd_child, const double2 *__restrict__ Lp,
+                                            const double2 *__restrict__ S, double2 *__restrict__ Lc)
+{
+    int C = blockIdx.x, row = threadIdx.x;
+    int cx = C % nd_child, cy = (C / nd_child) % nd_child, cz = C / (nd_child * nd_child);
+    int ndp = nd_child / 2;
+    int P = cx / 2 + ndp * (cy / 2 + ndp * (cz / 2));
+    int kc = (cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2);
+    if (row >= g.lk) return;
+    double2 acc = make_double2(0.0, 0.0);
+    contract_rows(g, S + (size_t)kc * g.lk * g.lp, Lp + (size_t)P * g.lp, acc, row);
+    store_rows(g, Lc + (size_t)C * g.lp, row, acc);
+}
+
+// ------------------------------------------------------------ per-site terms
+
+// L2P (src/evalLjk4.f90:38-68, the same formula for every receiver class) and
+// the 1/x^13 walls (src/eval_wall.f90:149-188), one thread per site.
+__global__ void k_site_terms(Geo g, unsigned what, const double *__restrict__ r,
+                             const int *__restrict__ type, const int *__restrict__ cbox,
+                             const int *__restrict__ rkind, const double2 *__restrict__ L4,
+                             const double *__restrict__ lfac, const sesmd_forcefield *__restrict__ ff,
+                             double *__restrict__ f)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    double x = r[i], y = r[g.n + i], z = r[2 * g.n + i];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if ((what & SESMD_FAR) && rkind[i] >= 0) {
+        int b = cbox[i], nb = g.nb, p = g.p;
+        double rxpb = x - d_cen4(g, b % nb), rypb = y - d_cen4(g, (b / nb) % nb),
+               rzpb = z - d_cen4(g, b / (nb * nb));
+        double rqq = ff->q[type[i]] * g.xpi;
+        double rho2 = rxpb * rxpb + rypb * rypb;
+        double rr = sqrt(rho2 + rzpb * rzpb);
+        double rp1 = rzpb / rr;
+        double sr = sqrt(rho2);
+        double2 rp2 = make_double2(rxpb / sr, rypb / sr);
+        double rp3 = 1.0 / rho2;
+        // Legendre table P[l][m], l = -1..p (P[-1][0] = 1, rLegendreP's l = -1 case)
+        double P[MAXP + 2][MAXP + 1];
+        for (int l = 0; l <= p + 1; l++)
+            for (int m = 0; m <= p; m++) P[l][m] = 0.0;
+        P[0][0] = 1.0;  // l = -1
+        double somx2 = sqrt((1.0 - rp1) * (1.0 + rp1));
+        for (int m = 0; m <= p; m++) {
+            double pmm = 1.0, fact = 1.0;
+            for (int i2 = 1; i2 <= m; i2++) {
+                pmm = -pmm * fact * somx2;
+                fact = fact + 2.0;
+            }
+            P[m + 1][m] = pmm;
+            if (m + 1 <= p) {
+                double pmmp1 = rp1 * (2 * m + 1) * pmm;
+                P[m + 2][m] = pmmp1;
+                double pm2 = pmm;
+                for (int ll = m + 2; ll <= p; ll++) {
+                    double pll = (rp1 * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pm2) / (ll - m);
+                    pm2 = pmmp1;
+                    pmmp1 = pll;
+                    P[ll + 1][m] = pll;
+                }
+            }
+        }
+        // e^{i k phi} for k = 0..p by repeated multiplication
+        double2 pw[MAXP + 1];
+        pw[0] = make_double2(1.0, 0.0);
+        for (int k2 = 1; k2 <= p; k2++) pw[k2] = cmul(pw[k2 - 1], rp2);
+        const double2 *L = L4 + (size_t)b * g.lp;
+        double rrj = 1.0 / rr;  // rr^(j-1)
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int j = 0; j <= p; j++) {
+            for (int k = -j; k <= j; k++) {
+                int ak = k < 0 ? -k : k;
+                double2 e = pw[ak];
+                if (k < 0) {  // 1 / e^{i|k|phi}
+                    double d = e.x * e.x + e.y * e.y;
+                    e = make_double2(e.x / d, -e.y / d);
+                }
+                double2 l = L[j * (j + 1) + k];
+                double sc = lfac[j * (j + 1) + k] * rrj;
+                double2 rq0 = cmul(make_double2(l.x * sc, l.y * sc), e);
+                double rq1 = (j + ak) * P[j][ak];      // P(j-1, |k|)
+                double rq2 = P[j + 1][ak];             // P(j, |k|)
+                double rq3 = rxpb * rzpb * rq1;
+                double2 rq4 = make_double2(-j * rxpb * rr * rq2, k * rypb * rr * rq2);
+                double rq5 = rypb * rzpb * rq1;
+                double2 rq6 = make_double2(-j * rypb * rr * rq2, -k * rxpb * rr * rq2);
+                double2 t1 = cmul(rq0, make_double2(rq3 + rq4.x, rq4.y));
+                double2 t2 = cmul(rq0, make_double2(rq5 + rq6.x, rq6.y));
+                ax += rqq * rp3 * t1.x;
+                ay += rqq * rp3 * t2.x;
+                az -= rqq * rq0.x * rq1;
+            }
+            rrj *= rr;
+        }
+        fx += ax;
+        fy += ay;
+        fz += az;
+    }
+    if (what & SESMD_WALL) {
+        double h = g.rhs;
+        auto p13 = [](double u) {
+            double u2 = u * u, u4 = u2 * u2, u8 = u4 * u4;
+            return u8 * u4 * u;
+        };
+        fx += 48.0 / p13(x - h) + 48.0 / p13(h + x);
+        fy += 48.0 / p13(y - h) + 48.0 / p13(h + y);
+        fz += 48.0 / p13(z - h) + 48.0 / p13(h + z);
+    }
+    f[i] += fx;
+    f[g.n + i] += fy;
+    f[2 * g.n + i] += fz;
+}
+
+// ------------------------------------------------------------ intramolecular
+
+__device__ __forceinline__ void ld3(const double *r, int n, int i, double *x)
+{
+    x[0] = r[i];
+    x[1] = r[n + i];
+    x[2] = r[2 * n + i];
+}
+
+// one dihedral's forces on its four sites (src/eval_dihedral.f90:116-271)
+__device__ void dihedral_forces(const sesmd_forcefield *ff, int d, const double *Ri,
+                                const double *Rj, const double *Rk, const double *Rl, double F[4][3])
+{
+    double rxji = Rj[0] - Ri[0], ryji = Rj[1] - Ri[1], rzji = Rj[2] - Ri[2];
+    double rxkj = Rk[0] - Rj[0], rykj = Rk[1] - Rj[1], rzkj = Rk[2] - Rj[2];
+    double rxnk = Rl[0] - Rk[0], rynk = Rl[1] - Rk[1], rznk = Rl[2] - Rk[2];
+    double ax = rxji * rykj - ryji * rxkj, ay = rxji * rzkj - rzji * rxkj, az = ryji * rzkj - rzji * rykj;
+    double bx = rxkj * rynk - rykj * rxnk, by = rxkj * rznk - rzkj * rxnk, bz = rykj * rznk - rzkj * rynk;
+    double na = sqrt(ax * ax + ay * ay + az * az), nbn = sqrt(bx * bx + by * by + bz * bz);
+    double arg = (ax * bx + ay * by + az * bz) / na / nbn;
+    double clamp = d < 18 ? ff->clamp_opls : ff->clamp_be;
+    if (arg <= -1.0) arg = -clamp;
+    if (arg >= 1.0) arg = clamp;
+    double th = acos(arg), costh = cos(th), sinth = sin(th);
+    double r1 = d < 18 ? -ff->dva[d] + 2.0 * ff->dvb[d] * sin(2.0 * th) / sinth -
+                             3.0 * ff->dvc[d] * sin(3.0 * th) / sinth
+                       : -3.0 * ff->dva[d] * sin(3.0 * th) / sinth;
+    double c1 = na * na, c2 = nbn * nbn, ab = na * nbn;
+    // site i
+    double fix = (rxkj * (-rykj * rynk - rzkj * rznk) + rxnk * (rykj * rykj + rzkj * rzkj)) / ab;
+    double fiy = (rykj * (-rxkj * rxnk - rzkj * rznk) + rynk * (rxkj * rxkj + rzkj * rzkj)) / ab;
+    double fiz = (rzkj * (-rxkj * rxnk - rykj * rynk) + rznk * (rxkj * rxkj + rykj * rykj)) / ab;
+    double fix2 = (2.0 * rxji * (-rykj * rykj - rzkj * rzkj) + 2.0 * rxkj * (ryji * rykj + rzji * rzkj)) / c1;
+    double fiy2 = (2.0 * ryji * (-rxkj * rxkj - rzkj * rzkj) + 2.0 * rykj * (rxji * rxkj + rzji * rzkj)) / c1;
+    double fiz2 = (2.0 * rzji * (-rxkj * rxkj - rykj * rykj) + 2.0 * rzkj * (rxji * rxkj + ryji * rykj)) / c1;
+    F[0][0] = r1 * (fix - 0.5 * costh * fix2);
+    F[0][1] = r1 * (fiy - 0.5 * costh * fiy2);
+    F[0][2] = r1 * (fiz - 0.5 * costh * fiz2);
+    // site j
+    double fjx = rxji * (-rykj * rynk - rzkj * rznk) + rxkj * (rykj * rynk + rzkj * rznk);
+    double fjy = ryji * (-rxkj * rxnk - rzkj * rznk) + rykj * (rxkj * rxnk + rzkj * rznk);
+    double fjz = rzji * (-rxkj * rxnk - rykj * rynk) + rzkj * (rxkj * rxnk + rykj * rynk);
+    fjx = (fjx + rxnk * (-ryji * rykj - rzji * rzkj - rykj * rykj - rzkj * rzkj) +
+           2.0 * rxkj * (ryji * rynk + rzji * rznk)) / ab;
+    fjy = (fjy + rynk * (-rxji * rxkj - rzji * rzkj - rxkj * rxkj - rzkj * rzkj) +
+           2.0 * rykj * (rxji * rxnk + rzji * rznk)) / ab;
+    fjz = (fjz + rznk * (-rxji * rxkj - ryji * rykj - rxkj * rxkj - rykj * rykj) +
+           2.0 * rzkj * (rxji * rxnk + ryji * rynk)) / ab;
+    double fjx3 = (2.0 * rxji * (rykj * rykj + rzkj * rzkj + ryji * rykj + rzji * rzkj) +
+                   2.0 * rxkj * (-ryji * ryji - rzji * rzji - ryji * rykj - rzji * rzkj)) / c1;
+    double fjy3 = (2.0 * ryji * (rxkj * rxkj + rzkj * rzkj + rxji * rxkj + rzji * rzkj) +
+                   2.0 * rykj * (-rxji * rxji - rzji * rzji - rxji * rxkj - rzji * rzkj)) / c1;
+    double fjz3 = (2.0 * rzji * (rxkj * rxkj + rykj * rykj + rxji * rxkj + ryji * rykj) +
+                   2.0 * rzkj * (-rxji * rxji - ryji * ryji - rxji * rxkj - ryji * rykj)) / c1;
+    double fjx5 = (2.0 * rxnk * (rykj * rynk + rzkj * rznk) + 2.0 * rxkj * (-rynk * rynk - rznk * rznk)) / c2;
+    double fjy5 = (2.0 * rynk * (rxkj * rxnk + rzkj * rznk) + 2.0 * rykj * (-rxnk * rxnk - rznk * rznk)) / c2;
+    double fjz5 = (2.0 * rznk * (rxkj * rxnk + rykj * rynk) + 2.0 * rzkj * (-rxnk * rxnk - rynk * rynk)) / c2;
+    F[1][0] = r1 * (fjx - 0.5 * costh * (fjx3 + fjx5));
+    F[1][1] = r1 * (fjy - 0.5 * costh * (fjy3 + fjy5));
+    F[1][2] = r1 * (fjz - 0.5 * costh * (fjz3 + fjz5));
+    // site k
+    double fkx = (rxji * (rykj * rykj + rzkj * rzkj + rykj * rynk + rzkj * rznk) +
+                  rxkj * (-ryji * rykj - rzji * rzkj) + rxnk * (ryji * rykj + rzji * rzkj) +
+                  2.0 * rxkj * (-ryji * rynk - rzji * rznk)) / ab;
+    double fky = (ryji * (rxkj * rxkj + rzkj * rzkj + rxkj * rxnk + rzkj * rznk) +
+                  rykj * (-rxji * rxkj - rzji * rzkj) + rynk * (rxji * rxkj + rzji * rzkj) +
+                  2.0 * rykj * (-rxji * rxnk - rzji * rznk)) / ab;
+    double fkz = (rzji * (rxkj * rxkj + rykj * rykj + rxkj * rxnk + rykj * rynk) +
+                  rzkj * (-rxji * rxkj - ryji * rykj) + rznk * (rxji * rxkj + ryji * rykj) +
+                  2.0 * rzkj * (-rxji * rxnk - ryji * rynk)) / ab;
+    double fkx4 = (2.0 * rxji * (-ryji * rykj - rzji * rzkj) + 2.0 * rxkj * (ryji * ryji + rzji * rzji)) / c1;
+    double fky4 = (2.0 * ryji * (-rxji * rxkj - rzji * rzkj) + 2.0 * rykj * (rxji * rxji + rzji * rzji)) / c1;
+    double fkz4 = (2.0 * rzji * (-rxji * rxkj - ryji * rykj) + 2.0 * rzkj * (rxji * rxji + ryji * ryji)) / c1;
+    double fkx5 = (2.0 * rxnk * (-rykj * rykj - rzkj * rzkj - rykj * rynk - rzkj * rznk) +
+                   2.0 * rxkj * (rynk * rynk + rznk * rznk + rykj * rynk + rzkj * rznk)) / c2;
+    double fky5 = (2.0 * rynk * (-rxkj * rxkj - rzkj * rzkj - rxkj * rxnk - rzkj * rznk) +
+                   2.0 * rykj * (rxnk * rxnk + rznk * rznk + rxkj * rxnk + rzkj * rznk)) / c2;
+    double fkz5 = (2.0 * rznk * (-rxkj * rxkj - rykj * rykj - rxkj * rxnk - rykj * rynk) +
+                   2.0 * rzkj * (rxnk * rxnk + rynk * rynk + rxkj * rxnk + rykj * rynk)) / c2;
+    F[2][0] = r1 * (fkx - 0.5 * costh * (fkx4 + fkx5));
+    F[2][1] = r1 * (fky - 0.5 * costh * (fky4 + fky5));
+    F[2][2] = r1 * (fkz - 0.5 * costh * (fkz4 + fkz5));
+    // site l
+    double fnx = (rxji * (-rykj * rykj - rzkj * rzkj) + rxkj * (ryji * rykj + rzji * rzkj)) / ab;
+    double fny = (ryji * (-rxkj * rxkj - rzkj * rzkj) + rykj * (rxji * rxkj + rzji * rzkj)) / ab;
+    double fnz = (rzji * (-rxkj * rxkj - rykj * rykj) + rzkj * (rxji * rxkj + ryji * rykj)) / ab;
+    double fnx2 = (2.0 * rxnk * (rykj * rykj + rzkj * rzkj) + 2.0 * rxkj * (-rykj * rynk - rzkj * rznk)) / c2;
+    double fny2 = (2.0 * rynk * (rxkj * rxkj + rzkj * rzkj) + 2.0 * rykj * (-rxkj * rxnk - rzkj * rznk)) / c2;
+    double fnz2 = (2.0 * rznk * (rxkj * rxkj + rykj * rykj) + 2.0 * rzkj * (-rxkj * rxnk - rykj * rynk)) / c2;
+    F[3][0] = r1 * (fnx - 0.5 * costh * fnx2);
+    F[3][1] = r1 * (fny - 0.5 * costh * fny2);
+    F[3][2] = r1 * (fnz - 0.5 * costh * fnz2);
+}
+
+// Intramolecular surfactant terms, one thread per surfactant site; the thread
+// evaluates every term its site takes part in (from per-site term lists), so
+// each site's force is written by exactly one thread:
+//   eval_surfactant_internal (src/eval_surfactant_internal.f90:12-110),
+//   eval_subtract_fma (src/eval_subtract_fma.f90:10-54),
+//   eval_valence (src/eval_valence.f90:97-158),
+//   eval_dihedral / eval_dihedral_berendsen (src/eval_dihedral.f90:102-295).
+__global__ void k_intra(Geo g, unsigned what, const double *__restrict__ r,
+                        const unsigned *__restrict__ isub, const sesmd_forcefield *__restrict__ ff,
+                        const Topo *__restrict__ topo, double *__restrict__ f)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 29 * g.ns) return;
+    int s = t / 29, j = t % 29;
+    int base = 3 * g.nw + 29 * s;
+    int n = g.n;
+    double xs[3];
+    ld3(r, n, base + j, xs);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (what & SESMD_INTERNAL) {
+        for (int q = 0; q < topo->npair[j]; q++) {
+            int k = topo->pair[j][q];
+            double xk[3];
+            ld3(r, n, base + k, xk);
+            double x = xs[0] - xk[0], y = xs[1] - xk[1], z = xs[2] - xk[2];
+            double rsq = x * x + y * y + z * z;
+            double rctr = sqrt(rsq);
+            int tj = 3 + j, tk = 3 + k;
+            double fel = ff->q[tj] * ff->q[tk] * g.xpi / (rctr * rctr * rctr);
+            double rsi = 1.0 / rsq, r6 = rsi * rsi * rsi;
+            double flj = ff->lj_c[tj * 32 + tk] * r6 * (ff->lj_s6[tj * 32 + tk] * r6 - 0.5) * rsi;
+            double ft = fel + flj;
+            fx += x * ft;
+            fy += y * ft;
+            fz += z * ft;
+        }
+    }
+    if (what & SESMD_SUBTRACT) {
+        unsigned mask = isub[base + j];
+        while (mask) {
+            int k = __ffs(mask) - 1;
+            mask &= mask - 1;
+            double xk[3];
+            ld3(r, n, base + k, xk);
+            double xmn = xs[0] - xk[0], ymn = xs[1] - xk[1], zmn = xs[2] - xk[2];
+            double rmn = sqrt(xmn * xmn + ymn * ymn + zmn * zmn);
+            double fel = ff->q[3 + j] * ff->q[3 + k] * g.xpi / (rmn * rmn * rmn);
+            fx -= xmn * fel;
+            fy -= ymn * fel;
+            fz -= zmn * fel;
+        }
+    }
+    if (what & SESMD_VALENCE) {
+        for (int q = 0; q < topo->nang[j]; q++) {
+            int a = topo->ang[j][q] / 3, role = topo->ang[j][q] % 3;
+            double ri[3], rj[3], rk[3];
+            ld3(r, n, base + ff->ang_i[a] - 1, ri);
+            ld3(r, n, base + ff->ang_j[a] - 1, rj);
+            ld3(r, n, base + ff->ang_k[a] - 1, rk);
+            double rxji = ri[0] - rj[0], ryji = ri[1] - rj[1], rzji = ri[2] - rj[2];
+            double rxki = rk[0] - rj[0], ryki = rk[1] - rj[1], rzki = rk[2] - rj[2];
+            double dp = rxji * rxki + ryji * ryki + rzji * rzki;
+            double rjin = sqrt(rxji * rxji + ryji * ryji + rzji * rzji);
+            double rkin = sqrt(rxki * rxki + ryki * ryki + rzki * rzki);
+            double th = acos(dp / rjin / rkin);
+            double costh = cos(th);
+            double r1 = ff->rkth[a] * (th - ff->th[a]) / sin(th);
+            double fjx = rxki / rjin / rkin - costh * rxji / (rjin * rjin);
+            double fjy = ryki / rjin / rkin - costh * ryji / (rjin * rjin);
+            double fjz = rzki / rjin / rkin - costh * rzji / (rjin * rjin);
+            double fkx = rxji / rjin / rkin - costh * rxki / (rkin * rkin);
+            double fky = ryji / rjin / rkin - costh * ryki / (rkin * rkin);
+            double fkz = rzji / rjin / rkin - costh * rzki / (rkin * rkin);
+            if (role == 0) {
+                fx += r1 * fjx; fy += r1 * fjy; fz += r1 * fjz;
+            } else if (role == 1) {
+                fx -= r1 * (fjx + fkx); fy -= r1 * (fjy + fky); fz -= r1 * (fjz + fkz);
+            } else {
+                fx += r1 * fkx; fy += r1 * fky; fz += r1 * fkz;
+            }
+        }
+    }
+    if (what & (SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE)) {
+        for (int q = 0; q < topo->ndih[j]; q++) {
+            int d = topo->dih[j][q] / 4, role = topo->dih[j][q] % 4;
+            if (d < 18 && !(what & SESMD_DIHEDRAL)) continue;
+            if (d >= 18 && !(what & SESMD_DIHEDRAL_BE)) continue;
+            double Ri[3], Rj[3], Rk[3], Rl[3];
+            ld3(r, n, base + ff->dih_i[d] - 1, Ri);
+            ld3(r, n, base + ff->dih_j[d] - 1, Rj);
+            ld3(r, n, base + ff->dih_k[d] - 1, Rk);
+            ld3(r, n, base + ff->dih_l[d] - 1, Rl);
+            double F[4][3];
+            dihedral_forces(ff, d, Ri, Rj, Rk, Rl, F);
+            fx += F[role][0];
+            fy += F[role][1];
+            fz += F[role][2];
+        }
+    }
+    f[base + j] += fx;
+    f[n + base + j] += fy;
+    f[2 * n + base + j] += fz;
+}
+
+// eval_graphite, src/eval_graphite.f90:13-132: frozen-carbon LJ within rc on
+// the receivers filed in their own box (W1/W2 oxygens, surfactant sites of
+// ring 0) of the bottom L4 layer; one block per bottom-layer box.
+__global__ void k_graphite(Geo g, const double *__restrict__ r, const int *__restrict__ perm,
+                           const int *__restrict__ bstart, const int *__restrict__ rkind,
+                           const int *__restrict__ type, const double *__restrict__ gr,
+                           const int *__restrict__ gstart, const int *__restrict__ gidx,
+                           const sesmd_forcefield *__restrict__ ff, double *__restrict__ f)
+{
+    int b = blockIdx.x;  // z = 0 layer: boxes 0 .. nb^2-1
+    for (int s = bstart[b] + threadIdx.x; s < bstart[b + 1]; s += blockDim.x) {
+        int i = perm[s];
+        int rk = rkind[i];
+        if (!(rk == 0 || rk == 1 || rk == 3)) continue;
+        int t = type[i];
+        double c = ff->gr_c[t], s6 = ff->gr_s6[t];
+        if (c == 0.0) continue;
+        double x = r[i], y = r[g.n + i], z = r[2 * g.n + i];
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int q = gstart[b]; q < gstart[b + 1]; q++) {
+            int a = gidx[q];
+            double xij = x - gr[3 * a], yij = y - gr[3 * a + 1], zij = z - gr[3 * a + 2];
+            double rijsq = xij * xij + yij * yij + zij * zij;
+            if (sqrt(rijsq) <= g.rc) {
+                double rsi = 1.0 / rijsq, r6 = rsi * rsi * rsi;
+                double flj = c * r6 * (s6 * r6 - 0.5) * rsi;
+                fx += xij * flj;
+                fy += yij * flj;
+                fz += zij * flj;
+            }
+        }
+        f[i] += fx;
+        f[g.n + i] += fy;
+        f[2 * g.n + i] += fz;
+    }
+}
+
+// ------------------------------------------------------------ integrator
+
+// NH partial sums per block: [0..8] before, [9..17] after; groups W1-O, W1-H,
+// W2-O, W2-H, S-H, S-O, S-CH2, S-Si, S-CH3 (src/nh_water1.f90:285-313,
+// src/nh_surfactant.f90:1-199)
+__device__ __constant__ int c_sgrp[29] = {0, 1, 2, 2, 1, 2, 2, 1, 2, 2, 1, 2, 2, 1, 2,
+                                          2, 2, 3, 4, 1, 3, 4, 4, 4, 1, 3, 4, 4, 4};
+
+template <int NV>
+__device__ void block_sum_store(double (&v)[NV], double *out)
+{
+    __shared__ double s[256][NV];
+    for (int c = 0; c < NV; c++) s[threadIdx.x][c] = v[c];
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int c = 0; c < NV; c++) s[threadIdx.x][c] += s[threadIdx.x + off][c];
+        __syncthreads();
+    }
+    if (threadIdx.x < NV) out[threadIdx.x] = s[0][threadIdx.x];
+}
+
+// Part A for waters (src/move_w1_a.f90:29-146 / move_w2_a): predictor with
+// the NH friction, SHAKE over the three cyclic constraints, kinetic sums before
+// and after, force zeroing.  One thread per molecule, 256 per block.
+__global__ __launch_bounds__(256) void k_move_a_water(Geo g, double *__restrict__ r,
+                                                      double *__restrict__ v,
+                                                      const double *__restrict__ a,
+                                                      double *__restrict__ f,
+                                                      const int *__restrict__ cls,
+                                                      const sesmd_forcefield *__restrict__ ff,
+                                                      const double *__restrict__ nh,
+                                                      double *__restrict__ partials,
+                                                      int *__restrict__ counters)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[NH_VALS];
+    for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
+    int n = g.n;
+    if (m < g.nw) {
+        double escale = 1.0 - nh[0] * g.dt / 2.0;
+        int base = 3 * m;
+        int grp = cls[m] == 0 ? 0 : 2;
+        double rx[3][3], px[3][3], vx[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int d = 0; d < 3; d++) {
+                double rr = r[d * n + base + i], vv = v[d * n + base + i], aa = a[d * n + base + i];
+                rx[i][d] = rr;
+                px[i][d] = rr + vv * escale + aa;
+                vx[i][d] = vv * escale + aa;
+            }
+        for (int i = 0; i < 3; i++) {
+            double vv = v[base + i] * v[base + i] + v[n + base + i] * v[n + base + i] +
+                        v[2 * n + base + i] * v[2 * n + base + i];
+            acc[grp + (i == 0 ? 0 : 1)] += vv;
+        }
+        double rm[3] = {1.0 / ff->mass[0], 1.0 / ff->mass[1], 1.0 / ff->mass[2]};
+        double tol2 = 2.0 * 0.0001;
+        int it = 0;
+        bool done = false;
+        while (!done && it <= 100) {
+            done = true;
+            for (int c = 0; c < 3; c++) {
+                int ia = c, ib = c + 1 > 2 ? 0 : c + 1;
+                double rabsq = ff->dsqw[c];
+                double pxab = px[ia][0] - px[ib][0], pyab = px[ia][1] - px[ib][1], pzab = px[ia][2] - px[ib][2];
+                double pabsq = pxab * pxab + pyab * pyab + pzab * pzab;
+                double diffsq = rabsq - pabsq;
+                if (fabs(diffsq) > rabsq * tol2) {
+                    double rxab = rx[ia][0] - rx[ib][0], ryab = rx[ia][1] - rx[ib][1], rzab = rx[ia][2] - rx[ib][2];
+                    double rpab = rxab * pxab + ryab * pyab + rzab * pzab;
+                    double gab = diffsq / (2.0 * (rm[ia] + rm[ib]) * rpab);
+                    double dd[3] = {rxab * gab, ryab * gab, rzab * gab};
+                    for (int d = 0; d < 3; d++) {
+                        px[ia][d] += rm[ia] * dd[d];
+                        px[ib][d] -= rm[ib] * dd[d];
+                        vx[ia][d] += rm[ia] * dd[d];
+                        vx[ib][d] -= rm[ib] * dd[d];
+                    }
+                    done = false;
+                }
+            }
+            it++;
+        }
+        if (!done) atomicAdd(&counters[0], 1);
+        for (int i = 0; i < 3; i++) {
+            for (int d = 0; d < 3; d++) {
+                r[d * n + base + i] = px[i][d];
+                v[d * n + base + i] = vx[i][d];
+                f[d * n + base + i] = 0.0;
+            }
+            acc[9 + grp + (i == 0 ? 0 : 1)] +=
+                vx[i][0] * vx[i][0] + vx[i][1] * vx[i][1] + vx[i][2] * vx[i][2];
+        }
+    }
+    block_sum_store<NH_VALS>(acc, partials + (size_t)blockIdx.x * NH_VALS);
+}
+
+// Part A for surfactants (src/move_surfactant_a.f90:19-147): 28 bond
+// constraints, Gauss-Seidel; one thread per molecule.
+__global__ __launch_bounds__(64) void k_move_a_surf(Geo g, double *__restrict__ r,
+                                                    double *__restrict__ v,
+                                                    const double *__restrict__ a,
+                                                    double *__restrict__ f,
+                                                    const sesmd_forcefield *__restrict__ ff,
+                                                    const double *__restrict__ nh,
+                                                    double *__restrict__ partials,
+                                                    int *__restrict__ counters)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[NH_VALS];
+    for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
+    int n = g.n;
+    if (s < g.ns) {
+        double escale = 1.0 - nh[0] * g.dt / 2.0;
+        int base = 3 * g.nw + 29 * s;
+        double rx[29][3], px[29][3], vx[29][3];
+        for (int i = 0; i < 29; i++) {
+            double vv = 0.0;
+            for (int d = 0; d < 3; d++) {
+                double rr = r[d * n + base + i], v0 = v[d * n + base + i], aa = a[d * n + base + i];
+                rx[i][d] = rr;
+                px[i][d] = rr + v0 * escale + aa;
+                vx[i][d] = v0 * escale + aa;
+                vv += v0 * v0;
+            }
+            acc[4 + c_sgrp[i]] += vv;
+        }
+        double tol2 = 2.0 * 0.0001;
+        int it = 0;
+        bool done = false;
+        while (!done && it <= 100) {
+            done = true;
+            for (int c = 0; c < 28; c++) {
+                int ia = ff->ibond[c] - 1, ib = ff->jbond[c] - 1;
+                double rabsq = ff->dsqs[c];
+                double pxab = px[ia][0] - px[ib][0], pyab = px[ia][1] - px[ib][1], pzab = px[ia][2] - px[ib][2];
+                double pabsq = pxab * pxab + pyab * pyab + pzab * pzab;
+                double diffsq = rabsq - pabsq;
+                if (fabs(diffsq) > rabsq * tol2) {
+                    double rxab = rx[ia][0] - rx[ib][0], ryab = rx[ia][1] - rx[ib][1], rzab = rx[ia][2] - rx[ib][2];
+                    double rpab = rxab * pxab + ryab * pyab + rzab * pzab;
+                    double rma = 1.0 / ff->mass[3 + ia], rmb = 1.0 / ff->mass[3 + ib];
+                    double gab = diffsq / (2.0 * (rma + rmb) * rpab);
+                    double dd[3] = {rxab * gab, ryab * gab, rzab * gab};
+                    for (int d = 0; d < 3; d++) {
+                        px[ia][d] += rma * dd[d];
+                        px[ib][d] -= rmb * dd[d];
+                        vx[ia][d] += rma * dd[d];
+                        vx[ib][d] -= rmb * dd[d];
+                    }
+                    done = false;
+                }
+            }
+            it++;
+        }
+        if (!done) atomicAdd(&counters[0], 1);
+        for (int i = 0; i < 29; i++) {
+            for (int d = 0; d < 3; d++) {
+                r[d * n + base + i] = px[i][d];
+                v[d * n + base + i] = vx[i][d];
+                f[d * n + base + i] = 0.0;
+            }
+            acc[13 + c_sgrp[i]] += vx[i][0] * vx[i][0] + vx[i][1] * vx[i][1] + vx[i][2] * vx[i][2];
+        }
+    }
+    block_sum_store<NH_VALS>(acc, partials + (size_t)(g.nw > 0 ? (g.nw + 255) / 256 : 0) * NH_VALS +
+                                      (size_t)blockIdx.x * NH_VALS);
+}
+
+// kinetic sums -> the eta update of src/move_w2_a.f90:145-157 (single block)
+__global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restrict__ partials,
+                                                   int nblocks, const sesmd_forcefield *__restrict__ ff,
+                                                   double *__restrict__ nh)
+{
+    double acc[NH_VALS];
+    for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int c = 0; c < NH_VALS; c++) acc[c] += partials[(size_t)b * NH_VALS + c];
+    __shared__ double tot[NH_VALS];
+    block_sum_store<NH_VALS>(acc, tot);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double *m = ff->mass;
+        double S[6];
+        for (int ph = 0; ph < 2; ph++) {
+            const double *t = tot + 9 * ph;
+            S[0 + ph] = m[3] * t[4] + m[4] * t[5] + m[5] * t[6] + m[3 + 17] * t[7] + m[3 + 18] * t[8];
+            S[2 + ph] = m[0] * t[0] + m[1] * t[1];
+            S[4 + ph] = m[0] * t[2] + m[1] * t[3];
+        }
+        double dt2 = g.dt * g.dt;
+        double sum1 = (S[0] + S[2] + S[4]) / dt2;
+        double sum2 = (S[1] + S[3] + S[5]) / dt2;
+        nh[0] = nh[0] + nh[8] * (sum1 + nh[9] + sum2);
+        for (int c = 0; c < 6; c++) nh[1 + c] = S[c];
+    }
+}
+
+// Part B for waters (src/move_w1_b.f90:23-123): a = F dt^2/2m, velocity with
+// the new eta, RATTLE.
+__global__ __launch_bounds__(256) void k_move_b_water(Geo g, const double *__restrict__ r,
+                                                      double *__restrict__ v, double *__restrict__ a,
+                                                      const double *__restrict__ f,
+                                                      const sesmd_forcefield *__restrict__ ff,
+                                                      const double *__restrict__ nh,
+                                                      int *__restrict__ counters)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.nw) return;
+    int n = g.n, base = 3 * m;
+    double escale = 2.0 / (2.0 + nh[0] * g.dt);
+    double rx[3][3], vx[3][3], rm[3];
+    for (int i = 0; i < 3; i++) {
+        double mi = ff->mass[i];
+        rm[i] = 1.0 / mi;
+        for (int d = 0; d < 3; d++) {
+            rx[i][d] = r[d * n + base + i];
+            double ac = f[d * n + base + i] * g.dtsq2 / mi;
+            a[d * n + base + i] = ac;
+            vx[i][d] = escale * (v[d * n + base + i] + ac);
+        }
+    }
+    int it = 0;
+    bool done = false;
+    while (!done && it <= 100) {
+        done = true;
+        for (int c = 0; c < 3; c++) {
+            int ia = c, ib = c + 1 > 2 ? 0 : c + 1;
+            double vxab = vx[ia][0] - vx[ib][0], vyab = vx[ia][1] - vx[ib][1], vzab = vx[ia][2] - vx[ib][2];
+            double rxab = rx[ia][0] - rx[ib][0], ryab = rx[ia][1] - rx[ib][1], rzab = rx[ia][2] - rx[ib][2];
+            double rvab = (rxab * vxab + ryab * vyab + rzab * vzab) / g.dt;
+            double gab = -rvab / ((rm[ia] + rm[ib]) * ff->dsqw[c]);
+            if (fabs(gab) > 0.0001) {
+                double dd[3] = {rxab * gab, ryab * gab, rzab * gab};
+                for (int d = 0; d < 3; d++) {
+                    vx[ia][d] += rm[ia] * dd[d] * g.dt;
+                    vx[ib][d] -= rm[ib] * dd[d] * g.dt;
+                }
+                done = false;
+            }
+        }
+        it++;
+    }
+    if (!done) atomicAdd(&counters[0], 1);
+    for (int i = 0; i < 3; i++)
+        for (int d = 0; d < 3; d++) v[d * n + base + i] = vx[i][d];
+}
+
+// Part B for surfactants (src/move_surfactant_b.f90:23-122)
+__global__ __launch_bounds__(64) void k_move_b_surf(Geo g, const double *__restrict__ r,
+                                                    double *__restrict__ v, double *__restrict__ a,
+                                                    const double *__restrict__ f,
+                                                    const sesmd_forcefield *__restrict__ ff,
+                                                    const double *__restrict__ nh,
+                                                    int *__restrict__ counters)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.ns) return;
+    int n = g.n, base = 3 * g.nw + 29 * s;
+    double escale = 2.0 / (2.0 + nh[0] * g.dt);
+    double rx[29][3], vx[29][3];
+    for (int i = 0; i < 29; i++) {
+        double mi = ff->mass[3 + i];
+        for (int d = 0; d < 3; d++) {
+            rx[i][d] = r[d * n + base + i];
+            double ac = f[d * n + base + i] * g.dtsq2 / mi;
+            a[d * n + base + i] = ac;
+            vx[i][d] = escale * (v[d * n + base + i] + ac);
+        }
+    }
+    int it = 0;
+    bool done = false;
+    while (!done && it <= 100) {
+        done = true;
+        for (int c = 0; c < 28; c++) {
+            int ia = ff->ibond[c] - 1, ib = ff->jbond[c] - 1;
+            double rma = 1.0 / ff->mass[3 + ia], rmb = 1.0 / ff->mass[3 + ib];
+            double vxab = vx[ia][0] - vx[ib][0], vyab = vx[ia][1] - vx[ib][1], vzab = vx[ia][2] - vx[ib][2];
+            double rxab = rx[ia][0] - rx[ib][0], ryab = rx[ia][1] - rx[ib][1], rzab = rx[ia][2] - rx[ib][2];
+            double rvab = (rxab * vxab + ryab * vyab + rzab * vzab) / g.dt;
+            double gab = -rvab / ((rma + rmb) * ff->dsqs[c]);
+            if (fabs(gab) > 0.0001) {
+                double dd[3] = {rxab * gab, ryab * gab, rzab * gab};
+                for (int d = 0; d < 3; d++) {
+                    vx[ia][d] += rma * dd[d] * g.dt;
+                    vx[ib][d] -= rmb * dd[d] * g.dt;
+                }
+                done = false;
+            }
+        }
+        it++;
+    }
+    if (!done) atomicAdd(&counters[0], 1);
+    for (int i = 0; i < 29; i++)
+        for (int d = 0; d < 3; d++) v[d * n + base + i] = vx[i][d];
+}
+
+// init_accel, src/init_accel.f90:1-49
+__global__ void k_init_accel(Geo g, const int *__restrict__ type, const double *__restrict__ f,
+                             const sesmd_forcefield *__restrict__ ff, double *__restrict__ a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    double mi = ff->mass[type[i]];
+    for (int d = 0; d < 3; d++) a[d * g.n + i] = f[d * g.n + i] * g.dtsq2 / mi;
+}
+
+// ------------------------------------------------------------ host side
+
+static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+template <typename T>
+static int dalloc(T **p, size_t n)
+{
+    HC(hipMalloc((void **)p, sizeof(T) * (n ? n : 1)));
+    HC(hipMemset(*p, 0, sizeof(T) * (n ? n : 1)));
+    return 0;
+}
+
+#define CK(x)                  \
+    do {                       \
+        if ((x) != 0) return 1; \
+    } while (0)
+
+static void build_topology(const sesmd_forcefield &ff, Topo &t)
+{
+    memset(&t, 0, sizeof t);
+    for (int p = 0; p < ff.n_internal; p++) {
+        int j = ff.int_j[p] - 1, k = ff.int_k[p] - 1;
+        t.pair[j][t.npair[j]++] = k;
+        t.pair[k][t.npair[k]++] = j;
+    }
+    for (int a = 0; a < 36; a++) {
+        int s[3] = {ff.ang_i[a] - 1, ff.ang_j[a] - 1, ff.ang_k[a] - 1};
+        for (int role = 0; role < 3; role++) t.ang[s[role]][t.nang[s[role]]++] = 3 * a + role;
+    }
+    for (int d = 0; d < 30; d++) {
+        int s[4] = {ff.dih_i[d] - 1, ff.dih_j[d] - 1, ff.dih_k[d] - 1, ff.dih_l[d] - 1};
+        for (int role = 0; role < 4; role++) t.dih[s[role]][t.ndih[s[role]]++] = 4 * d + role;
+    }
+}
+
+static int check_topology(const sesmd_forcefield &ff)
+{
+    int np[29] = {0}, na[29] = {0}, nd[29] = {0};
+    if (ff.n_internal < 0 || ff.n_internal > 320) return fail("n_internal out of range");
+    for (int p = 0; p < ff.n_internal; p++) {
+        int j = ff.int_j[p], k = ff.int_k[p];
+        if (j < 1 || j > 29 || k < 1 || k > 29 || j == k) return fail("bad internal pair %d", p);
+        if (++np[j - 1] > 32 || ++np[k - 1] > 32) return fail("too many internal pairs per site");
+    }
+    for (int a = 0; a < 36; a++)
+        for (int s : {ff.ang_i[a], ff.ang_j[a], ff.ang_k[a]}) {
+            if (s < 1 || s > 29) return fail("bad angle site");
+            if (++na[s - 1] > 12) return fail("too many angles per site");
+        }
+    for (int d = 0; d < 30; d++)
+        for (int s : {ff.dih_i[d], ff.dih_j[d], ff.dih_k[d], ff.dih_l[d]}) {
+            if (s < 1 || s > 29) return fail("bad dihedral site");
+            if (++nd[s - 1] > 16) return fail("too many dihedrals per site");
+        }
+    for (int b = 0; b < 28; b++)
+        if (ff.ibond[b] < 1 || ff.ibond[b] > 29 || ff.jbond[b] < 1 || ff.jbond[b] > 29)
+            return fail("bad bond");
+    return 0;
+}
+
+static int upload_ops(sesmd_ctx *c)
+{
+    const Geo &g = c->g;
+    int lp = g.lp, lk = g.lk, p = g.p;
+    const double *fact = c->ff.rifact;
+    std::vector<hostm::C> T((size_t)lk * lp);
+    std::vector<double2> buf;
+    auto put = [&](double2 *dst, size_t count) -> int {
+        HC(hipMemcpy(dst, buf.data(), sizeof(double2) * count, hipMemcpyHostToDevice));
+        return 0;
+    };
+    auto append = [&]() {
+        for (auto &z : T) buf.push_back(make_double2(z.real(), z.imag()));
+    };
+    // M2L at L4 and L3: offsets in [-3, 3]^3 (spacing rs and 2 rs)
+    for (int lvl = 0; lvl < 2; lvl++) {
+        buf.clear();
+        double sp = lvl == 0 ? g.rs : 2.0 * g.rs;
+        for (int dz = -3; dz <= 3; dz++)
+            for (int dy = -3; dy <= 3; dy++)
+                for (int dx = -3; dx <= 3; dx++) {
+                    if (std::abs(dx) <= 1 && std::abs(dy) <= 1 && std::abs(dz) <= 1) {
+                        std::fill(T.begin(), T.end(), hostm::C(0.0));
+                    } else {
+                        hostm::m2l_op(fact, p, dx * sp, dy * sp, dz * sp, T.data());
+                    }
+                    append();
+                }
+        CK(put(lvl == 0 ? c->T4 : c->T3, buf.size()));
+    }
+    // M2L at the top level (spacing 4 rs), offsets in [-(nb2-1), nb2-1]^3
+    {
+        buf.clear();
+        int w = g.nb2 - 1;
+        double sp = 4.0 * g.rs;
+        for (int dz = -w; dz <= w; dz++)
+            for (int dy = -w; dy <= w; dy++)
+                for (int dx = -w; dx <= w; dx++) {
+                    if (std::abs(dx) <= 1 && std::abs(dy) <= 1 && std::abs(dz) <= 1)
+                        std::fill(T.begin(), T.end(), hostm::C(0.0));
+                    else
+                        hostm::m2l_op(fact, p, dx * sp, dy * sp, dz * sp, T.data());
+                    append();
+                }
+        CK(put(c->T2, buf.size()));
+    }
+    // M2M 4->3 and 3->2 (child - parent = +-half the child spacing... per axis),
+    // L2L 2->3 and 3->4 (parent - child)
+    for (int which = 0; which < 4; which++) {
+        buf.clear();
+        double h = (which == 0 || which == 3) ? g.rs / 2.0 : g.rs;
+        for (int kc = 0; kc < 8; kc++) {
+            double ox = (kc & 1) ? h : -h, oy = ((kc >> 1) & 1) ? h : -h, oz = (kc >> 2) ? h : -h;
+            if (which < 2)
+                hostm::m2m_op(fact, p, ox, oy, oz, T.data());
+            else
+                hostm::l2l_op(fact, p, -ox, -oy, -oz, T.data());
+            append();
+        }
+        double2 *dst = which == 0 ? c->M43 : which == 1 ? c->M32 : which == 2 ? c->S23 : c->S34;
+        CK(put(dst, buf.size()));
+    }
+    // rfctr(j, k) table for L2P
+    std::vector<double> lf(lp);
+    for (int j = 0; j <= p; j++)
+        for (int k = -j; k <= j; k++) {
+            int ak = std::abs(k);
+            lf[j * (j + 1) + k] = std::sqrt(fact[j - ak] / fact[j + ak]);
+        }
+    HC(hipMemcpy(c->lfac, lf.data(), sizeof(double) * lp, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, int p,
+                            const sesmd_forcefield *ff, int precision)
+{
+    if (!out || !ff) return fail("null argument");
+    *out = nullptr;
+    if (n_water < 0 || n_surf < 0 || n_water + n_surf == 0) return fail("empty system");
+    if (nb2 < 1 || nb2 > 8) return fail("nb2 must be 1..8");
+    if (p < 1 || p > MAXP) return fail("p must be 1..%d", MAXP);
+    if (precision != SESMD_FP64 && precision != SESMD_MIXED) return fail("bad precision");
+    if (check_topology(*ff)) return 1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device");
+    sesmd_ctx *c = new sesmd_ctx();
+    memset((void *)c, 0, sizeof(*c));
+    Geo &g = c->g;
+    g.nw = n_water;
+    g.ns = n_surf;
+    g.n = 3 * n_water + 29 * n_surf;
+    g.nmol = n_water + n_surf;
+    g.nb2 = nb2;
+    g.nb = 4 * nb2;
+    g.nb3 = g.nb * g.nb * g.nb;
+    g.n3 = g.nb / 2;
+    g.n33 = g.n3 * g.n3 * g.n3;
+    g.n2 = nb2 * nb2 * nb2;
+    g.p = p;
+    g.lp = (p + 1) * (p + 1);
+    g.lk = (p + 1) * (p + 2) / 2;
+    g.rs = ff->rs;
+    g.rsd = g.nb * g.rs;
+    g.rhs = g.rsd / 2.0;
+    g.rc = ff->rc;
+    g.rc2 = ff->rc * ff->rc;
+    g.xpi = ff->xpi;
+    g.dt = ff->dt;
+    g.dtsq2 = ff->dtsq2;
+    c->precision = precision;
+    c->ff = *ff;
+    int rc = 0;
+    auto A = [&](int r2) {
+        if (r2) rc = 1;
+    };
+    A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess
+          ? fail("stream create failed") : 0);
+    size_t n = g.n;
+    A(dalloc(&c->r, 3 * n));
+    A(dalloc(&c->v, 3 * n));
+    A(dalloc(&c->a, 3 * n));
+    A(dalloc(&c->f, 3 * n));
+    A(dalloc(&c->type, n));
+    A(dalloc(&c->mol, n));
+    A(dalloc(&c->first, g.nmol + 1));
+    A(dalloc(&c->cls, g.nmol));
+    A(dalloc(&c->anc, g.nmol));
+    A(dalloc(&c->cbox, n));
+    A(dalloc(&c->rkind, n));
+    A(dalloc(&c->isub, n));
+    A(dalloc(&c->key_in, n));
+    A(dalloc(&c->key_out, n));
+    A(dalloc(&c->val_in, n));
+    A(dalloc(&c->perm, n));
+    A(dalloc(&c->bstart, g.nb3 + 1));
+    A(dalloc(&c->nvalid, 1));
+    c->max_work = g.nb3 + (int)(n / NEAR_TILE) + 1;
+    A(dalloc(&c->work, c->max_work));
+    A(dalloc(&c->nwork, 1));
+    A(dalloc(&c->spos, n));
+    A(dalloc(&c->smeta, n));
+    A(dalloc(&c->O4, (size_t)g.nb3 * g.lp));
+    A(dalloc(&c->L4, (size_t)g.nb3 * g.lp));
+    A(dalloc(&c->O3, (size_t)g.n33 * g.lp));
+    A(dalloc(&c->L3, (size_t)g.n33 * g.lp));
+    A(dalloc(&c->O2, (size_t)g.n2 * g.lp));
+    A(dalloc(&c->L2, (size_t)g.n2 * g.lp));
+    A(dalloc(&c->ne4, g.nb3));
+    A(dalloc(&c->ne3, g.n33));
+    A(dalloc(&c->ne2, g.n2));
+    A(dalloc(&c->T4, (size_t)343 * g.lk * g.lp));
+    A(dalloc(&c->T3, (size_t)343 * g.lk * g.lp));
+    int w2 = 2 * nb2 - 1;
+    A(dalloc(&c->T2, (size_t)w2 * w2 * w2 * g.lk * g.lp));
+    A(dalloc(&c->M43, (size_t)8 * g.lk * g.lp));
+    A(dalloc(&c->M32, (size_t)8 * g.lk * g.lp));
+    A(dalloc(&c->S23, (size_t)8 * g.lk * g.lp));
+    A(dalloc(&c->S34, (size_t)8 * g.lk * g.lp));
+    A(dalloc(&c->lfac, g.lp));
+    A(dalloc(&c->gstart, g.nb * g.nb + 1));
+    A(dalloc(&c->gidx, 1));
+    A(dalloc(&c->gr, 3));
+    A(dalloc(&c->dff, 1));
+    A(dalloc(&c->dtopo, 1));
+    c->nblk_w = g.nw > 0 ? (int)nblk(g.nw, 256) : 0;
+    c->nblk_s = g.ns > 0 ? (int)nblk(g.ns, 64) : 0;
+    A(dalloc(&c->partials, (size_t)(c->nblk_w + c->nblk_s + 1) * NH_VALS));
+    A(dalloc(&c->nh, 16));
+    A(dalloc(&c->counters, 4));
+    if (rc) {
+        std::string e = g_err;
+        sesmd_destroy(c);
+        g_err = e;
+        return 1;
+    }
+    // statics
+    std::vector<int> type(n), mol(n), first(g.nmol + 1);
+    for (int w = 0; w < n_water; w++) {
+        first[w] = 3 * w;
+        for (int k = 0; k < 3; k++) {
+            type[3 * w + k] = k;
+            mol[3 * w + k] = w;
+        }
+    }
+    for (int s = 0; s < n_surf; s++) {
+        int b = 3 * n_water + 29 * s;
+        first[n_water + s] = b;
+        for (int k = 0; k < 29; k++) {
+            type[b + k] = 3 + k;
+            mol[b + k] = n_water + s;
+        }
+    }
+    first[g.nmol] = (int)n;
+    Topo topo;
+    build_topology(*ff, topo);
+    if (hipMemcpy(c->type, type.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->mol, mol.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->first, first.data(), sizeof(int) * (g.nmol + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dff, ff, sizeof(*ff), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dtopo, &topo, sizeof topo, hipMemcpyHostToDevice) != hipSuccess) {
+        sesmd_destroy(c);
+        return fail("upload of statics failed");
+    }
+    if (upload_ops(c)) {
+        std::string e = g_err;
+        sesmd_destroy(c);
+        g_err = e;
+        return 1;
+    }
+    // radix-sort workspace
+    size_t bytes = 0;
+    int endbit = 1;
+    while ((1 << endbit) <= g.nb3) endbit++;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->key_in, c->key_out, c->val_in, c->perm,
+                                       (int)n, 0, endbit, c->stream);
+    c->cub_bytes = bytes;
+    if (hipMalloc(&c->cub_tmp, bytes ? bytes : 16) != hipSuccess) {
+        sesmd_destroy(c);
+        return fail("workspace alloc failed");
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int sesmd_destroy(sesmd_ctx *c)
+{
+    if (!c) return 0;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph_ok) hipGraphExecDestroy(c->graph);
+    void *ptrs[] = {c->r, c->v, c->a, c->f, c->type, c->mol, c->first, c->cls, c->anc, c->cbox,
+                    c->rkind, c->isub, c->key_in, c->key_out, c->val_in, c->perm, c->cub_tmp,
+                    c->bstart, c->nvalid, c->work, c->nwork, c->spos, c->smeta, c->O4, c->L4,
+                    c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2,
+                    c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
+                    c->dtopo, c->partials, c->nh, c->counters};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+// host AoS (N,3) <-> device SoA
+static int up_aos(sesmd_ctx *c, double *dst, const double *src)
+{
+    size_t n = c->g.n;
+    std::vector<double> t(3 * n);
+    for (size_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) t[d * n + i] = src[3 * i + d];
+    HC(hipMemcpyAsync(dst, t.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+static int down_aos(sesmd_ctx *c, double *dst, const double *src)
+{
+    size_t n = c->g.n;
+    std::vector<double> t(3 * n);
+    HC(hipMemcpyAsync(t.data(), src, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) dst[3 * i + d] = t[d * n + i];
+    return 0;
+}
+
+extern "C" int sesmd_set_state(sesmd_ctx *c, const double *r, const double *v, const double *a,
+                               double eta)
+{
+    if (!c) return fail("null context");
+    if (r) CK(up_aos(c, c->r, r));
+    if (v) CK(up_aos(c, c->v, v));
+    if (a) CK(up_aos(c, c->a, a));
+    HC(hipMemcpy(c->nh, &eta, sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int sesmd_get_state(sesmd_ctx *c, double *r, double *v, double *a, double *eta)
+{
+    if (!c) return fail("null context");
+    if (r) CK(down_aos(c, r, c->r));
+    if (v) CK(down_aos(c, v, c->v));
+    if (a) CK(down_aos(c, a, c->a));
+    if (eta) HC(hipMemcpy(eta, c->nh, sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int sesmd_set_nh(sesmd_ctx *c, double tr, double pnh1, double pnh2)
+{
+    if (!c) return fail("null context");
+    double t[3] = {tr, pnh1, pnh2};
+    HC(hipMemcpy(c->nh + 7, t, sizeof t, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int sesmd_set_graphite(sesmd_ctx *c, int n, const double *xyz)
+{
+    if (!c) return fail("null context");
+    if (n < 0 || (n > 0 && !xyz)) return fail("bad graphite");
+    const Geo &g = c->g;
+    int nc = g.nb * g.nb;
+    double rcut = g.rc + std::sqrt(2.0) * g.rs / 2.0;
+    rcut = rcut * rcut;
+    std::vector<int> start(nc + 1, 0), idx;
+    std::vector<double> cen(g.nb);
+    for (int q = 0; q < g.nb; q++) {
+        int i = q % 4 + 1;
+        cen[q] = 4.0 * g.rs * i / 4.0 - 2.0 * g.rs - 4.0 * g.rs / 8.0 + rcen2(g, q / 4);
+    }
+    for (int col = 0; col < nc; col++) {
+        start[col] = (int)idx.size();
+        double cx = cen[col % g.nb], cy = cen[col / g.nb];
+        for (int a = 0; a < n; a++) {
+            double dx = xyz[3 * a] - cx, dy = xyz[3 * a + 1] - cy;
+            if (dx * dx + dy * dy <= rcut) idx.push_back(a);
+        }
+    }
+    start[nc] = (int)idx.size();
+    hipFree(c->gidx);
+    hipFree(c->gr);
+    c->gidx = nullptr;
+    c->gr = nullptr;
+    CK(dalloc(&c->gidx, idx.size()));
+    CK(dalloc(&c->gr, 3 * (size_t)n));
+    if (!idx.empty()) HC(hipMemcpy(c->gidx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+    if (n) HC(hipMemcpy(c->gr, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    HC(hipMemcpy(c->gstart, start.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
+    c->ngr = n;
+    c->graph_ok = false;
+    if (c->graph) {
+        hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return 0;
+}
+
+static int launch_rebuild(sesmd_ctx *c)
+{
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
+    k_classify<<<nblk(g.nmol, 256), 256, 0, st>>>(g, c->r, c->first, c->cls, c->anc);
+    k_site_box<<<nblk(g.n, 256), 256, 0, st>>>(g, c->r, c->mol, c->cls, c->anc, c->cbox, c->rkind,
+                                               c->key_in, c->val_in);
+    if (g.ns) k_isub<<<nblk(29 * g.ns, 128), 128, 0, st>>>(g, c->r, c->anc, c->isub);
+    int endbit = 1;
+    while ((1 << endbit) <= g.nb3) endbit++;
+    size_t bytes = c->cub_bytes;
+    HC(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, bytes, c->key_in, c->key_out, c->val_in,
+                                          c->perm, g.n, 0, endbit, st));
+    k_bstart<<<nblk(g.nb3 + 1, 256), 256, 0, st>>>(g, c->key_out, c->bstart);
+    k_worklist<<<1, 1024, 0, st>>>(g, c->bstart, c->work, c->nwork, c->max_work);
+    k_gather_meta<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->type, c->mol, c->cls, c->anc,
+                                                  c->smeta);
+    HC(hipGetLastError());
+    c->built = true;
+    return 0;
+}
+
+extern "C" int sesmd_rebuild(sesmd_ctx *c)
+{
+    if (!c) return fail("null context");
+    CK(launch_rebuild(c));
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int launch_fmm(sesmd_ctx *c)
+{
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
+    k_p2m<<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
+    k_m2m<<<g.n33, 64, 0, st>>>(g, g.n3, c->O4, c->ne4, c->M43, c->O3, c->ne3);
+    k_m2m<<<g.n2, 64, 0, st>>>(g, g.nb2, c->O3, c->ne3, c->M32, c->O2, c->ne2);
+    k_m2l_top<<<g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->L2);
+    k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3);
+    k_m2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->O3, c->ne3, c->T3, c->L3);
+    k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4);
+    k_m2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->O4, c->ne4, c->T4, c->L4);
+    HC(hipGetLastError());
+    return 0;
+}
+
+extern "C" int sesmd_fmm(sesmd_ctx *c)
+{
+    if (!c) return fail("null context");
+    if (!c->built) return fail("sesmd_fmm before sesmd_rebuild");
+    CK(launch_fmm(c));
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask)
+{
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
+    size_t n3 = 3 * (size_t)g.n;
+    k_fill_d<<<nblk(n3, 256), 256, 0, st>>>(c->f, n3, 0.0);
+    if (what & (SESMD_NEAR | SESMD_FAR | SESMD_GRAPHITE))
+        k_gather_pos<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->r, c->type, c->dff, c->spos);
+    if (what & SESMD_NEAR) {
+        if (mask == ~0ull)
+            k_near<false><<<c->max_work, NEAR_TILE, 0, st>>>(g, c->spos, c->smeta, c->bstart, c->rkind,
+                                                           c->work, c->nwork, c->dff, mask, c->f);
+        else
+            k_near<true><<<c->max_work, NEAR_TILE, 0, st>>>(g, c->spos, c->smeta, c->bstart, c->rkind,
+                                                          c->work, c->nwork, c->dff, mask, c->f);
+    }
+    if (what & (SESMD_FAR | SESMD_WALL))
+        k_site_terms<<<nblk(g.n, 128), 128, 0, st>>>(g, what, c->r, c->type, c->cbox, c->rkind, c->L4,
+                                                     c->lfac, c->dff, c->f);
+    if (g.ns && (what & (SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL |
+                         SESMD_DIHEDRAL_BE)))
+        k_intra<<<nblk(29 * g.ns, 64), 64, 0, st>>>(g, what, c->r, c->isub, c->dff, c->dtopo, c->f);
+    if ((what & SESMD_GRAPHITE) && c->ngr > 0)
+        k_graphite<<<g.nb * g.nb, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->gr,
+                                               c->gstart, c->gidx, c->dff, c->f);
+    HC(hipGetLastError());
+    return 0;
+}
+
+extern "C" int sesmd_forces(sesmd_ctx *c, unsigned what, uint64_t near_mask)
+{
+    if (!c) return fail("null context");
+    if (!c->built) return fail("sesmd_forces before sesmd_rebuild");
+    CK(launch_forces(c, what, near_mask));
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int sesmd_near_only(sesmd_ctx *c)
+{
+    if (!c || !c->built) return fail("not built");
+    const Geo &g = c->g;
+    k_near<false><<<c->max_work, NEAR_TILE, 0, c->stream>>>(g, c->spos, c->smeta, c->bstart, c->rkind,
+                                                           c->work, c->nwork, c->dff, ~0ull, c->f);
+    HC(hipGetLastError());
+    return 0;
+}
+
+extern "C" int sesmd_get_forces(sesmd_ctx *c, double *f)
+{
+    if (!c || !f) return fail("null argument");
+    return down_aos(c, f, c->f);
+}
+
+extern "C" int sesmd_set_forces(sesmd_ctx *c, const double *f)
+{
+    if (!c || !f) return fail("null argument");
+    return up_aos(c, c->f, f);
+}
+
+static int launch_move_a(sesmd_ctx *c)
+{
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
+    if (c->nblk_w)
+        k_move_a_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->cls, c->dff, c->nh,
+                                                   c->partials, c->counters);
+    if (c->nblk_s)
+        k_move_a_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->partials,
+                                                c->counters);
+    k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh);
+    HC(hipGetLastError());
+    return 0;
+}
+
+extern "C" int sesmd_move_a(sesmd_ctx *c, double *sums6)
+{
+    if (!c) return fail("null context");
+    if (!c->built) return fail("sesmd_move_a before sesmd_rebuild (classes unknown)");
+    CK(launch_move_a(c));
+    if (sums6) HC(hipMemcpyAsync(sums6, c->nh + 1, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int launch_move_b(sesmd_ctx *c)
+{
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
+    if (c->nblk_w)
+        k_move_b_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->counters);
+    if (c->nblk_s)
+        k_move_b_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->counters);
+    HC(hipGetLastError());
+    return 0;
+}
+
+extern "C" int sesmd_move_b(sesmd_ctx *c)
+{
+    if (!c) return fail("null context");
+    CK(launch_move_b(c));
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int sesmd_init_accel(sesmd_ctx *c)
+{
+    if (!c) return fail("null context");
+    const Geo &g = c->g;
+    k_init_accel<<<nblk(g.n, 256), 256, 0, c->stream>>>(g, c->type, c->f, c->dff, c->a);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int regular_step(sesmd_ctx *c)
+{
+    CK(launch_move_a(c));
+    CK(launch_forces(c, SESMD_ALL, ~0ull));
+    CK(launch_move_b(c));
+    return 0;
+}
+
+extern "C" int sesmd_steps(sesmd_ctx *c, int nsteps, int ifar, int *kb)
+{
+    if (!c || !kb) return fail("null argument");
+    if (!c->built) return fail("sesmd_steps before sesmd_rebuild");
+    if (ifar < 1) return fail("ifar must be >= 1");
+    for (int s = 0; s < nsteps; s++) {
+        (*kb)++;
+        if (*kb % ifar == 0) {
+            CK(launch_move_a(c));
+            CK(launch_rebuild(c));
+            CK(launch_fmm(c));
+            CK(launch_forces(c, SESMD_ALL, ~0ull));
+            CK(launch_move_b(c));
+            continue;
+        }
+        if (!c->graph_ok) {
+            hipGraph_t gr;
+            HC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int r = regular_step(c);
+            hipError_t e = hipStreamEndCapture(c->stream, &gr);
+            if (r) return 1;
+            HC(e);
+            HC(hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0));
+            HC(hipGraphDestroy(gr));
+            c->graph_ok = true;
+        }
+        HC(hipGraphLaunch(c->graph, c->stream));
+    }
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int sesmd_get_classes(sesmd_ctx *c, int *cls, int *anc, int *cbox, int *rkind,
+                                 unsigned *isub)
+{
+    if (!c) return fail("null context");
+    const Geo &g = c->g;
+    if (cls) HC(hipMemcpy(cls, c->cls, sizeof(int) * g.nmol, hipMemcpyDeviceToHost));
+    if (anc) HC(hipMemcpy(anc, c->anc, sizeof(int) * g.nmol, hipMemcpyDeviceToHost));
+    if (cbox) HC(hipMemcpy(cbox, c->cbox, sizeof(int) * g.n, hipMemcpyDeviceToHost));
+    if (rkind) HC(hipMemcpy(rkind, c->rkind, sizeof(int) * g.n, hipMemcpyDeviceToHost));
+    if (isub) HC(hipMemcpy(isub, c->isub, sizeof(unsigned) * g.n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int sesmd_get_expansions(sesmd_ctx *c, double *O4, double *L4, double *O3, double *L3,
+                                    double *O2, double *L2)
+{
+    if (!c) return fail("null context");
+    const Geo &g = c->g;
+    size_t s4 = sizeof(double2) * g.nb3 * g.lp, s3 = sizeof(double2) * g.n33 * g.lp,
+           s2 = sizeof(double2) * g.n2 * g.lp;
+    if (O4) HC(hipMemcpy(O4, c->O4, s4, hipMemcpyDeviceToHost));
+    if (L4) HC(hipMemcpy(L4, c->L4, s4, hipMemcpyDeviceToHost));
+    if (O3) HC(hipMemcpy(O3, c->O3, s3, hipMemcpyDeviceToHost));
+    if (L3) HC(hipMemcpy(L3, c->L3, s3, hipMemcpyDeviceToHost));
+    if (O2) HC(hipMemcpy(O2, c->O2, s2, hipMemcpyDeviceToHost));
+    if (L2) HC(hipMemcpy(L2, c->L2, s2, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int sesmd_get_counters(sesmd_ctx *c, int *shake_fail)
+{
+    if (!c) return fail("null context");
+    int t[4];
+    HC(hipMemcpy(t, c->counters, sizeof t, hipMemcpyDeviceToHost));
+    if (shake_fail) *shake_fail = t[0];
+    return 0;
+}
+
+extern "C" int sesmd_device_arrays(sesmd_ctx *c, void **r, void **v, void **a, void **f)
+{
+    if (!c) return fail("null context");
+    if (r) *r = c->r;
+    if (v) *v = c->v;
+    if (a) *a = c->a;
+    if (f) *f = c->f;
+    return 0;
+}
+
+extern "C" void *sesmd_stream(sesmd_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" int sesmd_pair_counts(sesmd_ctx *c, double *pairs, double *lj_pairs)
+{
+    if (!c || !c->built) return fail("not built");
+    const Geo &g = c->g;
+    unsigned long long *cnt;
+    HC(hipMalloc(&cnt, 2 * sizeof(unsigned long long)));
+    HC(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), c->stream));
+    k_gather_pos<<<nblk(g.n, 256), 256, 0, c->stream>>>(g, c->perm, c->r, c->type, c->dff, c->spos);
+    k_pair_count<<<nblk(g.n, 128), 128, 0, c->stream>>>(g, c->spos, c->smeta, c->bstart, c->rkind,
+                                                         c->dff, cnt);
+    unsigned long long h[2];
+    HC(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    HC(hipFree(cnt));
+    if (pairs) *pairs = 0.5 * (double)h[0];
+    if (lj_pairs) *lj_pairs = 0.5 * (double)h[1];
+    return 0;
+}

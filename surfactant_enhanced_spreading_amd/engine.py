@@ -1,0 +1,271 @@
+"""Python front end of libsesmd.so (the HIP engine) over its C ABI.
+
+The method names follow the reference's routine families: rebuild() is the
+every-10-steps migration + list build, fmm() the far-field chain
+computeOnm4..computeLjk4, forces() the eval*/evalLjk4 force routines,
+move_a()/move_b() the move_*_a / move_*_b integrator halves and
+init_accel() src/init_accel.f90.  Arrays are (N, 3) in reduced units.
+
+There is no CPU fallback: without the built library or a GPU this module
+raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import forcefield as ff
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, 'libsesmd.so')
+
+NEAR, FAR, INTERNAL, SUBTRACT, VALENCE, DIHEDRAL, DIHEDRAL_BE, WALL, GRAPHITE = \
+    1, 2, 4, 8, 16, 32, 64, 128, 256
+ALL = 511
+ALL_MASK = (1 << 64) - 1
+FP64, MIXED = 0, 1
+
+
+class ForceField(ctypes.Structure):
+    """Mirror of sesmd_forcefield (include/sesmd.h)."""
+    _fields_ = [
+        ('q', ctypes.c_double * 32), ('mass', ctypes.c_double * 32),
+        ('lj_c', ctypes.c_double * 1024), ('lj_s6', ctypes.c_double * 1024),
+        ('gr_c', ctypes.c_double * 32), ('gr_s6', ctypes.c_double * 32),
+        ('dsqw', ctypes.c_double * 3), ('dsqs', ctypes.c_double * 28),
+        ('ibond', ctypes.c_int * 28), ('jbond', ctypes.c_int * 28),
+        ('th', ctypes.c_double * 36), ('rkth', ctypes.c_double * 36),
+        ('ang_i', ctypes.c_int * 36), ('ang_j', ctypes.c_int * 36), ('ang_k', ctypes.c_int * 36),
+        ('dva', ctypes.c_double * 30), ('dvb', ctypes.c_double * 30), ('dvc', ctypes.c_double * 30),
+        ('dih_i', ctypes.c_int * 30), ('dih_j', ctypes.c_int * 30),
+        ('dih_k', ctypes.c_int * 30), ('dih_l', ctypes.c_int * 30),
+        ('n_internal', ctypes.c_int),
+        ('int_j', ctypes.c_int * 320), ('int_k', ctypes.c_int * 320),
+        ('rifact', ctypes.c_double * 33),
+        ('xpi', ctypes.c_double), ('rc', ctypes.c_double), ('rs', ctypes.c_double),
+        ('dt', ctypes.c_double), ('dtsq2', ctypes.c_double),
+        ('clamp_opls', ctypes.c_double), ('clamp_be', ctypes.c_double),
+    ]
+
+
+def reference_forcefield():
+    """The reference's force field (src/init_arrays.f90) in the 32-type layout."""
+    T = ff.type_tables()
+    s = ForceField()
+    s.q[:] = list(T['q'])
+    s.mass[:] = list(T['mass'])
+    s.lj_c[:] = list(T['lj_c'].reshape(-1))
+    s.lj_s6[:] = list(T['lj_s6'].reshape(-1))
+    s.gr_c[:] = list(T['gr_c'])
+    s.gr_s6[:] = list(T['gr_s6'])
+    s.dsqw[:] = ff.DSQW
+    s.dsqs[:] = ff.DSQS
+    s.ibond[:] = ff.IBOND
+    s.jbond[:] = ff.JBOND
+    s.th[:] = ff.TH
+    s.rkth[:] = ff.RKTH
+    s.ang_i[:], s.ang_j[:], s.ang_k[:] = ff.ANG_I, ff.ANG_J, ff.ANG_K
+    s.dva[:], s.dvb[:], s.dvc[:] = ff.DVA, ff.DVB, ff.DVC
+    s.dih_i[:], s.dih_j[:], s.dih_k[:], s.dih_l[:] = ff.DIH_I, ff.DIH_J, ff.DIH_K, ff.DIH_L
+    pairs = ff.internal_pairs()
+    s.n_internal = len(pairs)
+    s.int_j[:len(pairs)] = [p[0] for p in pairs]
+    s.int_k[:len(pairs)] = [p[1] for p in pairs]
+    s.rifact[:] = ff.RIFACT
+    s.xpi, s.rc, s.rs, s.dt, s.dtsq2 = ff.XPI, ff.RC, ff.RS, ff.DT, ff.DTSQ2
+    s.clamp_opls, s.clamp_be = ff.dihedral_clamp(1), ff.dihedral_clamp(30)
+    return s
+
+
+_lib = None
+
+
+def lib():
+    """Load libsesmd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError('libsesmd.so not built: run '
+                              '`python -m surfactant_enhanced_spreading_amd.csrc.build`')
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        L.sesmd_last_error.restype = ctypes.c_char_p
+        L.sesmd_create.argtypes = [ctypes.POINTER(vp)] + [ctypes.c_int] * 4 + \
+            [ctypes.POINTER(ForceField), ctypes.c_int]
+        L.sesmd_destroy.argtypes = [vp]
+        L.sesmd_set_state.argtypes = [vp, dp, dp, dp, ctypes.c_double]
+        L.sesmd_get_state.argtypes = [vp, dp, dp, dp, dp]
+        L.sesmd_set_nh.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.sesmd_set_graphite.argtypes = [vp, ctypes.c_int, dp]
+        L.sesmd_rebuild.argtypes = [vp]
+        L.sesmd_fmm.argtypes = [vp]
+        L.sesmd_forces.argtypes = [vp, ctypes.c_uint, ctypes.c_uint64]
+        L.sesmd_get_forces.argtypes = [vp, dp]
+        L.sesmd_set_forces.argtypes = [vp, dp]
+        L.sesmd_move_a.argtypes = [vp, dp]
+        L.sesmd_move_b.argtypes = [vp]
+        L.sesmd_init_accel.argtypes = [vp]
+        L.sesmd_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip]
+        L.sesmd_get_classes.argtypes = [vp, ip, ip, ip, ip, ctypes.POINTER(ctypes.c_uint)]
+        L.sesmd_get_expansions.argtypes = [vp] + [dp] * 6
+        L.sesmd_get_counters.argtypes = [vp, ip]
+        L.sesmd_device_arrays.argtypes = [vp] + [ctypes.POINTER(vp)] * 4
+        L.sesmd_stream.argtypes = [vp]
+        L.sesmd_stream.restype = vp
+        L.sesmd_near_only.argtypes = [vp]
+        L.sesmd_pair_counts.argtypes = [vp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class SesmdError(RuntimeError):
+    pass
+
+
+def _ck(rc):
+    if rc != 0:
+        raise SesmdError(lib().sesmd_last_error().decode())
+
+
+class Engine:
+    """One system resident on the GPU."""
+
+    def __init__(self, sys_, nb2=3, p=ff.P_DEFAULT, precision=FP64, tr=None, eta=0.0,
+                 forcefield=None):
+        self.L = lib()
+        self._ff = forcefield if forcefield is not None else reference_forcefield()
+        self.n_water, self.n_surf = sys_.n_water, sys_.n_surf
+        self.n = sys_.n_sites
+        self.nb2, self.p = nb2, p
+        self.nb = 4 * nb2
+        self.lp = (p + 1) ** 2
+        h = ctypes.c_void_p()
+        _ck(self.L.sesmd_create(ctypes.byref(h), self.n_water, self.n_surf, nb2, p,
+                                ctypes.byref(self._ff), precision))
+        self.h = h
+        acc = sys_.acc if sys_.acc is not None else np.zeros_like(sys_.pos)
+        self.set_state(sys_.pos, sys_.vel, acc, eta)
+        tr = ff.TR_DEFAULT if tr is None else tr
+        npart = 3 * self.n_water + 29 * self.n_surf
+        nc = 3 * self.n_water + 28 * self.n_surf
+        df = 3 * npart - nc
+        self.df = df
+        _ck(self.L.sesmd_set_nh(self.h, tr, ff.DT / df / tr / (ff.RLXTIME * ff.RLXTIME),
+                                -2.0 * df * tr))
+        g = np.ascontiguousarray(sys_.graphite, dtype=np.float64).reshape(-1, 3)
+        _ck(self.L.sesmd_set_graphite(self.h, len(g), _p(g)))
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.L.sesmd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # state ------------------------------------------------------------------
+    def set_state(self, r, v, a, eta):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _ck(self.L.sesmd_set_state(self.h, _p(r), _p(v), _p(a), float(eta)))
+
+    def state(self):
+        r = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        a = np.zeros((self.n, 3))
+        eta = ctypes.c_double()
+        _ck(self.L.sesmd_get_state(self.h, _p(r), _p(v), _p(a), ctypes.byref(eta)))
+        return r, v, a, eta.value
+
+    # reference routine families ---------------------------------------------
+    def rebuild(self):
+        _ck(self.L.sesmd_rebuild(self.h))
+
+    def fmm(self):
+        _ck(self.L.sesmd_fmm(self.h))
+
+    def forces(self, what=ALL, mask=ALL_MASK, fetch=True):
+        _ck(self.L.sesmd_forces(self.h, what, mask))
+        if not fetch:
+            return None
+        f = np.zeros((self.n, 3))
+        _ck(self.L.sesmd_get_forces(self.h, _p(f)))
+        return f
+
+    def set_forces(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        _ck(self.L.sesmd_set_forces(self.h, _p(f)))
+
+    def move_a(self):
+        s = np.zeros(6)
+        _ck(self.L.sesmd_move_a(self.h, _p(s)))
+        return s
+
+    def move_b(self):
+        _ck(self.L.sesmd_move_b(self.h))
+
+    def init_accel(self):
+        _ck(self.L.sesmd_init_accel(self.h))
+
+    def steps(self, nsteps, ifar=10, kb=0):
+        k = ctypes.c_int(kb)
+        _ck(self.L.sesmd_steps(self.h, nsteps, ifar, ctypes.byref(k)))
+        return k.value
+
+    def bootstrap(self):
+        """The start-up force evaluation of src/mybox.f90:65-128."""
+        self.rebuild()
+        self.fmm()
+        self.forces(fetch=False)
+        self.init_accel()
+
+    # inspection -------------------------------------------------------------
+    def classes(self):
+        nm = self.n_water + self.n_surf
+        cls = np.zeros(nm, np.int32)
+        anc = np.zeros(nm, np.int32)
+        cbox = np.zeros(self.n, np.int32)
+        rk = np.zeros(self.n, np.int32)
+        isub = np.zeros(self.n, np.uint32)
+        _ck(self.L.sesmd_get_classes(self.h, _p(cls, ctypes.c_int), _p(anc, ctypes.c_int),
+                                     _p(cbox, ctypes.c_int), _p(rk, ctypes.c_int),
+                                     _p(isub, ctypes.c_uint)))
+        return dict(cls=cls, anc=anc, cbox=cbox, rkind=rk, isub=isub)
+
+    def expansions(self):
+        nb3, n33, n2, lp = self.nb ** 3, (self.nb // 2) ** 3, self.nb2 ** 3, self.lp
+        arrs = [np.zeros((nb3, lp), np.complex128), np.zeros((nb3, lp), np.complex128),
+                np.zeros((n33, lp), np.complex128), np.zeros((n33, lp), np.complex128),
+                np.zeros((n2, lp), np.complex128), np.zeros((n2, lp), np.complex128)]
+        _ck(self.L.sesmd_get_expansions(self.h, *[_p(a) for a in arrs]))
+        return dict(zip(('O4', 'L4', 'O3', 'L3', 'O2', 'L2'), arrs))
+
+    def shake_failures(self):
+        k = ctypes.c_int()
+        _ck(self.L.sesmd_get_counters(self.h, ctypes.byref(k)))
+        return k.value
+
+    def pair_counts(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _ck(self.L.sesmd_pair_counts(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def stream(self):
+        return self.L.sesmd_stream(self.h)
+
+    def near_only(self):
+        _ck(self.L.sesmd_near_only(self.h))
+
+    def temperature(self):
+        """Kinetic temperature sum m v^2 / dt^2 / df (src/compute_temperature.f90:240-244)."""
+        r, v, a, eta = self.state()
+        m = np.concatenate([np.tile(ff.EMW, self.n_water), np.tile(ff.EMS, self.n_surf)])
+        return float((m[:, None] * v * v).sum() / ff.DT ** 2 / self.df)

@@ -1,0 +1,64 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+
+from surfactant_enhanced_spreading_amd import builder  # noqa: E402
+
+
+def pytest_configure(config):
+    config.addinivalue_line('markers', 'gpu: needs an MI355X (runs through libsesmd.so)')
+
+
+def rel_err(a, b):
+    """Per-site relative force error with a floor: |dF_i| / max(|F_i|, 1e-3 F_rms),
+    maximum over sites (|.| = max-norm over x, y, z)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    scale = np.maximum(np.abs(b).max(axis=1), 1e-3 * np.sqrt((b ** 2).sum(axis=1).mean()) + 1e-300)
+    return float((np.abs(a - b).max(axis=1) / scale).max())
+
+
+def _thermal(s, seed, amp=0.012):
+    """Displace every site by ~0.04 A so that angles and dihedrals sit off
+    their equilibria, as in a thermal configuration (at the ideal template the
+    angle force k (theta - theta0) / sin(theta) is a difference of equal
+    numbers and its relative error is meaningless)."""
+    rng = np.random.default_rng(seed + 1000)
+    s.pos = s.pos + rng.normal(0.0, amp, size=s.pos.shape)
+    return s
+
+
+def small_system(seed=1):
+    """Two droplets: one on the corner shared by eight of the reference's L2
+    (rank) boxes, one in the bottom L4 layer under the graphite sheet, so the
+    halo rings, every receiver kind and the graphite term all occur."""
+    d1 = builder.droplet(150, 3, center_A=(24.0, 24.0, 24.0), seed=seed)
+    d2 = builder.droplet(40, 1, center_A=(-10.0, 30.0, -62.0), seed=seed + 1)
+    return _thermal(builder.assemble([d1, d2], graphite=builder.graphite_sheet(), seed=seed), seed)
+
+
+def medium_system(seed=3):
+    """A few thousand sites spread over many boxes (two droplets and a cluster
+    against the +x wall)."""
+    d1 = builder.droplet(1200, 12, center_A=(-20.0, -10.0, 5.0), seed=seed)
+    d2 = builder.droplet(300, 4, center_A=(40.0, 30.0, -40.0), seed=seed + 1)
+    d3 = builder.droplet(60, 0, center_A=(62.0, -50.0, 50.0), seed=seed + 2)
+    d4 = builder.droplet(50, 1, center_A=(0.0, 0.0, -62.0), seed=seed + 3)
+    return _thermal(builder.assemble([d1, d2, d3, d4], graphite=builder.graphite_sheet(),
+                                     seed=seed), seed)
+
+
+@pytest.fixture(scope='session')
+def sys_small():
+    return small_system()
+
+
+@pytest.fixture(scope='session')
+def sys_medium():
+    return medium_system()
