@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Throughput of the per-timestep force + integration path on one MI355X.
+
+Workload: one droplet of 20,000 rigid SPC/E waters carrying 200 29-site
+M(D'E4OH)M surfactants (65,800 sites) in the reference's geometry (144 A cube,
+12^3 L4 boxes of 12 A, p = 4, far field and lists refreshed every 10 steps,
+2 fs, Nose-Hoover at 298 K).  A step is the reference's full step
+(src/mybox.f90:130-227); the timed region covers K consecutive steps, rebuild
+steps included, with all inputs resident in HBM.
+
+Prints one JSON line.  The library and torch ship different HSA runtimes, so
+timing uses HIP events of the library's own runtime (/opt/rocm), not torch.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MD steps/s (ns/day) 20k water+surfactant: mixed vs FP64 GPU at RMS F err≤1e-5"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+class HipTimer:
+    """hipEvent pairs on a given stream (the runtime libsesmd.so uses)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL('/opt/rocm/lib/libamdhip64.so')
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float),
+                                                 ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipDeviceSynchronize.argtypes = []
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, ev, stream):
+        assert self.hip.hipEventRecord(ev, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, e0, e1):
+        assert self.hip.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        return ms.value
+
+    def sync(self):
+        assert self.hip.hipDeviceSynchronize() == 0
+
+
+def cpu_baseline(sys_, seconds=20.0):
+    """The C restatement (oracle) on the host cores: one rebuild step (classes,
+    far field, forces) then regular steps until ~`seconds` have passed."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle as orc
+    threads = min(16, os.cpu_count() or 1)
+    orc.lib().orc_set_threads(threads)
+    o = orc.Oracle(sys_)
+    o.rebuild()
+    o.fmm()
+    o.init_accel(o.forces())
+    kb = 9  # next step (10) is a rebuild step, as in the GPU timed region
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        kb = o.steps(1, 10, kb)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds or n >= 50:
+            break
+    return dict(value=n / el, unit='steps/s', cores=threads, kind='port',
+                sample='%d full steps (1 rebuild+far-field step, %d regular) of the same '
+                       '65,800-site system, C restatement with OpenMP' % (n, n - 1))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=500)
+    ap.add_argument('--warmup', type=int, default=30)
+    ap.add_argument('--precision', choices=['fp64', 'mixed'], default='fp64')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build runs the whole system on one MI355X '
+              'and has no multi-GPU path (see DESIGN.md, "How it would shard")', file=sys.stderr)
+        sys.exit(2)
+
+    from surfactant_enhanced_spreading_amd import builder, engine, forcefield as ff
+    sys_ = builder.bench_system()
+    prec = engine.FP64 if args.precision == 'fp64' else engine.MIXED
+    e = engine.Engine(sys_, precision=prec)
+    e.bootstrap()
+    pairs, lj_pairs = e.pair_counts()
+    timer = HipTimer()
+    stream = e.stream()
+    kb = 0
+    kb = e.steps(args.warmup, 10, kb)
+    timer.sync()
+    e0, e1 = timer.event(), timer.event()
+    t0 = time.perf_counter()
+    timer.record(e0, stream)
+    kb = e.steps(args.steps, 10, kb)
+    timer.record(e1, stream)
+    ms = timer.elapsed_ms(e0, e1)
+    timer.sync()
+    wall = time.perf_counter() - t0
+    ms_step = ms / args.steps
+    steps_s = 1000.0 / ms_step
+    ns_day = steps_s * 2.0 * 0.0864  # dt = 2 fs (src/mybox.cmns:43)
+
+    # dominant kernel: the near field, timed alone on the same stream
+    e.forces(fetch=False)
+    reps = 50
+    timer.sync()
+    timer.record(e0, stream)
+    for _ in range(reps):
+        e.near_only()
+    timer.record(e1, stream)
+    near_ms = timer.elapsed_ms(e0, e1) / reps
+    flop = 21.0 * pairs + 9.0 * lj_pairs
+    achieved = flop / (near_ms * 1e-3) / 1e12
+    roofline = dict(bound='fp64-valu', achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS,
+                    unit='TFLOP/s', frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=None,
+                    kernel='k_near', kernel_ms=round(near_ms, 4),
+                    pairs=pairs, lj_pairs=lj_pairs)
+
+    out = dict(metric=METRIC, value=round(steps_s, 2), unit='steps/s', n_gpus=1,
+               steps=args.steps, warmup=args.warmup, ms_per_step=round(ms_step, 4),
+               higher_is_better=True, scaling='weak', vs_baseline=None,
+               dtype='f64' if prec == engine.FP64 else 'f32+f64',
+               data='synthetic',
+               config=dict(workload='20k SPC/E water + 200 M(D\'E4OH)M surfactant droplet, '
+                                    'reference geometry (12^3 L4 boxes, p=4, ifar=10, 2 fs)',
+                           n_water=sys_.n_water, n_surf=sys_.n_surf, sites=sys_.n_sites,
+                           boxes=12 ** 3, p=4, ifar=10, dt_fs=2.0, precision=args.precision),
+               ns_per_day=round(ns_day, 2), wall_s=round(wall, 3),
+               shake_failures=e.shake_failures(), temperature=round(e.temperature(), 4),
+               roofline=roofline)
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline(sys_, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()
