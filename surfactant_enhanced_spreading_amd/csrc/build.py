@@ -15,7 +15,7 @@ ARCH = os.environ.get('SESMD_ARCH', 'gfx950')
 # -packed-fp32-ops off: no v_pk_*_f32 generated from scalar/float2 code.
 FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', '--offload-arch=' + ARCH,
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-         '-Wno-unused-result', '-I' + os.path.join(ROOT, 'include')]
+         '-Wno-unused-result', '-Wno-unused-value', '-I' + os.path.join(ROOT, 'include')]
 
 
 def build(verbose=False):

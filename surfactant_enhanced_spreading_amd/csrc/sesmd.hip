@@ -75,6 +75,19 @@ struct Topo {
     int dih[29][16];    // dihedral index * 4 + role
 };
 
+// The reference surfactant's 28 bonds (src/init_arrays.f90:229-264), 0-based
+// site pairs.  With the topology fixed at compile time the Gauss-Seidel loop
+// unrolls and the molecule's predicted positions live in registers.
+constexpr int kBI[28] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17,
+                         17, 19, 20, 20, 20, 17, 24, 25, 25, 25};
+constexpr int kBJ[28] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18,
+                         19, 20, 21, 22, 23, 24, 25, 26, 27, 28};
+struct SurfConst {
+    double m[29];    // mass / m_O
+    double rm[29];   // 1 / mass
+    double dsq[28];  // squared bond lengths
+};
+
 struct sesmd_ctx {
     Geo g;
     int precision;
@@ -89,6 +102,8 @@ struct sesmd_ctx {
     void *cub_tmp;
     size_t cub_bytes;
     int *bstart, *nvalid;
+    int *bs4;    // first sorted position of every (box, LJ class) key
+    int *bflag;  // box holds a site not listed from every neighbour box
     int2 *work;
     int *nwork;
     int max_work;
@@ -113,6 +128,8 @@ struct sesmd_ctx {
     double *nh;        // [0] eta, [1..6] sums, [7] tr, [8] pnh1, [9] pnh2
     int *counters;     // [0] shake failures
     int nblk_w, nblk_s;
+    bool surf_fast;    // topology equals the reference's: unrolled SHAKE/RATTLE
+    SurfConst sconst;
     // graph of the regular step
     hipGraphExec_t graph;
     bool graph_ok;
@@ -367,9 +384,9 @@ __global__ void k_classify(Geo g, const double *__restrict__ r, const int *__res
 // the box whose half-open cube [c - rs/2, c + rs/2) holds them (isCentral,
 // src/ibox.f90:101-124, as the list builders use it).  Receiver kinds follow.
 __global__ void k_site_box(Geo g, const double *__restrict__ r, const int *__restrict__ mol,
-                           const int *__restrict__ cls, const int *__restrict__ anc,
-                           int *__restrict__ cbox, int *__restrict__ rkind, int *__restrict__ key,
-                           int *__restrict__ val)
+                           const int *__restrict__ type, const int *__restrict__ cls,
+                           const int *__restrict__ anc, int *__restrict__ cbox,
+                           int *__restrict__ rkind, int *__restrict__ key, int *__restrict__ val)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
@@ -400,7 +417,10 @@ __global__ void k_site_box(Geo g, const double *__restrict__ r, const int *__res
     }
     cbox[i] = b;
     rkind[i] = recv_kind(c, cheb(anc[m], b, g.nb));
-    key[i] = b;
+    // sort key: box, then LJ class (water O, surfactant site, water H), so the
+    // near-field loops need no per-source type test
+    int t = type[i];
+    key[i] = 4 * b + (t == 0 ? 0 : (t >= 3 ? 1 : 2));
     val[i] = i;
 }
 
@@ -438,18 +458,21 @@ __global__ void k_isub(Geo g, const double *__restrict__ r, const int *__restric
     isub[base + j] = mask;
 }
 
-// first sorted position of every box (binary search in the sorted keys)
-__global__ void k_bstart(Geo g, const int *__restrict__ keys, int *__restrict__ bstart)
+// first sorted position of every (box, class) key and of every box (binary
+// search in the sorted keys)
+__global__ void k_bstart(Geo g, const int *__restrict__ keys, int *__restrict__ bs4,
+                         int *__restrict__ bstart)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b > g.nb3) return;
+    int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kk > 4 * g.nb3) return;
     int lo = 0, hi = g.n;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (keys[mid] < b) lo = mid + 1;
+        if (keys[mid] < kk) lo = mid + 1;
         else hi = mid;
     }
-    bstart[b] = lo;
+    bs4[kk] = lo;
+    if ((kk & 3) == 0) bstart[kk >> 2] = lo;
 }
 
 // near-field work list: one item per NEAR_TILE receivers of a box; single block
@@ -483,16 +506,27 @@ __global__ void k_worklist(Geo g, const int *__restrict__ bstart, int2 *__restri
     if (threadIdx.x == 0) *nwork = min(s_base, max_work);
 }
 
-// static metadata of the sorted sites: (site, type | cls << 8, mol, anc)
+// static metadata of the sorted sites: (site, type | cls << 8, mol, packed
+// storage box) with the storage box as x | y << 8 | z << 16 and bit 24 set when
+// the site is listed by the reference from every box of the 3^3 block around
+// its own box (its storage ring stays within the class's list rings: W1 <= 1,
+// W2 <= 2, S <= 3), which spares the near-field loop the per-box ring test.
 __global__ void k_gather_meta(Geo g, const int *__restrict__ perm, const int *__restrict__ type,
                               const int *__restrict__ mol, const int *__restrict__ cls,
-                              const int *__restrict__ anc, int4 *__restrict__ smeta)
+                              const int *__restrict__ anc, const int *__restrict__ cbox,
+                              int4 *__restrict__ smeta, int *__restrict__ bflag)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= g.n) return;
     int i = perm[k];
     int m = mol[i];
-    smeta[k] = make_int4(i, type[i] | (cls[m] << 8), m, anc[m]);
+    int a = anc[m], c = cls[m];
+    int nb = g.nb;
+    int ax = a % nb, ay = (a / nb) % nb, az = a / (nb * nb);
+    int maxring = c == 0 ? 1 : (c == 1 ? 2 : 3);
+    int always = cheb(a, cbox[i], nb) + 1 <= maxring ? 1 : 0;
+    smeta[k] = make_int4(i, type[i] | (c << 8), m, ax | (ay << 8) | (az << 16) | (always << 24));
+    if (!always) atomicOr(&bflag[cbox[i]], 1);
 }
 
 // sorted positions + charge, every step
@@ -516,102 +550,169 @@ __global__ void k_gather_pos(Geo g, const int *__restrict__ perm, const double *
 // are streamed through LDS tiles; each lane owns one receiver, so forces are
 // written without atomics.  MASKED keeps only the (receiver kind, source kind)
 // pairs of `mask` (one reference kernel or a subset of them).
-template <bool MASKED>
-__global__ __launch_bounds__(NEAR_TILE) void k_near(
-    Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
-    const int *__restrict__ bstart, const int *__restrict__ rkind, const int2 *__restrict__ work,
-    const int *__restrict__ nwork, const sesmd_forcefield *__restrict__ ff, unsigned long long mask,
-    double *__restrict__ f)
-{
-    __shared__ double s_ljc[32 * 32];
-    __shared__ double s_ljs6[32 * 32];
-    __shared__ double4 s_pos[NEAR_TILE];
-    __shared__ int s_mol[NEAR_TILE];
-    __shared__ int s_tk[NEAR_TILE];  // type | (source kind + 1) << 8
+constexpr int NEAR_ROWS = 9;  // (dz, dy) rows of the 3^3 neighbourhood, one wave each
 
+struct Recv {
+    double x, y, z, q;       // receiver position, charge * xpi
+    double cO, s6O;          // LJ against water oxygen
+    const double *ljc, *ljs; // LJ row of the receiver's type
+    int mol, rk;
+};
+
+// Sources t in [s0, s1) of one LJ class for one receiver per lane.
+// CAT 0: water O (LJ with the receiver's O parameters), 1: surfactant sites (LJ
+// from the type table), 2: water H (Coulomb only).  CHECK: test the storage
+// ring of every source (only rows holding a site that some neighbour box does
+// not list); MASKED: keep the (receiver kind, source kind) pairs of `mask`.
+template <int CAT, bool CHECK, bool MASKED>
+__device__ __forceinline__ void near_segment(const Geo &g, const double4 *__restrict__ spos,
+                                             const int4 *__restrict__ smeta, int s0, int s1,
+                                             int bx, int by, int bz, const Recv &R,
+                                             unsigned long long mask, double &ax, double &ay,
+                                             double &az)
+{
+    const double rc2 = g.rc2;
+    if (s0 >= s1) return;
+    // one-ahead prefetch: the next record is loaded while this one is used
+    int4 mn = smeta[s0];
+    double4 pn = spos[s0];
+    for (int t = s0; t < s1; t++) {
+        const int4 mj = mn;
+        const double4 pj = pn;
+        int tn = min(t + 1, s1 - 1);
+        mn = smeta[tn];
+        pn = spos[tn];
+        bool ok = mj.z != R.mol;
+        if (CHECK || MASKED) {
+            int cls = (mj.y >> 8) & 3;
+            int ring = max(abs((mj.w & 255) - bx),
+                           max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
+            int sk = cls == 0 ? (ring <= 1 ? ring : -1)
+                              : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
+            if (sk < 0) continue;  // outside every list of the reference (uniform)
+            if (MASKED) ok = ok && ((mask >> (max(R.rk, 0) * 9 + sk)) & 1ull);
+        }
+        double dx = R.x - pj.x, dy = R.y - pj.y, dz = R.z - pj.z;
+        double r2 = dx * dx + dy * dy + dz * dz;
+        double rinv = rsqrt64(r2);
+        double rinv2 = rinv * rinv;
+        double ft = R.q * pj.w * (rinv2 * rinv);
+        if (CAT != 2) {
+            double c, s6;
+            if (CAT == 0) {
+                c = R.cO;
+                s6 = R.s6O;
+            } else {
+                int tj = mj.y & 255;
+                c = R.ljc[tj];
+                s6 = R.ljs[tj];
+            }
+            double r6 = rinv2 * rinv2 * rinv2;
+            double flj = c * r6 * (s6 * r6 - 0.5) * rinv2;
+            ft = (c != 0.0 && r2 <= rc2) ? ft + flj : ft;
+        }
+        ft = ok ? ft : 0.0;
+        ax = fma(dx, ft, ax);
+        ay = fma(dy, ft, ay);
+        az = fma(dz, ft, az);
+    }
+}
+
+template <bool CHECK, bool MASKED>
+__device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict__ spos,
+                                         const int4 *__restrict__ smeta, const int *__restrict__ bs4,
+                                         int r0, int r1, int bx, int by, int bz, const Recv &R,
+                                         unsigned long long mask, double &ax, double &ay, double &az)
+{
+    // boxes r0..r1 (consecutive indices); per box the classes O, S, H
+    for (int bb = r0; bb <= r1; bb++) {
+        int k0 = __builtin_amdgcn_readfirstlane(bs4[4 * bb]);
+        int k1 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 1]);
+        int k2 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 2]);
+        int k3 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 3]);
+        near_segment<0, CHECK, MASKED>(g, spos, smeta, k0, k1, bx, by, bz, R, mask, ax, ay, az);
+        near_segment<1, CHECK, MASKED>(g, spos, smeta, k1, k2, bx, by, bz, R, mask, ax, ay, az);
+        near_segment<2, CHECK, MASKED>(g, spos, smeta, k2, k3, bx, by, bz, R, mask, ax, ay, az);
+    }
+}
+
+template <bool MASKED>
+__global__ __launch_bounds__(NEAR_TILE * NEAR_ROWS) void k_near(
+    Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
+    const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
+    const int *__restrict__ rkind, const int2 *__restrict__ work, const int *__restrict__ nwork,
+    const sesmd_forcefield *__restrict__ ff, unsigned long long mask, double *__restrict__ f)
+{
+    // A block owns NEAR_TILE receivers of box b.  Its nine wavefronts take
+    // one (dz, dy) row of the 3^3 neighbourhood each -- the three x-neighbours
+    // of a row are consecutive boxes -- and every lane of a wave walks the same
+    // source sequence, so each source record is a wave-uniform scalar load and
+    // the per-source integer work runs on the scalar unit; the vector unit only
+    // does the FP64 pair arithmetic.  Within a box the sources are ordered by LJ
+    // class, so the loops carry no per-source type branch.  The nine partial
+    // forces are added in row order through LDS: no atomics, deterministic.
+    __shared__ double s_part[NEAR_ROWS][3][NEAR_TILE];
     int item = blockIdx.x;
     if (item >= *nwork) return;
+    int row = threadIdx.x / NEAR_TILE, lane = threadIdx.x % NEAR_TILE;
     int2 w = work[item];
     int b = w.x;
-    for (int t = threadIdx.x; t < 1024; t += NEAR_TILE) {
-        s_ljc[t] = ff->lj_c[t];
-        s_ljs6[t] = ff->lj_s6[t];
-    }
     int kend = bstart[b + 1];
-    int k = w.y + threadIdx.x;
+    int k = w.y + lane;
     bool active = k < kend;
-    double xi = 0, yi = 0, zi = 0, qi = 0;
-    int ti = 0, mi = -1, rk = -1, site = 0;
+    Recv R;
+    R.x = R.y = R.z = R.q = 0.0;
+    R.mol = -1;
+    R.rk = -1;
+    int ti = 0, site = 0;
     if (active) {
         double4 pi = spos[k];
         int4 me = smeta[k];
-        xi = pi.x;
-        yi = pi.y;
-        zi = pi.z;
-        qi = pi.w * g.xpi;
+        R.x = pi.x;
+        R.y = pi.y;
+        R.z = pi.z;
+        R.q = pi.w * g.xpi;
         site = me.x;
         ti = me.y & 255;
-        mi = me.z;
-        rk = rkind[site];
-        if (rk < 0) active = false;
+        R.mol = me.z;
+        R.rk = rkind[site];
+        if (R.rk < 0) active = false;
     }
+    R.cO = ff->lj_c[ti * 32];
+    R.s6O = ff->lj_s6[ti * 32];
+    R.ljc = ff->lj_c + ti * 32;
+    R.ljs = ff->lj_s6 + ti * 32;
     int nb = g.nb;
     int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
-    double accx = 0.0, accy = 0.0, accz = 0.0;
-    const double rc2 = g.rc2;
-    int rowoff = ti * 32;
-    for (int dz = -1; dz <= 1; dz++) {
-        int z = bz + dz;
-        if (z < 0 || z >= nb) continue;
-        for (int dy = -1; dy <= 1; dy++) {
-            int y = by + dy;
-            if (y < 0 || y >= nb) continue;
-            int x0 = max(bx - 1, 0), x1 = min(bx + 1, nb - 1);
-            // the three x-neighbours are contiguous in the sorted order
-            int s0 = bstart[x0 + nb * (y + nb * z)];
-            int s1 = bstart[x1 + 1 + nb * (y + nb * z)];
-            for (int t0 = s0; t0 < s1; t0 += NEAR_TILE) {
-                __syncthreads();
-                int t = t0 + threadIdx.x;
-                if (t < s1) {
-                    int4 me = smeta[t];
-                    s_pos[threadIdx.x] = spos[t];
-                    s_mol[threadIdx.x] = me.z;
-                    int sk = src_kind(me.y >> 8, cheb(me.w, b, nb));
-                    s_tk[threadIdx.x] = (me.y & 255) | ((sk + 1) << 8);
-                }
-                __syncthreads();
-                int nt = min(NEAR_TILE, s1 - t0);
-                if (!active) continue;
-                for (int u = 0; u < nt; u++) {
-                    int tk = s_tk[u];
-                    int sk = (tk >> 8) - 1;
-                    bool ok = (s_mol[u] != mi) && (sk >= 0);
-                    if (MASKED) ok = ok && ((mask >> (rk * 9 + sk)) & 1ull);
-                    if (!ok) continue;
-                    double4 pj = s_pos[u];
-                    double dx = xi - pj.x, dy2 = yi - pj.y, dz2 = zi - pj.z;
-                    double r2 = dx * dx + dy2 * dy2 + dz2 * dz2;
-                    double rinv = rsqrt64(r2);
-                    double rinv2 = rinv * rinv;
-                    double ft = qi * pj.w * (rinv2 * rinv);
-                    double c = s_ljc[rowoff + (tk & 255)];
-                    if (c != 0.0 && r2 <= rc2) {
-                        double s6 = s_ljs6[rowoff + (tk & 255)];
-                        double r6 = rinv2 * rinv2 * rinv2;
-                        ft += c * r6 * (s6 * r6 - 0.5) * rinv2;
-                    }
-                    accx = fma(dx, ft, accx);
-                    accy = fma(dy2, ft, accy);
-                    accz = fma(dz2, ft, accz);
-                }
-            }
-        }
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    int z = bz + row / 3 - 1, y = by + row % 3 - 1;
+    if (z >= 0 && z < nb && y >= 0 && y < nb) {
+        int r0 = max(bx - 1, 0) + nb * (y + nb * z), r1 = min(bx + 1, nb - 1) + nb * (y + nb * z);
+        bool check = MASKED;
+        if (!MASKED)
+            for (int bb = r0; bb <= r1; bb++) check = check || bflag[bb];
+        if (MASKED)
+            near_row<true, true>(g, spos, smeta, bs4, r0, r1, bx, by, bz, R, mask, ax, ay, az);
+        else if (check)
+            near_row<true, false>(g, spos, smeta, bs4, r0, r1, bx, by, bz, R, mask, ax, ay, az);
+        else
+            near_row<false, false>(g, spos, smeta, bs4, r0, r1, bx, by, bz, R, mask, ax, ay, az);
     }
-    if (active) {
-        f[site] += accx;
-        f[g.n + site] += accy;
-        f[2 * g.n + site] += accz;
+    s_part[row][0][lane] = ax;
+    s_part[row][1][lane] = ay;
+    s_part[row][2][lane] = az;
+    __syncthreads();
+    if (row == 0 && active) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+        for (int q = 0; q < NEAR_ROWS; q++) {
+            sx += s_part[q][0][lane];
+            sy += s_part[q][1][lane];
+            sz += s_part[q][2][lane];
+        }
+        f[site] += sx;
+        f[g.n + site] += sy;
+        f[2 * g.n + site] += sz;
     }
 }
 
@@ -646,7 +747,9 @@ __global__ void k_pair_count(Geo g, const double4 *__restrict__ spos, const int4
                 for (int t = bstart[bb]; t < bstart[bb + 1]; t++) {
                     int4 mj = smeta[t];
                     if (mj.z == me.z) continue;
-                    if (src_kind(mj.y >> 8, cheb(mj.w, b, nb)) < 0) continue;
+                    int ring = max(abs((mj.w & 255) - bx),
+                                   max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
+                    if (src_kind((mj.y >> 8) & 3, ring) < 0) continue;
                     np++;
                     double4 pj = spos[t];
                     double ddx = pi.x - pj.x, ddy = pi.y - pj.y, ddz = pi.z - pj.z;
@@ -663,6 +766,7 @@ __global__ void k_pair_count(Geo g, const double4 *__restrict__ spos, const int4
 // P2M, src/computeOnm4.f90:142-162: O_n^m = sum q rho^n Y_n^{-m}(alpha, beta)
 // about the box centre over the box's receivers; one block per box, one
 // thread per site, deterministic tree reduction in LDS.
+template <int PP>
 __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
                                             const int *__restrict__ perm,
                                             const int *__restrict__ bstart,
@@ -673,10 +777,11 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
 {
     __shared__ double s_re[64][MAXLK], s_im[64][MAXLK];
     int b = blockIdx.x;
-    int p = g.p, lk = g.lk, lp = g.lp;
+    constexpr int p = PP, lk = (PP + 1) * (PP + 2) / 2, lp = (PP + 1) * (PP + 1);
     int nb = g.nb;
     double cx = d_cen4(g, b % nb), cy = d_cen4(g, (b / nb) % nb), cz = d_cen4(g, b / (nb * nb));
     double are[MAXLK], aim[MAXLK];
+#pragma unroll
     for (int c = 0; c < lk; c++) are[c] = aim[c] = 0.0;
     int s0 = bstart[b], s1 = bstart[b + 1];
     int any = 0;
@@ -692,8 +797,10 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
         // P_n^m(cos alpha) by the recurrence of rLegendreP, columns m = 0..p
         double P[MAXP + 1][MAXP + 1];
         double somx2 = sqrt((1.0 - ca) * (1.0 + ca));
+#pragma unroll
         for (int m = 0; m <= p; m++) {
             double pmm = 1.0, fact = 1.0;
+#pragma unroll
             for (int i2 = 1; i2 <= m; i2++) {
                 pmm = -pmm * fact * somx2;
                 fact = fact + 2.0;
@@ -703,6 +810,7 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
                 double pmmp1 = ca * (2 * m + 1) * pmm;
                 P[m + 1][m] = pmmp1;
                 double pm2 = pmm;
+#pragma unroll
                 for (int ll = m + 2; ll <= p; ll++) {
                     double pll = (ca * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pm2) / (ll - m);
                     pm2 = pmmp1;
@@ -712,7 +820,9 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
             }
         }
         double rhon = 1.0;
+#pragma unroll
         for (int n = 0; n <= p; n++) {
+#pragma unroll
             for (int m = 0; m <= n; m++) {
                 // Y_n^{-m}: phase exp(-i m beta) rounded to single precision
                 float cr = (float)cos(-m * be), ci = (float)sin(-m * be);
@@ -725,19 +835,23 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
             rhon *= rho;
         }
     }
+#pragma unroll
     for (int c = 0; c < lk; c++) {
         s_re[threadIdx.x][c] = are[c];
         s_im[threadIdx.x][c] = aim[c];
     }
     any = __syncthreads_or(any);
+#pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off)
+#pragma unroll
             for (int c = 0; c < lk; c++) {
                 s_re[threadIdx.x][c] += s_re[threadIdx.x + off][c];
                 s_im[threadIdx.x][c] += s_im[threadIdx.x + off][c];
             }
         __syncthreads();
     }
+#pragma unroll
     for (int c = threadIdx.x; c < lp; c += 64) {
         int n = (int)floor(sqrt((double)c) + 1e-9);
         int m = c - n * (n + 1);
@@ -805,12 +919,15 @@ __global__ __launch_bounds__(64) void k_m2m(Geo g, int nd_parent, const double2 
 // children of the parent's neighbours that are not neighbours of the target.
 // accumulate = the table already holds the L2L part (shiftLjk zeroes, then
 // computeLjk accumulates).
-__global__ __launch_bounds__(64) void k_m2l(Geo g, int nd, const double2 *__restrict__ O,
-                                            const int *__restrict__ ne, const double2 *__restrict__ T,
-                                            double2 *__restrict__ L)
+__global__ __launch_bounds__(256) void k_m2l(Geo g, int nd, const double2 *__restrict__ O,
+                                             const int *__restrict__ ne, const double2 *__restrict__ T,
+                                             double2 *__restrict__ L)
 {
+    // 16 lanes per source group (one per k >= 0 row), 16 source groups; the
+    // partial sums are added in group order, so the result is deterministic.
+    __shared__ double2 s_acc[16][16];
     int Tb = blockIdx.x;
-    int row = threadIdx.x;
+    int row = threadIdx.x & 15, grp = threadIdx.x >> 4;
     int tx = Tb % nd, ty = (Tb / nd) % nd, tz = Tb / (nd * nd);
     int lo[3], hi[3], t[3] = {tx, ty, tz};
     for (int a = 0; a < 3; a++) {
@@ -818,18 +935,28 @@ __global__ __launch_bounds__(64) void k_m2l(Geo g, int nd, const double2 *__rest
         lo[a] = max(2 * (par - 1), 0);
         hi[a] = min(2 * (par + 1) + 1, nd - 1);
     }
-    if (row >= g.lk) return;
     double2 acc = make_double2(0.0, 0.0);
-    for (int sz = lo[2]; sz <= hi[2]; sz++)
-        for (int sy = lo[1]; sy <= hi[1]; sy++)
-            for (int sx = lo[0]; sx <= hi[0]; sx++) {
-                int dx = sx - tx, dy = sy - ty, dz = sz - tz;
-                if (iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1) continue;
-                int S = sx + nd * (sy + nd * sz);
-                if (!ne[S]) continue;
-                int off = (dx + 3) + 7 * ((dy + 3) + 7 * (dz + 3));
-                contract_rows(g, T + (size_t)off * g.lk * g.lp, O + (size_t)S * g.lp, acc, row);
-            }
+    if (row < g.lk) {
+        int ord = 0;
+        for (int sz = lo[2]; sz <= hi[2]; sz++)
+            for (int sy = lo[1]; sy <= hi[1]; sy++)
+                for (int sx = lo[0]; sx <= hi[0]; sx++) {
+                    int dx = sx - tx, dy = sy - ty, dz = sz - tz;
+                    if (iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1) continue;
+                    int S = sx + nd * (sy + nd * sz);
+                    if (!ne[S]) continue;
+                    if ((ord++ & 15) != grp) continue;
+                    int off = (dx + 3) + 7 * ((dy + 3) + 7 * (dz + 3));
+                    contract_rows(g, T + (size_t)off * g.lk * g.lp, O + (size_t)S * g.lp, acc, row);
+                }
+    }
+    s_acc[grp][row] = acc;
+    __syncthreads();
+    if (grp != 0 || row >= g.lk) return;
+    for (int q = 1; q < 16; q++) {
+        acc.x += s_acc[q][row].x;
+        acc.y += s_acc[q][row].y;
+    }
     // add onto the L2L part (row k >= 0), rebuild the k < 0 half
     int j = 0;
     while ((j + 1) * (j + 2) / 2 <= row) j++;
@@ -884,7 +1011,8 @@ __global__ __launch_bounds__(64) void k_l2l(Geo g, int nd_child, const double2 *
 
 // L2P (src/evalLjk4.f90:38-68, the same formula for every receiver class) and
 // the 1/x^13 walls (src/eval_wall.f90:149-188), one thread per site.
-__global__ void k_site_terms(Geo g, unsigned what, const double *__restrict__ r,
+template <int PP>
+__global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const double *__restrict__ r,
                              const int *__restrict__ type, const int *__restrict__ cbox,
                              const int *__restrict__ rkind, const double2 *__restrict__ L4,
                              const double *__restrict__ lfac, const sesmd_forcefield *__restrict__ ff,
@@ -895,7 +1023,8 @@ __global__ void k_site_terms(Geo g, unsigned what, const double *__restrict__ r,
     double x = r[i], y = r[g.n + i], z = r[2 * g.n + i];
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if ((what & SESMD_FAR) && rkind[i] >= 0) {
-        int b = cbox[i], nb = g.nb, p = g.p;
+        int b = cbox[i], nb = g.nb;
+        constexpr int p = PP;
         double rxpb = x - d_cen4(g, b % nb), rypb = y - d_cen4(g, (b / nb) % nb),
                rzpb = z - d_cen4(g, b / (nb * nb));
         double rqq = ff->q[type[i]] * g.xpi;
@@ -907,12 +1036,16 @@ __global__ void k_site_terms(Geo g, unsigned what, const double *__restrict__ r,
         double rp3 = 1.0 / rho2;
         // Legendre table P[l][m], l = -1..p (P[-1][0] = 1, rLegendreP's l = -1 case)
         double P[MAXP + 2][MAXP + 1];
+#pragma unroll
         for (int l = 0; l <= p + 1; l++)
+#pragma unroll
             for (int m = 0; m <= p; m++) P[l][m] = 0.0;
         P[0][0] = 1.0;  // l = -1
         double somx2 = sqrt((1.0 - rp1) * (1.0 + rp1));
+#pragma unroll
         for (int m = 0; m <= p; m++) {
             double pmm = 1.0, fact = 1.0;
+#pragma unroll
             for (int i2 = 1; i2 <= m; i2++) {
                 pmm = -pmm * fact * somx2;
                 fact = fact + 2.0;
@@ -922,6 +1055,7 @@ __global__ void k_site_terms(Geo g, unsigned what, const double *__restrict__ r,
                 double pmmp1 = rp1 * (2 * m + 1) * pmm;
                 P[m + 2][m] = pmmp1;
                 double pm2 = pmm;
+#pragma unroll
                 for (int ll = m + 2; ll <= p; ll++) {
                     double pll = (rp1 * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pm2) / (ll - m);
                     pm2 = pmmp1;
@@ -933,11 +1067,14 @@ __global__ void k_site_terms(Geo g, unsigned what, const double *__restrict__ r,
         // e^{i k phi} for k = 0..p by repeated multiplication
         double2 pw[MAXP + 1];
         pw[0] = make_double2(1.0, 0.0);
+#pragma unroll
         for (int k2 = 1; k2 <= p; k2++) pw[k2] = cmul(pw[k2 - 1], rp2);
         const double2 *L = L4 + (size_t)b * g.lp;
         double rrj = 1.0 / rr;  // rr^(j-1)
         double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
         for (int j = 0; j <= p; j++) {
+#pragma unroll
             for (int k = -j; k <= j; k++) {
                 int ak = k < 0 ? -k : k;
                 double2 e = pw[ak];
@@ -1082,7 +1219,7 @@ __device__ void dihedral_forces(const sesmd_forcefield *ff, int d, const double 
 //   eval_subtract_fma (src/eval_subtract_fma.f90:10-54),
 //   eval_valence (src/eval_valence.f90:97-158),
 //   eval_dihedral / eval_dihedral_berendsen (src/eval_dihedral.f90:102-295).
-__global__ void k_intra(Geo g, unsigned what, const double *__restrict__ r,
+__global__ __launch_bounds__(64) void k_intra(Geo g, unsigned what, const double *__restrict__ r,
                         const unsigned *__restrict__ isub, const sesmd_forcefield *__restrict__ ff,
                         const Topo *__restrict__ topo, double *__restrict__ f)
 {
@@ -1169,9 +1306,13 @@ __global__ void k_intra(Geo g, unsigned what, const double *__restrict__ r,
             ld3(r, n, base + ff->dih_l[d] - 1, Rl);
             double F[4][3];
             dihedral_forces(ff, d, Ri, Rj, Rk, Rl, F);
-            fx += F[role][0];
-            fy += F[role][1];
-            fz += F[role][2];
+            // select the site's row without indexing F at run time (keeps F in registers)
+            double gx = role == 0 ? F[0][0] : role == 1 ? F[1][0] : role == 2 ? F[2][0] : F[3][0];
+            double gy = role == 0 ? F[0][1] : role == 1 ? F[1][1] : role == 2 ? F[2][1] : F[3][1];
+            double gz = role == 0 ? F[0][2] : role == 1 ? F[1][2] : role == 2 ? F[2][2] : F[3][2];
+            fx += gx;
+            fy += gy;
+            fz += gz;
         }
     }
     f[base + j] += fx;
@@ -1384,6 +1525,165 @@ __global__ __launch_bounds__(64) void k_move_a_surf(Geo g, double *__restrict__ 
     }
     block_sum_store<NH_VALS>(acc, partials + (size_t)(g.nw > 0 ? (g.nw + 255) / 256 : 0) * NH_VALS +
                                       (size_t)blockIdx.x * NH_VALS);
+}
+
+constexpr int SURF_NT = 32;  // molecules (threads) per block
+
+// Part A for surfactants (src/move_surfactant_a.f90:19-147), one thread per
+// molecule: p in registers, v and the bond vectors r_a - r_b in lane-strided
+// LDS; constraint order, tolerance test and updates exactly as the reference.
+__global__ __launch_bounds__(SURF_NT) void k_move_a_surf_fast(
+    Geo g, double *__restrict__ r, double *__restrict__ v, const double *__restrict__ a,
+    double *__restrict__ f, SurfConst sc, const double *__restrict__ nh,
+    double *__restrict__ partials, int *__restrict__ counters)
+{
+    __shared__ double s_v[87][SURF_NT];
+    __shared__ double s_rb[84][SURF_NT];
+    int t = threadIdx.x;
+    int s = blockIdx.x * SURF_NT + t;
+    double acc[NH_VALS];
+#pragma unroll
+    for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
+    int n = g.n;
+    if (s < g.ns) {
+        double escale = 1.0 - nh[0] * g.dt / 2.0;
+        int base = 3 * g.nw + 29 * s;
+        double p[29][3];
+#pragma unroll
+        for (int i = 0; i < 29; i++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) p[i][d] = r[d * n + base + i];
+#pragma unroll
+        for (int c = 0; c < 28; c++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) s_rb[3 * c + d][t] = p[kBI[c]][d] - p[kBJ[c]][d];
+#pragma unroll
+        for (int i = 0; i < 29; i++) {
+            double vv = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double v0 = v[d * n + base + i], aa = a[d * n + base + i];
+                p[i][d] = p[i][d] + v0 * escale + aa;
+                s_v[3 * i + d][t] = v0 * escale + aa;
+                vv += v0 * v0;
+            }
+            acc[4 + c_sgrp[i]] += vv;
+        }
+        const double tol2 = 2.0 * 0.0001;
+        int it = 0;
+        bool done = false;
+        while (!done && it <= 100) {
+            done = true;
+#pragma unroll
+            for (int c = 0; c < 28; c++) {
+                const int ia = kBI[c], ib = kBJ[c];
+                double pxab = p[ia][0] - p[ib][0], pyab = p[ia][1] - p[ib][1], pzab = p[ia][2] - p[ib][2];
+                double pabsq = pxab * pxab + pyab * pyab + pzab * pzab;
+                double rabsq = sc.dsq[c];
+                double diffsq = rabsq - pabsq;
+                if (fabs(diffsq) > rabsq * tol2) {
+                    double rxab = s_rb[3 * c][t], ryab = s_rb[3 * c + 1][t], rzab = s_rb[3 * c + 2][t];
+                    double rpab = rxab * pxab + ryab * pyab + rzab * pzab;
+                    double rma = sc.rm[ia], rmb = sc.rm[ib];
+                    double gab = diffsq / (2.0 * (rma + rmb) * rpab);
+                    double dx = rxab * gab, dy = ryab * gab, dz = rzab * gab;
+                    p[ia][0] = p[ia][0] + rma * dx;
+                    p[ia][1] = p[ia][1] + rma * dy;
+                    p[ia][2] = p[ia][2] + rma * dz;
+                    p[ib][0] = p[ib][0] - rmb * dx;
+                    p[ib][1] = p[ib][1] - rmb * dy;
+                    p[ib][2] = p[ib][2] - rmb * dz;
+                    s_v[3 * ia][t] = s_v[3 * ia][t] + rma * dx;
+                    s_v[3 * ia + 1][t] = s_v[3 * ia + 1][t] + rma * dy;
+                    s_v[3 * ia + 2][t] = s_v[3 * ia + 2][t] + rma * dz;
+                    s_v[3 * ib][t] = s_v[3 * ib][t] - rmb * dx;
+                    s_v[3 * ib + 1][t] = s_v[3 * ib + 1][t] - rmb * dy;
+                    s_v[3 * ib + 2][t] = s_v[3 * ib + 2][t] - rmb * dz;
+                    done = false;
+                }
+            }
+            it++;
+        }
+        if (!done) atomicAdd(&counters[0], 1);
+#pragma unroll
+        for (int i = 0; i < 29; i++) {
+            double vv = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double vn = s_v[3 * i + d][t];
+                r[d * n + base + i] = p[i][d];
+                v[d * n + base + i] = vn;
+                f[d * n + base + i] = 0.0;
+                vv += vn * vn;
+            }
+            acc[13 + c_sgrp[i]] += vv;
+        }
+    }
+    block_sum_store<NH_VALS>(acc, partials + (size_t)(g.nw > 0 ? (g.nw + 255) / 256 : 0) * NH_VALS +
+                                      (size_t)blockIdx.x * NH_VALS);
+}
+
+// Part B for surfactants (src/move_surfactant_b.f90:23-122): velocities in
+// registers, bond vectors in LDS, RATTLE in the reference's order.
+__global__ __launch_bounds__(SURF_NT) void k_move_b_surf_fast(
+    Geo g, const double *__restrict__ r, double *__restrict__ v, double *__restrict__ a,
+    const double *__restrict__ f, SurfConst sc, const double *__restrict__ nh,
+    int *__restrict__ counters)
+{
+    __shared__ double s_rb[84][SURF_NT];
+    int t = threadIdx.x;
+    int s = blockIdx.x * SURF_NT + t;
+    if (s >= g.ns) return;
+    int n = g.n, base = 3 * g.nw + 29 * s;
+    double escale = 2.0 / (2.0 + nh[0] * g.dt);
+    double vx[29][3];
+#pragma unroll
+    for (int i = 0; i < 29; i++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) vx[i][d] = r[d * n + base + i];
+#pragma unroll
+    for (int c = 0; c < 28; c++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) s_rb[3 * c + d][t] = vx[kBI[c]][d] - vx[kBJ[c]][d];
+#pragma unroll
+    for (int i = 0; i < 29; i++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double ac = f[d * n + base + i] * g.dtsq2 / sc.m[i];
+            a[d * n + base + i] = ac;
+            vx[i][d] = escale * (v[d * n + base + i] + ac);
+        }
+    const double dt = g.dt;
+    int it = 0;
+    bool done = false;
+    while (!done && it <= 100) {
+        done = true;
+#pragma unroll
+        for (int c = 0; c < 28; c++) {
+            const int ia = kBI[c], ib = kBJ[c];
+            double rma = sc.rm[ia], rmb = sc.rm[ib];
+            double vxab = vx[ia][0] - vx[ib][0], vyab = vx[ia][1] - vx[ib][1], vzab = vx[ia][2] - vx[ib][2];
+            double rxab = s_rb[3 * c][t], ryab = s_rb[3 * c + 1][t], rzab = s_rb[3 * c + 2][t];
+            double rvab = (rxab * vxab + ryab * vyab + rzab * vzab) / dt;
+            double gab = -rvab / ((rma + rmb) * sc.dsq[c]);
+            if (fabs(gab) > 0.0001) {
+                double dx = rxab * gab, dy = ryab * gab, dz = rzab * gab;
+                vx[ia][0] = vx[ia][0] + rma * dx * dt;
+                vx[ia][1] = vx[ia][1] + rma * dy * dt;
+                vx[ia][2] = vx[ia][2] + rma * dz * dt;
+                vx[ib][0] = vx[ib][0] - rmb * dx * dt;
+                vx[ib][1] = vx[ib][1] - rmb * dy * dt;
+                vx[ib][2] = vx[ib][2] - rmb * dz * dt;
+                done = false;
+            }
+        }
+        it++;
+    }
+    if (!done) atomicAdd(&counters[0], 1);
+#pragma unroll
+    for (int i = 0; i < 29; i++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[d * n + base + i] = vx[i][d];
 }
 
 // kinetic sums -> the eta update of src/move_w2_a.f90:145-157 (single block)
@@ -1664,7 +1964,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     *out = nullptr;
     if (n_water < 0 || n_surf < 0 || n_water + n_surf == 0) return fail("empty system");
     if (nb2 < 1 || nb2 > 8) return fail("nb2 must be 1..8");
-    if (p < 1 || p > MAXP) return fail("p must be 1..%d", MAXP);
+    if (p != 4 && p != 8) return fail("p must be 4 (the reference) or 8");
     if (precision != SESMD_FP64 && precision != SESMD_MIXED) return fail("bad precision");
     if (check_topology(*ff)) return 1;
     int ndev = 0;
@@ -1719,6 +2019,8 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->val_in, n));
     A(dalloc(&c->perm, n));
     A(dalloc(&c->bstart, g.nb3 + 1));
+    A(dalloc(&c->bs4, 4 * (size_t)g.nb3 + 1));
+    A(dalloc(&c->bflag, g.nb3));
     A(dalloc(&c->nvalid, 1));
     c->max_work = g.nb3 + (int)(n / NEAR_TILE) + 1;
     A(dalloc(&c->work, c->max_work));
@@ -1748,8 +2050,16 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->gr, 3));
     A(dalloc(&c->dff, 1));
     A(dalloc(&c->dtopo, 1));
+    c->surf_fast = true;
+    for (int b = 0; b < 28; b++)
+        if (ff->ibond[b] - 1 != kBI[b] || ff->jbond[b] - 1 != kBJ[b]) c->surf_fast = false;
+    for (int i = 0; i < 29; i++) {
+        c->sconst.m[i] = ff->mass[3 + i];
+        c->sconst.rm[i] = 1.0 / ff->mass[3 + i];
+    }
+    for (int b = 0; b < 28; b++) c->sconst.dsq[b] = ff->dsqs[b];
     c->nblk_w = g.nw > 0 ? (int)nblk(g.nw, 256) : 0;
-    c->nblk_s = g.ns > 0 ? (int)nblk(g.ns, 64) : 0;
+    c->nblk_s = g.ns > 0 ? (int)nblk(g.ns, c->surf_fast ? SURF_NT : 64) : 0;
     A(dalloc(&c->partials, (size_t)(c->nblk_w + c->nblk_s + 1) * NH_VALS));
     A(dalloc(&c->nh, 16));
     A(dalloc(&c->counters, 4));
@@ -1796,7 +2106,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     // radix-sort workspace
     size_t bytes = 0;
     int endbit = 1;
-    while ((1 << endbit) <= g.nb3) endbit++;
+    while ((1 << endbit) <= 4 * g.nb3) endbit++;
     hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->key_in, c->key_out, c->val_in, c->perm,
                                        (int)n, 0, endbit, c->stream);
     c->cub_bytes = bytes;
@@ -1815,7 +2125,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
     if (c->graph_ok) hipGraphExecDestroy(c->graph);
     void *ptrs[] = {c->r, c->v, c->a, c->f, c->type, c->mol, c->first, c->cls, c->anc, c->cbox,
                     c->rkind, c->isub, c->key_in, c->key_out, c->val_in, c->perm, c->cub_tmp,
-                    c->bstart, c->nvalid, c->work, c->nwork, c->spos, c->smeta, c->O4, c->L4,
+                    c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
                     c->dtopo, c->partials, c->nh, c->counters};
@@ -1923,18 +2233,19 @@ static int launch_rebuild(sesmd_ctx *c)
     const Geo &g = c->g;
     hipStream_t st = c->stream;
     k_classify<<<nblk(g.nmol, 256), 256, 0, st>>>(g, c->r, c->first, c->cls, c->anc);
-    k_site_box<<<nblk(g.n, 256), 256, 0, st>>>(g, c->r, c->mol, c->cls, c->anc, c->cbox, c->rkind,
-                                               c->key_in, c->val_in);
+    k_site_box<<<nblk(g.n, 256), 256, 0, st>>>(g, c->r, c->mol, c->type, c->cls, c->anc, c->cbox,
+                                               c->rkind, c->key_in, c->val_in);
     if (g.ns) k_isub<<<nblk(29 * g.ns, 128), 128, 0, st>>>(g, c->r, c->anc, c->isub);
     int endbit = 1;
-    while ((1 << endbit) <= g.nb3) endbit++;
+    while ((1 << endbit) <= 4 * g.nb3) endbit++;
     size_t bytes = c->cub_bytes;
     HC(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, bytes, c->key_in, c->key_out, c->val_in,
                                           c->perm, g.n, 0, endbit, st));
-    k_bstart<<<nblk(g.nb3 + 1, 256), 256, 0, st>>>(g, c->key_out, c->bstart);
+    k_bstart<<<nblk(4 * g.nb3 + 1, 256), 256, 0, st>>>(g, c->key_out, c->bs4, c->bstart);
+    k_fill_i<<<nblk(g.nb3, 256), 256, 0, st>>>(c->bflag, g.nb3, 0);
     k_worklist<<<1, 1024, 0, st>>>(g, c->bstart, c->work, c->nwork, c->max_work);
     k_gather_meta<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->type, c->mol, c->cls, c->anc,
-                                                  c->smeta);
+                                                  c->cbox, c->smeta, c->bflag);
     HC(hipGetLastError());
     c->built = true;
     return 0;
@@ -1952,14 +2263,17 @@ static int launch_fmm(sesmd_ctx *c)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
-    k_p2m<<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
+    if (g.p == 4)
+        k_p2m<4><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
+    else
+        k_p2m<8><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
     k_m2m<<<g.n33, 64, 0, st>>>(g, g.n3, c->O4, c->ne4, c->M43, c->O3, c->ne3);
     k_m2m<<<g.n2, 64, 0, st>>>(g, g.nb2, c->O3, c->ne3, c->M32, c->O2, c->ne2);
     k_m2l_top<<<g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->L2);
     k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3);
-    k_m2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->O3, c->ne3, c->T3, c->L3);
+    k_m2l<<<g.n33, 256, 0, st>>>(g, g.n3, c->O3, c->ne3, c->T3, c->L3);
     k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4);
-    k_m2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->O4, c->ne4, c->T4, c->L4);
+    k_m2l<<<g.nb3, 256, 0, st>>>(g, g.nb, c->O4, c->ne4, c->T4, c->L4);
     HC(hipGetLastError());
     return 0;
 }
@@ -1983,14 +2297,14 @@ static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask)
         k_gather_pos<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->r, c->type, c->dff, c->spos);
     if (what & SESMD_NEAR) {
         if (mask == ~0ull)
-            k_near<false><<<c->max_work, NEAR_TILE, 0, st>>>(g, c->spos, c->smeta, c->bstart, c->rkind,
+            k_near<false><<<c->max_work, NEAR_TILE * NEAR_ROWS, 0, st>>>(g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind,
                                                            c->work, c->nwork, c->dff, mask, c->f);
         else
-            k_near<true><<<c->max_work, NEAR_TILE, 0, st>>>(g, c->spos, c->smeta, c->bstart, c->rkind,
+            k_near<true><<<c->max_work, NEAR_TILE * NEAR_ROWS, 0, st>>>(g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind,
                                                           c->work, c->nwork, c->dff, mask, c->f);
     }
     if (what & (SESMD_FAR | SESMD_WALL))
-        k_site_terms<<<nblk(g.n, 128), 128, 0, st>>>(g, what, c->r, c->type, c->cbox, c->rkind, c->L4,
+        (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<nblk(g.n, 128), 128, 0, st>>>(g, what, c->r, c->type, c->cbox, c->rkind, c->L4,
                                                      c->lfac, c->dff, c->f);
     if (g.ns && (what & (SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL |
                          SESMD_DIHEDRAL_BE)))
@@ -2015,7 +2329,7 @@ extern "C" int sesmd_near_only(sesmd_ctx *c)
 {
     if (!c || !c->built) return fail("not built");
     const Geo &g = c->g;
-    k_near<false><<<c->max_work, NEAR_TILE, 0, c->stream>>>(g, c->spos, c->smeta, c->bstart, c->rkind,
+    k_near<false><<<c->max_work, NEAR_TILE * NEAR_ROWS, 0, c->stream>>>(g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind,
                                                            c->work, c->nwork, c->dff, ~0ull, c->f);
     HC(hipGetLastError());
     return 0;
@@ -2040,7 +2354,10 @@ static int launch_move_a(sesmd_ctx *c)
     if (c->nblk_w)
         k_move_a_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->cls, c->dff, c->nh,
                                                    c->partials, c->counters);
-    if (c->nblk_s)
+    if (c->nblk_s && c->surf_fast)
+        k_move_a_surf_fast<<<c->nblk_s, SURF_NT, 0, st>>>(g, c->r, c->v, c->a, c->f, c->sconst, c->nh,
+                                                         c->partials, c->counters);
+    else if (c->nblk_s)
         k_move_a_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->partials,
                                                 c->counters);
     k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh);
@@ -2064,7 +2381,10 @@ static int launch_move_b(sesmd_ctx *c)
     hipStream_t st = c->stream;
     if (c->nblk_w)
         k_move_b_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->counters);
-    if (c->nblk_s)
+    if (c->nblk_s && c->surf_fast)
+        k_move_b_surf_fast<<<c->nblk_s, SURF_NT, 0, st>>>(g, c->r, c->v, c->a, c->f, c->sconst, c->nh,
+                                                         c->counters);
+    else if (c->nblk_s)
         k_move_b_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->counters);
     HC(hipGetLastError());
     return 0;
