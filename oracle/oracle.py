@@ -37,9 +37,14 @@ class OrcFF(ctypes.Structure):
     ]
 
 
-def make_ff():
+def make_ff(lj=True):
+    """The reference force field as an orc_ff; lj=False drops every LJ term
+    (for Coulomb-only checks against a direct sum)."""
     from surfactant_enhanced_spreading_amd import forcefield as ff
     T = ff.type_tables()
+    if not lj:
+        T['lj_c'][:] = 0.0
+        T['gr_c'][:] = 0.0
     s = OrcFF()
     s.q[:] = list(T['q'])
     s.mass[:] = list(T['mass'])
@@ -95,6 +100,9 @@ def lib():
         L.orc_shake_failures.restype = ctypes.c_int
         L.orc_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_ylm.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp]
+        L.orc_legendre.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.orc_legendre.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -106,11 +114,11 @@ def _p(a, t=ctypes.c_double):
 class Oracle:
     """The C restatement for one system (reduced units, (N,3) arrays)."""
 
-    def __init__(self, sys_, nb2=3, p=4, tr=None, eta=0.0):
+    def __init__(self, sys_, nb2=3, p=4, tr=None, eta=0.0, lj=True):
         from surfactant_enhanced_spreading_amd import forcefield as ff
         self.ff = ff
         self.L = lib()
-        self._ffs = make_ff()
+        self._ffs = make_ff(lj)
         self.n_water, self.n_surf = sys_.n_water, sys_.n_surf
         self.n = sys_.n_sites
         self.nb2, self.p = nb2, p
@@ -194,6 +202,11 @@ class Oracle:
 
     def shake_failures(self):
         return self.L.orc_shake_failures(self.h)
+
+    def ylm(self, n, m, a, b):
+        out = np.zeros(2)
+        self.L.orc_ylm(self.h, n, m, a, b, _p(out))
+        return complex(out[0], out[1])
 
     def steps(self, nsteps, ifar=10, kb=0):
         k = ctypes.c_int(kb)

@@ -137,6 +137,9 @@ class RefLib:
         self.lib.ylm_.restype = _C2
         self.lib.ylm_.argtypes = [ctypes.POINTER(ctypes.c_int)] * 2 + \
             [ctypes.POINTER(ctypes.c_double)] * 2
+        for fn in ('ibox2_', 'ibox64_'):
+            getattr(self.lib, fn).restype = ctypes.c_int
+            getattr(self.lib, fn).argtypes = [ctypes.POINTER(ctypes.c_double)] * 3
         self.lib.rlegendrep_.restype = ctypes.c_double
         self.lib.rlegendrep_.argtypes = [ctypes.POINTER(ctypes.c_int)] * 2 + \
             [ctypes.POINTER(ctypes.c_double)]
@@ -163,6 +166,14 @@ class RefLib:
         r = self.lib.ylm_(ctypes.byref(ctypes.c_int(n)), ctypes.byref(ctypes.c_int(m)),
                           ctypes.byref(ctypes.c_double(a)), ctypes.byref(ctypes.c_double(b)))
         return complex(r.re, r.im)
+
+    def ibox2(self, x, y, z):
+        d = ctypes.c_double
+        return self.lib.ibox2_(ctypes.byref(d(x)), ctypes.byref(d(y)), ctypes.byref(d(z)))
+
+    def ibox64(self, x, y, z):
+        d = ctypes.c_double
+        return self.lib.ibox64_(ctypes.byref(d(x)), ctypes.byref(d(y)), ctypes.byref(d(z)))
 
     def legendre(self, l, m, x):
         return self.lib.rlegendrep_(ctypes.byref(ctypes.c_int(l)), ctypes.byref(ctypes.c_int(m)),
@@ -273,6 +284,46 @@ def classify(pos, n_water, n_surf, rs):
     return cls, anc
 
 
+def classify_ref(ref, pos, n_water, n_surf, rs):
+    """classify() with the reference's compiled ibox2/ibox64 (src/ibox.f90:127-186)
+    doing every box decision, exactly as read_config calls them."""
+    rsd, rhs = geometry(rs)
+    ref['rsdx'][0] = ref['rsdy'][0] = ref['rsdz'][0] = rsd
+    ref['rhsx'][0] = ref['rhsy'][0] = ref['rhsz'][0] = rhs
+    cls = np.empty(n_water + n_surf, dtype=np.int32)
+    anc = np.empty((n_water + n_surf, 3), dtype=np.int32)
+
+    def set_rank(rk):
+        ref['rcen2xtmp'][0] = rcen2(rk % 3, rs)
+        ref['rcen2ytmp'][0] = rcen2((rk // 3) % 3, rs)
+        ref['rcen2ztmp'][0] = rcen2(rk // 9, rs)
+
+    def glob(rk, b64):
+        rx, ry, rz = rk % 3, (rk // 3) % 3, rk // 9
+        l = b64 - 1
+        return (4 * rx + l % 4, 4 * ry + (l // 4) % 4, 4 * rz + l // 16)
+
+    for w in range(n_water):
+        O, G, H = pos[3 * w], pos[3 * w + 1], pos[3 * w + 2]
+        ibO, ibG, ibH = ref.ibox2(*O), ref.ibox2(*G), ref.ibox2(*H)
+        rk = ibO - 1
+        set_rank(rk)
+        bO = ref.ibox64(*O)
+        w1 = False
+        if ibO == ibG == ibH:
+            w1 = bO == ref.ibox64(*G) == ref.ibox64(*H)
+        cls[w] = 0 if w1 else 1
+        anc[w] = glob(rk, bO)
+    base = 3 * n_water
+    for s_ in range(n_surf):
+        X = pos[base + 29 * s_ + 13]
+        rk = ref.ibox2(*X) - 1
+        set_rank(rk)
+        cls[n_water + s_] = 2
+        anc[n_water + s_] = glob(rk, ref.ibox64(*X))
+    return cls, anc
+
+
 class Emulator:
     """Runs the reference routines over all 27 ranks for one global system."""
 
@@ -302,7 +353,7 @@ class Emulator:
     def reclassify(self):
         """Re-file molecules (what the every-10-steps migration does,
         src/delete_w1.f90:31-120): molecules keep global order within a box."""
-        self.cls, self.anc = classify(self.pos, self.n_water, self.n_surf, self.rs)
+        self.cls, self.anc = classify_ref(self.ref, self.pos, self.n_water, self.n_surf, self.rs)
         self.by_box = [dict(), dict(), dict()]
         for m in range(self.n_water + self.n_surf):
             key = tuple(self.anc[m])
@@ -400,6 +451,32 @@ class Emulator:
         for name in ('buildw2', 'builds', 'buildinternal', 'buildsnnn', 'buildsnn',
                      'buildsn', 'buildw2nn', 'buildw2n'):
             self.ref.call(name, myrank)
+
+    def isub_masks(self):
+        """Per surfactant site, the same-molecule partners buildinternal lists in
+        isub (src/buildinternal.f90:11-63) as a bit mask (bit k: site k+1)."""
+        out = np.zeros(len(self.pos), dtype=np.uint32)
+        R = self.ref
+        for r, rk in enumerate(self.ranks()):
+            self.load_rank(rk)
+            self.ref.call('buildinternal', r)
+            for i in range(64):
+                ib1000 = self._ib(i, 10, 3)
+                n = R['icts'][ib1000 - 1]
+                g = self._glob(rk, i)
+                mols = self.by_box[2].get(tuple(g), [])
+                assert len(mols) == n
+                for k, m in enumerate(mols):
+                    sites = self.mol_sites(m)
+                    for j in range(29):
+                        mask = 0
+                        for q in range(28):
+                            v = R['isub'][q, j, k, i]
+                            if v == 0:
+                                break
+                            mask |= 1 << (int(v) - 1)
+                        out[sites[j]] = mask
+        return out
 
     def load_graphite(self, rk):
         """ictgraphite/xgr selection of src/read_graphite.f90:203-232 (restated:

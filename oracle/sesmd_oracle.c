@@ -109,6 +109,15 @@ static cplx ylm(const orc *o, int n, int m, double a, double b)
     return s * (double)cr + I * (s * (double)ci);
 }
 
+void orc_ylm(void *h, int n, int m, double a, double b, double *out)
+{
+    cplx y = ylm((orc *)h, n, m, a, b);
+    out[0] = creal(y);
+    out[1] = cimag(y);
+}
+
+double orc_legendre(int l, int m, double x) { return legendre(l, m, x); }
+
 /* rAnm, src/ifma.f90:41-52 */
 static double anm(const orc *o, int n, int m)
 {

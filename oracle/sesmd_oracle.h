@@ -45,6 +45,9 @@ void orc_get_state(void *h, double *r, double *v, double *a, double *eta);
 void orc_set_nh(void *h, double tr, double pnh1, double pnh2);
 void orc_set_graphite(void *h, int n, const double *xyz);
 void orc_set_threads(int n);
+/* the expansion primitives Ylm (src/ifma.f90:12-26) and rLegendreP */
+void orc_ylm(void *h, int n, int m, double a, double b, double *out);
+double orc_legendre(int l, int m, double x);
 
 /* every-10-steps work: molecule classes, storage boxes, site boxes, the
  * intramolecular subtraction masks and the far-field coefficients */

@@ -1,0 +1,122 @@
+"""Write the golden fixtures from the reference's own compiled Fortran routines.
+
+Run in a checkout that has the reference tree and the compiled subset
+(`make -C oracle ref`):
+
+    python tests/golden/make_goldens.py
+
+Outputs (numpy .npz, compressed), all values produced by the reference code
+through oracle/refharness.py (the 27-rank COMMON emulation):
+  tables.npz        the COMMON parameter tables init_arrays fills
+  ylm.npz           Ylm / rLegendreP on a grid of arguments
+  small_system.npz  inputs + bootstrap forces per component, per reference
+                    kernel, classes, isub, far-field expansions, one full step
+  medium_total.npz  inputs + total bootstrap force of a larger system
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+sys.path.insert(0, os.path.join(ROOT, 'tests'))
+
+import refharness as rh  # noqa: E402
+from conftest import small_system, medium_system  # noqa: E402
+
+
+def tables(ref):
+    keys = ['emw', 'ems', 'sgw', 'sgs', 'epw', 'eps', 'rqw', 'rqs', 'dsqw', 'dsqs', 'th',
+            'rkth', 'dva', 'dvb', 'dvc', 'sgww6', 'epww', 'sgss6', 'epss', 'sgws6', 'epws',
+            'sgcs6', 'epcs', 'rifact', 'iself', 'jself', 'ibond', 'jbond']
+    np.savez_compressed(os.path.join(HERE, 'tables.npz'),
+                        **{k: np.array(ref[k]) for k in keys})
+
+
+def ylm(ref):
+    rng = np.random.default_rng(11)
+    ang = rng.uniform([0.0, -np.pi], [np.pi, np.pi], size=(12, 2))
+    ang[0] = (0.3, 0.0)
+    ang[1] = (np.pi / 2, np.pi / 3)
+    rows = []
+    vals = []
+    for a, b in ang:
+        for n in range(0, 9):
+            for m in range(-n, n + 1):
+                rows.append((n, m, a, b))
+                vals.append(ref.ylm(n, m, a, b))
+    xs = np.concatenate([np.linspace(-1, 1, 9), rng.uniform(-1, 1, 8)])
+    leg = [(l, m, x, ref.legendre(l, m, x)) for x in xs for l in range(-1, 9) for m in range(0, 9)
+           if not (l == -1 and m > 0)]
+    np.savez_compressed(os.path.join(HERE, 'ylm.npz'), args=np.array(rows), vals=np.array(vals),
+                        leg=np.array(leg))
+
+
+def small(ref):
+    s = small_system()
+    em = rh.Emulator(ref, s)
+    out = dict(n_water=s.n_water, n_surf=s.n_surf, pos=s.pos, vel=s.vel, graphite=s.graphite)
+    out['cls'] = em.cls
+    out['anc'] = em.anc[:, 0] + 12 * (em.anc[:, 1] + 12 * em.anc[:, 2])
+    out['isub'] = em.isub_masks()
+    out['F_near'] = em.forces(far=False, intra=False, wall=False, graphite=False)
+    out['F_far'] = em.forces(near=False, intra=False, wall=False, graphite=False)
+    out['O4'] = em.O4.reshape(-1, rh.LP)
+    L4 = np.zeros((12, 12, 12, rh.LP), dtype=np.complex128)
+    for r, rk in enumerate(em.ranks()):
+        for i in range(64):
+            g = em._glob(rk, i)
+            L4[g[2], g[1], g[0]] = em.cL4[r][:, i]
+    L4 = L4.reshape(-1, rh.LP)
+    occ = np.nonzero(np.abs(out['O4']).max(axis=1))[0].astype(np.int32)
+    out['occ'] = occ  # boxes holding receivers: O4 and L4 are kept there
+    out['O4'] = out['O4'][occ]
+    out['L4'] = L4[occ]
+    out['O3'] = em.O3.reshape(-1, rh.LP)
+    out['O2'] = em.O2
+    out['F_intra'] = em.forces(near=False, far=False, wall=False, graphite=False)
+    out['F_wall'] = em.forces(near=False, far=False, intra=False, graphite=False)
+    out['F_graphite'] = em.forces(near=False, far=False, intra=False, wall=False)
+    out['F'] = em.forces()
+    names = []
+    for k in rh.NEAR_KERNELS:
+        fk = em.forces(far=False, intra=False, wall=False, graphite=False, kernels=[k])
+        names.append(k)
+        rows = np.nonzero(np.abs(fk).max(axis=1))[0].astype(np.int32)
+        out['Ki_' + k] = rows
+        out['Kv_' + k] = fk[rows]
+    out['kernels'] = np.array(names)
+    # one full step: init_accel, part A, rebuild, forces, part B
+    em.init_accel(out['F'])
+    out['acc0'] = em.acc.copy()
+    sums = em.step_a()
+    out['A_pos'], out['A_vel'], out['A_eta'], out['A_sums'] = em.pos.copy(), em.vel.copy(), \
+        em.eta, sums
+    em.reclassify()
+    F1 = em.forces()
+    out['F1'] = F1
+    em.step_b(F1)
+    out['B_vel'], out['B_acc'] = em.vel.copy(), em.acc.copy()
+    np.savez_compressed(os.path.join(HERE, 'small_system.npz'), **out)
+
+
+def medium(ref):
+    s = medium_system()
+    em = rh.Emulator(ref, s)
+    np.savez_compressed(os.path.join(HERE, 'medium_total.npz'), n_water=s.n_water,
+                        n_surf=s.n_surf, pos=s.pos, vel=s.vel, graphite=s.graphite,
+                        F=em.forces())
+
+
+if __name__ == '__main__':
+    ref = rh.RefLib()
+    tables(ref)
+    ylm(ref)
+    small(ref)
+    medium(ref)
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith('.npz'):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
