@@ -139,6 +139,38 @@ int sesmd_near_only(sesmd_ctx *ctx);
  * 3^3 blocks, and those within rc with an LJ term), for roofline accounting */
 int sesmd_pair_counts(sesmd_ctx *ctx, double *pairs, double *lj_pairs);
 
+/* ---- finer-grained calls, used by the Fortran drop-in shim (sesmd_fortran.h)
+ * to run one reference routine at a time ---- */
+
+/* Fix molecule classes (0 W1, 1 W2, 2 S) and storage boxes instead of
+ * classifying in sesmd_rebuild -- for a caller that files molecules itself,
+ * as the reference's delete_w1/add_w1 migration does (src/delete_w1.f90:31-120).
+ * NULL, NULL returns to automatic classification. */
+int sesmd_set_classes(sesmd_ctx *ctx, const int *cls, const int *anc);
+int sesmd_set_eta(sesmd_ctx *ctx, double eta);
+/* load expansion tables (layout of sesmd_get_expansions; NULL leaves a table) */
+int sesmd_set_expansions(sesmd_ctx *ctx, const double *O4, const double *L4, const double *O3,
+                         const double *L3, const double *O2, const double *L2);
+/* one stage of the far-field chain; each matches one reference routine */
+enum {
+    SESMD_P2M = 0,     /* computeOnm4 (src/computeOnm4.f90:22-162)  */
+    SESMD_M2M_43 = 1,  /* computeOnm3 (src/computeOnm3.f90:16-98)   */
+    SESMD_M2M_32 = 2,  /* computeOnm2 (src/computeOnm2.f90:14-63)   */
+    SESMD_M2L_2 = 3,   /* computeLjk2 (src/computeLjk2.f90:17-73)   */
+    SESMD_L2L_23 = 4,  /* shiftLjk2 (src/shiftLjk2.f90:14-41)       */
+    SESMD_M2L_3 = 5,   /* computeLjk3 (src/computeLjk3.f90:45-102)  */
+    SESMD_L2L_34 = 6,  /* shiftLjk3 (src/shiftLjk3.f90:16-70)       */
+    SESMD_M2L_4 = 7    /* computeLjk4 (src/computeLjk4.f90:58-170)  */
+};
+int sesmd_fmm_stage(sesmd_ctx *ctx, int stage);
+/* One integrator half for the molecule classes in cls_mask (bit 0 W1, 1 W2,
+ * 2 S) without the eta update: part 0 = A (move_*_a without the all-reduce),
+ * part 1 = B (move_*_b).  For part A, sums18 (may be NULL) receives the
+ * unweighted sums of |v dt|^2 per site group, before SHAKE [0..8] and after
+ * [9..17]: W1 O, W1 H, W2 O, W2 H, S H, S O, S CH2, S Si, S CH3 (each group
+ * shares one mass; nh_water1/2 and nh_surfactant weight them). */
+int sesmd_move_species(sesmd_ctx *ctx, int part, unsigned cls_mask, double *sums18);
+
 #ifdef __cplusplus
 }
 #endif

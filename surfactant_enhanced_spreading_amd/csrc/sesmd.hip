@@ -134,6 +134,7 @@ struct sesmd_ctx {
     hipGraphExec_t graph;
     bool graph_ok;
     bool built;
+    bool classes_fixed;  // classes/storage boxes set by the caller (sesmd_set_classes)
 };
 
 // ------------------------------------------------------------ host math
@@ -1390,13 +1391,13 @@ __global__ __launch_bounds__(256) void k_move_a_water(Geo g, double *__restrict_
                                                       const sesmd_forcefield *__restrict__ ff,
                                                       const double *__restrict__ nh,
                                                       double *__restrict__ partials,
-                                                      int *__restrict__ counters)
+                                                      int *__restrict__ counters, unsigned clsmask)
 {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[NH_VALS];
     for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
     int n = g.n;
-    if (m < g.nw) {
+    if (m < g.nw && ((clsmask >> cls[m]) & 1u)) {
         double escale = 1.0 - nh[0] * g.dt / 2.0;
         int base = 3 * m;
         int grp = cls[m] == 0 ? 0 : 2;
@@ -1689,7 +1690,7 @@ __global__ __launch_bounds__(SURF_NT) void k_move_b_surf_fast(
 // kinetic sums -> the eta update of src/move_w2_a.f90:145-157 (single block)
 __global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restrict__ partials,
                                                    int nblocks, const sesmd_forcefield *__restrict__ ff,
-                                                   double *__restrict__ nh)
+                                                   double *__restrict__ nh, int update_eta)
 {
     double acc[NH_VALS];
     for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
@@ -1710,8 +1711,9 @@ __global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restri
         double dt2 = g.dt * g.dt;
         double sum1 = (S[0] + S[2] + S[4]) / dt2;
         double sum2 = (S[1] + S[3] + S[5]) / dt2;
-        nh[0] = nh[0] + nh[8] * (sum1 + nh[9] + sum2);
+        if (update_eta) nh[0] = nh[0] + nh[8] * (sum1 + nh[9] + sum2);
         for (int c = 0; c < 6; c++) nh[1 + c] = S[c];
+        for (int c = 0; c < NH_VALS; c++) nh[16 + c] = tot[c];
     }
 }
 
@@ -1722,10 +1724,11 @@ __global__ __launch_bounds__(256) void k_move_b_water(Geo g, const double *__res
                                                       const double *__restrict__ f,
                                                       const sesmd_forcefield *__restrict__ ff,
                                                       const double *__restrict__ nh,
-                                                      int *__restrict__ counters)
+                                                      int *__restrict__ counters,
+                                                      const int *__restrict__ cls, unsigned clsmask)
 {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= g.nw) return;
+    if (m >= g.nw || !((clsmask >> cls[m]) & 1u)) return;
     int n = g.n, base = 3 * m;
     double escale = 2.0 / (2.0 + nh[0] * g.dt);
     double rx[3][3], vx[3][3], rm[3];
@@ -2061,7 +2064,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     c->nblk_w = g.nw > 0 ? (int)nblk(g.nw, 256) : 0;
     c->nblk_s = g.ns > 0 ? (int)nblk(g.ns, c->surf_fast ? SURF_NT : 64) : 0;
     A(dalloc(&c->partials, (size_t)(c->nblk_w + c->nblk_s + 1) * NH_VALS));
-    A(dalloc(&c->nh, 16));
+    A(dalloc(&c->nh, 16 + NH_VALS));
     A(dalloc(&c->counters, 4));
     if (rc) {
         std::string e = g_err;
@@ -2232,7 +2235,8 @@ static int launch_rebuild(sesmd_ctx *c)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
-    k_classify<<<nblk(g.nmol, 256), 256, 0, st>>>(g, c->r, c->first, c->cls, c->anc);
+    if (!c->classes_fixed)
+        k_classify<<<nblk(g.nmol, 256), 256, 0, st>>>(g, c->r, c->first, c->cls, c->anc);
     k_site_box<<<nblk(g.n, 256), 256, 0, st>>>(g, c->r, c->mol, c->type, c->cls, c->anc, c->cbox,
                                                c->rkind, c->key_in, c->val_in);
     if (g.ns) k_isub<<<nblk(29 * g.ns, 128), 128, 0, st>>>(g, c->r, c->anc, c->isub);
@@ -2347,20 +2351,23 @@ extern "C" int sesmd_set_forces(sesmd_ctx *c, const double *f)
     return up_aos(c, c->f, f);
 }
 
-static int launch_move_a(sesmd_ctx *c)
+static int launch_move_a(sesmd_ctx *c, unsigned clsmask = 7u, int update_eta = 1)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
     if (c->nblk_w)
         k_move_a_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->cls, c->dff, c->nh,
-                                                   c->partials, c->counters);
-    if (c->nblk_s && c->surf_fast)
+                                                   c->partials, c->counters, clsmask);
+    if (c->nblk_s && (clsmask & 4u) && c->surf_fast)
         k_move_a_surf_fast<<<c->nblk_s, SURF_NT, 0, st>>>(g, c->r, c->v, c->a, c->f, c->sconst, c->nh,
                                                          c->partials, c->counters);
-    else if (c->nblk_s)
+    else if (c->nblk_s && (clsmask & 4u))
         k_move_a_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->partials,
                                                 c->counters);
-    k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh);
+    else if (c->nblk_s)
+        k_fill_d<<<nblk((size_t)c->nblk_s * NH_VALS, 256), 256, 0, st>>>(
+            c->partials + (size_t)c->nblk_w * NH_VALS, (size_t)c->nblk_s * NH_VALS, 0.0);
+    k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, update_eta);
     HC(hipGetLastError());
     return 0;
 }
@@ -2375,12 +2382,17 @@ extern "C" int sesmd_move_a(sesmd_ctx *c, double *sums6)
     return 0;
 }
 
-static int launch_move_b(sesmd_ctx *c)
+static int launch_move_b(sesmd_ctx *c, unsigned clsmask = 7u)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
     if (c->nblk_w)
-        k_move_b_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->counters);
+        k_move_b_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->counters,
+                                                   c->cls, clsmask);
+    if (!(clsmask & 4u)) {
+        HC(hipGetLastError());
+        return 0;
+    }
     if (c->nblk_s && c->surf_fast)
         k_move_b_surf_fast<<<c->nblk_s, SURF_NT, 0, st>>>(g, c->r, c->v, c->a, c->f, c->sconst, c->nh,
                                                          c->counters);
@@ -2514,5 +2526,108 @@ extern "C" int sesmd_pair_counts(sesmd_ctx *c, double *pairs, double *lj_pairs)
     HC(hipFree(cnt));
     if (pairs) *pairs = 0.5 * (double)h[0];
     if (lj_pairs) *lj_pairs = 0.5 * (double)h[1];
+    return 0;
+}
+
+// ------------------------------------------------------------ finer-grained API
+
+extern "C" int sesmd_set_classes(sesmd_ctx *c, const int *cls, const int *anc)
+{
+    if (!c) return fail("null context");
+    const Geo &g = c->g;
+    if (!cls || !anc) {
+        c->classes_fixed = false;
+        return 0;
+    }
+    for (int m = 0; m < g.nmol; m++) {
+        if (anc[m] < 0 || anc[m] >= g.nb3) return fail("storage box of molecule %d out of range", m);
+        int want = m < g.nw ? (cls[m] == 0 || cls[m] == 1) : (cls[m] == 2);
+        if (!want) return fail("class %d of molecule %d does not fit its kind", cls[m], m);
+    }
+    HC(hipMemcpy(c->cls, cls, sizeof(int) * g.nmol, hipMemcpyHostToDevice));
+    HC(hipMemcpy(c->anc, anc, sizeof(int) * g.nmol, hipMemcpyHostToDevice));
+    c->classes_fixed = true;
+    return 0;
+}
+
+extern "C" int sesmd_set_eta(sesmd_ctx *c, double eta)
+{
+    if (!c) return fail("null context");
+    HC(hipMemcpy(c->nh, &eta, sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+__global__ void k_nonempty(const double2 *__restrict__ O, int nbox, int lp, int *__restrict__ ne)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    int any = 0;
+    for (int q = 0; q < lp; q++) any |= (O[(size_t)b * lp + q].x != 0.0 || O[(size_t)b * lp + q].y != 0.0);
+    ne[b] = any;
+}
+
+extern "C" int sesmd_set_expansions(sesmd_ctx *c, const double *O4, const double *L4,
+                                    const double *O3, const double *L3, const double *O2,
+                                    const double *L2)
+{
+    if (!c) return fail("null context");
+    const Geo &g = c->g;
+    size_t s4 = sizeof(double2) * g.nb3 * g.lp, s3 = sizeof(double2) * g.n33 * g.lp,
+           s2 = sizeof(double2) * g.n2 * g.lp;
+    if (O4) HC(hipMemcpy(c->O4, O4, s4, hipMemcpyHostToDevice));
+    if (L4) HC(hipMemcpy(c->L4, L4, s4, hipMemcpyHostToDevice));
+    if (O3) HC(hipMemcpy(c->O3, O3, s3, hipMemcpyHostToDevice));
+    if (L3) HC(hipMemcpy(c->L3, L3, s3, hipMemcpyHostToDevice));
+    if (O2) HC(hipMemcpy(c->O2, O2, s2, hipMemcpyHostToDevice));
+    if (L2) HC(hipMemcpy(c->L2, L2, s2, hipMemcpyHostToDevice));
+    if (O4) k_nonempty<<<nblk(g.nb3, 128), 128, 0, c->stream>>>(c->O4, g.nb3, g.lp, c->ne4);
+    if (O3) k_nonempty<<<nblk(g.n33, 128), 128, 0, c->stream>>>(c->O3, g.n33, g.lp, c->ne3);
+    if (O2) k_nonempty<<<nblk(g.n2, 128), 128, 0, c->stream>>>(c->O2, g.n2, g.lp, c->ne2);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int sesmd_fmm_stage(sesmd_ctx *c, int stage)
+{
+    if (!c) return fail("null context");
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
+    switch (stage) {
+    case SESMD_P2M:
+        if (!c->built) return fail("P2M before sesmd_rebuild");
+        if (g.p == 4)
+            k_p2m<4><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4,
+                                           c->ne4);
+        else
+            k_p2m<8><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4,
+                                           c->ne4);
+        break;
+    case SESMD_M2M_43: k_m2m<<<g.n33, 64, 0, st>>>(g, g.n3, c->O4, c->ne4, c->M43, c->O3, c->ne3); break;
+    case SESMD_M2M_32: k_m2m<<<g.n2, 64, 0, st>>>(g, g.nb2, c->O3, c->ne3, c->M32, c->O2, c->ne2); break;
+    case SESMD_M2L_2: k_m2l_top<<<g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->L2); break;
+    case SESMD_L2L_23: k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3); break;
+    case SESMD_M2L_3: k_m2l<<<g.n33, 256, 0, st>>>(g, g.n3, c->O3, c->ne3, c->T3, c->L3); break;
+    case SESMD_L2L_34: k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4); break;
+    case SESMD_M2L_4: k_m2l<<<g.nb3, 256, 0, st>>>(g, g.nb, c->O4, c->ne4, c->T4, c->L4); break;
+    default: return fail("unknown far-field stage %d", stage);
+    }
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int sesmd_move_species(sesmd_ctx *c, int part, unsigned cls_mask, double *sums18)
+{
+    if (!c) return fail("null context");
+    if (!c->built && !c->classes_fixed) return fail("molecule classes unknown: rebuild or set_classes");
+    if (part == 0) {
+        CK(launch_move_a(c, cls_mask, 0));
+        if (sums18) HC(hipMemcpyAsync(sums18, c->nh + 16, NH_VALS * sizeof(double), hipMemcpyDeviceToHost,
+                                      c->stream));
+    } else {
+        CK(launch_move_b(c, cls_mask));
+    }
+    HC(hipStreamSynchronize(c->stream));
     return 0;
 }
