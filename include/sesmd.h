@@ -170,6 +170,10 @@ int sesmd_fmm_stage(sesmd_ctx *ctx, int stage);
  * [9..17]: W1 O, W1 H, W2 O, W2 H, S H, S O, S CH2, S Si, S CH3 (each group
  * shares one mass; nh_water1/2 and nh_surfactant weight them). */
 int sesmd_move_species(sesmd_ctx *ctx, int part, unsigned cls_mask, double *sums18);
+/* The nine group sums of |v dt|^2 (order of sums18[0..8]) at the current
+ * velocities: what nh_water1, nh_water2 and nh_surfactant weight by mass
+ * (src/nh_water1.f90:1-43, src/nh_surfactant.f90:1-201). */
+int sesmd_kinetic_sums(sesmd_ctx *ctx, double *sums9);
 
 #ifdef __cplusplus
 }

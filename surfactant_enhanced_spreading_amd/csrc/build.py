@@ -1,4 +1,5 @@
-"""Build libsesmd.so (the HIP engine) in-tree for gfx950.
+"""Build libsesmd.so (the HIP engine) and libsesmd_fortran.so (the routines
+under the reference's Fortran names, over libsesmd) in-tree for gfx950.
 
     python -m surfactant_enhanced_spreading_amd.csrc.build
 """
@@ -10,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 OUT = os.path.join(PKG, 'libsesmd.so')
+OUT_FORTRAN = os.path.join(PKG, 'libsesmd_fortran.so')
 ARCH = os.environ.get('SESMD_ARCH', 'gfx950')
 
 # -packed-fp32-ops off: no v_pk_*_f32 generated from scalar/float2 code.
@@ -33,7 +35,25 @@ def build(verbose=False):
     if verbose and err.strip():
         print(err)
     os.replace(tmp, OUT)
+    build_fortran(verbose)
     return OUT
+
+
+def build_fortran(verbose=False):
+    """The Fortran-named library: host C++ only, linked against libsesmd.so."""
+    from surfactant_enhanced_spreading_amd.csrc import gen_ff_header
+    gen_ff_header.write()
+    tmp = OUT_FORTRAN + '.tmp'
+    cmd = [os.environ.get('CXX', 'g++'), '-O2', '-std=c++17', '-fPIC', '-shared', '-Wall',
+           '-I' + os.path.join(ROOT, 'include'), '-I' + HERE, os.path.join(HERE, 'dropin.cpp'),
+           '-L' + PKG, '-lsesmd', '-Wl,-rpath,$ORIGIN', '-o', tmp]
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError('g++ failed:\n' + r.stderr)
+    os.replace(tmp, OUT_FORTRAN)
+    return OUT_FORTRAN
 
 
 if __name__ == '__main__':

@@ -2631,3 +2631,57 @@ extern "C" int sesmd_move_species(sesmd_ctx *c, int part, unsigned cls_mask, dou
     HC(hipStreamSynchronize(c->stream));
     return 0;
 }
+
+// Unweighted |v dt|^2 per thermostat site group (nh_water1, nh_water2,
+// nh_surfactant: src/nh_water1.f90:1-43, src/nh_surfactant.f90:1-201), one
+// thread per molecule; groups as in the part-A kernels' first nine sums.
+__global__ __launch_bounds__(256) void k_kinetic(Geo g, const double *__restrict__ v,
+                                                 const int *__restrict__ cls,
+                                                 double *__restrict__ partials)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[NH_VALS];
+    for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
+    int n = g.n;
+    if (m < g.nw) {
+        int grp = cls[m] == 0 ? 0 : 2;
+        for (int i = 0; i < 3; i++) {
+            int s = 3 * m + i;
+            acc[grp + (i == 0 ? 0 : 1)] += v[s] * v[s] + v[n + s] * v[n + s] + v[2 * n + s] * v[2 * n + s];
+        }
+    } else if (m < g.nmol) {
+        int base = 3 * g.nw + 29 * (m - g.nw);
+        for (int i = 0; i < 29; i++) {
+            int s = base + i;
+            acc[4 + c_sgrp[i]] += v[s] * v[s] + v[n + s] * v[n + s] + v[2 * n + s] * v[2 * n + s];
+        }
+    }
+    block_sum_store<NH_VALS>(acc, partials + (size_t)blockIdx.x * NH_VALS);
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partials, int nblocks,
+                                                      double *__restrict__ out)
+{
+    double acc[NH_VALS];
+    for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int c = 0; c < NH_VALS; c++) acc[c] += partials[(size_t)b * NH_VALS + c];
+    block_sum_store<NH_VALS>(acc, out);
+}
+
+extern "C" int sesmd_kinetic_sums(sesmd_ctx *c, double *sums9)
+{
+    if (!c) return fail("null context");
+    if (!c->built && !c->classes_fixed) return fail("molecule classes unknown: rebuild or set_classes");
+    const Geo &g = c->g;
+    int nb = nblk(g.nmol, 256);
+    if (nb > c->nblk_w + c->nblk_s + 1) return fail("kinetic sums: partial buffer too small");
+    k_kinetic<<<nb, 256, 0, c->stream>>>(g, c->v, c->cls, c->partials);
+    k_sum_partials<<<1, 256, 0, c->stream>>>(c->partials, nb, c->nh + 16);
+    HC(hipGetLastError());
+    double t[NH_VALS];
+    HC(hipMemcpyAsync(t, c->nh + 16, sizeof t, hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    if (sums9) memcpy(sums9, t, 9 * sizeof(double));
+    return 0;
+}
