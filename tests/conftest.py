@@ -62,3 +62,14 @@ def sys_small():
 @pytest.fixture(scope='session')
 def sys_medium():
     return medium_system()
+
+
+@pytest.fixture(autouse=True, scope='session')
+def _scratch_cwd(tmp_path_factory):
+    """The reference's integrator routines write ERROR.txt into the working
+    directory when SHAKE/RATTLE does not converge (src/move_w1_a.f90:116);
+    run the session from a scratch directory so that stays out of the tree."""
+    old = os.getcwd()
+    os.chdir(str(tmp_path_factory.mktemp('cwd')))
+    yield
+    os.chdir(old)
