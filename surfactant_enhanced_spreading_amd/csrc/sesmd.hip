@@ -283,12 +283,11 @@ static double rcen2(const Geo &g, int ix)
 
 __device__ __forceinline__ double rsqrt64(double x)
 {
-    // v_rsq_f64 estimate refined by two Newton steps (~1 ulp)
+    // v_rsq_f64 (relative error below 2^-22) refined by one Newton step,
+    // y (3 - x y^2) / 2: the error squares, to ~1e-14 relative
     double y = __builtin_amdgcn_rsq(x);
     double hx = 0.5 * x;
-    y = y * fma(-hx * y, y, 1.5);
-    y = y * fma(-hx * y, y, 1.5);
-    return y;
+    return y * fma(-hx * y, y, 1.5);
 }
 
 __device__ __forceinline__ int iabs_d(int a) { return a < 0 ? -a : a; }
@@ -513,9 +512,10 @@ __global__ void k_worklist(Geo g, const int *__restrict__ bstart, int2 *__restri
 // its own box (its storage ring stays within the class's list rings: W1 <= 1,
 // W2 <= 2, S <= 3), which spares the near-field loop the per-box ring test.
 __global__ void k_gather_meta(Geo g, const int *__restrict__ perm, const int *__restrict__ type,
-                              const int *__restrict__ mol, const int *__restrict__ cls,
-                              const int *__restrict__ anc, const int *__restrict__ cbox,
-                              int4 *__restrict__ smeta, int *__restrict__ bflag)
+                              const int *__restrict__ mol, const int *__restrict__ first,
+                              const int *__restrict__ cls, const int *__restrict__ anc,
+                              const int *__restrict__ cbox, int4 *__restrict__ smeta,
+                              int *__restrict__ bflag)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= g.n) return;
@@ -526,7 +526,13 @@ __global__ void k_gather_meta(Geo g, const int *__restrict__ perm, const int *__
     int ax = a % nb, ay = (a / nb) % nb, az = a / (nb * nb);
     int maxring = c == 0 ? 1 : (c == 1 ? 2 : 3);
     int always = cheb(a, cbox[i], nb) + 1 <= maxring ? 1 : 0;
-    smeta[k] = make_int4(i, type[i] | (c << 8), m, ax | (ay << 8) | (az << 16) | (always << 24));
+    // split: the molecule has sites in more than one box, so a receiver of
+    // another box can belong to it (a W1 molecule never does)
+    int split = 0;
+    if (c != 0)
+        for (int s = first[m]; s < first[m + 1]; s++) split |= cbox[s] != cbox[i];
+    smeta[k] = make_int4(i, type[i] | (c << 8) | (split << 10), m,
+                         ax | (ay << 8) | (az << 16) | (always << 24));
     if (!always) atomicOr(&bflag[cbox[i]], 1);
 }
 
@@ -555,6 +561,7 @@ constexpr int NEAR_ROWS = 9;  // (dz, dy) rows of the 3^3 neighbourhood, one wav
 
 struct Recv {
     double x, y, z, q;       // receiver position, charge * xpi
+    double qO, qH;           // q * (water O charge), q * (water H charge)
     double cO, s6O;          // LJ against water oxygen
     const double *ljc, *ljs; // LJ row of the receiver's type
     int mol, rk;
@@ -565,7 +572,10 @@ struct Recv {
 // from the type table), 2: water H (Coulomb only).  CHECK: test the storage
 // ring of every source (only rows holding a site that some neighbour box does
 // not list); MASKED: keep the (receiver kind, source kind) pairs of `mask`.
-template <int CAT, bool CHECK, bool MASKED>
+// CEN: the segment lies in the receivers' own box (every source may share a
+// molecule with a receiver); elsewhere only sources of split molecules can.
+// HQ: both water hydrogens carry one charge, so q_i q_j is per lane for CAT 2.
+template <int CAT, bool CHECK, bool MASKED, bool CEN, bool HQ>
 __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__restrict__ spos,
                                              const int4 *__restrict__ smeta, int s0, int s1,
                                              int bx, int by, int bz, const Recv &R,
@@ -597,7 +607,8 @@ __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__rest
         double r2 = dx * dx + dy * dy + dz * dz;
         double rinv = rsqrt64(r2);
         double rinv2 = rinv * rinv;
-        double ft = R.q * pj.w * (rinv2 * rinv);
+        double qq = CAT == 0 ? R.qO : (CAT == 2 && HQ) ? R.qH : R.q * pj.w;
+        double ft = qq * (rinv2 * rinv);
         if (CAT != 2) {
             double c, s6;
             if (CAT == 0) {
@@ -612,32 +623,50 @@ __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__rest
             double flj = c * r6 * (s6 * r6 - 0.5) * rinv2;
             ft = (c != 0.0 && r2 <= rc2) ? ft + flj : ft;
         }
-        ft = ok ? ft : 0.0;
+        if (CHECK || MASKED || CEN)
+            ft = ok ? ft : 0.0;
+        else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
+            __asm__ volatile("");
+            ft = mj.z != R.mol ? ft : 0.0;
+        }
         ax = fma(dx, ft, ax);
         ay = fma(dy, ft, ay);
         az = fma(dz, ft, az);
     }
 }
 
-template <bool CHECK, bool MASKED>
-__device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict__ spos,
+template <bool CHECK, bool MASKED, bool CEN, bool HQ>
+__device__ __forceinline__ void near_box(const Geo &g, const double4 *__restrict__ spos,
                                          const int4 *__restrict__ smeta, const int *__restrict__ bs4,
-                                         int r0, int r1, int bx, int by, int bz, const Recv &R,
+                                         int bb, int bx, int by, int bz, const Recv &R,
                                          unsigned long long mask, double &ax, double &ay, double &az)
 {
-    // boxes r0..r1 (consecutive indices); per box the classes O, S, H
+    // one source box: the classes O, S, H
+    int k0 = __builtin_amdgcn_readfirstlane(bs4[4 * bb]);
+    int k1 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 1]);
+    int k2 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 2]);
+    int k3 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 3]);
+    near_segment<0, CHECK, MASKED, CEN, HQ>(g, spos, smeta, k0, k1, bx, by, bz, R, mask, ax, ay, az);
+    near_segment<1, CHECK, MASKED, CEN, HQ>(g, spos, smeta, k1, k2, bx, by, bz, R, mask, ax, ay, az);
+    near_segment<2, CHECK, MASKED, CEN, HQ>(g, spos, smeta, k2, k3, bx, by, bz, R, mask, ax, ay, az);
+}
+
+template <bool CHECK, bool MASKED, bool HQ>
+__device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict__ spos,
+                                         const int4 *__restrict__ smeta, const int *__restrict__ bs4,
+                                         int r0, int r1, int bc, int bx, int by, int bz, const Recv &R,
+                                         unsigned long long mask, double &ax, double &ay, double &az)
+{
+    // boxes r0..r1 (consecutive indices); bc: the receivers' own box
     for (int bb = r0; bb <= r1; bb++) {
-        int k0 = __builtin_amdgcn_readfirstlane(bs4[4 * bb]);
-        int k1 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 1]);
-        int k2 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 2]);
-        int k3 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 3]);
-        near_segment<0, CHECK, MASKED>(g, spos, smeta, k0, k1, bx, by, bz, R, mask, ax, ay, az);
-        near_segment<1, CHECK, MASKED>(g, spos, smeta, k1, k2, bx, by, bz, R, mask, ax, ay, az);
-        near_segment<2, CHECK, MASKED>(g, spos, smeta, k2, k3, bx, by, bz, R, mask, ax, ay, az);
+        if (bb == bc)
+            near_box<CHECK, MASKED, true, HQ>(g, spos, smeta, bs4, bb, bx, by, bz, R, mask, ax, ay, az);
+        else
+            near_box<CHECK, MASKED, false, HQ>(g, spos, smeta, bs4, bb, bx, by, bz, R, mask, ax, ay, az);
     }
 }
 
-template <bool MASKED>
+template <bool MASKED, bool HQ>
 __global__ __launch_bounds__(NEAR_TILE * NEAR_ROWS) void k_near(
     Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
     const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
@@ -679,6 +708,8 @@ __global__ __launch_bounds__(NEAR_TILE * NEAR_ROWS) void k_near(
         R.rk = rkind[site];
         if (R.rk < 0) active = false;
     }
+    R.qO = R.q * ff->q[0];
+    R.qH = R.q * ff->q[1];
     R.cO = ff->lj_c[ti * 32];
     R.s6O = ff->lj_s6[ti * 32];
     R.ljc = ff->lj_c + ti * 32;
@@ -693,11 +724,11 @@ __global__ __launch_bounds__(NEAR_TILE * NEAR_ROWS) void k_near(
         if (!MASKED)
             for (int bb = r0; bb <= r1; bb++) check = check || bflag[bb];
         if (MASKED)
-            near_row<true, true>(g, spos, smeta, bs4, r0, r1, bx, by, bz, R, mask, ax, ay, az);
+            near_row<true, true, HQ>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, mask, ax, ay, az);
         else if (check)
-            near_row<true, false>(g, spos, smeta, bs4, r0, r1, bx, by, bz, R, mask, ax, ay, az);
+            near_row<true, false, HQ>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, mask, ax, ay, az);
         else
-            near_row<false, false>(g, spos, smeta, bs4, r0, r1, bx, by, bz, R, mask, ax, ay, az);
+            near_row<false, false, HQ>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, mask, ax, ay, az);
     }
     s_part[row][0][lane] = ax;
     s_part[row][1][lane] = ay;
@@ -2248,7 +2279,7 @@ static int launch_rebuild(sesmd_ctx *c)
     k_bstart<<<nblk(4 * g.nb3 + 1, 256), 256, 0, st>>>(g, c->key_out, c->bs4, c->bstart);
     k_fill_i<<<nblk(g.nb3, 256), 256, 0, st>>>(c->bflag, g.nb3, 0);
     k_worklist<<<1, 1024, 0, st>>>(g, c->bstart, c->work, c->nwork, c->max_work);
-    k_gather_meta<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->type, c->mol, c->cls, c->anc,
+    k_gather_meta<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->type, c->mol, c->first, c->cls, c->anc,
                                                   c->cbox, c->smeta, c->bflag);
     HC(hipGetLastError());
     c->built = true;
@@ -2291,6 +2322,20 @@ extern "C" int sesmd_fmm(sesmd_ctx *c)
     return 0;
 }
 
+// The near-field kernel for `mask` (~0: all 39 reference kernels).
+static void launch_near(sesmd_ctx *c, uint64_t mask)
+{
+    const Geo &g = c->g;
+    bool hq = c->ff.q[1] == c->ff.q[2];
+    dim3 grid(c->max_work), block(NEAR_TILE * NEAR_ROWS);
+#define NEAR_ARGS g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind, c->work, c->nwork, c->dff, mask, c->f
+    if (mask == ~0ull && hq) k_near<false, true><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+    else if (mask == ~0ull) k_near<false, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+    else if (hq) k_near<true, true><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+    else k_near<true, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+#undef NEAR_ARGS
+}
+
 static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask)
 {
     const Geo &g = c->g;
@@ -2299,14 +2344,7 @@ static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask)
     k_fill_d<<<nblk(n3, 256), 256, 0, st>>>(c->f, n3, 0.0);
     if (what & (SESMD_NEAR | SESMD_FAR | SESMD_GRAPHITE))
         k_gather_pos<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->r, c->type, c->dff, c->spos);
-    if (what & SESMD_NEAR) {
-        if (mask == ~0ull)
-            k_near<false><<<c->max_work, NEAR_TILE * NEAR_ROWS, 0, st>>>(g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind,
-                                                           c->work, c->nwork, c->dff, mask, c->f);
-        else
-            k_near<true><<<c->max_work, NEAR_TILE * NEAR_ROWS, 0, st>>>(g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind,
-                                                          c->work, c->nwork, c->dff, mask, c->f);
-    }
+    if (what & SESMD_NEAR) launch_near(c, mask);
     if (what & (SESMD_FAR | SESMD_WALL))
         (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<nblk(g.n, 128), 128, 0, st>>>(g, what, c->r, c->type, c->cbox, c->rkind, c->L4,
                                                      c->lfac, c->dff, c->f);
@@ -2333,8 +2371,7 @@ extern "C" int sesmd_near_only(sesmd_ctx *c)
 {
     if (!c || !c->built) return fail("not built");
     const Geo &g = c->g;
-    k_near<false><<<c->max_work, NEAR_TILE * NEAR_ROWS, 0, c->stream>>>(g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind,
-                                                           c->work, c->nwork, c->dff, ~0ull, c->f);
+    launch_near(c, ~0ull);
     HC(hipGetLastError());
     return 0;
 }
