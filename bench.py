@@ -6,7 +6,10 @@ M(D'E4OH)M surfactants (65,800 sites) in the reference's geometry (144 A cube,
 12^3 L4 boxes of 12 A, p = 4, far field and lists refreshed every 10 steps,
 2 fs, Nose-Hoover at 298 K).  A step is the reference's full step
 (src/mybox.f90:130-227); the timed region covers K consecutive steps, rebuild
-steps included, with all inputs resident in HBM.
+steps included, with all inputs resident in HBM.  `value` is the FP64 path
+(the drop-in's precision); the "mixed" object reports the mixed-precision mode
+(FP32 pair arithmetic) on the same workload, its speedup and its RMS force
+error against FP64.
 
 Prints one JSON line.  The library and torch ship different HSA runtimes, so
 timing uses HIP events of the library's own runtime (/opt/rocm), not torch.
@@ -23,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MD steps/s (ns/day) 20k water+surfactant: mixed vs FP64 GPU at RMS F err≤1e-5"
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
@@ -82,44 +86,24 @@ def cpu_baseline(sys_, seconds=20.0):
                        '65,800-site system, C restatement with OpenMP' % (n, n - 1))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=500)
-    ap.add_argument('--warmup', type=int, default=30)
-    ap.add_argument('--precision', choices=['fp64', 'mixed'], default='fp64')
-    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
-    ap.add_argument('--cpu-seconds', type=float, default=20.0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print('bench.py measures one GPU: this build runs the whole system on one MI355X '
-              'and has no multi-GPU path (see DESIGN.md, "How it would shard")', file=sys.stderr)
-        sys.exit(2)
-
-    from surfactant_enhanced_spreading_amd import builder, engine, forcefield as ff
-    sys_ = builder.bench_system()
-    prec = engine.FP64 if args.precision == 'fp64' else engine.MIXED
+def run(sys_, prec, steps, warmup, timer):
+    """Bootstrap, warm up, then time `steps` full steps and the near-field
+    kernel alone; returns the figures and the engine (still open)."""
+    from surfactant_enhanced_spreading_amd import engine
     e = engine.Engine(sys_, precision=prec)
     e.bootstrap()
-    pairs, lj_pairs = e.pair_counts()
-    timer = HipTimer()
     stream = e.stream()
-    kb = 0
-    kb = e.steps(args.warmup, 10, kb)
+    kb = e.steps(warmup, 10, 0)
     timer.sync()
     e0, e1 = timer.event(), timer.event()
     t0 = time.perf_counter()
     timer.record(e0, stream)
-    kb = e.steps(args.steps, 10, kb)
+    kb = e.steps(steps, 10, kb)
     timer.record(e1, stream)
     ms = timer.elapsed_ms(e0, e1)
     timer.sync()
     wall = time.perf_counter() - t0
-    ms_step = ms / args.steps
-    steps_s = 1000.0 / ms_step
-    ns_day = steps_s * 2.0 * 0.0864  # dt = 2 fs (src/mybox.cmns:43)
-
-    # dominant kernel: the near field, timed alone on the same stream
+    # dominant kernel: the near field, launched alone on the same stream
     e.forces(fetch=False)
     reps = 50
     timer.sync()
@@ -128,25 +112,77 @@ def main():
         e.near_only()
     timer.record(e1, stream)
     near_ms = timer.elapsed_ms(e0, e1) / reps
+    timer.sync()
+    ms_step = ms / steps
+    return dict(ms_step=ms_step, steps_s=1000.0 / ms_step, wall=wall, near_ms=near_ms), e
+
+
+def rms_force_error(sys_):
+    """||F_mixed - F_fp64|| / ||F_fp64|| over all sites and components, total
+    force at the start-up configuration."""
+    import numpy as np
+    from surfactant_enhanced_spreading_amd import engine
+    f = []
+    for prec in (engine.FP64, engine.MIXED):
+        e = engine.Engine(sys_, precision=prec)
+        e.rebuild()
+        e.fmm()
+        f.append(e.forces())
+        e.close()
+    return float(np.sqrt(((f[1] - f[0]) ** 2).sum() / (f[0] ** 2).sum()))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=500)
+    ap.add_argument('--warmup', type=int, default=30)
+    ap.add_argument('--no-mixed', action='store_true', help='skip the mixed-precision leg')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build runs the whole system on one MI355X '
+              'and has no multi-GPU path (see DESIGN.md, "How it would shard")', file=sys.stderr)
+        sys.exit(2)
+
+    from surfactant_enhanced_spreading_amd import builder, engine
+    sys_ = builder.bench_system()
+    timer = HipTimer()
+    r64, e = run(sys_, engine.FP64, args.steps, args.warmup, timer)
+    pairs, lj_pairs = e.pair_counts()
+    shake_failures, temperature = e.shake_failures(), e.temperature()
+    e.close()
+    steps_s = r64['steps_s']
     flop = 21.0 * pairs + 9.0 * lj_pairs
-    achieved = flop / (near_ms * 1e-3) / 1e12
+    achieved = flop / (r64['near_ms'] * 1e-3) / 1e12
     roofline = dict(bound='fp64-valu', achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS,
                     unit='TFLOP/s', frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=None,
-                    kernel='k_near', kernel_ms=round(near_ms, 4),
+                    kernel='k_near', kernel_ms=round(r64['near_ms'], 4),
                     pairs=pairs, lj_pairs=lj_pairs)
 
     out = dict(metric=METRIC, value=round(steps_s, 2), unit='steps/s', n_gpus=1,
-               steps=args.steps, warmup=args.warmup, ms_per_step=round(ms_step, 4),
-               higher_is_better=True, scaling='weak', vs_baseline=None,
-               dtype='f64' if prec == engine.FP64 else 'f32+f64',
+               steps=args.steps, warmup=args.warmup, ms_per_step=round(r64['ms_step'], 4),
+               higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f64',
                data='synthetic',
                config=dict(workload='20k SPC/E water + 200 M(D\'E4OH)M surfactant droplet, '
                                     'reference geometry (12^3 L4 boxes, p=4, ifar=10, 2 fs)',
                            n_water=sys_.n_water, n_surf=sys_.n_surf, sites=sys_.n_sites,
-                           boxes=12 ** 3, p=4, ifar=10, dt_fs=2.0, precision=args.precision),
-               ns_per_day=round(ns_day, 2), wall_s=round(wall, 3),
-               shake_failures=e.shake_failures(), temperature=round(e.temperature(), 4),
-               roofline=roofline)
+                           boxes=12 ** 3, p=4, ifar=10, dt_fs=2.0, precision='fp64'),
+               ns_per_day=round(steps_s * 2.0 * 0.0864, 2),  # dt = 2 fs (src/mybox.cmns:43)
+               wall_s=round(r64['wall'], 3), shake_failures=shake_failures,
+               temperature=round(temperature, 4), roofline=roofline)
+    if not args.no_mixed:
+        rm, e = run(sys_, engine.MIXED, args.steps, args.warmup, timer)
+        e.close()
+        ach = flop / (rm['near_ms'] * 1e-3) / 1e12
+        out['mixed'] = dict(steps_per_s=round(rm['steps_s'], 2), ms_per_step=round(rm['ms_step'], 4),
+                            ns_per_day=round(rm['steps_s'] * 2.0 * 0.0864, 2),
+                            speedup_vs_fp64=round(rm['steps_s'] / steps_s, 3),
+                            rms_force_err=rms_force_error(sys_), dtype='f32 pair / f64',
+                            near_kernel_ms=round(rm['near_ms'], 4),
+                            near_achieved_tflops=round(ach, 3),
+                            near_frac_fp32_peak=round(ach / FP32_PEAK_TFLOPS, 4))
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(sys_, args.cpu_seconds)
     print(json.dumps(out), flush=True)

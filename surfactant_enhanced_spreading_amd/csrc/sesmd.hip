@@ -109,6 +109,8 @@ struct sesmd_ctx {
     int max_work;
     double4 *spos;
     int4 *smeta;
+    float4 *spos32;  // mixed precision: position relative to its box centre, charge
+    float *lj32;     // mixed precision: lj_c (1024) then lj_s6 (1024) as float
     // far field
     double2 *O4, *L4, *O3, *L3, *O2, *L2;
     int *ne4, *ne3, *ne2;         // nonempty flags per box
@@ -547,6 +549,20 @@ __global__ void k_gather_pos(Geo g, const int *__restrict__ perm, const double *
     spos[k] = make_double4(r[i], r[g.n + i], r[2 * g.n + i], ff->q[type[i]]);
 }
 
+// Mixed precision: each sorted site's position relative to the centre of
+// its own L4 box, in FP32 (|x - c| < rs: an absolute rounding of ~1e-7 sigma),
+// and its charge.
+__global__ void k_gather_pos32(Geo g, const int *__restrict__ perm, const int *__restrict__ key,
+                               const double *__restrict__ r, const int *__restrict__ type,
+                               const sesmd_forcefield *__restrict__ ff, float4 *__restrict__ spos32)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    int i = perm[k], b = key[k] >> 2, nb = g.nb;
+    spos32[k] = make_float4((float)(r[i] - d_cen4(g, b % nb)), (float)(r[g.n + i] - d_cen4(g, (b / nb) % nb)),
+                            (float)(r[2 * g.n + i] - d_cen4(g, b / (nb * nb))), (float)ff->q[type[i]]);
+}
+
 // ------------------------------------------------------------ near field
 
 // The near field of the 39 eval[wS]* kernels (e.g. src/evalw1w1.f90:28-245,
@@ -666,12 +682,109 @@ __device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict
     }
 }
 
-template <bool MASKED, bool HQ>
+// Mixed precision (SESMD_MIXED): the same pair set and order, pair arithmetic
+// in FP32 on coordinates relative to the source box's centre, the forces of
+// one (box, class) segment summed in FP32 and then added to the FP64 totals.
+struct Recv32 {
+    float q, qO, qH, cO, s6O, rc2;
+    const float *ljc, *ljs;
+};
+
+template <int CAT, bool CHECK, bool CEN, bool HQ>
+__device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32,
+                                               const int4 *__restrict__ smeta, int s0, int s1,
+                                               int bx, int by, int bz, float xr, float yr, float zr,
+                                               const Recv &R, const Recv32 &Q, double &ax,
+                                               double &ay, double &az)
+{
+    if (s0 >= s1) return;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    int4 mn = smeta[s0];
+    float4 pn = spos32[s0];
+    for (int t = s0; t < s1; t++) {
+        const int4 mj = mn;
+        const float4 pj = pn;
+        int tn = min(t + 1, s1 - 1);
+        mn = smeta[tn];
+        pn = spos32[tn];
+        bool ok = mj.z != R.mol;
+        if (CHECK) {
+            int cls = (mj.y >> 8) & 3;
+            int ring = max(abs((mj.w & 255) - bx),
+                           max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
+            int sk = cls == 0 ? (ring <= 1 ? ring : -1)
+                              : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
+            if (sk < 0) continue;
+        }
+        float dx = xr - pj.x, dy = yr - pj.y, dz = zr - pj.z;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float rinv = __builtin_amdgcn_rsqf(r2);
+        float rinv2 = rinv * rinv;
+        float qq = CAT == 0 ? Q.qO : (CAT == 2 && HQ) ? Q.qH : Q.q * pj.w;
+        float ft = qq * (rinv2 * rinv);
+        if (CAT != 2) {
+            float c, s6;
+            if (CAT == 0) {
+                c = Q.cO;
+                s6 = Q.s6O;
+            } else {
+                int tj = mj.y & 255;
+                c = Q.ljc[tj];
+                s6 = Q.ljs[tj];
+            }
+            float r6 = rinv2 * rinv2 * rinv2;
+            float flj = c * r6 * (s6 * r6 - 0.5f) * rinv2;
+            ft = (c != 0.0f && r2 <= Q.rc2) ? ft + flj : ft;
+        }
+        if (CHECK || CEN)
+            ft = ok ? ft : 0.0f;
+        else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
+            __asm__ volatile("");
+            ft = mj.z != R.mol ? ft : 0.0f;
+        }
+        fx = fmaf(dx, ft, fx);
+        fy = fmaf(dy, ft, fy);
+        fz = fmaf(dz, ft, fz);
+    }
+    ax += (double)fx;
+    ay += (double)fy;
+    az += (double)fz;
+}
+
+template <bool CHECK, bool HQ>
+__device__ __forceinline__ void near_row32(const Geo &g, const float4 *__restrict__ spos32,
+                                           const int4 *__restrict__ smeta, const int *__restrict__ bs4,
+                                           int r0, int r1, int bc, int bx, int by, int bz,
+                                           const Recv &R, const Recv32 &Q, double &ax, double &ay,
+                                           double &az)
+{
+    int nb = g.nb;
+    float yr = (float)(R.y - d_cen4(g, (r0 / nb) % nb)), zr = (float)(R.z - d_cen4(g, r0 / (nb * nb)));
+    for (int bb = r0; bb <= r1; bb++) {
+        float xr = (float)(R.x - d_cen4(g, bb % nb));
+        int k0 = __builtin_amdgcn_readfirstlane(bs4[4 * bb]);
+        int k1 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 1]);
+        int k2 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 2]);
+        int k3 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 3]);
+        if (bb == bc) {
+            near_segment32<0, CHECK, true, HQ>(spos32, smeta, k0, k1, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
+            near_segment32<1, CHECK, true, HQ>(spos32, smeta, k1, k2, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
+            near_segment32<2, CHECK, true, HQ>(spos32, smeta, k2, k3, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
+        } else {
+            near_segment32<0, CHECK, false, HQ>(spos32, smeta, k0, k1, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
+            near_segment32<1, CHECK, false, HQ>(spos32, smeta, k1, k2, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
+            near_segment32<2, CHECK, false, HQ>(spos32, smeta, k2, k3, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
+        }
+    }
+}
+
+template <bool MASKED, bool HQ, bool MIX>
 __global__ __launch_bounds__(NEAR_TILE * NEAR_ROWS) void k_near(
     Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
     const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
     const int *__restrict__ rkind, const int2 *__restrict__ work, const int *__restrict__ nwork,
-    const sesmd_forcefield *__restrict__ ff, unsigned long long mask, double *__restrict__ f)
+    const sesmd_forcefield *__restrict__ ff, unsigned long long mask, double *__restrict__ f,
+    const float4 *__restrict__ spos32, const float *__restrict__ lj32)
 {
     // A block owns NEAR_TILE receivers of box b.  Its nine wavefronts take
     // one (dz, dy) row of the 3^3 neighbourhood each -- the three x-neighbours
@@ -723,7 +836,21 @@ __global__ __launch_bounds__(NEAR_TILE * NEAR_ROWS) void k_near(
         bool check = MASKED;
         if (!MASKED)
             for (int bb = r0; bb <= r1; bb++) check = check || bflag[bb];
-        if (MASKED)
+        if (MIX) {
+            Recv32 Q;
+            Q.q = (float)R.q;
+            Q.qO = (float)R.qO;
+            Q.qH = (float)R.qH;
+            Q.cO = lj32[ti * 32];
+            Q.s6O = lj32[1024 + ti * 32];
+            Q.rc2 = (float)g.rc2;
+            Q.ljc = lj32 + ti * 32;
+            Q.ljs = lj32 + 1024 + ti * 32;
+            if (check)
+                near_row32<true, HQ>(g, spos32, smeta, bs4, r0, r1, b, bx, by, bz, R, Q, ax, ay, az);
+            else
+                near_row32<false, HQ>(g, spos32, smeta, bs4, r0, r1, b, bx, by, bz, R, Q, ax, ay, az);
+        } else if (MASKED)
             near_row<true, true, HQ>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, mask, ax, ay, az);
         else if (check)
             near_row<true, false, HQ>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, mask, ax, ay, az);
@@ -1991,6 +2118,16 @@ static int upload_ops(sesmd_ctx *c)
     return 0;
 }
 
+static hipError_t upload_lj32(sesmd_ctx *c, const sesmd_forcefield *ff)
+{
+    std::vector<float> t(2048);
+    for (int k = 0; k < 1024; k++) {
+        t[k] = (float)ff->lj_c[k];
+        t[1024 + k] = (float)ff->lj_s6[k];
+    }
+    return hipMemcpy(c->lj32, t.data(), sizeof(float) * 2048, hipMemcpyHostToDevice);
+}
+
 extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, int p,
                             const sesmd_forcefield *ff, int precision)
 {
@@ -2061,6 +2198,10 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->nwork, 1));
     A(dalloc(&c->spos, n));
     A(dalloc(&c->smeta, n));
+    if (precision == SESMD_MIXED) {
+        A(dalloc(&c->spos32, n));
+        A(dalloc(&c->lj32, 2048));
+    }
     A(dalloc(&c->O4, (size_t)g.nb3 * g.lp));
     A(dalloc(&c->L4, (size_t)g.nb3 * g.lp));
     A(dalloc(&c->O3, (size_t)g.n33 * g.lp));
@@ -2127,7 +2268,8 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
         hipMemcpy(c->mol, mol.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->first, first.data(), sizeof(int) * (g.nmol + 1), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->dff, ff, sizeof(*ff), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->dtopo, &topo, sizeof topo, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(c->dtopo, &topo, sizeof topo, hipMemcpyHostToDevice) != hipSuccess ||
+        (c->lj32 && upload_lj32(c, ff) != hipSuccess)) {
         sesmd_destroy(c);
         return fail("upload of statics failed");
     }
@@ -2162,7 +2304,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
                     c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
-                    c->dtopo, c->partials, c->nh, c->counters};
+                    c->dtopo, c->partials, c->nh, c->counters, c->spos32, c->lj32};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -2327,12 +2469,17 @@ static void launch_near(sesmd_ctx *c, uint64_t mask)
 {
     const Geo &g = c->g;
     bool hq = c->ff.q[1] == c->ff.q[2];
+    bool mix = c->precision == SESMD_MIXED && mask == ~0ull;
     dim3 grid(c->max_work), block(NEAR_TILE * NEAR_ROWS);
-#define NEAR_ARGS g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind, c->work, c->nwork, c->dff, mask, c->f
-    if (mask == ~0ull && hq) k_near<false, true><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
-    else if (mask == ~0ull) k_near<false, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
-    else if (hq) k_near<true, true><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
-    else k_near<true, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+#define NEAR_ARGS \
+    g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind, c->work, c->nwork, c->dff, mask, c->f, \
+        c->spos32, c->lj32
+    if (mix && hq) k_near<false, true, true><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+    else if (mix) k_near<false, false, true><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+    else if (mask == ~0ull && hq) k_near<false, true, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+    else if (mask == ~0ull) k_near<false, false, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+    else if (hq) k_near<true, true, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+    else k_near<true, false, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
 #undef NEAR_ARGS
 }
 
@@ -2344,6 +2491,9 @@ static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask)
     k_fill_d<<<nblk(n3, 256), 256, 0, st>>>(c->f, n3, 0.0);
     if (what & (SESMD_NEAR | SESMD_FAR | SESMD_GRAPHITE))
         k_gather_pos<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->r, c->type, c->dff, c->spos);
+    if ((what & SESMD_NEAR) && c->precision == SESMD_MIXED && mask == ~0ull)
+        k_gather_pos32<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->key_out, c->r, c->type, c->dff,
+                                                       c->spos32);
     if (what & SESMD_NEAR) launch_near(c, mask);
     if (what & (SESMD_FAR | SESMD_WALL))
         (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<nblk(g.n, 128), 128, 0, st>>>(g, what, c->r, c->type, c->cbox, c->rkind, c->L4,
