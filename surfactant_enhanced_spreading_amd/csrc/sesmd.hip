@@ -22,6 +22,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -86,6 +87,9 @@ struct SurfConst {
     double m[29];    // mass / m_O
     double rm[29];   // 1 / mass
     double dsq[28];  // squared bond lengths
+    double k2[28];   // SHAKE: 2 (1/m_a + 1/m_b)
+    double kr[28];   // RATTLE: -1 / (dt (1/m_a + 1/m_b) d^2)
+    double wa[28], wb[28];  // RATTLE: dt / m_a, dt / m_b
 };
 
 struct sesmd_ctx {
@@ -93,6 +97,9 @@ struct sesmd_ctx {
     int precision;
     sesmd_forcefield ff;
     hipStream_t stream;
+    hipStream_t side;      // surfactant work of a step, beside the main stream
+    hipEvent_t ev[4];      // fork/join points of a step
+    double *fs;            // intramolecular forces of a step, 29 ns sites (x, y, z blocks)
     // device state
     double *r, *v, *a, *f;  // each 3*N: x block, y block, z block
     int *type, *mol, *first;
@@ -132,9 +139,12 @@ struct sesmd_ctx {
     int nblk_w, nblk_s;
     bool surf_fast;    // topology equals the reference's: unrolled SHAKE/RATTLE
     SurfConst sconst;
-    // graph of the regular step
-    hipGraphExec_t graph;
+    // graphs of one regular step and of the ifar - 1 regular steps between
+    // two rebuilds
+    hipGraphExec_t graph, graph_run;
     bool graph_ok;
+    int graph_run_len;
+    bool use_run_graph;
     bool built;
     bool classes_fixed;  // classes/storage boxes set by the caller (sesmd_set_classes)
 };
@@ -1378,9 +1388,13 @@ __device__ void dihedral_forces(const sesmd_forcefield *ff, int d, const double 
 //   eval_subtract_fma (src/eval_subtract_fma.f90:10-54),
 //   eval_valence (src/eval_valence.f90:97-158),
 //   eval_dihedral / eval_dihedral_berendsen (src/eval_dihedral.f90:102-295).
+// fs: when non-null, the site's total is stored there (x, y, z blocks of 29 ns
+// surfactant sites) instead of being added to f -- the step pipeline keeps the
+// intramolecular forces apart so this kernel can run beside the near field.
 __global__ __launch_bounds__(64) void k_intra(Geo g, unsigned what, const double *__restrict__ r,
                         const unsigned *__restrict__ isub, const sesmd_forcefield *__restrict__ ff,
-                        const Topo *__restrict__ topo, double *__restrict__ f)
+                        const Topo *__restrict__ topo, double *__restrict__ f,
+                        double *__restrict__ fs)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 29 * g.ns) return;
@@ -1474,9 +1488,16 @@ __global__ __launch_bounds__(64) void k_intra(Geo g, unsigned what, const double
             fz += gz;
         }
     }
-    f[base + j] += fx;
-    f[n + base + j] += fy;
-    f[2 * n + base + j] += fz;
+    if (fs) {
+        int ns29 = 29 * g.ns;
+        fs[t] = fx;
+        fs[ns29 + t] = fy;
+        fs[2 * ns29 + t] = fz;
+    } else {
+        f[base + j] += fx;
+        f[n + base + j] += fy;
+        f[2 * n + base + j] += fz;
+    }
 }
 
 // eval_graphite, src/eval_graphite.f90:13-132: frozen-carbon LJ within rc on
@@ -1541,7 +1562,7 @@ __device__ void block_sum_store(double (&v)[NV], double *out)
 // Part A for waters (src/move_w1_a.f90:29-146 / move_w2_a): predictor with
 // the NH friction, SHAKE over the three cyclic constraints, kinetic sums before
 // and after, force zeroing.  One thread per molecule, 256 per block.
-__global__ __launch_bounds__(256) void k_move_a_water(Geo g, double *__restrict__ r,
+__device__ __forceinline__ void water_a(int blk, Geo g, double *__restrict__ r,
                                                       double *__restrict__ v,
                                                       const double *__restrict__ a,
                                                       double *__restrict__ f,
@@ -1551,7 +1572,7 @@ __global__ __launch_bounds__(256) void k_move_a_water(Geo g, double *__restrict_
                                                       double *__restrict__ partials,
                                                       int *__restrict__ counters, unsigned clsmask)
 {
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    int m = blk * blockDim.x + threadIdx.x;
     double acc[NH_VALS];
     for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
     int n = g.n;
@@ -1611,7 +1632,7 @@ __global__ __launch_bounds__(256) void k_move_a_water(Geo g, double *__restrict_
                 vx[i][0] * vx[i][0] + vx[i][1] * vx[i][1] + vx[i][2] * vx[i][2];
         }
     }
-    block_sum_store<NH_VALS>(acc, partials + (size_t)blockIdx.x * NH_VALS);
+    block_sum_store<NH_VALS>(acc, partials + (size_t)blk * NH_VALS);
 }
 
 // Part A for surfactants (src/move_surfactant_a.f90:19-147): 28 bond
@@ -1688,23 +1709,44 @@ __global__ __launch_bounds__(64) void k_move_a_surf(Geo g, double *__restrict__ 
 
 constexpr int SURF_NT = 32;  // molecules (threads) per block
 
+// Per-bond constants of the unrolled surfactant SHAKE/RATTLE, staged in LDS and
+// read inside the iteration (an empty asm statement with a memory clobber per
+// bond keeps the compiler from hoisting the loads): kept in registers they
+// would crowd out the molecule's 87 coordinates and spill.
+struct SurfLds {
+    double rb[84][SURF_NT];  // bond vectors r_a - r_b of the old positions, per molecule
+    double k[5][28];         // A: d^2, d^2 tol2, 2 (1/m_a + 1/m_b), 1/m_a, 1/m_b
+                             // B: -1/(dt (1/m_a+1/m_b) d^2), dt/m_a, dt/m_b
+    double pre[5][SURF_NT];  // A: kinetic group sums before SHAKE
+};
+
 // Part A for surfactants (src/move_surfactant_a.f90:19-147), one thread per
-// molecule: p in registers, v and the bond vectors r_a - r_b in lane-strided
-// LDS; constraint order, tolerance test and updates exactly as the reference.
-__global__ __launch_bounds__(SURF_NT) void k_move_a_surf_fast(
-    Geo g, double *__restrict__ r, double *__restrict__ v, const double *__restrict__ a,
+// molecule: predicted positions p in registers; constraint order, tolerance
+// test and position updates exactly as the reference.  The reference applies
+// every correction to p and v alike, so v = p - r_old at the end (equal up to
+// the rounding of |r| ~ 20 sigma, ~1e-12 of v dt) instead of a second set of
+// updates.
+__device__ __forceinline__ void surf_a(
+    int blk, Geo g, double *__restrict__ r, double *__restrict__ v, const double *__restrict__ a,
     double *__restrict__ f, SurfConst sc, const double *__restrict__ nh,
     double *__restrict__ partials, int *__restrict__ counters)
 {
-    __shared__ double s_v[87][SURF_NT];
-    __shared__ double s_rb[84][SURF_NT];
+    __shared__ SurfLds L;
     int t = threadIdx.x;
-    int s = blockIdx.x * SURF_NT + t;
-    double acc[NH_VALS];
-#pragma unroll
-    for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
+    for (int c = t; c < 28; c += SURF_NT) {
+        L.k[0][c] = sc.dsq[c];
+        L.k[1][c] = sc.dsq[c] * (2.0 * 0.0001);
+        L.k[2][c] = sc.k2[c];
+        L.k[3][c] = sc.rm[kBI[c]];
+        L.k[4][c] = sc.rm[kBJ[c]];
+    }
+    __syncthreads();
+    const double *K = &L.k[0][0];
+    const double *RB = &L.rb[0][0];
+    int s = blk * SURF_NT + t;
     int n = g.n;
-    if (s < g.ns) {
+    bool live = t < SURF_NT && s < g.ns;
+    if (live) {
         double escale = 1.0 - nh[0] * g.dt / 2.0;
         int base = 3 * g.nw + 29 * s;
         double p[29][3];
@@ -1715,7 +1757,8 @@ __global__ __launch_bounds__(SURF_NT) void k_move_a_surf_fast(
 #pragma unroll
         for (int c = 0; c < 28; c++)
 #pragma unroll
-            for (int d = 0; d < 3; d++) s_rb[3 * c + d][t] = p[kBI[c]][d] - p[kBJ[c]][d];
+            for (int d = 0; d < 3; d++) L.rb[3 * c + d][t] = p[kBI[c]][d] - p[kBJ[c]][d];
+        double pre[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < 29; i++) {
             double vv = 0.0;
@@ -1723,28 +1766,29 @@ __global__ __launch_bounds__(SURF_NT) void k_move_a_surf_fast(
             for (int d = 0; d < 3; d++) {
                 double v0 = v[d * n + base + i], aa = a[d * n + base + i];
                 p[i][d] = p[i][d] + v0 * escale + aa;
-                s_v[3 * i + d][t] = v0 * escale + aa;
                 vv += v0 * v0;
             }
-            acc[4 + c_sgrp[i]] += vv;
+            pre[c_sgrp[i]] += vv;
         }
-        const double tol2 = 2.0 * 0.0001;
+#pragma unroll
+        for (int q = 0; q < 5; q++) L.pre[q][t] = pre[q];
         int it = 0;
         bool done = false;
         while (!done && it <= 100) {
             done = true;
 #pragma unroll
             for (int c = 0; c < 28; c++) {
+                __asm__ volatile("" ::: "memory");
                 const int ia = kBI[c], ib = kBJ[c];
                 double pxab = p[ia][0] - p[ib][0], pyab = p[ia][1] - p[ib][1], pzab = p[ia][2] - p[ib][2];
                 double pabsq = pxab * pxab + pyab * pyab + pzab * pzab;
-                double rabsq = sc.dsq[c];
-                double diffsq = rabsq - pabsq;
-                if (fabs(diffsq) > rabsq * tol2) {
-                    double rxab = s_rb[3 * c][t], ryab = s_rb[3 * c + 1][t], rzab = s_rb[3 * c + 2][t];
+                double diffsq = K[c] - pabsq;
+                if (fabs(diffsq) > K[28 + c]) {
+                    double rxab = RB[(3 * c) * SURF_NT + t], ryab = RB[(3 * c + 1) * SURF_NT + t],
+                           rzab = RB[(3 * c + 2) * SURF_NT + t];
                     double rpab = rxab * pxab + ryab * pyab + rzab * pzab;
-                    double rma = sc.rm[ia], rmb = sc.rm[ib];
-                    double gab = diffsq / (2.0 * (rma + rmb) * rpab);
+                    double rma = K[84 + c], rmb = K[112 + c];
+                    double gab = diffsq / (K[56 + c] * rpab);
                     double dx = rxab * gab, dy = ryab * gab, dz = rzab * gab;
                     p[ia][0] = p[ia][0] + rma * dx;
                     p[ia][1] = p[ia][1] + rma * dy;
@@ -1752,47 +1796,64 @@ __global__ __launch_bounds__(SURF_NT) void k_move_a_surf_fast(
                     p[ib][0] = p[ib][0] - rmb * dx;
                     p[ib][1] = p[ib][1] - rmb * dy;
                     p[ib][2] = p[ib][2] - rmb * dz;
-                    s_v[3 * ia][t] = s_v[3 * ia][t] + rma * dx;
-                    s_v[3 * ia + 1][t] = s_v[3 * ia + 1][t] + rma * dy;
-                    s_v[3 * ia + 2][t] = s_v[3 * ia + 2][t] + rma * dz;
-                    s_v[3 * ib][t] = s_v[3 * ib][t] - rmb * dx;
-                    s_v[3 * ib + 1][t] = s_v[3 * ib + 1][t] - rmb * dy;
-                    s_v[3 * ib + 2][t] = s_v[3 * ib + 2][t] - rmb * dz;
                     done = false;
                 }
             }
             it++;
         }
         if (!done) atomicAdd(&counters[0], 1);
+        double post[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < 29; i++) {
             double vv = 0.0;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                double vn = s_v[3 * i + d][t];
+                double vn = p[i][d] - r[d * n + base + i];
                 r[d * n + base + i] = p[i][d];
                 v[d * n + base + i] = vn;
                 f[d * n + base + i] = 0.0;
                 vv += vn * vn;
             }
-            acc[13 + c_sgrp[i]] += vv;
+            post[c_sgrp[i]] += vv;
+        }
+#pragma unroll
+        for (int q = 0; q < 5; q++) L.rb[q][t] = post[q];  // bond vectors no longer needed
+    }
+    double acc[NH_VALS];
+#pragma unroll
+    for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
+    if (live) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            acc[4 + q] = L.pre[q][t];
+            acc[13 + q] = L.rb[q][t];
         }
     }
     block_sum_store<NH_VALS>(acc, partials + (size_t)(g.nw > 0 ? (g.nw + 255) / 256 : 0) * NH_VALS +
-                                      (size_t)blockIdx.x * NH_VALS);
+                                      (size_t)blk * NH_VALS);
 }
 
 // Part B for surfactants (src/move_surfactant_b.f90:23-122): velocities in
-// registers, bond vectors in LDS, RATTLE in the reference's order.
-__global__ __launch_bounds__(SURF_NT) void k_move_b_surf_fast(
-    Geo g, const double *__restrict__ r, double *__restrict__ v, double *__restrict__ a,
-    const double *__restrict__ f, SurfConst sc, const double *__restrict__ nh,
-    int *__restrict__ counters)
+// registers, bond vectors and per-bond constants in LDS, RATTLE in the
+// reference's order.  A correction below the tolerance is not applied: it
+// adds an exact zero, so the update needs no branch.
+__device__ __forceinline__ void surf_b(
+    int blk, Geo g, const double *__restrict__ r, double *__restrict__ v, double *__restrict__ a,
+    const double *__restrict__ f, const double *__restrict__ fs, SurfConst sc,
+    const double *__restrict__ nh, int *__restrict__ counters)
 {
-    __shared__ double s_rb[84][SURF_NT];
+    __shared__ SurfLds L;
     int t = threadIdx.x;
-    int s = blockIdx.x * SURF_NT + t;
-    if (s >= g.ns) return;
+    for (int c = t; c < 28; c += SURF_NT) {
+        L.k[0][c] = sc.kr[c];
+        L.k[1][c] = sc.wa[c];
+        L.k[2][c] = sc.wb[c];
+    }
+    __syncthreads();
+    const double *K = &L.k[0][0];
+    const double *RB = &L.rb[0][0];
+    int s = blk * SURF_NT + t;
+    if (t >= SURF_NT || s >= g.ns) return;
     int n = g.n, base = 3 * g.nw + 29 * s;
     double escale = 2.0 / (2.0 + nh[0] * g.dt);
     double vx[29][3];
@@ -1803,38 +1864,42 @@ __global__ __launch_bounds__(SURF_NT) void k_move_b_surf_fast(
 #pragma unroll
     for (int c = 0; c < 28; c++)
 #pragma unroll
-        for (int d = 0; d < 3; d++) s_rb[3 * c + d][t] = vx[kBI[c]][d] - vx[kBJ[c]][d];
+        for (int d = 0; d < 3; d++) L.rb[3 * c + d][t] = vx[kBI[c]][d] - vx[kBJ[c]][d];
 #pragma unroll
     for (int i = 0; i < 29; i++)
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            double ac = f[d * n + base + i] * g.dtsq2 / sc.m[i];
+            double fv = f[d * n + base + i];
+            if (fs) fv += fs[d * 29 * g.ns + 29 * s + i];
+            double ac = fv * g.dtsq2 / sc.m[i];
             a[d * n + base + i] = ac;
             vx[i][d] = escale * (v[d * n + base + i] + ac);
         }
-    const double dt = g.dt;
     int it = 0;
     bool done = false;
     while (!done && it <= 100) {
         done = true;
 #pragma unroll
         for (int c = 0; c < 28; c++) {
+            // gab = -(r_ab . v_ab / dt) / ((1/m_a + 1/m_b) d^2), the two
+            // divisions folded into one factor per bond (within 2 ulp)
+            __asm__ volatile("" ::: "memory");
             const int ia = kBI[c], ib = kBJ[c];
-            double rma = sc.rm[ia], rmb = sc.rm[ib];
             double vxab = vx[ia][0] - vx[ib][0], vyab = vx[ia][1] - vx[ib][1], vzab = vx[ia][2] - vx[ib][2];
-            double rxab = s_rb[3 * c][t], ryab = s_rb[3 * c + 1][t], rzab = s_rb[3 * c + 2][t];
-            double rvab = (rxab * vxab + ryab * vyab + rzab * vzab) / dt;
-            double gab = -rvab / ((rma + rmb) * sc.dsq[c]);
-            if (fabs(gab) > 0.0001) {
-                double dx = rxab * gab, dy = ryab * gab, dz = rzab * gab;
-                vx[ia][0] = vx[ia][0] + rma * dx * dt;
-                vx[ia][1] = vx[ia][1] + rma * dy * dt;
-                vx[ia][2] = vx[ia][2] + rma * dz * dt;
-                vx[ib][0] = vx[ib][0] - rmb * dx * dt;
-                vx[ib][1] = vx[ib][1] - rmb * dy * dt;
-                vx[ib][2] = vx[ib][2] - rmb * dz * dt;
-                done = false;
-            }
+            double rxab = RB[(3 * c) * SURF_NT + t], ryab = RB[(3 * c + 1) * SURF_NT + t],
+                   rzab = RB[(3 * c + 2) * SURF_NT + t];
+            double gab = (rxab * vxab + ryab * vyab + rzab * vzab) * K[c];
+            bool app = fabs(gab) > 0.0001;
+            gab = app ? gab : 0.0;
+            done = done && !app;
+            double dx = rxab * gab, dy = ryab * gab, dz = rzab * gab;
+            double wa = K[28 + c], wb = K[56 + c];
+            vx[ia][0] = vx[ia][0] + wa * dx;
+            vx[ia][1] = vx[ia][1] + wa * dy;
+            vx[ia][2] = vx[ia][2] + wa * dz;
+            vx[ib][0] = vx[ib][0] - wb * dx;
+            vx[ib][1] = vx[ib][1] - wb * dy;
+            vx[ib][2] = vx[ib][2] - wb * dz;
         }
         it++;
     }
@@ -1877,7 +1942,7 @@ __global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restri
 
 // Part B for waters (src/move_w1_b.f90:23-123): a = F dt^2/2m, velocity with
 // the new eta, RATTLE.
-__global__ __launch_bounds__(256) void k_move_b_water(Geo g, const double *__restrict__ r,
+__device__ __forceinline__ void water_b(int blk, Geo g, const double *__restrict__ r,
                                                       double *__restrict__ v, double *__restrict__ a,
                                                       const double *__restrict__ f,
                                                       const sesmd_forcefield *__restrict__ ff,
@@ -1885,7 +1950,7 @@ __global__ __launch_bounds__(256) void k_move_b_water(Geo g, const double *__res
                                                       int *__restrict__ counters,
                                                       const int *__restrict__ cls, unsigned clsmask)
 {
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    int m = blk * blockDim.x + threadIdx.x;
     if (m >= g.nw || !((clsmask >> cls[m]) & 1u)) return;
     int n = g.n, base = 3 * m;
     double escale = 2.0 / (2.0 + nh[0] * g.dt);
@@ -1926,10 +1991,88 @@ __global__ __launch_bounds__(256) void k_move_b_water(Geo g, const double *__res
         for (int d = 0; d < 3; d++) v[d * n + base + i] = vx[i][d];
 }
 
+// Kernels over the integrator bodies above.  The fused forms put the
+// surfactant blocks (first) and the water blocks of one half-step in a single
+// launch so the two run side by side on the device.
+__global__ __launch_bounds__(256) void k_move_a_water(Geo g, double *__restrict__ r, double *__restrict__ v,
+                                                      const double *__restrict__ a, double *__restrict__ f,
+                                                      const int *__restrict__ cls,
+                                                      const sesmd_forcefield *__restrict__ ff,
+                                                      const double *__restrict__ nh,
+                                                      double *__restrict__ partials,
+                                                      int *__restrict__ counters, unsigned clsmask)
+{
+    water_a(blockIdx.x, g, r, v, a, f, cls, ff, nh, partials, counters, clsmask);
+}
+
+__global__ __launch_bounds__(SURF_NT) void k_move_a_surf_fast(Geo g, double *__restrict__ r,
+                                                              double *__restrict__ v,
+                                                              const double *__restrict__ a,
+                                                              double *__restrict__ f, SurfConst sc,
+                                                              const double *__restrict__ nh,
+                                                              double *__restrict__ partials,
+                                                              int *__restrict__ counters)
+{
+    surf_a(blockIdx.x, g, r, v, a, f, sc, nh, partials, counters);
+}
+
+__global__ __launch_bounds__(256) void k_move_a_fused(Geo g, double *__restrict__ r, double *__restrict__ v,
+                                                      const double *__restrict__ a, double *__restrict__ f,
+                                                      const int *__restrict__ cls,
+                                                      const sesmd_forcefield *__restrict__ ff,
+                                                      SurfConst sc, const double *__restrict__ nh,
+                                                      double *__restrict__ partials,
+                                                      int *__restrict__ counters, int nblk_s)
+{
+    if ((int)blockIdx.x < nblk_s)
+        surf_a(blockIdx.x, g, r, v, a, f, sc, nh, partials, counters);
+    else
+        water_a(blockIdx.x - nblk_s, g, r, v, a, f, cls, ff, nh, partials, counters, 7u);
+}
+
+__global__ __launch_bounds__(256) void k_move_b_water(Geo g, const double *__restrict__ r,
+                                                      double *__restrict__ v, double *__restrict__ a,
+                                                      const double *__restrict__ f,
+                                                      const sesmd_forcefield *__restrict__ ff,
+                                                      const double *__restrict__ nh,
+                                                      int *__restrict__ counters,
+                                                      const int *__restrict__ cls, unsigned clsmask)
+{
+    water_b(blockIdx.x, g, r, v, a, f, ff, nh, counters, cls, clsmask);
+}
+
+__global__ __launch_bounds__(SURF_NT) void k_move_b_surf_fast(Geo g, const double *__restrict__ r,
+                                                              double *__restrict__ v,
+                                                              double *__restrict__ a,
+                                                              const double *__restrict__ f,
+                                                              const double *__restrict__ fs,
+                                                              SurfConst sc,
+                                                              const double *__restrict__ nh,
+                                                              int *__restrict__ counters)
+{
+    surf_b(blockIdx.x, g, r, v, a, f, fs, sc, nh, counters);
+}
+
+__global__ __launch_bounds__(256) void k_move_b_fused(Geo g, const double *__restrict__ r,
+                                                      double *__restrict__ v, double *__restrict__ a,
+                                                      const double *__restrict__ f,
+                                                      const double *__restrict__ fs,
+                                                      const sesmd_forcefield *__restrict__ ff,
+                                                      SurfConst sc, const double *__restrict__ nh,
+                                                      int *__restrict__ counters,
+                                                      const int *__restrict__ cls, int nblk_s)
+{
+    if ((int)blockIdx.x < nblk_s)
+        surf_b(blockIdx.x, g, r, v, a, f, fs, sc, nh, counters);
+    else
+        water_b(blockIdx.x - nblk_s, g, r, v, a, f, ff, nh, counters, cls, 7u);
+}
+
 // Part B for surfactants (src/move_surfactant_b.f90:23-122)
 __global__ __launch_bounds__(64) void k_move_b_surf(Geo g, const double *__restrict__ r,
                                                     double *__restrict__ v, double *__restrict__ a,
                                                     const double *__restrict__ f,
+                                                    const double *__restrict__ fs,
                                                     const sesmd_forcefield *__restrict__ ff,
                                                     const double *__restrict__ nh,
                                                     int *__restrict__ counters)
@@ -1943,7 +2086,9 @@ __global__ __launch_bounds__(64) void k_move_b_surf(Geo g, const double *__restr
         double mi = ff->mass[3 + i];
         for (int d = 0; d < 3; d++) {
             rx[i][d] = r[d * n + base + i];
-            double ac = f[d * n + base + i] * g.dtsq2 / mi;
+            double fv = f[d * n + base + i];
+            if (fs) fv += fs[d * 29 * g.ns + 29 * s + i];
+            double ac = fv * g.dtsq2 / mi;
             a[d * n + base + i] = ac;
             vx[i][d] = escale * (v[d * n + base + i] + ac);
         }
@@ -2166,12 +2311,16 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     g.dtsq2 = ff->dtsq2;
     c->precision = precision;
     c->ff = *ff;
+    c->use_run_graph = !getenv("SESMD_STEP_GRAPH_ONE");
     int rc = 0;
     auto A = [&](int r2) {
         if (r2) rc = 1;
     };
-    A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess
+    A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+              hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess
           ? fail("stream create failed") : 0);
+    for (hipEvent_t &e : c->ev)
+        A(hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ? fail("event create failed") : 0);
     size_t n = g.n;
     A(dalloc(&c->r, 3 * n));
     A(dalloc(&c->v, 3 * n));
@@ -2232,10 +2381,18 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
         c->sconst.m[i] = ff->mass[3 + i];
         c->sconst.rm[i] = 1.0 / ff->mass[3 + i];
     }
-    for (int b = 0; b < 28; b++) c->sconst.dsq[b] = ff->dsqs[b];
+    for (int b = 0; b < 28; b++) {
+        double rma = c->sconst.rm[kBI[b]], rmb = c->sconst.rm[kBJ[b]];
+        c->sconst.dsq[b] = ff->dsqs[b];
+        c->sconst.k2[b] = 2.0 * (rma + rmb);
+        c->sconst.kr[b] = -1.0 / (ff->dt * ((rma + rmb) * ff->dsqs[b]));
+        c->sconst.wa[b] = rma * ff->dt;
+        c->sconst.wb[b] = rmb * ff->dt;
+    }
     c->nblk_w = g.nw > 0 ? (int)nblk(g.nw, 256) : 0;
     c->nblk_s = g.ns > 0 ? (int)nblk(g.ns, c->surf_fast ? SURF_NT : 64) : 0;
     A(dalloc(&c->partials, (size_t)(c->nblk_w + c->nblk_s + 1) * NH_VALS));
+    A(dalloc(&c->fs, 3 * 29 * (size_t)g.ns + 1));
     A(dalloc(&c->nh, 16 + NH_VALS));
     A(dalloc(&c->counters, 4));
     if (rc) {
@@ -2299,14 +2456,19 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
     if (!c) return 0;
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->graph_ok) hipGraphExecDestroy(c->graph);
+    if (c->graph_run_len) hipGraphExecDestroy(c->graph_run);
     void *ptrs[] = {c->r, c->v, c->a, c->f, c->type, c->mol, c->first, c->cls, c->anc, c->cbox,
                     c->rkind, c->isub, c->key_in, c->key_out, c->val_in, c->perm, c->cub_tmp,
                     c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
-                    c->dtopo, c->partials, c->nh, c->counters, c->spos32, c->lj32};
+                    c->dtopo, c->partials, c->nh, c->counters, c->spos32, c->lj32, c->fs};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    if (c->side) hipStreamSynchronize(c->side);
+    for (hipEvent_t e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->side) hipStreamDestroy(c->side);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -2400,6 +2562,10 @@ extern "C" int sesmd_set_graphite(sesmd_ctx *c, int n, const double *xyz)
     if (c->graph) {
         hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
+    }
+    if (c->graph_run_len) {
+        hipGraphExecDestroy(c->graph_run);
+        c->graph_run_len = 0;
     }
     return 0;
 }
@@ -2500,7 +2666,8 @@ static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask)
                                                      c->lfac, c->dff, c->f);
     if (g.ns && (what & (SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL |
                          SESMD_DIHEDRAL_BE)))
-        k_intra<<<nblk(29 * g.ns, 64), 64, 0, st>>>(g, what, c->r, c->isub, c->dff, c->dtopo, c->f);
+        k_intra<<<nblk(29 * g.ns, 64), 64, 0, st>>>(g, what, c->r, c->isub, c->dff, c->dtopo, c->f,
+                                                    nullptr);
     if ((what & SESMD_GRAPHITE) && c->ngr > 0)
         k_graphite<<<g.nb * g.nb, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->gr,
                                                c->gstart, c->gidx, c->dff, c->f);
@@ -2581,10 +2748,11 @@ static int launch_move_b(sesmd_ctx *c, unsigned clsmask = 7u)
         return 0;
     }
     if (c->nblk_s && c->surf_fast)
-        k_move_b_surf_fast<<<c->nblk_s, SURF_NT, 0, st>>>(g, c->r, c->v, c->a, c->f, c->sconst, c->nh,
-                                                         c->counters);
+        k_move_b_surf_fast<<<c->nblk_s, SURF_NT, 0, st>>>(g, c->r, c->v, c->a, c->f, nullptr, c->sconst,
+                                                         c->nh, c->counters);
     else if (c->nblk_s)
-        k_move_b_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->counters);
+        k_move_b_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, nullptr, c->dff, c->nh,
+                                                c->counters);
     HC(hipGetLastError());
     return 0;
 }
@@ -2607,11 +2775,76 @@ extern "C" int sesmd_init_accel(sesmd_ctx *c)
     return 0;
 }
 
-static int regular_step(sesmd_ctx *c)
+// One full step (src/mybox.f90:130-227).  Part A and part B each run the
+// surfactant and water molecules in one launch (the few, latency-bound
+// surfactant blocks first); the intramolecular surfactant forces run on a
+// second stream beside the near field, L2P and graphite, into their own
+// buffer fs, which part B adds:
+//   main:  A(surf+water) - nh(eta) - [rebuild, far field] -+- fill, gather, near,
+//                                                          |  L2P + wall, graphite -+- B(surf+water)
+//   side:                                                  +- intra -> fs ----------+
+static int launch_step(sesmd_ctx *c, bool rebuild)
 {
-    CK(launch_move_a(c));
-    CK(launch_forces(c, SESMD_ALL, ~0ull));
-    CK(launch_move_b(c));
+    const Geo &g = c->g;
+    hipStream_t st = c->stream, ss = c->side;
+    if (c->surf_fast) {
+        k_move_a_fused<<<c->nblk_s + c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->cls, c->dff,
+                                                              c->sconst, c->nh, c->partials, c->counters,
+                                                              c->nblk_s);
+    } else {
+        if (c->nblk_w)
+            k_move_a_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->cls, c->dff, c->nh,
+                                                       c->partials, c->counters, 7u);
+        if (c->nblk_s)
+            k_move_a_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->partials,
+                                                    c->counters);
+    }
+    k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1);
+    HC(hipGetLastError());
+    if (rebuild) {
+        CK(launch_rebuild(c));
+        CK(launch_fmm(c));
+    }
+    if (g.ns) {
+        HC(hipEventRecord(c->ev[0], st));
+        HC(hipStreamWaitEvent(ss, c->ev[0], 0));
+        k_intra<<<nblk(29 * g.ns, 64), 64, 0, ss>>>(g, SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE |
+                                                          SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE,
+                                                    c->r, c->isub, c->dff, c->dtopo, c->f, c->fs);
+        HC(hipEventRecord(c->ev[1], ss));
+    }
+    CK(launch_forces(c, SESMD_ALL & ~(SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE |
+                                       SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE),
+                     ~0ull));
+    if (g.ns) HC(hipStreamWaitEvent(st, c->ev[1], 0));
+    if (c->surf_fast) {
+        k_move_b_fused<<<c->nblk_s + c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->fs, c->dff,
+                                                              c->sconst, c->nh, c->counters, c->cls,
+                                                              c->nblk_s);
+    } else {
+        if (c->nblk_w)
+            k_move_b_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->counters,
+                                                       c->cls, 7u);
+        if (c->nblk_s)
+            k_move_b_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->fs, c->dff, c->nh,
+                                                    c->counters);
+    }
+    HC(hipGetLastError());
+    return 0;
+}
+
+static int capture_steps(sesmd_ctx *c, int nsteps, hipGraphExec_t *out)
+{
+    hipGraph_t gr;
+    HC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = 0;
+    for (int k = 0; k < nsteps && !r; k++) r = launch_step(c, false);
+    hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    if (r) return 1;
+    HC(e);
+    hipError_t ei = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    HC(ei);
     return 0;
 }
 
@@ -2620,28 +2853,34 @@ extern "C" int sesmd_steps(sesmd_ctx *c, int nsteps, int ifar, int *kb)
     if (!c || !kb) return fail("null argument");
     if (!c->built) return fail("sesmd_steps before sesmd_rebuild");
     if (ifar < 1) return fail("ifar must be >= 1");
-    for (int s = 0; s < nsteps; s++) {
-        (*kb)++;
-        if (*kb % ifar == 0) {
-            CK(launch_move_a(c));
-            CK(launch_rebuild(c));
-            CK(launch_fmm(c));
-            CK(launch_forces(c, SESMD_ALL, ~0ull));
-            CK(launch_move_b(c));
+    int run = ifar - 1;  // regular steps between two rebuild steps
+    for (int s = 0; s < nsteps;) {
+        if ((*kb + 1) % ifar == 0) {
+            (*kb)++;
+            s++;
+            CK(launch_step(c, true));
+            continue;
+        }
+        if (c->use_run_graph && run > 1 && (*kb + 1) % ifar == 1 && nsteps - s >= run) {
+            // a whole run of regular steps as one graph
+            if (c->graph_run_len != run) {
+                if (c->graph_run_len) HC(hipGraphExecDestroy(c->graph_run));
+                c->graph_run_len = 0;
+                CK(capture_steps(c, run, &c->graph_run));
+                c->graph_run_len = run;
+            }
+            HC(hipGraphLaunch(c->graph_run, c->stream));
+            *kb += run;
+            s += run;
             continue;
         }
         if (!c->graph_ok) {
-            hipGraph_t gr;
-            HC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int r = regular_step(c);
-            hipError_t e = hipStreamEndCapture(c->stream, &gr);
-            if (r) return 1;
-            HC(e);
-            HC(hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0));
-            HC(hipGraphDestroy(gr));
+            CK(capture_steps(c, 1, &c->graph));
             c->graph_ok = true;
         }
         HC(hipGraphLaunch(c->graph, c->stream));
+        (*kb)++;
+        s++;
     }
     HC(hipStreamSynchronize(c->stream));
     return 0;

@@ -106,3 +106,37 @@ def test_one_step(pair, sys_small, sys_medium):
     re_, ve, ae, ee = e.state()
     assert rel_err(ve, vo) <= TOL
     assert rel_err(ae, ao) <= TOL
+
+
+@pytest.mark.parametrize('which', ['small', 'medium'])
+def test_steps_pipeline(which, sys_small, sys_medium):
+    """sesmd_steps -- the two-stream step with the surfactant work beside the
+    near field, regular steps replayed from captured graphs (one step, and a
+    whole run of ifar - 1 regular steps), a rebuild + far-field step every
+    ifar -- against the oracle's own step loop: steps 9..22 with ifar = 10
+    (step 9 from the one-step graph, 10 and 20 rebuilds, 11..19 as one
+    graph, 21..22 from the one-step graph)."""
+    import oracle as orc
+    from surfactant_enhanced_spreading_amd import engine
+    s = sys_small if which == 'small' else sys_medium
+    o = orc.Oracle(s)
+    o.rebuild()
+    o.fmm()
+    o.init_accel(o.forces())
+    e = engine.Engine(s)
+    try:
+        e.bootstrap()
+        ko = o.steps(14, 10, 8)
+        ke = e.steps(14, 10, 8)
+        assert ko == ke == 22
+        ro, vo, ao, eo = o.state()
+        re_, ve, ae, ee = e.state()
+        print(which, 'r', np.abs(re_ - ro).max(), 'v', rel_err(ve, vo), 'a', rel_err(ae, ao),
+              'eta', ee, eo, flush=True)
+        assert np.abs(re_ - ro).max() <= 1e-12 * np.abs(ro).max()
+        assert rel_err(ve, vo) <= TOL
+        assert rel_err(ae, ao) <= TOL
+        assert abs(ee - eo) <= 1e-10 * abs(eo)
+        assert e.shake_failures() == o.shake_failures()
+    finally:
+        e.close()
