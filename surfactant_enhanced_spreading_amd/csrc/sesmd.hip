@@ -22,7 +22,6 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -122,6 +121,7 @@ struct sesmd_ctx {
     double2 *O4, *L4, *O3, *L3, *O2, *L2;
     int *ne4, *ne3, *ne2;         // nonempty flags per box
     double2 *T4, *T3, *T2;        // M2L operators per offset
+    double2 *top_part;            // top-level M2L partials per (target, source)
     double2 *M43, *M32, *S23, *S34; // M2M / L2L operators per child
     double *c4;                   // L4 centres (3 per box)
     double *lfac;                 // rfctr(j,|k|) table
@@ -144,7 +144,6 @@ struct sesmd_ctx {
     hipGraphExec_t graph, graph_run;
     bool graph_ok;
     int graph_run_len;
-    bool use_run_graph;
     bool built;
     bool classes_fixed;  // classes/storage boxes set by the caller (sesmd_set_classes)
 };
@@ -488,6 +487,7 @@ __global__ void k_bstart(Geo g, const int *__restrict__ keys, int *__restrict__ 
 }
 
 // near-field work list: one item per NEAR_TILE receivers of a box; single block
+// Near-field work items: per box, chunks of NEAR_TILE receivers.
 __global__ void k_worklist(Geo g, const int *__restrict__ bstart, int2 *__restrict__ work,
                            int *__restrict__ nwork, int max_work)
 {
@@ -1064,24 +1064,39 @@ __device__ __forceinline__ void store_rows(const Geo &g, double2 *out, int row, 
 }
 
 // M2M (computeOnm3 / computeOnm2): parent = sum over 8 children of M[child] O_child
-__global__ __launch_bounds__(64) void k_m2m(Geo g, int nd_parent, const double2 *__restrict__ Oc,
-                                            const int *__restrict__ nec, const double2 *__restrict__ M,
-                                            double2 *__restrict__ Op, int *__restrict__ nep)
+// M2M: the eight children of a parent contracted side by side (64 lanes
+// each), then summed in child order -- deterministic.
+__global__ __launch_bounds__(512) void k_m2m(Geo g, int nd_parent, const double2 *__restrict__ Oc,
+                                             const int *__restrict__ nec, const double2 *__restrict__ M,
+                                             double2 *__restrict__ Op, int *__restrict__ nep)
 {
+    __shared__ double2 s_part[8][64];
     int P = blockIdx.x;
-    int row = threadIdx.x;
+    int kc = threadIdx.x >> 6, row = threadIdx.x & 63;
     int px = P % nd_parent, py = (P / nd_parent) % nd_parent, pz = P / (nd_parent * nd_parent);
     int ndc = 2 * nd_parent;
+    int C = (2 * px + (kc & 1)) + ndc * ((2 * py + ((kc >> 1) & 1)) + ndc * (2 * pz + (kc >> 2)));
     double2 acc = make_double2(0.0, 0.0);
-    int any = 0;
-    for (int kc = 0; kc < 8; kc++) {
-        int C = (2 * px + (kc & 1)) + ndc * ((2 * py + ((kc >> 1) & 1)) + ndc * (2 * pz + (kc >> 2)));
-        if (!nec[C]) continue;
-        any = 1;
-        if (row < g.lk) contract_rows(g, M + (size_t)kc * g.lk * g.lp, Oc + (size_t)C * g.lp, acc, row);
+    if (nec[C] && row < g.lk) contract_rows(g, M + (size_t)kc * g.lk * g.lp, Oc + (size_t)C * g.lp, acc, row);
+    s_part[kc][row] = acc;
+    __syncthreads();
+    if (kc != 0) return;
+    if (row < g.lk) {
+        double2 sum = s_part[0][row];
+        for (int q = 1; q < 8; q++) {
+            sum.x += s_part[q][row].x;
+            sum.y += s_part[q][row].y;
+        }
+        store_rows(g, Op + (size_t)P * g.lp, row, sum);
     }
-    if (row < g.lk) store_rows(g, Op + (size_t)P * g.lp, row, acc);
-    if (row == 0) nep[P] = any;
+    if (row == 0) {
+        int any = 0;
+        for (int q = 0; q < 8; q++) {
+            int Cq = (2 * px + (q & 1)) + ndc * ((2 * py + ((q >> 1) & 1)) + ndc * (2 * pz + (q >> 2)));
+            any |= nec[Cq];
+        }
+        nep[P] = any;
+    }
 }
 
 // M2L over the interaction list (computeLjk4 / computeLjk3): sources are the
@@ -1139,24 +1154,39 @@ __global__ __launch_bounds__(256) void k_m2l(Geo g, int nd, const double2 *__res
 }
 
 // top level (computeLjk2): every L2 box that is not adjacent to the target
-__global__ __launch_bounds__(64) void k_m2l_top(Geo g, const double2 *__restrict__ O2,
-                                                const int *__restrict__ ne2,
-                                                const double2 *__restrict__ T2,
-                                                double2 *__restrict__ L2)
+// M2L at the top level (computeLjk2): one block per (target, source) pair of
+// L2 boxes, then a per-target sum over sources in index order.
+__global__ __launch_bounds__(64) void k_m2l_top_pair(Geo g, const double2 *__restrict__ O2,
+                                                     const int *__restrict__ ne2,
+                                                     const double2 *__restrict__ T2,
+                                                     double2 *__restrict__ part)
 {
-    int Tb = blockIdx.x, row = threadIdx.x;
+    int Tb = blockIdx.x / g.n2, S = blockIdx.x % g.n2, row = threadIdx.x;
     int nd = g.nb2;
-    int tx = Tb % nd, ty = (Tb / nd) % nd, tz = Tb / (nd * nd);
     if (row >= g.lk) return;
+    int tx = Tb % nd, ty = (Tb / nd) % nd, tz = Tb / (nd * nd);
+    int sx = S % nd, sy = (S / nd) % nd, sz = S / (nd * nd);
+    int dx = sx - tx, dy = sy - ty, dz = sz - tz;
     double2 acc = make_double2(0.0, 0.0);
-    int w = 2 * nd - 1;
-    for (int S = 0; S < g.n2; S++) {
-        int sx = S % nd, sy = (S / nd) % nd, sz = S / (nd * nd);
-        int dx = sx - tx, dy = sy - ty, dz = sz - tz;
-        if (iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1) continue;
-        if (!ne2[S]) continue;
+    bool adjacent = iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1;
+    if (!adjacent && ne2[S]) {
+        int w = 2 * nd - 1;
         int off = (dx + nd - 1) + w * ((dy + nd - 1) + w * (dz + nd - 1));
         contract_rows(g, T2 + (size_t)off * g.lk * g.lp, O2 + (size_t)S * g.lp, acc, row);
+    }
+    part[((size_t)Tb * g.n2 + S) * g.lk + row] = acc;
+}
+
+__global__ __launch_bounds__(64) void k_m2l_top_sum(Geo g, const double2 *__restrict__ part,
+                                                    double2 *__restrict__ L2)
+{
+    int Tb = blockIdx.x, row = threadIdx.x;
+    if (row >= g.lk) return;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int S = 0; S < g.n2; S++) {
+        double2 v = part[((size_t)Tb * g.n2 + S) * g.lk + row];
+        acc.x += v.x;
+        acc.y += v.y;
     }
     store_rows(g, L2 + (size_t)Tb * g.lp, row, acc);
 }
@@ -2311,7 +2341,6 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     g.dtsq2 = ff->dtsq2;
     c->precision = precision;
     c->ff = *ff;
-    c->use_run_graph = !getenv("SESMD_STEP_GRAPH_ONE");
     int rc = 0;
     auto A = [&](int r2) {
         if (r2) rc = 1;
@@ -2364,6 +2393,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->T3, (size_t)343 * g.lk * g.lp));
     int w2 = 2 * nb2 - 1;
     A(dalloc(&c->T2, (size_t)w2 * w2 * w2 * g.lk * g.lp));
+    A(dalloc(&c->top_part, (size_t)g.n2 * g.n2 * g.lk));
     A(dalloc(&c->M43, (size_t)8 * g.lk * g.lp));
     A(dalloc(&c->M32, (size_t)8 * g.lk * g.lp));
     A(dalloc(&c->S23, (size_t)8 * g.lk * g.lp));
@@ -2460,7 +2490,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
     void *ptrs[] = {c->r, c->v, c->a, c->f, c->type, c->mol, c->first, c->cls, c->anc, c->cbox,
                     c->rkind, c->isub, c->key_in, c->key_out, c->val_in, c->perm, c->cub_tmp,
                     c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->spos, c->smeta, c->O4, c->L4,
-                    c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2,
+                    c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
                     c->dtopo, c->partials, c->nh, c->counters, c->spos32, c->lj32, c->fs};
     for (void *p : ptrs)
@@ -2610,9 +2640,10 @@ static int launch_fmm(sesmd_ctx *c)
         k_p2m<4><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
     else
         k_p2m<8><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
-    k_m2m<<<g.n33, 64, 0, st>>>(g, g.n3, c->O4, c->ne4, c->M43, c->O3, c->ne3);
-    k_m2m<<<g.n2, 64, 0, st>>>(g, g.nb2, c->O3, c->ne3, c->M32, c->O2, c->ne2);
-    k_m2l_top<<<g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->L2);
+    k_m2m<<<g.n33, 512, 0, st>>>(g, g.n3, c->O4, c->ne4, c->M43, c->O3, c->ne3);
+    k_m2m<<<g.n2, 512, 0, st>>>(g, g.nb2, c->O3, c->ne3, c->M32, c->O2, c->ne2);
+    k_m2l_top_pair<<<g.n2 * g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->top_part);
+    k_m2l_top_sum<<<g.n2, 64, 0, st>>>(g, c->top_part, c->L2);
     k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3);
     k_m2l<<<g.n33, 256, 0, st>>>(g, g.n3, c->O3, c->ne3, c->T3, c->L3);
     k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4);
@@ -2861,7 +2892,7 @@ extern "C" int sesmd_steps(sesmd_ctx *c, int nsteps, int ifar, int *kb)
             CK(launch_step(c, true));
             continue;
         }
-        if (c->use_run_graph && run > 1 && (*kb + 1) % ifar == 1 && nsteps - s >= run) {
+        if (run > 1 && (*kb + 1) % ifar == 1 && nsteps - s >= run) {
             // a whole run of regular steps as one graph
             if (c->graph_run_len != run) {
                 if (c->graph_run_len) HC(hipGraphExecDestroy(c->graph_run));
@@ -3029,9 +3060,12 @@ extern "C" int sesmd_fmm_stage(sesmd_ctx *c, int stage)
             k_p2m<8><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4,
                                            c->ne4);
         break;
-    case SESMD_M2M_43: k_m2m<<<g.n33, 64, 0, st>>>(g, g.n3, c->O4, c->ne4, c->M43, c->O3, c->ne3); break;
-    case SESMD_M2M_32: k_m2m<<<g.n2, 64, 0, st>>>(g, g.nb2, c->O3, c->ne3, c->M32, c->O2, c->ne2); break;
-    case SESMD_M2L_2: k_m2l_top<<<g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->L2); break;
+    case SESMD_M2M_43: k_m2m<<<g.n33, 512, 0, st>>>(g, g.n3, c->O4, c->ne4, c->M43, c->O3, c->ne3); break;
+    case SESMD_M2M_32: k_m2m<<<g.n2, 512, 0, st>>>(g, g.nb2, c->O3, c->ne3, c->M32, c->O2, c->ne2); break;
+    case SESMD_M2L_2:
+        k_m2l_top_pair<<<g.n2 * g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->top_part);
+        k_m2l_top_sum<<<g.n2, 64, 0, st>>>(g, c->top_part, c->L2);
+        break;
     case SESMD_L2L_23: k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3); break;
     case SESMD_M2L_3: k_m2l<<<g.n33, 256, 0, st>>>(g, g.n3, c->O3, c->ne3, c->T3, c->L3); break;
     case SESMD_L2L_34: k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4); break;
