@@ -132,6 +132,26 @@ def rms_force_error(sys_):
     return float(np.sqrt(((f[1] - f[0]) ** 2).sum() / (f[0] ** 2).sum()))
 
 
+def copy_calibration(nbytes, reps=5):
+    """A plain device-to-device copy of `nbytes` (hipMemcpy, the runtime's copy
+    kernel), for scaling the FETCH_SIZE / WRITE_SIZE counters of the profiled
+    kernels: each copy reads and writes exactly nbytes."""
+    hip = ctypes.CDLL('/opt/rocm/lib/libamdhip64.so')
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), nbytes) == 0 and hip.hipMalloc(ctypes.byref(b), nbytes) == 0
+    assert hip.hipMemset(a, 1, nbytes) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps(dict(copy_bytes=nbytes, reps=reps)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -140,7 +160,12 @@ def main():
     ap.add_argument('--no-mixed', action='store_true', help='skip the mixed-precision leg')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    ap.add_argument('--copy-calibrate', type=int, default=0, metavar='BYTES',
+                    help='only run a device copy of BYTES (counter calibration) and exit')
     args = ap.parse_args()
+    if args.copy_calibrate:
+        copy_calibration(args.copy_calibrate)
+        return
     if args.gpus != 1:
         print('bench.py measures one GPU: this build runs the whole system on one MI355X '
               'and has no multi-GPU path (see DESIGN.md, "How it would shard")', file=sys.stderr)

@@ -113,8 +113,12 @@ struct sesmd_ctx {
     int2 *work;
     int *nwork;
     int max_work;
+    int2 *work_raw;                  // items in box order, before the cost sort
+    unsigned *wkey_in, *wkey_out;    // 1 + estimated cost per item (0: padding)
+    int *widx_in, *widx_out;
     double4 *spos;
     int4 *smeta;
+    double *npart;   // near-field partial forces: 9 rows x (x, y, z) x N, sorted order
     float4 *spos32;  // mixed precision: position relative to its box centre, charge
     float *lj32;     // mixed precision: lj_c (1024) then lj_s6 (1024) as float
     // far field
@@ -487,8 +491,12 @@ __global__ void k_bstart(Geo g, const int *__restrict__ keys, int *__restrict__ 
 }
 
 // near-field work list: one item per NEAR_TILE receivers of a box; single block
-// Near-field work items: per box, chunks of NEAR_TILE receivers.
+// Near-field work items: per box, chunks of NEAR_TILE receivers, in box order,
+// each with a cost estimate (receivers x sites in the 3^3 block) as a sort key;
+// the items are then ordered by decreasing cost (k_work_gather), so the longest
+// workgroups start first and the short ones fill in behind them.
 __global__ void k_worklist(Geo g, const int *__restrict__ bstart, int2 *__restrict__ work,
+                           unsigned *__restrict__ wkey, int *__restrict__ widx,
                            int *__restrict__ nwork, int max_work)
 {
     __shared__ int s_cnt[1024];
@@ -509,13 +517,39 @@ __global__ void k_worklist(Geo g, const int *__restrict__ bstart, int2 *__restri
             __syncthreads();
         }
         int excl = s_base + s_cnt[threadIdx.x] - nch;
+        int src = 0;
+        if (nch) {
+            int nb = g.nb, bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+            for (int z = max(bz - 1, 0); z <= min(bz + 1, nb - 1); z++)
+                for (int y = max(by - 1, 0); y <= min(by + 1, nb - 1); y++) {
+                    int r0 = max(bx - 1, 0) + nb * (y + nb * z), r1 = min(bx + 1, nb - 1) + nb * (y + nb * z);
+                    src += bstart[r1 + 1] - bstart[r0];
+                }
+        }
         for (int c = 0; c < nch; c++)
-            if (excl + c < max_work) work[excl + c] = make_int2(b, bstart[b] + c * NEAR_TILE);
+            if (excl + c < max_work) {
+                int k0 = bstart[b] + c * NEAR_TILE;
+                int nr = min(NEAR_TILE, bstart[b + 1] - k0);
+                work[excl + c] = make_int2(b, k0);
+                wkey[excl + c] = 1u + (unsigned)min(nr * src, 1 << 30);
+                widx[excl + c] = excl + c;
+            }
         __syncthreads();
         if (threadIdx.x == blockDim.x - 1) s_base += s_cnt[threadIdx.x];
         __syncthreads();
     }
+    for (int i = min(s_base, max_work) + threadIdx.x; i < max_work; i += blockDim.x) {
+        wkey[i] = 0u;
+        widx[i] = i;
+    }
     if (threadIdx.x == 0) *nwork = min(s_base, max_work);
+}
+
+__global__ void k_work_gather(const int2 *__restrict__ raw, const int *__restrict__ order,
+                              const int *__restrict__ nwork, int2 *__restrict__ work)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *nwork) work[i] = raw[order[i]];
 }
 
 // static metadata of the sorted sites: (site, type | cls << 8, mol, packed
@@ -579,10 +613,9 @@ __global__ void k_gather_pos32(Geo g, const int *__restrict__ perm, const int *_
 // src/evalSS.f90:13-125): each receiver (a site central in box b at the last
 // rebuild) gets, from every site of another molecule in the 3^3 block around b,
 // Coulomb xpi q_i q_j r/r^3 plus LJ c r^-6 (s6 r^-6 - 1/2) r/r^2 for r <= rc.
-// One 64-lane wavefront per NEAR_TILE receivers of a box; the 27 source boxes
-// are streamed through LDS tiles; each lane owns one receiver, so forces are
-// written without atomics.  MASKED keeps only the (receiver kind, source kind)
-// pairs of `mask` (one reference kernel or a subset of them).
+// Each lane owns one receiver, so forces are written without atomics.  MASKED
+// keeps only the (receiver kind, source kind) pairs of `mask` (one reference
+// kernel or a subset of them).
 constexpr int NEAR_ROWS = 9;  // (dz, dy) rows of the 3^3 neighbourhood, one wave each
 
 struct Recv {
@@ -602,62 +635,75 @@ struct Recv {
 // molecule with a receiver); elsewhere only sources of split molecules can.
 // HQ: both water hydrogens carry one charge, so q_i q_j is per lane for CAT 2.
 template <int CAT, bool CHECK, bool MASKED, bool CEN, bool HQ>
+__device__ __forceinline__ void near_pair(const Geo &g, const double4 &pj, const int4 &mj, int bx, int by,
+                                          int bz, const Recv &R, unsigned long long mask, double &ax,
+                                          double &ay, double &az)
+{
+    bool ok = mj.z != R.mol;
+    if (CHECK || MASKED) {
+        int cls = (mj.y >> 8) & 3;
+        int ring = max(abs((mj.w & 255) - bx),
+                       max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
+        int sk = cls == 0 ? (ring <= 1 ? ring : -1)
+                          : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
+        if (sk < 0) return;  // outside every list of the reference (uniform)
+        if (MASKED) ok = ok && ((mask >> (max(R.rk, 0) * 9 + sk)) & 1ull);
+    }
+    double dx = R.x - pj.x, dy = R.y - pj.y, dz = R.z - pj.z;
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double rinv = rsqrt64(r2);
+    double rinv2 = rinv * rinv;
+    double qq = CAT == 0 ? R.qO : (CAT == 2 && HQ) ? R.qH : R.q * pj.w;
+    double ft = qq * (rinv2 * rinv);
+    if (CAT != 2) {
+        double c, s6;
+        if (CAT == 0) {
+            c = R.cO;
+            s6 = R.s6O;
+        } else {
+            int tj = mj.y & 255;
+            c = R.ljc[tj];
+            s6 = R.ljs[tj];
+        }
+        double r6 = rinv2 * rinv2 * rinv2;
+        double flj = c * r6 * (s6 * r6 - 0.5) * rinv2;
+        ft = (c != 0.0 && r2 <= g.rc2) ? ft + flj : ft;
+    }
+    if (CHECK || MASKED || CEN)
+        ft = ok ? ft : 0.0;
+    else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
+        __asm__ volatile("");
+        ft = mj.z != R.mol ? ft : 0.0;
+    }
+    ax = fma(dx, ft, ax);
+    ay = fma(dy, ft, ay);
+    az = fma(dz, ft, az);
+}
+
+template <int CAT, bool CHECK, bool MASKED, bool CEN, bool HQ>
 __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__restrict__ spos,
                                              const int4 *__restrict__ smeta, int s0, int s1,
                                              int bx, int by, int bz, const Recv &R,
                                              unsigned long long mask, double &ax, double &ay,
                                              double &az)
 {
-    const double rc2 = g.rc2;
     if (s0 >= s1) return;
-    // one-ahead prefetch: the next record is loaded while this one is used
-    int4 mn = smeta[s0];
-    double4 pn = spos[s0];
-    for (int t = s0; t < s1; t++) {
-        const int4 mj = mn;
-        const double4 pj = pn;
-        int tn = min(t + 1, s1 - 1);
-        mn = smeta[tn];
-        pn = spos[tn];
-        bool ok = mj.z != R.mol;
-        if (CHECK || MASKED) {
-            int cls = (mj.y >> 8) & 3;
-            int ring = max(abs((mj.w & 255) - bx),
-                           max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
-            int sk = cls == 0 ? (ring <= 1 ? ring : -1)
-                              : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
-            if (sk < 0) continue;  // outside every list of the reference (uniform)
-            if (MASKED) ok = ok && ((mask >> (max(R.rk, 0) * 9 + sk)) & 1ull);
-        }
-        double dx = R.x - pj.x, dy = R.y - pj.y, dz = R.z - pj.z;
-        double r2 = dx * dx + dy * dy + dz * dz;
-        double rinv = rsqrt64(r2);
-        double rinv2 = rinv * rinv;
-        double qq = CAT == 0 ? R.qO : (CAT == 2 && HQ) ? R.qH : R.q * pj.w;
-        double ft = qq * (rinv2 * rinv);
-        if (CAT != 2) {
-            double c, s6;
-            if (CAT == 0) {
-                c = R.cO;
-                s6 = R.s6O;
-            } else {
-                int tj = mj.y & 255;
-                c = R.ljc[tj];
-                s6 = R.ljs[tj];
-            }
-            double r6 = rinv2 * rinv2 * rinv2;
-            double flj = c * r6 * (s6 * r6 - 0.5) * rinv2;
-            ft = (c != 0.0 && r2 <= rc2) ? ft + flj : ft;
-        }
-        if (CHECK || MASKED || CEN)
-            ft = ok ? ft : 0.0;
-        else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
-            __asm__ volatile("");
-            ft = mj.z != R.mol ? ft : 0.0;
-        }
-        ax = fma(dx, ft, ax);
-        ay = fma(dy, ft, ay);
-        az = fma(dz, ft, az);
+    // Sources two at a time, the next two loaded while these are used: scalar
+    // loads return out of order, so each use waits for every load in flight;
+    // pairing halves the waits per source.
+    int t1 = min(s0 + 1, s1 - 1);
+    int4 mA = smeta[s0], mB = smeta[t1];
+    double4 pA = spos[s0], pB = spos[t1];
+    for (int t = s0; t < s1; t += 2) {
+        const int4 ma = mA, mb = mB;
+        const double4 pa = pA, pb = pB;
+        int tn0 = min(t + 2, s1 - 1), tn1 = min(t + 3, s1 - 1);
+        mA = smeta[tn0];
+        mB = smeta[tn1];
+        pA = spos[tn0];
+        pB = spos[tn1];
+        near_pair<CAT, CHECK, MASKED, CEN, HQ>(g, pa, ma, bx, by, bz, R, mask, ax, ay, az);
+        if (t + 1 < s1) near_pair<CAT, CHECK, MASKED, CEN, HQ>(g, pb, mb, bx, by, bz, R, mask, ax, ay, az);
     }
 }
 
@@ -701,6 +747,51 @@ struct Recv32 {
 };
 
 template <int CAT, bool CHECK, bool CEN, bool HQ>
+__device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, int bx, int by, int bz,
+                                            float xr, float yr, float zr, const Recv &R,
+                                            const Recv32 &Q, float &fx, float &fy, float &fz)
+{
+    bool ok = mj.z != R.mol;
+    if (CHECK) {
+        int cls = (mj.y >> 8) & 3;
+        int ring = max(abs((mj.w & 255) - bx),
+                       max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
+        int sk = cls == 0 ? (ring <= 1 ? ring : -1)
+                          : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
+        if (sk < 0) return;
+    }
+    float dx = xr - pj.x, dy = yr - pj.y, dz = zr - pj.z;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    float rinv2 = rinv * rinv;
+    float qq = CAT == 0 ? Q.qO : (CAT == 2 && HQ) ? Q.qH : Q.q * pj.w;
+    float ft = qq * (rinv2 * rinv);
+    if (CAT != 2) {
+        float c, s6;
+        if (CAT == 0) {
+            c = Q.cO;
+            s6 = Q.s6O;
+        } else {
+            int tj = mj.y & 255;
+            c = Q.ljc[tj];
+            s6 = Q.ljs[tj];
+        }
+        float r6 = rinv2 * rinv2 * rinv2;
+        float flj = c * r6 * (s6 * r6 - 0.5f) * rinv2;
+        ft = (c != 0.0f && r2 <= Q.rc2) ? ft + flj : ft;
+    }
+    if (CHECK || CEN)
+        ft = ok ? ft : 0.0f;
+    else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
+        __asm__ volatile("");
+        ft = mj.z != R.mol ? ft : 0.0f;
+    }
+    fx = fmaf(dx, ft, fx);
+    fy = fmaf(dy, ft, fy);
+    fz = fmaf(dz, ft, fz);
+}
+
+template <int CAT, bool CHECK, bool CEN, bool HQ>
 __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32,
                                                const int4 *__restrict__ smeta, int s0, int s1,
                                                int bx, int by, int bz, float xr, float yr, float zr,
@@ -709,52 +800,19 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
 {
     if (s0 >= s1) return;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    int4 mn = smeta[s0];
-    float4 pn = spos32[s0];
-    for (int t = s0; t < s1; t++) {
-        const int4 mj = mn;
-        const float4 pj = pn;
-        int tn = min(t + 1, s1 - 1);
-        mn = smeta[tn];
-        pn = spos32[tn];
-        bool ok = mj.z != R.mol;
-        if (CHECK) {
-            int cls = (mj.y >> 8) & 3;
-            int ring = max(abs((mj.w & 255) - bx),
-                           max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
-            int sk = cls == 0 ? (ring <= 1 ? ring : -1)
-                              : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
-            if (sk < 0) continue;
-        }
-        float dx = xr - pj.x, dy = yr - pj.y, dz = zr - pj.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        float rinv = __builtin_amdgcn_rsqf(r2);
-        float rinv2 = rinv * rinv;
-        float qq = CAT == 0 ? Q.qO : (CAT == 2 && HQ) ? Q.qH : Q.q * pj.w;
-        float ft = qq * (rinv2 * rinv);
-        if (CAT != 2) {
-            float c, s6;
-            if (CAT == 0) {
-                c = Q.cO;
-                s6 = Q.s6O;
-            } else {
-                int tj = mj.y & 255;
-                c = Q.ljc[tj];
-                s6 = Q.ljs[tj];
-            }
-            float r6 = rinv2 * rinv2 * rinv2;
-            float flj = c * r6 * (s6 * r6 - 0.5f) * rinv2;
-            ft = (c != 0.0f && r2 <= Q.rc2) ? ft + flj : ft;
-        }
-        if (CHECK || CEN)
-            ft = ok ? ft : 0.0f;
-        else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
-            __asm__ volatile("");
-            ft = mj.z != R.mol ? ft : 0.0f;
-        }
-        fx = fmaf(dx, ft, fx);
-        fy = fmaf(dy, ft, fy);
-        fz = fmaf(dz, ft, fz);
+    int t1 = min(s0 + 1, s1 - 1);
+    int4 mA = smeta[s0], mB = smeta[t1];
+    float4 pA = spos32[s0], pB = spos32[t1];
+    for (int t = s0; t < s1; t += 2) {
+        const int4 ma = mA, mb = mB;
+        const float4 pa = pA, pb = pB;
+        int tn0 = min(t + 2, s1 - 1), tn1 = min(t + 3, s1 - 1);
+        mA = smeta[tn0];
+        mB = smeta[tn1];
+        pA = spos32[tn0];
+        pB = spos32[tn1];
+        near_pair32<CAT, CHECK, CEN, HQ>(pa, ma, bx, by, bz, xr, yr, zr, R, Q, fx, fy, fz);
+        if (t + 1 < s1) near_pair32<CAT, CHECK, CEN, HQ>(pb, mb, bx, by, bz, xr, yr, zr, R, Q, fx, fy, fz);
     }
     ax += (double)fx;
     ay += (double)fy;
@@ -789,25 +847,25 @@ __device__ __forceinline__ void near_row32(const Geo &g, const float4 *__restric
 }
 
 template <bool MASKED, bool HQ, bool MIX>
-__global__ __launch_bounds__(NEAR_TILE * NEAR_ROWS) void k_near(
+__global__ __launch_bounds__(NEAR_TILE) void k_near(
     Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
     const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
     const int *__restrict__ rkind, const int2 *__restrict__ work, const int *__restrict__ nwork,
     const sesmd_forcefield *__restrict__ ff, unsigned long long mask, double *__restrict__ f,
-    const float4 *__restrict__ spos32, const float *__restrict__ lj32)
+    const float4 *__restrict__ spos32, const float *__restrict__ lj32, double *__restrict__ part)
 {
-    // A block owns NEAR_TILE receivers of box b.  Its nine wavefronts take
-    // one (dz, dy) row of the 3^3 neighbourhood each -- the three x-neighbours
-    // of a row are consecutive boxes -- and every lane of a wave walks the same
+    // One 64-lane wavefront per (work item, row): NEAR_TILE receivers of box
+    // b against one (dz, dy) row of the 3^3 neighbourhood -- the three
+    // x-neighbours of a row are consecutive boxes.  Every lane walks the same
     // source sequence, so each source record is a wave-uniform scalar load and
     // the per-source integer work runs on the scalar unit; the vector unit only
-    // does the FP64 pair arithmetic.  Within a box the sources are ordered by LJ
-    // class, so the loops carry no per-source type branch.  The nine partial
-    // forces are added in row order through LDS: no atomics, deterministic.
-    __shared__ double s_part[NEAR_ROWS][3][NEAR_TILE];
-    int item = blockIdx.x;
+    // does the pair arithmetic.  Within a box the sources are ordered by LJ
+    // class, so the loops carry no per-source type branch.  Each wave writes
+    // its partial forces; k_near_sum adds the nine rows in order (no atomics,
+    // deterministic), and no wave waits for the other rows of its item.
+    int item = blockIdx.x / NEAR_ROWS;  // items come longest first (see k_worklist)
     if (item >= *nwork) return;
-    int row = threadIdx.x / NEAR_TILE, lane = threadIdx.x % NEAR_TILE;
+    int row = blockIdx.x % NEAR_ROWS, lane = threadIdx.x;
     int2 w = work[item];
     int b = w.x;
     int kend = bstart[b + 1];
@@ -867,22 +925,31 @@ __global__ __launch_bounds__(NEAR_TILE * NEAR_ROWS) void k_near(
         else
             near_row<false, false, HQ>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, mask, ax, ay, az);
     }
-    s_part[row][0][lane] = ax;
-    s_part[row][1][lane] = ay;
-    s_part[row][2][lane] = az;
-    __syncthreads();
-    if (row == 0 && active) {
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-#pragma unroll
-        for (int q = 0; q < NEAR_ROWS; q++) {
-            sx += s_part[q][0][lane];
-            sy += s_part[q][1][lane];
-            sz += s_part[q][2][lane];
-        }
-        f[site] += sx;
-        f[g.n + site] += sy;
-        f[2 * g.n + site] += sz;
+    if (k < kend) {
+        part[(size_t)(3 * row) * g.n + k] = ax;
+        part[(size_t)(3 * row + 1) * g.n + k] = ay;
+        part[(size_t)(3 * row + 2) * g.n + k] = az;
     }
+}
+
+// Sum of the nine row partials of every receiver, in row order, onto f.
+__global__ void k_near_sum(Geo g, const int4 *__restrict__ smeta, const int *__restrict__ rkind,
+                           const double *__restrict__ part, double *__restrict__ f)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    int site = smeta[k].x;
+    if (rkind[site] < 0) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+    for (int q = 0; q < NEAR_ROWS; q++) {
+        sx += part[(size_t)(3 * q) * g.n + k];
+        sy += part[(size_t)(3 * q + 1) * g.n + k];
+        sz += part[(size_t)(3 * q + 2) * g.n + k];
+    }
+    f[site] += sx;
+    f[g.n + site] += sy;
+    f[2 * g.n + site] += sz;
 }
 
 // pair counts for roofline accounting (unordered pairs, counted from both sides / 2)
@@ -2373,8 +2440,14 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->nvalid, 1));
     c->max_work = g.nb3 + (int)(n / NEAR_TILE) + 1;
     A(dalloc(&c->work, c->max_work));
+    A(dalloc(&c->work_raw, c->max_work));
+    A(dalloc(&c->wkey_in, c->max_work));
+    A(dalloc(&c->wkey_out, c->max_work));
+    A(dalloc(&c->widx_in, c->max_work));
+    A(dalloc(&c->widx_out, c->max_work));
     A(dalloc(&c->nwork, 1));
     A(dalloc(&c->spos, n));
+    A(dalloc(&c->npart, 27 * n));
     A(dalloc(&c->smeta, n));
     if (precision == SESMD_MIXED) {
         A(dalloc(&c->spos32, n));
@@ -2472,6 +2545,10 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     while ((1 << endbit) <= 4 * g.nb3) endbit++;
     hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->key_in, c->key_out, c->val_in, c->perm,
                                        (int)n, 0, endbit, c->stream);
+    size_t wb = 0;
+    hipcub::DeviceRadixSort::SortPairsDescending(nullptr, wb, c->wkey_in, c->wkey_out, c->widx_in,
+                                                 c->widx_out, c->max_work, 0, 32, c->stream);
+    bytes = std::max(bytes, wb);
     c->cub_bytes = bytes;
     if (hipMalloc(&c->cub_tmp, bytes ? bytes : 16) != hipSuccess) {
         sesmd_destroy(c);
@@ -2489,7 +2566,8 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
     if (c->graph_run_len) hipGraphExecDestroy(c->graph_run);
     void *ptrs[] = {c->r, c->v, c->a, c->f, c->type, c->mol, c->first, c->cls, c->anc, c->cbox,
                     c->rkind, c->isub, c->key_in, c->key_out, c->val_in, c->perm, c->cub_tmp,
-                    c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->spos, c->smeta, c->O4, c->L4,
+                    c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->work_raw, c->npart, c->wkey_in,
+                    c->wkey_out, c->widx_in, c->widx_out, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
                     c->dtopo, c->partials, c->nh, c->counters, c->spos32, c->lj32, c->fs};
@@ -2616,7 +2694,12 @@ static int launch_rebuild(sesmd_ctx *c)
                                           c->perm, g.n, 0, endbit, st));
     k_bstart<<<nblk(4 * g.nb3 + 1, 256), 256, 0, st>>>(g, c->key_out, c->bs4, c->bstart);
     k_fill_i<<<nblk(g.nb3, 256), 256, 0, st>>>(c->bflag, g.nb3, 0);
-    k_worklist<<<1, 1024, 0, st>>>(g, c->bstart, c->work, c->nwork, c->max_work);
+    k_worklist<<<1, 1024, 0, st>>>(g, c->bstart, c->work_raw, c->wkey_in, c->widx_in, c->nwork,
+                                   c->max_work);
+    size_t wbytes = c->cub_bytes;
+    HC(hipcub::DeviceRadixSort::SortPairsDescending(c->cub_tmp, wbytes, c->wkey_in, c->wkey_out, c->widx_in,
+                                                    c->widx_out, c->max_work, 0, 32, st));
+    k_work_gather<<<nblk(c->max_work, 256), 256, 0, st>>>(c->work_raw, c->widx_out, c->nwork, c->work);
     k_gather_meta<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->type, c->mol, c->first, c->cls, c->anc,
                                                   c->cbox, c->smeta, c->bflag);
     HC(hipGetLastError());
@@ -2667,10 +2750,10 @@ static void launch_near(sesmd_ctx *c, uint64_t mask)
     const Geo &g = c->g;
     bool hq = c->ff.q[1] == c->ff.q[2];
     bool mix = c->precision == SESMD_MIXED && mask == ~0ull;
-    dim3 grid(c->max_work), block(NEAR_TILE * NEAR_ROWS);
+    dim3 grid(c->max_work * NEAR_ROWS), block(NEAR_TILE);
 #define NEAR_ARGS \
     g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind, c->work, c->nwork, c->dff, mask, c->f, \
-        c->spos32, c->lj32
+        c->spos32, c->lj32, c->npart
     if (mix && hq) k_near<false, true, true><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
     else if (mix) k_near<false, false, true><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
     else if (mask == ~0ull && hq) k_near<false, true, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
@@ -2678,6 +2761,7 @@ static void launch_near(sesmd_ctx *c, uint64_t mask)
     else if (hq) k_near<true, true, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
     else k_near<true, false, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
 #undef NEAR_ARGS
+    k_near_sum<<<nblk(g.n, 256), 256, 0, c->stream>>>(g, c->smeta, c->rkind, c->npart, c->f);
 }
 
 static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask)
