@@ -21,6 +21,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -119,6 +120,7 @@ struct sesmd_ctx {
     double4 *spos;
     int4 *smeta;
     double *npart;   // near-field partial forces: 9 rows x (x, y, z) x N, sorted order
+    int *iperm;      // site -> sorted position
     float4 *spos32;  // mixed precision: position relative to its box centre, charge
     float *lj32;     // mixed precision: lj_c (1024) then lj_s6 (1024) as float
     // far field
@@ -475,6 +477,12 @@ __global__ void k_isub(Geo g, const double *__restrict__ r, const int *__restric
 
 // first sorted position of every (box, class) key and of every box (binary
 // search in the sorted keys)
+__global__ void k_invperm(Geo g, const int *__restrict__ perm, int *__restrict__ iperm)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < g.n) iperm[perm[k]] = k;
+}
+
 __global__ void k_bstart(Geo g, const int *__restrict__ keys, int *__restrict__ bs4,
                          int *__restrict__ bstart)
 {
@@ -932,25 +940,6 @@ __global__ __launch_bounds__(NEAR_TILE) void k_near(
     }
 }
 
-// Sum of the nine row partials of every receiver, in row order, onto f.
-__global__ void k_near_sum(Geo g, const int4 *__restrict__ smeta, const int *__restrict__ rkind,
-                           const double *__restrict__ part, double *__restrict__ f)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= g.n) return;
-    int site = smeta[k].x;
-    if (rkind[site] < 0) return;
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-#pragma unroll
-    for (int q = 0; q < NEAR_ROWS; q++) {
-        sx += part[(size_t)(3 * q) * g.n + k];
-        sy += part[(size_t)(3 * q + 1) * g.n + k];
-        sz += part[(size_t)(3 * q + 2) * g.n + k];
-    }
-    f[site] += sx;
-    f[g.n + site] += sy;
-    f[2 * g.n + site] += sz;
-}
 
 // pair counts for roofline accounting (unordered pairs, counted from both sides / 2)
 __global__ void k_pair_count(Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
@@ -1278,22 +1267,38 @@ __global__ __launch_bounds__(64) void k_l2l(Geo g, int nd_child, const double2 *
 // L2P (src/evalLjk4.f90:38-68, the same formula for every receiver class) and
 // the 1/x^13 walls (src/eval_wall.f90:149-188), one thread per site.
 template <int PP>
+// Per-site terms, and the first write of f: f = (near field: the nine row
+// partials of k_near, in row order) + L2P + wall.  The pair kernels before it
+// leave their results in `part`; the kernels after it add to f.
 __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const double *__restrict__ r,
                              const int *__restrict__ type, const int *__restrict__ cbox,
                              const int *__restrict__ rkind, const double2 *__restrict__ L4,
                              const double *__restrict__ lfac, const sesmd_forcefield *__restrict__ ff,
-                             double *__restrict__ f)
+                             const double *__restrict__ part, const double4 *__restrict__ spos,
+                             const int4 *__restrict__ smeta, double *__restrict__ f)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= g.n) return;
-    double x = r[i], y = r[g.n + i], z = r[2 * g.n + i];
+    // one thread per sorted position k (coalesced reads of the sorted copies)
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    int i = smeta[k].x;
+    double4 pk = spos[k];
+    double x = pk.x, y = pk.y, z = pk.z;
+    double nx = 0.0, ny = 0.0, nz = 0.0;
+    if ((what & SESMD_NEAR) && rkind[i] >= 0) {
+#pragma unroll
+        for (int q = 0; q < NEAR_ROWS; q++) {
+            nx += part[(size_t)(3 * q) * g.n + k];
+            ny += part[(size_t)(3 * q + 1) * g.n + k];
+            nz += part[(size_t)(3 * q + 2) * g.n + k];
+        }
+    }
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if ((what & SESMD_FAR) && rkind[i] >= 0) {
         int b = cbox[i], nb = g.nb;
         constexpr int p = PP;
         double rxpb = x - d_cen4(g, b % nb), rypb = y - d_cen4(g, (b / nb) % nb),
                rzpb = z - d_cen4(g, b / (nb * nb));
-        double rqq = ff->q[type[i]] * g.xpi;
+        double rqq = pk.w * g.xpi;
         double rho2 = rxpb * rxpb + rypb * rypb;
         double rr = sqrt(rho2 + rzpb * rzpb);
         double rp1 = rzpb / rr;
@@ -1379,9 +1384,9 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
         fy += 48.0 / p13(y - h) + 48.0 / p13(h + y);
         fz += 48.0 / p13(z - h) + 48.0 / p13(h + z);
     }
-    f[i] += fx;
-    f[g.n + i] += fy;
-    f[2 * g.n + i] += fz;
+    f[i] = nx + fx;
+    f[g.n + i] = ny + fy;
+    f[2 * g.n + i] = nz + fz;
 }
 
 // ------------------------------------------------------------ intramolecular
@@ -1659,6 +1664,33 @@ __device__ void block_sum_store(double (&v)[NV], double *out)
 // Part A for waters (src/move_w1_a.f90:29-146 / move_w2_a): predictor with
 // the NH friction, SHAKE over the three cyclic constraints, kinetic sums before
 // and after, force zeroing.  One thread per molecule, 256 per block.
+__device__ void nh_update_block(Geo g, const double *__restrict__ partials, int nblocks,
+                                const sesmd_forcefield *__restrict__ ff, double *__restrict__ nh,
+                                int update_eta);
+
+// Sorted-order copies of a site's new position for the next force evaluation
+// (what k_gather_pos / k_gather_pos32 would write), from the integrator.
+struct SortedOut {
+    const int *iperm, *key;
+    const double *q;  // charge per type
+    double4 *spos;
+    float4 *spos32;   // null unless mixed precision
+};
+
+__device__ __forceinline__ void put_sorted(const Geo &g, const SortedOut &so, int site, int type,
+                                           double x, double y, double z)
+{
+    if (!so.spos) return;
+    int k = so.iperm[site];
+    double q = so.q[type];
+    so.spos[k] = make_double4(x, y, z, q);
+    if (so.spos32) {
+        int b = so.key[k] >> 2, nb = g.nb;
+        so.spos32[k] = make_float4((float)(x - d_cen4(g, b % nb)), (float)(y - d_cen4(g, (b / nb) % nb)),
+                                   (float)(z - d_cen4(g, b / (nb * nb))), (float)q);
+    }
+}
+
 __device__ __forceinline__ void water_a(int blk, Geo g, double *__restrict__ r,
                                                       double *__restrict__ v,
                                                       const double *__restrict__ a,
@@ -1667,7 +1699,8 @@ __device__ __forceinline__ void water_a(int blk, Geo g, double *__restrict__ r,
                                                       const sesmd_forcefield *__restrict__ ff,
                                                       const double *__restrict__ nh,
                                                       double *__restrict__ partials,
-                                                      int *__restrict__ counters, unsigned clsmask)
+                                                      int *__restrict__ counters, unsigned clsmask,
+                                                      SortedOut so = SortedOut())
 {
     int m = blk * blockDim.x + threadIdx.x;
     double acc[NH_VALS];
@@ -1725,6 +1758,7 @@ __device__ __forceinline__ void water_a(int blk, Geo g, double *__restrict__ r,
                 v[d * n + base + i] = vx[i][d];
                 f[d * n + base + i] = 0.0;
             }
+            put_sorted(g, so, base + i, i, px[i][0], px[i][1], px[i][2]);
             acc[9 + grp + (i == 0 ? 0 : 1)] +=
                 vx[i][0] * vx[i][0] + vx[i][1] * vx[i][1] + vx[i][2] * vx[i][2];
         }
@@ -1826,7 +1860,7 @@ struct SurfLds {
 __device__ __forceinline__ void surf_a(
     int blk, Geo g, double *__restrict__ r, double *__restrict__ v, const double *__restrict__ a,
     double *__restrict__ f, SurfConst sc, const double *__restrict__ nh,
-    double *__restrict__ partials, int *__restrict__ counters)
+    double *__restrict__ partials, int *__restrict__ counters, SortedOut so = SortedOut())
 {
     __shared__ SurfLds L;
     int t = threadIdx.x;
@@ -1915,6 +1949,13 @@ __device__ __forceinline__ void surf_a(
         }
 #pragma unroll
         for (int q = 0; q < 5; q++) L.rb[q][t] = post[q];  // bond vectors no longer needed
+    }
+    // sorted copies of the new positions, read back once the molecule's
+    // registers are free
+    if (live && so.spos) {
+        int base = 3 * g.nw + 29 * s;
+        for (int i = 0; i < 29; i++)
+            put_sorted(g, so, base + i, 3 + i, r[base + i], r[n + base + i], r[2 * n + base + i]);
     }
     double acc[NH_VALS];
 #pragma unroll
@@ -2008,9 +2049,9 @@ __device__ __forceinline__ void surf_b(
 }
 
 // kinetic sums -> the eta update of src/move_w2_a.f90:145-157 (single block)
-__global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restrict__ partials,
-                                                   int nblocks, const sesmd_forcefield *__restrict__ ff,
-                                                   double *__restrict__ nh, int update_eta)
+__device__ void nh_update_block(Geo g, const double *__restrict__ partials, int nblocks,
+                                const sesmd_forcefield *__restrict__ ff, double *__restrict__ nh,
+                                int update_eta)
 {
     double acc[NH_VALS];
     for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
@@ -2035,6 +2076,13 @@ __global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restri
         for (int c = 0; c < 6; c++) nh[1 + c] = S[c];
         for (int c = 0; c < NH_VALS; c++) nh[16 + c] = tot[c];
     }
+}
+
+__global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restrict__ partials,
+                                                   int nblocks, const sesmd_forcefield *__restrict__ ff,
+                                                   double *__restrict__ nh, int update_eta)
+{
+    nh_update_block(g, partials, nblocks, ff, nh, update_eta);
 }
 
 // Part B for waters (src/move_w1_b.f90:23-123): a = F dt^2/2m, velocity with
@@ -2113,18 +2161,20 @@ __global__ __launch_bounds__(SURF_NT) void k_move_a_surf_fast(Geo g, double *__r
     surf_a(blockIdx.x, g, r, v, a, f, sc, nh, partials, counters);
 }
 
+// Part A of a regular step in one launch: surfactant blocks and water blocks,
+// the new positions also written in sorted order for the force evaluation.
 __global__ __launch_bounds__(256) void k_move_a_fused(Geo g, double *__restrict__ r, double *__restrict__ v,
                                                       const double *__restrict__ a, double *__restrict__ f,
                                                       const int *__restrict__ cls,
                                                       const sesmd_forcefield *__restrict__ ff,
-                                                      SurfConst sc, const double *__restrict__ nh,
+                                                      SurfConst sc, double *__restrict__ nh,
                                                       double *__restrict__ partials,
-                                                      int *__restrict__ counters, int nblk_s)
+                                                      int *__restrict__ counters, int nblk_s, SortedOut so)
 {
     if ((int)blockIdx.x < nblk_s)
-        surf_a(blockIdx.x, g, r, v, a, f, sc, nh, partials, counters);
+        surf_a(blockIdx.x, g, r, v, a, f, sc, nh, partials, counters, so);
     else
-        water_a(blockIdx.x - nblk_s, g, r, v, a, f, cls, ff, nh, partials, counters, 7u);
+        water_a(blockIdx.x - nblk_s, g, r, v, a, f, cls, ff, nh, partials, counters, 7u, so);
 }
 
 __global__ __launch_bounds__(256) void k_move_b_water(Geo g, const double *__restrict__ r,
@@ -2448,6 +2498,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->nwork, 1));
     A(dalloc(&c->spos, n));
     A(dalloc(&c->npart, 27 * n));
+    A(dalloc(&c->iperm, n));
     A(dalloc(&c->smeta, n));
     if (precision == SESMD_MIXED) {
         A(dalloc(&c->spos32, n));
@@ -2566,7 +2617,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
     if (c->graph_run_len) hipGraphExecDestroy(c->graph_run);
     void *ptrs[] = {c->r, c->v, c->a, c->f, c->type, c->mol, c->first, c->cls, c->anc, c->cbox,
                     c->rkind, c->isub, c->key_in, c->key_out, c->val_in, c->perm, c->cub_tmp,
-                    c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->work_raw, c->npart, c->wkey_in,
+                    c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->work_raw, c->npart, c->iperm, c->wkey_in,
                     c->wkey_out, c->widx_in, c->widx_out, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
@@ -2693,6 +2744,7 @@ static int launch_rebuild(sesmd_ctx *c)
     HC(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, bytes, c->key_in, c->key_out, c->val_in,
                                           c->perm, g.n, 0, endbit, st));
     k_bstart<<<nblk(4 * g.nb3 + 1, 256), 256, 0, st>>>(g, c->key_out, c->bs4, c->bstart);
+    k_invperm<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->iperm);
     k_fill_i<<<nblk(g.nb3, 256), 256, 0, st>>>(c->bflag, g.nb3, 0);
     k_worklist<<<1, 1024, 0, st>>>(g, c->bstart, c->work_raw, c->wkey_in, c->widx_in, c->nwork,
                                    c->max_work);
@@ -2761,24 +2813,23 @@ static void launch_near(sesmd_ctx *c, uint64_t mask)
     else if (hq) k_near<true, true, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
     else k_near<true, false, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
 #undef NEAR_ARGS
-    k_near_sum<<<nblk(g.n, 256), 256, 0, c->stream>>>(g, c->smeta, c->rkind, c->npart, c->f);
 }
 
-static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask)
+// gathered: the sorted copies of the positions are already current (written
+// by the fused part A of a regular step)
+static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask, bool gathered = false)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
-    size_t n3 = 3 * (size_t)g.n;
-    k_fill_d<<<nblk(n3, 256), 256, 0, st>>>(c->f, n3, 0.0);
-    if (what & (SESMD_NEAR | SESMD_FAR | SESMD_GRAPHITE))
+    if (!gathered)  // the site terms read the sorted copies too
         k_gather_pos<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->r, c->type, c->dff, c->spos);
-    if ((what & SESMD_NEAR) && c->precision == SESMD_MIXED && mask == ~0ull)
+    if ((what & SESMD_NEAR) && c->precision == SESMD_MIXED && mask == ~0ull && !gathered)
         k_gather_pos32<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->key_out, c->r, c->type, c->dff,
                                                        c->spos32);
     if (what & SESMD_NEAR) launch_near(c, mask);
-    if (what & (SESMD_FAR | SESMD_WALL))
-        (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<nblk(g.n, 128), 128, 0, st>>>(g, what, c->r, c->type, c->cbox, c->rkind, c->L4,
-                                                     c->lfac, c->dff, c->f);
+    (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<nblk(g.n, 128), 128, 0, st>>>(
+        g, what, c->r, c->type, c->cbox, c->rkind, c->L4, c->lfac, c->dff, c->npart, c->spos, c->smeta,
+        c->f);
     if (g.ns && (what & (SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL |
                          SESMD_DIHEDRAL_BE)))
         k_intra<<<nblk(29 * g.ns, 64), 64, 0, st>>>(g, what, c->r, c->isub, c->dff, c->dtopo, c->f,
@@ -2902,10 +2953,20 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream, ss = c->side;
+    bool gathered = false;
     if (c->surf_fast) {
+        SortedOut so;
+        so.iperm = c->iperm;
+        so.key = c->key_out;
+        so.q = reinterpret_cast<const double *>(reinterpret_cast<const char *>(c->dff) +
+                                                offsetof(sesmd_forcefield, q));  // device table
+        so.spos = rebuild ? nullptr : c->spos;  // a rebuild re-sorts: gathered afterwards
+        so.spos32 = rebuild ? nullptr : c->spos32;
+        gathered = !rebuild;
         k_move_a_fused<<<c->nblk_s + c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->cls, c->dff,
                                                               c->sconst, c->nh, c->partials, c->counters,
-                                                              c->nblk_s);
+                                                              c->nblk_s, so);
+        k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1);
     } else {
         if (c->nblk_w)
             k_move_a_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->cls, c->dff, c->nh,
@@ -2913,8 +2974,8 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
         if (c->nblk_s)
             k_move_a_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->partials,
                                                     c->counters);
+        k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1);
     }
-    k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1);
     HC(hipGetLastError());
     if (rebuild) {
         CK(launch_rebuild(c));
@@ -2930,7 +2991,7 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
     }
     CK(launch_forces(c, SESMD_ALL & ~(SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE |
                                        SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE),
-                     ~0ull));
+                     ~0ull, gathered));
     if (g.ns) HC(hipStreamWaitEvent(st, c->ev[1], 0));
     if (c->surf_fast) {
         k_move_b_fused<<<c->nblk_s + c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->fs, c->dff,
