@@ -21,6 +21,8 @@ FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', '--offload-arch=' + ARCH,
 
 
 def build(verbose=False):
+    from surfactant_enhanced_spreading_amd.csrc import gen_ff_header
+    gen_ff_header.write()
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     src = os.path.join(HERE, 'sesmd.hip')
     tmp = OUT + '.tmp'

@@ -12,8 +12,9 @@
 //   near field      spos[N] (double4 x y z q, sorted), smeta[N] (int4), work list
 //   far field       O4/L4 [B][lp], O3/L3, O2/L2 as double2; translation operators
 //
-// Every kernel launches on the context's one stream; nothing synchronises the
-// host inside a step, so sesmd_steps replays the regular step from a HIP graph.
+// Kernels launch on the context's stream (and, inside a step, a second stream
+// for the surfactant intramolecular forces); nothing synchronises the host
+// inside a step, so sesmd_steps replays runs of regular steps from HIP graphs.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -1905,21 +1906,32 @@ __device__ __forceinline__ void surf_a(
         for (int q = 0; q < 5; q++) L.pre[q][t] = pre[q];
         int it = 0;
         bool done = false;
+        // bond c + 1's vector and constants are loaded while bond c is worked on
+        double nrx = RB[t], nry = RB[SURF_NT + t], nrz = RB[2 * SURF_NT + t];
+        double nd = K[0], nt = K[28], nk = K[56], nma = K[84], nmb = K[112];
         while (!done && it <= 100) {
             done = true;
 #pragma unroll
             for (int c = 0; c < 28; c++) {
-                __asm__ volatile("" ::: "memory");
                 const int ia = kBI[c], ib = kBJ[c];
+                double rxab = nrx, ryab = nry, rzab = nrz, dsq = nd, tolc = nt, k2 = nk, rma = nma,
+                       rmb = nmb;
+                __asm__ volatile("" ::: "memory");
+                const int cn = c + 1 < 28 ? c + 1 : 0;
+                nrx = RB[(3 * cn) * SURF_NT + t];
+                nry = RB[(3 * cn + 1) * SURF_NT + t];
+                nrz = RB[(3 * cn + 2) * SURF_NT + t];
+                nd = K[cn];
+                nt = K[28 + cn];
+                nk = K[56 + cn];
+                nma = K[84 + cn];
+                nmb = K[112 + cn];
                 double pxab = p[ia][0] - p[ib][0], pyab = p[ia][1] - p[ib][1], pzab = p[ia][2] - p[ib][2];
                 double pabsq = pxab * pxab + pyab * pyab + pzab * pzab;
-                double diffsq = K[c] - pabsq;
-                if (fabs(diffsq) > K[28 + c]) {
-                    double rxab = RB[(3 * c) * SURF_NT + t], ryab = RB[(3 * c + 1) * SURF_NT + t],
-                           rzab = RB[(3 * c + 2) * SURF_NT + t];
+                double diffsq = dsq - pabsq;
+                if (fabs(diffsq) > tolc) {
                     double rpab = rxab * pxab + ryab * pyab + rzab * pzab;
-                    double rma = K[84 + c], rmb = K[112 + c];
-                    double gab = diffsq / (K[56 + c] * rpab);
+                    double gab = diffsq / (k2 * rpab);
                     double dx = rxab * gab, dy = ryab * gab, dz = rzab * gab;
                     p[ia][0] = p[ia][0] + rma * dx;
                     p[ia][1] = p[ia][1] + rma * dy;
@@ -2015,23 +2027,31 @@ __device__ __forceinline__ void surf_b(
         }
     int it = 0;
     bool done = false;
+    // bond c + 1's vector and constants are loaded while bond c is worked on
+    double nrx = RB[t], nry = RB[SURF_NT + t], nrz = RB[2 * SURF_NT + t];
+    double nk = K[0], nwa = K[28], nwb = K[56];
     while (!done && it <= 100) {
         done = true;
 #pragma unroll
         for (int c = 0; c < 28; c++) {
             // gab = -(r_ab . v_ab / dt) / ((1/m_a + 1/m_b) d^2), the two
             // divisions folded into one factor per bond (within 2 ulp)
-            __asm__ volatile("" ::: "memory");
             const int ia = kBI[c], ib = kBJ[c];
+            double rxab = nrx, ryab = nry, rzab = nrz, kc = nk, wa = nwa, wb = nwb;
+            __asm__ volatile("" ::: "memory");
+            const int cn = c + 1 < 28 ? c + 1 : 0;
+            nrx = RB[(3 * cn) * SURF_NT + t];
+            nry = RB[(3 * cn + 1) * SURF_NT + t];
+            nrz = RB[(3 * cn + 2) * SURF_NT + t];
+            nk = K[cn];
+            nwa = K[28 + cn];
+            nwb = K[56 + cn];
             double vxab = vx[ia][0] - vx[ib][0], vyab = vx[ia][1] - vx[ib][1], vzab = vx[ia][2] - vx[ib][2];
-            double rxab = RB[(3 * c) * SURF_NT + t], ryab = RB[(3 * c + 1) * SURF_NT + t],
-                   rzab = RB[(3 * c + 2) * SURF_NT + t];
-            double gab = (rxab * vxab + ryab * vyab + rzab * vzab) * K[c];
+            double gab = (rxab * vxab + ryab * vyab + rzab * vzab) * kc;
             bool app = fabs(gab) > 0.0001;
             gab = app ? gab : 0.0;
             done = done && !app;
             double dx = rxab * gab, dy = ryab * gab, dz = rzab * gab;
-            double wa = K[28 + c], wb = K[56 + c];
             vx[ia][0] = vx[ia][0] + wa * dx;
             vx[ia][1] = vx[ia][1] + wa * dy;
             vx[ia][2] = vx[ia][2] + wa * dz;
