@@ -1098,15 +1098,29 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 // Dense contraction out_jk (+)= sum_c Op[jk][c] in_c for k >= 0 rows, then the
 // k < 0 half by conjugation.  One block of 64 threads per target; lane jk
 // (< lk) accumulates; used by M2M (8 children), M2L (interaction list) and L2L.
+// acc += sum_c op(row, c) in[c], c in order; the loads of a chunk of CH terms
+// are issued together (one memory round trip per chunk, not per term).
+template <int CH = 8>
 __device__ void contract_rows(const Geo &g, const double2 *__restrict__ op, const double2 *__restrict__ in,
                               double2 &acc, int row)
 {
-    int lp = g.lp;
-    const double2 *o = op + (size_t)row * lp;
-    for (int c = 0; c < lp; c++) {
-        double2 t = cmul(o[c], in[c]);
-        acc.x += t.x;
-        acc.y += t.y;
+    int lp = g.lp, lk = g.lk;
+    const double2 *o = op + row;  // column-major: element (row, c) at c * lk + row
+    for (int c0 = 0; c0 < lp; c0 += CH) {
+        double2 ov[CH], iv[CH];
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (c0 + k < lp) {
+                ov[k] = o[(size_t)(c0 + k) * lk];
+                iv[k] = in[c0 + k];
+            }
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (c0 + k < lp) {
+                double2 t = cmul(ov[k], iv[k]);
+                acc.x += t.x;
+                acc.y += t.y;
+            }
     }
 }
 
@@ -1160,54 +1174,61 @@ __global__ __launch_bounds__(512) void k_m2m(Geo g, int nd_parent, const double2
 // children of the parent's neighbours that are not neighbours of the target.
 // accumulate = the table already holds the L2L part (shiftLjk zeroes, then
 // computeLjk accumulates).
-__global__ __launch_bounds__(256) void k_m2l(Geo g, int nd, const double2 *__restrict__ O,
-                                             const int *__restrict__ ne, const double2 *__restrict__ T,
-                                             double2 *__restrict__ L)
+// The targets of one parity class (t mod 2 per axis) share their candidate
+// source offsets: d in [-2 - p, 3 - p]^3 per axis (the children of the
+// parent's neighbours), less the 27 near neighbours.  A wave holds TPW = 64 / R
+// such targets, a 2 x 2 tile in (x, y) when R = 16, R lanes each (one per
+// k >= 0 row): at every offset the lanes of all its targets read the same
+// operator, so one load instruction serves them all.  The W waves of a block
+// take the 216 candidate offsets in turn, and their partial sums are added in
+// wave order: deterministic.
+template <int R, int W>
+__global__ __launch_bounds__(64 * W) void k_m2l(Geo g, int nd, const double2 *__restrict__ O,
+                                               const int *__restrict__ ne, const double2 *__restrict__ T,
+                                               double2 *__restrict__ L)
 {
-    // 16 lanes per source group (one per k >= 0 row), 16 source groups; the
-    // partial sums are added in group order, so the result is deterministic.
-    __shared__ double2 s_acc[16][16];
-    int Tb = blockIdx.x;
-    int row = threadIdx.x & 15, grp = threadIdx.x >> 4;
-    int tx = Tb % nd, ty = (Tb / nd) % nd, tz = Tb / (nd * nd);
-    int lo[3], hi[3], t[3] = {tx, ty, tz};
-    for (int a = 0; a < 3; a++) {
-        int par = t[a] / 2;
-        lo[a] = max(2 * (par - 1), 0);
-        hi[a] = min(2 * (par + 1) + 1, nd - 1);
-    }
+    constexpr int TPW = 64 / R, TX = TPW == 4 ? 2 : 1, TY = TPW == 4 ? 2 : 1;
+    __shared__ double2 s_acc[W][64];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int row = lane % R, j = lane / R;
+    int h = nd / 2, qx = (h + TX - 1) / TX, qy = (h + TY - 1) / TY;
+    int b = blockIdx.x;
+    int par = b & 7;
+    b >>= 3;
+    int bx = b % qx, by = (b / qx) % qy, hz = b / (qx * qy);
+    int px = par & 1, py = (par >> 1) & 1, pz = par >> 2;
+    int hx = bx * TX + j % TX, hy = by * TY + j / TX;
+    bool valid = hx < h && hy < h && row < g.lk;
+    int tx = 2 * hx + px, ty = 2 * hy + py, tz = 2 * hz + pz;
     double2 acc = make_double2(0.0, 0.0);
-    if (row < g.lk) {
-        int ord = 0;
-        for (int sz = lo[2]; sz <= hi[2]; sz++)
-            for (int sy = lo[1]; sy <= hi[1]; sy++)
-                for (int sx = lo[0]; sx <= hi[0]; sx++) {
-                    int dx = sx - tx, dy = sy - ty, dz = sz - tz;
-                    if (iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1) continue;
-                    int S = sx + nd * (sy + nd * sz);
-                    if (!ne[S]) continue;
-                    if ((ord++ & 15) != grp) continue;
-                    int off = (dx + 3) + 7 * ((dy + 3) + 7 * (dz + 3));
-                    contract_rows(g, T + (size_t)off * g.lk * g.lp, O + (size_t)S * g.lp, acc, row);
-                }
+    for (int i = w; i < 216; i += W) {
+        int dx = i % 6 - 2 - px, dy = (i / 6) % 6 - 2 - py, dz = i / 36 - 2 - pz;
+        if (iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1) continue;  // wave-uniform
+        int sx = tx + dx, sy = ty + dy, sz = tz + dz;
+        if (!valid || sx < 0 || sx >= nd || sy < 0 || sy >= nd || sz < 0 || sz >= nd) continue;
+        int S = sx + nd * (sy + nd * sz);
+        if (!ne[S]) continue;
+        int off = (dx + 3) + 7 * ((dy + 3) + 7 * (dz + 3));
+        contract_rows(g, T + (size_t)off * g.lk * g.lp, O + (size_t)S * g.lp, acc, row);
     }
-    s_acc[grp][row] = acc;
+    s_acc[w][lane] = acc;
     __syncthreads();
-    if (grp != 0 || row >= g.lk) return;
-    for (int q = 1; q < 16; q++) {
-        acc.x += s_acc[q][row].x;
-        acc.y += s_acc[q][row].y;
+    if (w != 0 || !valid) return;
+    acc = s_acc[0][lane];
+    for (int q = 1; q < W; q++) {
+        acc.x += s_acc[q][lane].x;
+        acc.y += s_acc[q][lane].y;
     }
     // add onto the L2L part (row k >= 0), rebuild the k < 0 half
-    int j = 0;
-    while ((j + 1) * (j + 2) / 2 <= row) j++;
-    int k = row - j * (j + 1) / 2;
-    double2 *Lt = L + (size_t)Tb * g.lp;
-    double2 cur = Lt[j * (j + 1) + k];
+    int jj = 0;
+    while ((jj + 1) * (jj + 2) / 2 <= row) jj++;
+    int k = row - jj * (jj + 1) / 2;
+    double2 *Lt = L + (size_t)(tx + nd * (ty + nd * tz)) * g.lp;
+    double2 cur = Lt[jj * (jj + 1) + k];
     cur.x += acc.x;
     cur.y += acc.y;
-    Lt[j * (j + 1) + k] = cur;
-    if (k > 0) Lt[j * (j + 1) - k] = make_double2(cur.x, -cur.y);
+    Lt[jj * (jj + 1) + k] = cur;
+    if (k > 0) Lt[jj * (jj + 1) - k] = make_double2(cur.x, -cur.y);
 }
 
 // top level (computeLjk2): every L2 box that is not adjacent to the target
@@ -2368,8 +2389,14 @@ static int upload_ops(sesmd_ctx *c)
         HC(hipMemcpy(dst, buf.data(), sizeof(double2) * count, hipMemcpyHostToDevice));
         return 0;
     };
+    // operators are stored column-major (element (row, c) at c * lk + row), so
+    // the lanes of a contraction (one row each) read consecutive elements
     auto append = [&]() {
-        for (auto &z : T) buf.push_back(make_double2(z.real(), z.imag()));
+        for (int cc = 0; cc < lp; cc++)
+            for (int rr = 0; rr < lk; rr++) {
+                const hostm::C &z = T[(size_t)rr * lp + cc];
+                buf.push_back(make_double2(z.real(), z.imag()));
+            }
     };
     // M2L at L4 and L3: offsets in [-3, 3]^3 (spacing rs and 2 rs)
     for (int lvl = 0; lvl < 2; lvl++) {
@@ -2787,6 +2814,20 @@ extern "C" int sesmd_rebuild(sesmd_ctx *c)
     return 0;
 }
 
+// M2L over the interaction lists of one level (computeLjk3 / computeLjk4);
+// nd is even at every level (each level halves the one below it)
+static void launch_m2l(sesmd_ctx *c, int nd, const double2 *O, const int *ne, const double2 *T, double2 *L)
+{
+    const Geo &g = c->g;
+    int h = nd / 2;
+    if (g.lk <= 16) {
+        int nblk = 8 * ((h + 1) / 2) * ((h + 1) / 2) * h;
+        k_m2l<16, 16><<<nblk, 64 * 16, 0, c->stream>>>(g, nd, O, ne, T, L);
+    } else {
+        k_m2l<64, 8><<<8 * h * h * h, 64 * 8, 0, c->stream>>>(g, nd, O, ne, T, L);
+    }
+}
+
 static int launch_fmm(sesmd_ctx *c)
 {
     const Geo &g = c->g;
@@ -2800,9 +2841,9 @@ static int launch_fmm(sesmd_ctx *c)
     k_m2l_top_pair<<<g.n2 * g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->top_part);
     k_m2l_top_sum<<<g.n2, 64, 0, st>>>(g, c->top_part, c->L2);
     k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3);
-    k_m2l<<<g.n33, 256, 0, st>>>(g, g.n3, c->O3, c->ne3, c->T3, c->L3);
+    launch_m2l(c, g.n3, c->O3, c->ne3, c->T3, c->L3);
     k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4);
-    k_m2l<<<g.nb3, 256, 0, st>>>(g, g.nb, c->O4, c->ne4, c->T4, c->L4);
+    launch_m2l(c, g.nb, c->O4, c->ne4, c->T4, c->L4);
     HC(hipGetLastError());
     return 0;
 }
@@ -3232,9 +3273,9 @@ extern "C" int sesmd_fmm_stage(sesmd_ctx *c, int stage)
         k_m2l_top_sum<<<g.n2, 64, 0, st>>>(g, c->top_part, c->L2);
         break;
     case SESMD_L2L_23: k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3); break;
-    case SESMD_M2L_3: k_m2l<<<g.n33, 256, 0, st>>>(g, g.n3, c->O3, c->ne3, c->T3, c->L3); break;
+    case SESMD_M2L_3: launch_m2l(c, g.n3, c->O3, c->ne3, c->T3, c->L3); break;
     case SESMD_L2L_34: k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4); break;
-    case SESMD_M2L_4: k_m2l<<<g.nb3, 256, 0, st>>>(g, g.nb, c->O4, c->ne4, c->T4, c->L4); break;
+    case SESMD_M2L_4: launch_m2l(c, g.nb, c->O4, c->ne4, c->T4, c->L4); break;
     default: return fail("unknown far-field stage %d", stage);
     }
     HC(hipGetLastError());

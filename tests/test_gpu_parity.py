@@ -140,3 +140,30 @@ def test_steps_pipeline(which, sys_small, sys_medium):
         assert e.shake_failures() == o.shake_failures()
     finally:
         e.close()
+
+
+@pytest.mark.parametrize('nb2,p', [(3, 8), (4, 8), (4, 4)])
+def test_other_orders_and_domains(nb2, p, sys_small):
+    """The engine beyond the reference's own p = 4, 27-rank geometry: expansion
+    order 8 and a 16^3-box domain (nb2 = 4), against the oracle run with the
+    same parameters -- classes bit-exact, expansions 1e-12, total force 1e-10."""
+    import oracle as orc
+    from surfactant_enhanced_spreading_amd import engine
+    o = orc.Oracle(sys_small, nb2=nb2, p=p)
+    o.rebuild()
+    o.fmm()
+    e = engine.Engine(sys_small, nb2=nb2, p=p)
+    try:
+        e.rebuild()
+        e.fmm()
+        a, b = o.classes(), e.classes()
+        for k in ('cls', 'anc', 'cbox', 'rkind', 'isub'):
+            assert np.array_equal(a[k], b[k]), k
+        A, B = o.expansions(), e.expansions()
+        for k in ('O4', 'O3', 'O2', 'L2', 'L3', 'L4'):
+            scale = np.abs(A[k]).max()
+            assert scale > 0, k
+            assert np.abs(A[k] - B[k]).max() <= 1e-12 * scale, k
+        assert rel_err(e.forces(), o.forces()) <= TOL
+    finally:
+        e.close()
