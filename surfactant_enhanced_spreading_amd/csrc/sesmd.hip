@@ -1860,233 +1860,221 @@ __global__ __launch_bounds__(64) void k_move_a_surf(Geo g, double *__restrict__ 
                                       (size_t)blockIdx.x * NH_VALS);
 }
 
-constexpr int SURF_NT = 32;  // molecules (threads) per block
+constexpr int SURF_MPB = 4;  // molecules per 256-thread block, one wavefront each
 
-// Per-bond constants of the unrolled surfactant SHAKE/RATTLE, staged in LDS and
-// read inside the iteration (an empty asm statement with a memory clobber per
-// bond keeps the compiler from hoisting the loads): kept in registers they
-// would crowd out the molecule's 87 coordinates and spill.
-struct SurfLds {
-    double rb[84][SURF_NT];  // bond vectors r_a - r_b of the old positions, per molecule
-    double k[5][28];         // A: d^2, d^2 tol2, 2 (1/m_a + 1/m_b), 1/m_a, 1/m_b
-                             // B: -1/(dt (1/m_a+1/m_b) d^2), dt/m_a, dt/m_b
-    double pre[5][SURF_NT];  // A: kinetic group sums before SHAKE
+// Pipelined Gauss-Seidel.  Bond c of sweep s runs at step kOff[c] + 4 s, one
+// lane per bond.  For any two bonds j < k that share a site,
+// 0 < kOff[k] - kOff[j] < 4, so (i) they never run in the same step, (ii) in a
+// sweep j runs before k, and (iii) k of sweep s runs before j of sweep s + 1:
+// every site receives its corrections in exactly the sequential order of
+// src/move_surfactant_a.f90:70-118 (bonds that share no site commute exactly),
+// while up to six sweeps are in flight.  A sweep ends at step kEnd + 4 s; if
+// none of its bonds corrected, the sequential loop would stop there, and the
+// bonds already run for later sweeps saw unchanged positions, so they made no
+// correction either.
+constexpr int kOff[28] = {0,  1,  2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12, 13,
+                          14, 15, 16, 17, 18, 19, 20, 21, 22, 19, 20, 21, 22, 23};
+constexpr int kEnd = 23;
+
+constexpr bool bond_schedule_ok()
+{
+    for (int j = 0; j < 28; j++)
+        for (int k = j + 1; k < 28; k++) {
+            bool share = kBI[j] == kBI[k] || kBI[j] == kBJ[k] || kBJ[j] == kBI[k] || kBJ[j] == kBJ[k];
+            if (share && !(kOff[k] > kOff[j] && kOff[k] - kOff[j] < 4)) return false;
+        }
+    for (int k = 0; k < 28; k++)
+        if (kOff[k] < 0 || kOff[k] > kEnd) return false;
+    return true;
+}
+static_assert(bond_schedule_ok(), "bond schedule breaks the sequential update order");
+
+__device__ __constant__ int c_bi[28] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17,
+                                        17, 19, 20, 20, 20, 17, 24, 25, 25, 25};
+__device__ __constant__ int c_bj[28] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18,
+                                        19, 20, 21, 22, 23, 24, 25, 26, 27, 28};
+__device__ __constant__ int c_boff[28] = {0,  1,  2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12, 13,
+                                          14, 15, 16, 17, 18, 19, 20, 21, 22, 19, 20, 21, 22, 23};
+
+// The molecule's 29 sites (x, y, z, pad) in LDS, shared by its wavefront's lanes.
+struct SurfMolLds {
+    double p[SURF_MPB][29][4];  // positions being constrained (A) / velocities (B)
+    double r[SURF_MPB][29][4];  // positions that define the bond vectors
 };
 
-// Part A for surfactants (src/move_surfactant_a.f90:19-147), one thread per
-// molecule: predicted positions p in registers; constraint order, tolerance
-// test and position updates exactly as the reference.  The reference applies
+// Runs the pipelined sweeps for one molecule (wave-uniform call).  `upd`
+// applies bond c's correction to P[ia], P[ib] and returns whether it corrected;
+// returns whether the iteration converged within `smax` sweeps.
+template <typename Upd>
+__device__ __forceinline__ bool pipelined_sweeps(bool bond, int off, int smax, Upd upd)
+{
+    unsigned hist = 0;  // bit (s & 7): this bond corrected in sweep s
+    for (int T = 0;; T++) {
+        int u = T - off;
+        if (bond && u >= 0 && (u & 3) == 0 && (u >> 2) < smax) {
+            int bit = (u >> 2) & 7;
+            bool corr = upd();
+            hist = (hist & ~(1u << bit)) | ((unsigned)corr << bit);
+        }
+        // the next step's LDS reads must see this step's writes (one
+        // wavefront: its LDS operations complete in order)
+        __asm__ volatile("" ::: "memory");
+        int ue = T - kEnd;
+        if (ue >= 0 && (ue & 3) == 0) {
+            int se = ue >> 2;
+            bool any = __ballot(bond && ((hist >> (se & 7)) & 1u)) != 0;
+            if (!any) return true;
+            if (se + 1 >= smax) return false;
+        }
+    }
+}
+
+// Part A for surfactants (src/move_surfactant_a.f90:19-147), one wavefront per
+// molecule: lane i < 29 predicts site i, lane c < 28 owns bond c (old bond
+// vector and constants in registers) in the pipelined SHAKE; constraint order,
+// tolerance test and position updates as the reference.  The reference applies
 // every correction to p and v alike, so v = p - r_old at the end (equal up to
-// the rounding of |r| ~ 20 sigma, ~1e-12 of v dt) instead of a second set of
-// updates.
+// the rounding of |r| ~ 20 sigma, ~1e-12 of v dt).
 __device__ __forceinline__ void surf_a(
     int blk, Geo g, double *__restrict__ r, double *__restrict__ v, const double *__restrict__ a,
     double *__restrict__ f, SurfConst sc, const double *__restrict__ nh,
     double *__restrict__ partials, int *__restrict__ counters, SortedOut so = SortedOut())
 {
-    __shared__ SurfLds L;
-    int t = threadIdx.x;
-    for (int c = t; c < 28; c += SURF_NT) {
-        L.k[0][c] = sc.dsq[c];
-        L.k[1][c] = sc.dsq[c] * (2.0 * 0.0001);
-        L.k[2][c] = sc.k2[c];
-        L.k[3][c] = sc.rm[kBI[c]];
-        L.k[4][c] = sc.rm[kBJ[c]];
-    }
-    __syncthreads();
-    const double *K = &L.k[0][0];
-    const double *RB = &L.rb[0][0];
-    int s = blk * SURF_NT + t;
-    int n = g.n;
-    bool live = t < SURF_NT && s < g.ns;
-    if (live) {
-        double escale = 1.0 - nh[0] * g.dt / 2.0;
-        int base = 3 * g.nw + 29 * s;
-        double p[29][3];
-#pragma unroll
-        for (int i = 0; i < 29; i++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) p[i][d] = r[d * n + base + i];
-#pragma unroll
-        for (int c = 0; c < 28; c++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) L.rb[3 * c + d][t] = p[kBI[c]][d] - p[kBJ[c]][d];
-        double pre[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int i = 0; i < 29; i++) {
-            double vv = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                double v0 = v[d * n + base + i], aa = a[d * n + base + i];
-                p[i][d] = p[i][d] + v0 * escale + aa;
-                vv += v0 * v0;
-            }
-            pre[c_sgrp[i]] += vv;
-        }
-#pragma unroll
-        for (int q = 0; q < 5; q++) L.pre[q][t] = pre[q];
-        int it = 0;
-        bool done = false;
-        // bond c + 1's vector and constants are loaded while bond c is worked on
-        double nrx = RB[t], nry = RB[SURF_NT + t], nrz = RB[2 * SURF_NT + t];
-        double nd = K[0], nt = K[28], nk = K[56], nma = K[84], nmb = K[112];
-        while (!done && it <= 100) {
-            done = true;
-#pragma unroll
-            for (int c = 0; c < 28; c++) {
-                const int ia = kBI[c], ib = kBJ[c];
-                double rxab = nrx, ryab = nry, rzab = nrz, dsq = nd, tolc = nt, k2 = nk, rma = nma,
-                       rmb = nmb;
-                __asm__ volatile("" ::: "memory");
-                const int cn = c + 1 < 28 ? c + 1 : 0;
-                nrx = RB[(3 * cn) * SURF_NT + t];
-                nry = RB[(3 * cn + 1) * SURF_NT + t];
-                nrz = RB[(3 * cn + 2) * SURF_NT + t];
-                nd = K[cn];
-                nt = K[28 + cn];
-                nk = K[56 + cn];
-                nma = K[84 + cn];
-                nmb = K[112 + cn];
-                double pxab = p[ia][0] - p[ib][0], pyab = p[ia][1] - p[ib][1], pzab = p[ia][2] - p[ib][2];
-                double pabsq = pxab * pxab + pyab * pyab + pzab * pzab;
-                double diffsq = dsq - pabsq;
-                if (fabs(diffsq) > tolc) {
-                    double rpab = rxab * pxab + ryab * pyab + rzab * pzab;
-                    double gab = diffsq / (k2 * rpab);
-                    double dx = rxab * gab, dy = ryab * gab, dz = rzab * gab;
-                    p[ia][0] = p[ia][0] + rma * dx;
-                    p[ia][1] = p[ia][1] + rma * dy;
-                    p[ia][2] = p[ia][2] + rma * dz;
-                    p[ib][0] = p[ib][0] - rmb * dx;
-                    p[ib][1] = p[ib][1] - rmb * dy;
-                    p[ib][2] = p[ib][2] - rmb * dz;
-                    done = false;
-                }
-            }
-            it++;
-        }
-        if (!done) atomicAdd(&counters[0], 1);
-        double post[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int i = 0; i < 29; i++) {
-            double vv = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                double vn = p[i][d] - r[d * n + base + i];
-                r[d * n + base + i] = p[i][d];
-                v[d * n + base + i] = vn;
-                f[d * n + base + i] = 0.0;
-                vv += vn * vn;
-            }
-            post[c_sgrp[i]] += vv;
-        }
-#pragma unroll
-        for (int q = 0; q < 5; q++) L.rb[q][t] = post[q];  // bond vectors no longer needed
-    }
-    // sorted copies of the new positions, read back once the molecule's
-    // registers are free
-    if (live && so.spos) {
-        int base = 3 * g.nw + 29 * s;
-        for (int i = 0; i < 29; i++)
-            put_sorted(g, so, base + i, 3 + i, r[base + i], r[n + base + i], r[2 * n + base + i]);
-    }
+    __shared__ SurfMolLds L;
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int s = blk * SURF_MPB + w;
+    int n = g.n, base = 3 * g.nw + 29 * s;
+    bool live = s < g.ns;  // wave-uniform
     double acc[NH_VALS];
 #pragma unroll
     for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
-    if (live) {
+    bool site = live && lane < 29;
+    double ro[3] = {0.0, 0.0, 0.0};
+    if (site) {
+        double escale = 1.0 - nh[0] * g.dt / 2.0;
+        double vv = 0.0;
 #pragma unroll
-        for (int q = 0; q < 5; q++) {
-            acc[4 + q] = L.pre[q][t];
-            acc[13 + q] = L.rb[q][t];
+        for (int d = 0; d < 3; d++) {
+            double v0 = v[d * n + base + lane], aa = a[d * n + base + lane];
+            ro[d] = r[d * n + base + lane];
+            L.r[w][lane][d] = ro[d];
+            L.p[w][lane][d] = ro[d] + v0 * escale + aa;
+            vv += v0 * v0;
         }
+        int grp = c_sgrp[lane];
+#pragma unroll
+        for (int q = 0; q < 5; q++) acc[4 + q] = grp == q ? vv : 0.0;  // no dynamic register index
+    }
+    __asm__ volatile("" ::: "memory");
+    if (live) {
+        bool bond = lane < 28;
+        int ia = bond ? c_bi[lane] : 0, ib = bond ? c_bj[lane] : 1;
+        int off = bond ? c_boff[lane] : 0;
+        double rbx = L.r[w][ia][0] - L.r[w][ib][0], rby = L.r[w][ia][1] - L.r[w][ib][1],
+               rbz = L.r[w][ia][2] - L.r[w][ib][2];
+        int cb = bond ? lane : 0;
+        double dsq = sc.dsq[cb], tolc = sc.dsq[cb] * (2.0 * 0.0001), k2 = sc.k2[cb];
+        double rma = sc.rm[ia], rmb = sc.rm[ib];
+        double *Pa = &L.p[w][ia][0], *Pb = &L.p[w][ib][0];
+        bool conv = pipelined_sweeps(bond, off, 101, [&]() {
+            double ax = Pa[0], ay = Pa[1], az = Pa[2], bx = Pb[0], by = Pb[1], bz = Pb[2];
+            double pxab = ax - bx, pyab = ay - by, pzab = az - bz;
+            double pabsq = pxab * pxab + pyab * pyab + pzab * pzab;
+            double diffsq = dsq - pabsq;
+            bool corr = fabs(diffsq) > tolc;
+            if (corr) {
+                double rpab = rbx * pxab + rby * pyab + rbz * pzab;
+                double gab = diffsq / (k2 * rpab);
+                double dx = rbx * gab, dy = rby * gab, dz = rbz * gab;
+                Pa[0] = ax + rma * dx;
+                Pa[1] = ay + rma * dy;
+                Pa[2] = az + rma * dz;
+                Pb[0] = bx - rmb * dx;
+                Pb[1] = by - rmb * dy;
+                Pb[2] = bz - rmb * dz;
+            }
+            return corr;
+        });
+        if (!conv && lane == 0) atomicAdd(&counters[0], 1);
+    }
+    if (site) {
+        double pn[3], vv = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            pn[d] = L.p[w][lane][d];
+            double vn = pn[d] - ro[d];
+            r[d * n + base + lane] = pn[d];
+            v[d * n + base + lane] = vn;
+            f[d * n + base + lane] = 0.0;
+            vv += vn * vn;
+        }
+        int grp = c_sgrp[lane];
+#pragma unroll
+        for (int q = 0; q < 5; q++) acc[13 + q] = grp == q ? vv : 0.0;
+        put_sorted(g, so, base + lane, 3 + lane, pn[0], pn[1], pn[2]);
     }
     block_sum_store<NH_VALS>(acc, partials + (size_t)(g.nw > 0 ? (g.nw + 255) / 256 : 0) * NH_VALS +
                                       (size_t)blk * NH_VALS);
 }
 
-// Part B for surfactants (src/move_surfactant_b.f90:23-122): velocities in
-// registers, bond vectors and per-bond constants in LDS, RATTLE in the
-// reference's order.  A correction below the tolerance is not applied: it
-// adds an exact zero, so the update needs no branch.
+// Part B for surfactants (src/move_surfactant_b.f90:23-122): lane i < 29 forms
+// site i's new acceleration and velocity, lane c < 28 owns bond c in the
+// pipelined RATTLE (reference order and tolerance).  The two divisions of the
+// correction are folded into one factor per bond (within 2 ulp).
 __device__ __forceinline__ void surf_b(
     int blk, Geo g, const double *__restrict__ r, double *__restrict__ v, double *__restrict__ a,
     const double *__restrict__ f, const double *__restrict__ fs, SurfConst sc,
     const double *__restrict__ nh, int *__restrict__ counters)
 {
-    __shared__ SurfLds L;
-    int t = threadIdx.x;
-    for (int c = t; c < 28; c += SURF_NT) {
-        L.k[0][c] = sc.kr[c];
-        L.k[1][c] = sc.wa[c];
-        L.k[2][c] = sc.wb[c];
-    }
-    __syncthreads();
-    const double *K = &L.k[0][0];
-    const double *RB = &L.rb[0][0];
-    int s = blk * SURF_NT + t;
-    if (t >= SURF_NT || s >= g.ns) return;
+    __shared__ SurfMolLds L;
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int s = blk * SURF_MPB + w;
+    if (s >= g.ns) return;  // wave-uniform; no block-wide barrier follows
     int n = g.n, base = 3 * g.nw + 29 * s;
-    double escale = 2.0 / (2.0 + nh[0] * g.dt);
-    double vx[29][3];
-#pragma unroll
-    for (int i = 0; i < 29; i++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) vx[i][d] = r[d * n + base + i];
-#pragma unroll
-    for (int c = 0; c < 28; c++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) L.rb[3 * c + d][t] = vx[kBI[c]][d] - vx[kBJ[c]][d];
-#pragma unroll
-    for (int i = 0; i < 29; i++)
+    bool site = lane < 29;
+    if (site) {
+        double escale = 2.0 / (2.0 + nh[0] * g.dt);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            double fv = f[d * n + base + i];
-            if (fs) fv += fs[d * 29 * g.ns + 29 * s + i];
-            double ac = fv * g.dtsq2 / sc.m[i];
-            a[d * n + base + i] = ac;
-            vx[i][d] = escale * (v[d * n + base + i] + ac);
+            L.r[w][lane][d] = r[d * n + base + lane];
+            double fv = f[d * n + base + lane];
+            if (fs) fv += fs[d * 29 * g.ns + 29 * s + lane];
+            double ac = fv * g.dtsq2 / sc.m[lane];
+            a[d * n + base + lane] = ac;
+            L.p[w][lane][d] = escale * (v[d * n + base + lane] + ac);
         }
-    int it = 0;
-    bool done = false;
-    // bond c + 1's vector and constants are loaded while bond c is worked on
-    double nrx = RB[t], nry = RB[SURF_NT + t], nrz = RB[2 * SURF_NT + t];
-    double nk = K[0], nwa = K[28], nwb = K[56];
-    while (!done && it <= 100) {
-        done = true;
-#pragma unroll
-        for (int c = 0; c < 28; c++) {
-            // gab = -(r_ab . v_ab / dt) / ((1/m_a + 1/m_b) d^2), the two
-            // divisions folded into one factor per bond (within 2 ulp)
-            const int ia = kBI[c], ib = kBJ[c];
-            double rxab = nrx, ryab = nry, rzab = nrz, kc = nk, wa = nwa, wb = nwb;
-            __asm__ volatile("" ::: "memory");
-            const int cn = c + 1 < 28 ? c + 1 : 0;
-            nrx = RB[(3 * cn) * SURF_NT + t];
-            nry = RB[(3 * cn + 1) * SURF_NT + t];
-            nrz = RB[(3 * cn + 2) * SURF_NT + t];
-            nk = K[cn];
-            nwa = K[28 + cn];
-            nwb = K[56 + cn];
-            double vxab = vx[ia][0] - vx[ib][0], vyab = vx[ia][1] - vx[ib][1], vzab = vx[ia][2] - vx[ib][2];
-            double gab = (rxab * vxab + ryab * vyab + rzab * vzab) * kc;
-            bool app = fabs(gab) > 0.0001;
-            gab = app ? gab : 0.0;
-            done = done && !app;
-            double dx = rxab * gab, dy = ryab * gab, dz = rzab * gab;
-            vx[ia][0] = vx[ia][0] + wa * dx;
-            vx[ia][1] = vx[ia][1] + wa * dy;
-            vx[ia][2] = vx[ia][2] + wa * dz;
-            vx[ib][0] = vx[ib][0] - wb * dx;
-            vx[ib][1] = vx[ib][1] - wb * dy;
-            vx[ib][2] = vx[ib][2] - wb * dz;
-        }
-        it++;
     }
-    if (!done) atomicAdd(&counters[0], 1);
+    __asm__ volatile("" ::: "memory");
+    bool bond = lane < 28;
+    int ia = bond ? c_bi[lane] : 0, ib = bond ? c_bj[lane] : 1;
+    int off = bond ? c_boff[lane] : 0;
+    double rbx = L.r[w][ia][0] - L.r[w][ib][0], rby = L.r[w][ia][1] - L.r[w][ib][1],
+           rbz = L.r[w][ia][2] - L.r[w][ib][2];
+    int cb = bond ? lane : 0;
+    double kc = sc.kr[cb], wa = sc.wa[cb], wb = sc.wb[cb];
+    double *Va = &L.p[w][ia][0], *Vb = &L.p[w][ib][0];
+    bool conv = pipelined_sweeps(bond, off, 101, [&]() {
+        // gab = -(r_ab . v_ab / dt) / ((1/m_a + 1/m_b) d^2)
+        double ax = Va[0], ay = Va[1], az = Va[2], bx = Vb[0], by = Vb[1], bz = Vb[2];
+        double vxab = ax - bx, vyab = ay - by, vzab = az - bz;
+        double gab = (rbx * vxab + rby * vyab + rbz * vzab) * kc;
+        bool app = fabs(gab) > 0.0001;
+        if (app) {
+            double dx = rbx * gab, dy = rby * gab, dz = rbz * gab;
+            Va[0] = ax + wa * dx;
+            Va[1] = ay + wa * dy;
+            Va[2] = az + wa * dz;
+            Vb[0] = bx - wb * dx;
+            Vb[1] = by - wb * dy;
+            Vb[2] = bz - wb * dz;
+        }
+        return app;
+    });
+    if (!conv && lane == 0) atomicAdd(&counters[0], 1);
+    if (site) {
 #pragma unroll
-    for (int i = 0; i < 29; i++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) v[d * n + base + i] = vx[i][d];
+        for (int d = 0; d < 3; d++) v[d * n + base + lane] = L.p[w][lane][d];
+    }
 }
 
 // kinetic sums -> the eta update of src/move_w2_a.f90:145-157 (single block)
@@ -2191,7 +2179,7 @@ __global__ __launch_bounds__(256) void k_move_a_water(Geo g, double *__restrict_
     water_a(blockIdx.x, g, r, v, a, f, cls, ff, nh, partials, counters, clsmask);
 }
 
-__global__ __launch_bounds__(SURF_NT) void k_move_a_surf_fast(Geo g, double *__restrict__ r,
+__global__ __launch_bounds__(256) void k_move_a_surf_fast(Geo g, double *__restrict__ r,
                                                               double *__restrict__ v,
                                                               const double *__restrict__ a,
                                                               double *__restrict__ f, SurfConst sc,
@@ -2229,7 +2217,7 @@ __global__ __launch_bounds__(256) void k_move_b_water(Geo g, const double *__res
     water_b(blockIdx.x, g, r, v, a, f, ff, nh, counters, cls, clsmask);
 }
 
-__global__ __launch_bounds__(SURF_NT) void k_move_b_surf_fast(Geo g, const double *__restrict__ r,
+__global__ __launch_bounds__(256) void k_move_b_surf_fast(Geo g, const double *__restrict__ r,
                                                               double *__restrict__ v,
                                                               double *__restrict__ a,
                                                               const double *__restrict__ f,
@@ -2591,7 +2579,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
         c->sconst.wb[b] = rmb * ff->dt;
     }
     c->nblk_w = g.nw > 0 ? (int)nblk(g.nw, 256) : 0;
-    c->nblk_s = g.ns > 0 ? (int)nblk(g.ns, c->surf_fast ? SURF_NT : 64) : 0;
+    c->nblk_s = g.ns > 0 ? (int)nblk(g.ns, c->surf_fast ? SURF_MPB : 64) : 0;
     A(dalloc(&c->partials, (size_t)(c->nblk_w + c->nblk_s + 1) * NH_VALS));
     A(dalloc(&c->fs, 3 * 29 * (size_t)g.ns + 1));
     A(dalloc(&c->nh, 16 + NH_VALS));
@@ -2940,7 +2928,7 @@ static int launch_move_a(sesmd_ctx *c, unsigned clsmask = 7u, int update_eta = 1
         k_move_a_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->cls, c->dff, c->nh,
                                                    c->partials, c->counters, clsmask);
     if (c->nblk_s && (clsmask & 4u) && c->surf_fast)
-        k_move_a_surf_fast<<<c->nblk_s, SURF_NT, 0, st>>>(g, c->r, c->v, c->a, c->f, c->sconst, c->nh,
+        k_move_a_surf_fast<<<c->nblk_s, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->sconst, c->nh,
                                                          c->partials, c->counters);
     else if (c->nblk_s && (clsmask & 4u))
         k_move_a_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->partials,
@@ -2975,7 +2963,7 @@ static int launch_move_b(sesmd_ctx *c, unsigned clsmask = 7u)
         return 0;
     }
     if (c->nblk_s && c->surf_fast)
-        k_move_b_surf_fast<<<c->nblk_s, SURF_NT, 0, st>>>(g, c->r, c->v, c->a, c->f, nullptr, c->sconst,
+        k_move_b_surf_fast<<<c->nblk_s, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, nullptr, c->sconst,
                                                          c->nh, c->counters);
     else if (c->nblk_s)
         k_move_b_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, nullptr, c->dff, c->nh,
