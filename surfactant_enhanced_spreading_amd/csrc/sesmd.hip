@@ -626,6 +626,7 @@ __global__ void k_gather_pos32(Geo g, const int *__restrict__ perm, const int *_
 // keeps only the (receiver kind, source kind) pairs of `mask` (one reference
 // kernel or a subset of them).
 constexpr int NEAR_ROWS = 9;  // (dz, dy) rows of the 3^3 neighbourhood, one wave each
+constexpr int SRC_PAD = 4;    // spare records after the sorted source arrays (read ahead, never used)
 
 struct Recv {
     double x, y, z, q;       // receiver position, charge * xpi
@@ -699,20 +700,25 @@ __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__rest
     if (s0 >= s1) return;
     // Sources two at a time, the next two loaded while these are used: scalar
     // loads return out of order, so each use waits for every load in flight;
-    // pairing halves the waits per source.
-    int t1 = min(s0 + 1, s1 - 1);
-    int4 mA = smeta[s0], mB = smeta[t1];
-    double4 pA = spos[s0], pB = spos[t1];
-    for (int t = s0; t < s1; t += 2) {
+    // pairing halves the waits per source.  The loads run up to three records
+    // past s1 (the sorted arrays carry SRC_PAD spare records) and use one
+    // incremented pointer per array, which keeps the scalar unit's address
+    // arithmetic to a few instructions per pair of sources.
+    const double4 *ps = spos + s0;
+    const int4 *pm = smeta + s0;
+    int4 mA = pm[0], mB = pm[1];
+    double4 pA = ps[0], pB = ps[1];
+    for (int t = s1 - s0; t > 0; t -= 2) {
         const int4 ma = mA, mb = mB;
         const double4 pa = pA, pb = pB;
-        int tn0 = min(t + 2, s1 - 1), tn1 = min(t + 3, s1 - 1);
-        mA = smeta[tn0];
-        mB = smeta[tn1];
-        pA = spos[tn0];
-        pB = spos[tn1];
+        ps += 2;
+        pm += 2;
+        mA = pm[0];
+        mB = pm[1];
+        pA = ps[0];
+        pB = ps[1];
         near_pair<CAT, CHECK, MASKED, CEN, HQ>(g, pa, ma, bx, by, bz, R, mask, ax, ay, az);
-        if (t + 1 < s1) near_pair<CAT, CHECK, MASKED, CEN, HQ>(g, pb, mb, bx, by, bz, R, mask, ax, ay, az);
+        if (t > 1) near_pair<CAT, CHECK, MASKED, CEN, HQ>(g, pb, mb, bx, by, bz, R, mask, ax, ay, az);
     }
 }
 
@@ -809,19 +815,21 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
 {
     if (s0 >= s1) return;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    int t1 = min(s0 + 1, s1 - 1);
-    int4 mA = smeta[s0], mB = smeta[t1];
-    float4 pA = spos32[s0], pB = spos32[t1];
-    for (int t = s0; t < s1; t += 2) {
+    const float4 *ps = spos32 + s0;
+    const int4 *pm = smeta + s0;
+    int4 mA = pm[0], mB = pm[1];
+    float4 pA = ps[0], pB = ps[1];
+    for (int t = s1 - s0; t > 0; t -= 2) {
         const int4 ma = mA, mb = mB;
         const float4 pa = pA, pb = pB;
-        int tn0 = min(t + 2, s1 - 1), tn1 = min(t + 3, s1 - 1);
-        mA = smeta[tn0];
-        mB = smeta[tn1];
-        pA = spos32[tn0];
-        pB = spos32[tn1];
+        ps += 2;
+        pm += 2;
+        mA = pm[0];
+        mB = pm[1];
+        pA = ps[0];
+        pB = ps[1];
         near_pair32<CAT, CHECK, CEN, HQ>(pa, ma, bx, by, bz, xr, yr, zr, R, Q, fx, fy, fz);
-        if (t + 1 < s1) near_pair32<CAT, CHECK, CEN, HQ>(pb, mb, bx, by, bz, xr, yr, zr, R, Q, fx, fy, fz);
+        if (t > 1) near_pair32<CAT, CHECK, CEN, HQ>(pb, mb, bx, by, bz, xr, yr, zr, R, Q, fx, fy, fz);
     }
     ax += (double)fx;
     ay += (double)fy;
@@ -2531,12 +2539,12 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->widx_in, c->max_work));
     A(dalloc(&c->widx_out, c->max_work));
     A(dalloc(&c->nwork, 1));
-    A(dalloc(&c->spos, n));
+    A(dalloc(&c->spos, n + SRC_PAD));
     A(dalloc(&c->npart, 27 * n));
     A(dalloc(&c->iperm, n));
-    A(dalloc(&c->smeta, n));
+    A(dalloc(&c->smeta, n + SRC_PAD));
     if (precision == SESMD_MIXED) {
-        A(dalloc(&c->spos32, n));
+        A(dalloc(&c->spos32, n + SRC_PAD));
         A(dalloc(&c->lj32, 2048));
     }
     A(dalloc(&c->O4, (size_t)g.nb3 * g.lp));
