@@ -717,6 +717,7 @@ __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__rest
         mB = pm[1];
         pA = ps[0];
         pB = ps[1];
+        __asm__ volatile("" ::: "memory");  // keeps these loads ahead of the pair arithmetic
         near_pair<CAT, CHECK, MASKED, CEN, HQ>(g, pa, ma, bx, by, bz, R, mask, ax, ay, az);
         if (t > 1) near_pair<CAT, CHECK, MASKED, CEN, HQ>(g, pb, mb, bx, by, bz, R, mask, ax, ay, az);
     }
@@ -828,6 +829,7 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
         mB = pm[1];
         pA = ps[0];
         pB = ps[1];
+        __asm__ volatile("" ::: "memory");  // keeps these loads ahead of the pair arithmetic
         near_pair32<CAT, CHECK, CEN, HQ>(pa, ma, bx, by, bz, xr, yr, zr, R, Q, fx, fy, fz);
         if (t > 1) near_pair32<CAT, CHECK, CEN, HQ>(pb, mb, bx, by, bz, xr, yr, zr, R, Q, fx, fy, fz);
     }
