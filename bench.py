@@ -132,6 +132,23 @@ def rms_force_error(sys_):
     return float(np.sqrt(((f[1] - f[0]) ** 2).sum() / (f[0] ** 2).sum()))
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_near_counters_bench_fp64.md')
+
+
+def near_traffic():
+    """HBM bytes per k_near launch on this workload, from the committed
+    FETCH_SIZE / WRITE_SIZE counter runs (each scaled by what the counter reads
+    for a plain copy of known size); None if that summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith('traffic_bytes_per_launch:'):
+                    return int(line.split(':', 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def copy_calibration(nbytes, reps=5):
     """A plain device-to-device copy of `nbytes` (hipMemcpy, the runtime's copy
     kernel), for scaling the FETCH_SIZE / WRITE_SIZE counters of the profiled
@@ -182,7 +199,7 @@ def main():
     flop = 21.0 * pairs + 9.0 * lj_pairs
     achieved = flop / (r64['near_ms'] * 1e-3) / 1e12
     roofline = dict(bound='fp64-valu', achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS,
-                    unit='TFLOP/s', frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=None,
+                    unit='TFLOP/s', frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=near_traffic(),
                     kernel='k_near', kernel_ms=round(r64['near_ms'], 4),
                     pairs=pairs, lj_pairs=lj_pairs)
 
