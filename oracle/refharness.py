@@ -30,6 +30,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, '_ref', 'libmybox_ref.so')
+MAKE_GRAPHITE = os.path.join(HERE, '_ref', 'make_graphite')
 
 P = 4
 LP = (P + 1) ** 2
@@ -178,6 +179,22 @@ class RefLib:
     def legendre(self, l, m, x):
         return self.lib.rlegendrep_(ctypes.byref(ctypes.c_int(l)), ctypes.byref(ctypes.c_int(m)),
                                     ctypes.byref(ctypes.c_double(x)))
+
+    def write_pdb(self, sys_, icq, directory):
+        """The compiled write_pdb (src/write_pdb.f90) on sys_, writing into
+        `directory`; it always writes the PARAMETER iwater (9,997) waters."""
+        nw = sys_.n_water
+        cols = [np.ascontiguousarray(sys_.pos[:3 * nw, a]) for a in range(3)] + \
+               [np.ascontiguousarray(sys_.pos[3 * nw:, a]) for a in range(3)]
+        ptr = [c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for c in cols]
+        i = lambda v: ctypes.byref(ctypes.c_int(int(v)))
+        cwd = os.getcwd()
+        os.chdir(directory)
+        try:
+            self.lib.write_pdb_(ptr[0], ptr[1], ptr[2], i(0), ptr[3], ptr[4], ptr[5],
+                                i(sys_.n_surf), i(icq))
+        finally:
+            os.chdir(cwd)
 
     def _init_tables(self):
         cwd = os.getcwd()
@@ -430,6 +447,45 @@ class Emulator:
                                     R['a%s%s' % (ax, tag)][sl, b64 - 1] = self.acc[sites, a]
                             self.smap[c].append((s0, box, sites, own, b64))
                         cnt[box - 1] = len(mols)
+
+    def read_config_mismatches(self, directory):
+        """For every rank, the own-box storage (counts, positions, velocities)
+        that the compiled read_config leaves after reading
+        directory/data_cont2.dat, against what load_rank fills from this
+        emulator's system; returns the (rank, class) pairs that differ."""
+        R = self.ref
+        bad = []
+        tags = [('w1', 'ictw1'), ('w2', 'ictw2'), ('s', 'icts')]
+        for rk_i, rk in enumerate(self.ranks()):
+            self.load_rank(rk)
+            want = []
+            for c, (tag, cnt) in enumerate(tags):
+                grid, halo = self.RING[c]
+                own = np.zeros((grid, grid, grid), bool)
+                own[halo:halo + 4, halo:halo + 4, halo:halo + 4] = True
+                idx = np.nonzero(own.reshape(-1))[0]
+                want.append((idx, R[cnt][idx].copy(),
+                             [R['r%s%s' % (a, tag)][:, idx].copy() for a in 'xyz'],
+                             [R['v%s%s' % (a, tag)].copy() for a in 'xyz']))
+                R[cnt][:] = 0
+                for a in 'xyz':
+                    R['r%s%s' % (a, tag)][:] = 0.0
+                    R['v%s%s' % (a, tag)][:] = 0.0
+            cwd = os.getcwd()
+            os.chdir(directory)
+            try:
+                self.ref.call('read_config', rk_i)
+            finally:
+                os.chdir(cwd)
+            for c, (tag, cnt) in enumerate(tags):
+                idx, n0, r0, v0 = want[c]
+                same = np.array_equal(R[cnt][idx], n0)
+                for k, a in enumerate('xyz'):
+                    same = same and np.array_equal(R['r%s%s' % (a, tag)][:, idx], r0[k])
+                    same = same and np.array_equal(R['v%s%s' % (a, tag)], v0[k])
+                if not same:
+                    bad.append((rk_i, c))
+        return bad
 
     def zero_forces(self):
         for n in ('fxw1', 'fyw1', 'fzw1', 'fxw2', 'fyw2', 'fzw2', 'fxs', 'fys', 'fzs'):

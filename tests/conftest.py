@@ -54,6 +54,19 @@ def medium_system(seed=3):
                                      seed=seed), seed)
 
 
+def file_format_system(n_water=9997, n_surf=475):
+    """A system of the reference's PARAMETER size (read_config reads exactly
+    9,997 waters + 475 surfactants, src/mybox.cmns:59-60) with positions and
+    velocities from a fixed arithmetic pattern, for the file-format tests: the
+    readers and writers do not care whether the geometry is physical."""
+    n = 3 * n_water + 29 * n_surf
+    i = np.arange(n, dtype=np.float64)
+    pos = np.stack([(i * 0.3711) % 44.0 - 22.0, (i * 0.5317) % 44.0 - 22.0,
+                    (i * 0.7129) % 44.0 - 22.0], axis=1)
+    vel = np.stack([np.sin(i * 0.1), np.cos(i * 0.07), np.sin(i * 0.013 + 1.0)], axis=1) * 2e-3
+    return builder.System(n_water, n_surf, pos, vel)
+
+
 @pytest.fixture(scope='session')
 def sys_small():
     return small_system()

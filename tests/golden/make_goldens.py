@@ -12,7 +12,15 @@ through oracle/refharness.py (the 27-rank COMMON emulation):
   small_system.npz  inputs + bootstrap forces per component, per reference
                     kernel, classes, isub, far-field expansions, one full step
   medium_total.npz  inputs + total bootstrap force of a larger system
+  formats.json      SHA-256 digests of the reference's own file output
+                    (make_graphite's graphite.dat, write_pdb's snapshot of the
+                    file-format test system) and of the positions/velocities
+                    read_config stores from that system's data_cont2.dat
 """
+import hashlib
+import json
+import subprocess
+import tempfile
 import os
 import sys
 
@@ -25,7 +33,8 @@ sys.path.insert(0, os.path.join(ROOT, 'oracle'))
 sys.path.insert(0, os.path.join(ROOT, 'tests'))
 
 import refharness as rh  # noqa: E402
-from conftest import small_system, medium_system  # noqa: E402
+from conftest import small_system, medium_system, file_format_system  # noqa: E402
+from surfactant_enhanced_spreading_amd import formats as fmt  # noqa: E402
 
 
 def tables(ref):
@@ -111,8 +120,75 @@ def medium(ref):
                         F=em.forces())
 
 
+def sha(path):
+    with open(path, 'rb') as fh:
+        return hashlib.sha256(fh.read()).hexdigest()
+
+
+def stored_digest(ref, em):
+    """SHA-256 over the rows (x, y, z, vx, vy, vz) that read_config stores in
+    the own boxes of all 27 ranks, rows sorted, as little-endian float64."""
+    rows = []
+    cwd = os.getcwd()
+    for rk_i, rk in enumerate(em.ranks()):
+        em.load_rank(rk)
+        for c, tag in enumerate(('w1', 'w2', 's')):
+            ref['ictw1' if c == 0 else 'ictw2' if c == 1 else 'icts'][:] = 0
+            for a in 'xyz':
+                ref['r%s%s' % (a, tag)][:] = 0.0
+                ref['v%s%s' % (a, tag)][:] = 0.0
+        os.chdir(em.workdir)
+        try:
+            ref.call('read_config', rk_i)
+        finally:
+            os.chdir(cwd)
+        for c, tag in enumerate(('w1', 'w2', 's')):
+            grid, halo = em.RING[c]
+            nsite = 29 if c == 2 else 3
+            cnt = ref['ictw1' if c == 0 else 'ictw2' if c == 1 else 'icts']
+            for lz in range(halo, halo + 4):
+                for ly in range(halo, halo + 4):
+                    for lx in range(halo, halo + 4):
+                        box = lx + grid * ly + grid * grid * lz
+                        b64 = (lx - halo) + 4 * (ly - halo) + 16 * (lz - halo)
+                        k = cnt[box] * nsite
+                        if k == 0:
+                            continue
+                        r = [ref['r%s%s' % (a, tag)][:k, box] for a in 'xyz']
+                        v = [ref['v%s%s' % (a, tag)][:k, b64] for a in 'xyz']
+                        rows.append(np.stack(r + v, axis=1))
+    a = np.concatenate(rows)
+    a = a[np.lexsort(a.T[::-1])]
+    return hashlib.sha256(np.ascontiguousarray(a, dtype='<f8').tobytes()).hexdigest(), len(a)
+
+
+def formats(ref):
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        subprocess.run([rh.MAKE_GRAPHITE], cwd=d, check=True, capture_output=True)
+        g = os.path.join(d, 'graphite.dat')
+        lines = open(g).read().split('\n')
+        out['graphite'] = dict(sha256=sha(g), lines=len(lines) - 1, first=lines[0], last=lines[-2])
+        s = file_format_system()
+        ref.write_pdb(s, 42, d)
+        _, sname = fmt.pdb_names(42, s.n_water)
+        out['pdb'] = {f: sha(os.path.join(d, f)) for f in sorted(os.listdir(d)) if f.endswith('.pdb')}
+        fmt.write_config(os.path.join(d, 'data_cont2.dat'), s)
+        out['data_cont2_sha256'] = sha(os.path.join(d, 'data_cont2.dat'))
+        em = rh.Emulator(ref, fmt.read_config(os.path.join(d, 'data_cont2.dat'), s.n_water, s.n_surf))
+        em.workdir = d
+        out['read_config_sha256'], out['read_config_rows'] = stored_digest(ref, em)
+    with open(os.path.join(HERE, 'formats.json'), 'w') as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+        fh.write('\n')
+
+
 if __name__ == '__main__':
     ref = rh.RefLib()
+    if len(sys.argv) > 1 and sys.argv[1] == 'formats':
+        formats(ref)
+        sys.exit(0)
+    formats(ref)
     tables(ref)
     ylm(ref)
     small(ref)
