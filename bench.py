@@ -117,6 +117,37 @@ def run(sys_, prec, steps, warmup, timer):
     return dict(ms_step=ms_step, steps_s=1000.0 / ms_step, wall=wall, near_ms=near_ms), e
 
 
+def energy_drift(sys_, prec, nsteps, every=500, settle=100, ifar=10):
+    """Extended (Nose-Hoover) energy over `nsteps` steps after `settle` steps
+    (the first steps project the start-up velocities onto the constraints):
+    drift and largest excursion relative to the kinetic energy, energies
+    evaluated in FP64 (sesmd_energies) every `every` steps.  ifar: steps
+    between list / far-field refreshes (the reference's 10)."""
+    from surfactant_enhanced_spreading_amd import engine
+    e = engine.Engine(sys_, precision=prec)
+    try:
+        e.bootstrap()
+        kb = e.steps(settle, ifar, 0)
+        e0 = e.energies()
+        hs = [e0['total']]
+        done = 0
+        while done < nsteps:
+            k = min(every, nsteps - done)
+            kb = e.steps(k, ifar, kb)
+            done += k
+            hs.append(e.energies()['total'])
+        e1 = e.energies()
+    finally:
+        e.close()
+    k0 = e0['kinetic']
+    ns = nsteps * 2.0e-6  # 2 fs steps
+    return dict(steps=nsteps, ifar=ifar, kinetic0=round(k0, 3), total0=round(hs[0], 4), total_end=round(hs[-1], 4),
+                drift_rel_kinetic=(hs[-1] - hs[0]) / k0,
+                drift_rel_kinetic_per_ns=(hs[-1] - hs[0]) / k0 / ns,
+                max_excursion_rel_kinetic=max(abs(h - hs[0]) for h in hs) / k0,
+                temperature0=round(e0['temperature'], 4), temperature_end=round(e1['temperature'], 4))
+
+
 def rms_force_error(sys_):
     """||F_mixed - F_fp64|| / ||F_fp64|| over all sites and components, total
     force at the start-up configuration."""
@@ -177,6 +208,8 @@ def main():
     ap.add_argument('--no-mixed', action='store_true', help='skip the mixed-precision leg')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    ap.add_argument('--drift-steps', type=int, default=10000,
+                    help='steps of the energy-drift runs (FP64 and mixed); 0 skips them')
     ap.add_argument('--copy-calibrate', type=int, default=0, metavar='BYTES',
                     help='only run a device copy of BYTES (counter calibration) and exit')
     args = ap.parse_args()
@@ -225,6 +258,14 @@ def main():
                             near_kernel_ms=round(rm['near_ms'], 4),
                             near_achieved_tflops=round(ach, 3),
                             near_frac_fp32_peak=round(ach / FP32_PEAK_TFLOPS, 4))
+    if args.drift_steps > 0:
+        out['energy_drift'] = dict(fp64=energy_drift(sys_, engine.FP64, args.drift_steps))
+        if not args.no_mixed:
+            out['energy_drift']['mixed'] = energy_drift(sys_, engine.MIXED, args.drift_steps)
+        # the same dynamics with lists and far field refreshed every step: the
+        # drift above is the reference's 10-step refresh, not the integrator
+        out['energy_drift']['fp64_refresh_every_step'] = energy_drift(
+            sys_, engine.FP64, min(args.drift_steps, 2000), ifar=1)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(sys_, args.cpu_seconds)
     print(json.dumps(out), flush=True)

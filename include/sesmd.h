@@ -175,6 +175,34 @@ int sesmd_move_species(sesmd_ctx *ctx, int part, unsigned cls_mask, double *sums
  * (src/nh_water1.f90:1-43, src/nh_surfactant.f90:1-201). */
 int sesmd_kinetic_sums(sesmd_ctx *ctx, double *sums9);
 
+/* ---- energies (no counterpart in the reference, which reports none: the
+ * potentials whose gradients are the force terms above, for energy and drift
+ * reporting) ---- */
+enum {
+    SESMD_E_NEAR_COUL = 0,     /* 1/2 sum over receivers of xpi q_i q_j / r (the 39 kernels' pairs) */
+    SESMD_E_NEAR_LJ = 1,       /* 1/2 sum of (c/12) r^-6 (s6 r^-6 - 1), r <= rc */
+    SESMD_E_FAR = 2,           /* 1/2 sum q_i Phi(x_i), Phi the frozen local expansion (evalLjk4) */
+    SESMD_E_SUBTRACT = 3,      /* minus the intramolecular Coulomb eval_subtract_fma removes */
+    SESMD_E_INTERNAL_COUL = 4, /* eval_surfactant_internal, Coulomb */
+    SESMD_E_INTERNAL_LJ = 5,   /* eval_surfactant_internal, LJ (no cutoff) */
+    SESMD_E_ANGLE = 6,         /* 1/2 k (theta - theta0)^2 (eval_valence) */
+    SESMD_E_DIHEDRAL = 7,      /* a(1+cos phi) + b(1-cos 2phi) + c(1+cos 3phi) (eval_dihedral) */
+    SESMD_E_DIHEDRAL_BE = 8,   /* a(1+cos 3phi) (eval_dihedral_berendsen) */
+    SESMD_E_WALL = 9,          /* 4/(x-h)^12 + 4/(x+h)^12 per axis (eval_wall) */
+    SESMD_E_GRAPHITE = 10,     /* carbon LJ within rc (eval_graphite) */
+    SESMD_E_POTENTIAL = 11,    /* sum of 0..10 */
+    SESMD_E_KINETIC = 12,      /* 1/2 sum m v^2 (v = stored v dt / dt) */
+    SESMD_E_THERMOSTAT = 13,   /* Q eta^2 / 2 + df tr xi, Q = dt / (2 pnh1), xi = integral of eta dt */
+    SESMD_E_TOTAL = 14,        /* potential + kinetic + thermostat: conserved by the dynamics */
+    SESMD_E_VIRIAL = 15,       /* near-field pair virial, 1/2 sum r . F */
+    SESMD_E_TEMPERATURE = 16,  /* 2 K / df (src/compute_temperature.f90:242) */
+    SESMD_E_COUNT = 17
+};
+/* All components at the current state, in FP64 whatever the context's
+ * precision (reduced units: energies in epsilon).  xi accumulates over the
+ * steps run since the context was created. */
+int sesmd_energies(sesmd_ctx *ctx, double *e);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,6 +103,10 @@ def lib():
         L.orc_ylm.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp]
         L.orc_legendre.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.orc_legendre.restype = ctypes.c_double
+        L.orc_energies.argtypes = [vp, dp]
+        L.orc_xi.argtypes = [vp]
+        L.orc_xi.restype = ctypes.c_double
+        L.orc_set_xi.argtypes = [vp, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -203,6 +207,18 @@ class Oracle:
     def shake_failures(self):
         return self.L.orc_shake_failures(self.h)
 
+    def energies(self):
+        """Energy components (ENERGY_NAMES order) at the current state."""
+        e = np.zeros(len(ENERGY_NAMES))
+        self.L.orc_energies(self.h, _p(e))
+        return dict(zip(ENERGY_NAMES, e))
+
+    def xi(self):
+        return self.L.orc_xi(self.h)
+
+    def set_xi(self, xi):
+        self.L.orc_set_xi(self.h, float(xi))
+
     def ylm(self, n, m, a, b):
         out = np.zeros(2)
         self.L.orc_ylm(self.h, n, m, a, b, _p(out))
@@ -213,6 +229,12 @@ class Oracle:
         f = np.zeros((self.n, 3))
         self.L.orc_steps(self.h, nsteps, ifar, ctypes.byref(k), _p(f))
         return k.value
+
+
+# orc_energies order (sesmd_oracle.h)
+ENERGY_NAMES = ('near_coul', 'near_lj', 'far', 'subtract', 'internal_coul', 'internal_lj',
+                'angle', 'dihedral', 'dihedral_be', 'wall', 'graphite', 'potential', 'kinetic',
+                'thermostat', 'total', 'virial', 'temperature')
 
 
 def near_mask(pairs):

@@ -22,6 +22,7 @@ typedef struct {
     orc_ff ff;
     double *r, *v, *a;
     double eta, tr, pnh1, pnh2;
+    double xi;               /* time integral of eta (thermostat energy) */
     int ngr;
     double *gr;
     int *type, *mol, *first; /* per site: type, molecule; per molecule: first site */
@@ -1220,7 +1221,9 @@ void orc_move_a(void *h, double *sums6)
     s[5] = kin_sum(o, 1);
     double sum1 = (s[0] + s[2] + s[4]) / (dt * dt);
     double sum2 = (s[1] + s[3] + s[5]) / (dt * dt);
+    double eta_old = o->eta;
     o->eta = o->eta + o->pnh1 * (sum1 + o->pnh2 + sum2);
+    o->xi += 0.5 * dt * (eta_old + o->eta);
     if (sums6) memcpy(sums6, s, sizeof(s));
 }
 
@@ -1304,6 +1307,201 @@ void orc_init_accel(void *h, const double *F)
 }
 
 int orc_shake_failures(void *h) { return ((orc *)h)->shake_fail; }
+
+double orc_xi(void *h) { return ((orc *)h)->xi; }
+void orc_set_xi(void *h, double xi) { ((orc *)h)->xi = xi; }
+
+/* ------------------------------------------------------------- energies
+ *
+ * Not computed by the reference: the potential whose gradient is each force
+ * term above, for energy and drift reporting.  Pair terms are summed over
+ * receivers with weight 1/2 (every pair is seen from both ends); the far-field
+ * energy is 1/2 sum q_i Phi(x_i) with Phi the frozen local expansion; the
+ * thermostat adds Q eta^2/2 + df tr xi with Q = df tr tau^2/2 = dt/(2 pnh1). */
+
+/* LJ energy of the force c r^-6 (s6 r^-6 - 1/2) r/r^2: (c/12) r^-6 (s6 r^-6 - 1) */
+static double lj_energy(double c, double s6, double rsq)
+{
+    double rsi = 1.0 / rsq, r6 = rsi * rsi * rsi;
+    return c / 12.0 * r6 * (s6 * r6 - 1.0);
+}
+
+/* rqq * Phi(x_i), Phi = Re sum_jk rfctr L_jk e^{ik phi} r^j P_j^|k|(cos theta):
+ * the potential whose gradient l2p_site evaluates */
+static double l2p_energy_site(const orc *o, int i)
+{
+    int b = o->cbox[i], p = o->p;
+    const double *c = o->c4 + 3 * b;
+    const cplx *L = o->L4 + (size_t)b * o->lp;
+    const double *x = o->r + 3 * i;
+    double rqq = o->ff.q[o->type[i]] * o->ff.xpi;
+    double rxpb = x[0] - c[0], rypb = x[1] - c[1], rzpb = x[2] - c[2];
+    double rr = sqrt(rxpb * rxpb + rypb * rypb + rzpb * rzpb);
+    double rp1 = rzpb / rr;
+    cplx rp2 = (rxpb + rypb * I) / sqrt(rxpb * rxpb + rypb * rypb);
+    double phi = 0.0;
+    for (int j = 0; j <= p; j++)
+        for (int k = -j; k <= j; k++) {
+            int ak = iabs_(k);
+            double rfctr = sqrt(o->ff.rifact[j - ak] / o->ff.rifact[j + ak]);
+            phi += creal(L[idx(j, k)] * rfctr * cipow(rp2, k)) * ipow(rr, j) * legendre(j, ak, rp1);
+        }
+    return rqq * phi;
+}
+
+static double angle_energy(const orc *o, int base, int a)
+{
+    const double *ri = o->r + 3 * (base + o->ff.ang_i[a] - 1), *rj = o->r + 3 * (base + o->ff.ang_j[a] - 1),
+                 *rk = o->r + 3 * (base + o->ff.ang_k[a] - 1);
+    double rxji = ri[0] - rj[0], ryji = ri[1] - rj[1], rzji = ri[2] - rj[2];
+    double rxki = rk[0] - rj[0], ryki = rk[1] - rj[1], rzki = rk[2] - rj[2];
+    double dp = rxji * rxki + ryji * ryki + rzji * rzki;
+    double rjin = sqrt(rxji * rxji + ryji * ryji + rzji * rzji);
+    double rkin = sqrt(rxki * rxki + ryki * ryki + rzki * rzki);
+    double d = acos(dp / rjin / rkin) - o->ff.th[a];
+    return 0.5 * o->ff.rkth[a] * d * d;
+}
+
+static double dihedral_energy(const orc *o, int base, int d)
+{
+    const double *Ri = o->r + 3 * (base + o->ff.dih_i[d] - 1), *Rj = o->r + 3 * (base + o->ff.dih_j[d] - 1),
+                 *Rk = o->r + 3 * (base + o->ff.dih_k[d] - 1), *Rl = o->r + 3 * (base + o->ff.dih_l[d] - 1);
+    double rxji = Rj[0] - Ri[0], ryji = Rj[1] - Ri[1], rzji = Rj[2] - Ri[2];
+    double rxkj = Rk[0] - Rj[0], rykj = Rk[1] - Rj[1], rzkj = Rk[2] - Rj[2];
+    double rxnk = Rl[0] - Rk[0], rynk = Rl[1] - Rk[1], rznk = Rl[2] - Rk[2];
+    double ax = rxji * rykj - ryji * rxkj, ay = rxji * rzkj - rzji * rxkj, az = ryji * rzkj - rzji * rykj;
+    double bx = rxkj * rynk - rykj * rxnk, by = rxkj * rznk - rzkj * rxnk, bz = rykj * rznk - rzkj * rynk;
+    double na = sqrt(ax * ax + ay * ay + az * az), nbn = sqrt(bx * bx + by * by + bz * bz);
+    double arg = (ax * bx + ay * by + az * bz) / na / nbn;
+    double clamp = d < 18 ? o->ff.clamp_opls : o->ff.clamp_be;
+    if (arg <= -1.0) arg = -clamp;
+    if (arg >= 1.0) arg = clamp;
+    double th = acos(arg);
+    if (d < 18)
+        return o->ff.dva[d] * (1.0 + cos(th)) + o->ff.dvb[d] * (1.0 - cos(2.0 * th)) +
+               o->ff.dvc[d] * (1.0 + cos(3.0 * th));
+    return o->ff.dva[d] * (1.0 + cos(3.0 * th));
+}
+
+void orc_energies(void *h, double *e)
+{
+    orc *o = h;
+    memset(e, 0, sizeof(double) * ORC_E_COUNT);
+    int nb = o->nb;
+    double ec = 0.0, el = 0.0, vir = 0.0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : ec, el, vir)
+    for (int b = 0; b < o->nb3; b++) {
+        int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+        for (int s = o->bstart[b]; s < o->bstart[b + 1]; s++) {
+            int i = o->bsite[s];
+            if (o->rkind[i] < 0) continue;
+            int mi = o->mol[i], ti = o->type[i];
+            const double *xi = o->r + 3 * i;
+            for (int dz = -1; dz <= 1; dz++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        int x = bx + dx, y = by + dy, z = bz + dz;
+                        if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
+                        int nbx = x + nb * (y + nb * z);
+                        for (int t = o->bstart[nbx]; t < o->bstart[nbx + 1]; t++) {
+                            int j = o->bsite[t], mj = o->mol[j], tj = o->type[j];
+                            if (mj == mi) continue;
+                            if (src_kind(o->cls[mj], cheb(o, o->anc[mj], b)) < 0) continue;
+                            const double *xj = o->r + 3 * j;
+                            double u = xi[0] - xj[0], v = xi[1] - xj[1], w = xi[2] - xj[2];
+                            double rsq = u * u + v * v + w * w, r = sqrt(rsq);
+                            double qq = o->ff.q[ti] * o->ff.xpi * o->ff.q[tj];
+                            ec += 0.5 * qq / r;
+                            vir += 0.5 * qq / r;
+                            double c = o->ff.lj_c[ti * 32 + tj];
+                            if (c != 0.0 && r <= o->ff.rc) {
+                                double s6 = o->ff.lj_s6[ti * 32 + tj];
+                                double rsi = 1.0 / rsq, r6 = rsi * rsi * rsi;
+                                el += 0.5 * lj_energy(c, s6, rsq);
+                                vir += 0.5 * c * r6 * (s6 * r6 - 0.5);
+                            }
+                        }
+                    }
+        }
+    }
+    e[ORC_E_NEAR_COUL] = ec;
+    e[ORC_E_NEAR_LJ] = el;
+    e[ORC_E_VIRIAL] = vir;
+    double far = 0.0, wl = 0.0;
+    for (int i = 0; i < o->n; i++) {
+        if (o->rkind[i] >= 0) far += 0.5 * l2p_energy_site(o, i);
+        for (int a = 0; a < 3; a++) {
+            double x = o->r[3 * i + a], h1 = x - o->rhs, h2 = x + o->rhs;
+            double u1 = 1.0 / (h1 * h1), u2 = 1.0 / (h2 * h2);
+            double u16 = u1 * u1 * u1, u26 = u2 * u2 * u2;
+            wl += 4.0 * u16 * u16 + 4.0 * u26 * u26;
+        }
+    }
+    e[ORC_E_FAR] = far;
+    e[ORC_E_WALL] = wl;
+    double gr = 0.0;
+    for (int b = 0; b < nb * nb; b++)
+        for (int s = o->bstart[b]; s < o->bstart[b + 1]; s++) {
+            int i = o->bsite[s], rk = o->rkind[i];
+            if (!(rk == 0 || rk == 1 || rk == 3)) continue;
+            int t = o->type[i];
+            double c = o->ff.gr_c[t], s6 = o->ff.gr_s6[t];
+            if (c == 0.0) continue;
+            const double *x = o->r + 3 * i;
+            for (int g = o->gstart[b]; g < o->gstart[b + 1]; g++) {
+                const double *cg = o->gr + 3 * o->gidx[g];
+                double u = x[0] - cg[0], v = x[1] - cg[1], w = x[2] - cg[2];
+                double rsq = u * u + v * v + w * w;
+                if (sqrt(rsq) <= o->ff.rc) gr += lj_energy(c, s6, rsq);
+            }
+        }
+    e[ORC_E_GRAPHITE] = gr;
+    double ic = 0.0, il = 0.0, sb = 0.0, an = 0.0, dh = 0.0, db = 0.0;
+    for (int s = 0; s < o->ns; s++) {
+        int base = o->first[o->nw + s];
+        for (int p = 0; p < o->ff.n_internal; p++) {
+            int j = o->ff.int_j[p] - 1, k = o->ff.int_k[p] - 1;
+            const double *xj = o->r + 3 * (base + j), *xk = o->r + 3 * (base + k);
+            double u = xj[0] - xk[0], v = xj[1] - xk[1], w = xj[2] - xk[2];
+            double rsq = u * u + v * v + w * w;
+            int tj = 3 + j, tk = 3 + k;
+            ic += o->ff.q[tj] * o->ff.q[tk] * o->ff.xpi / sqrt(rsq);
+            il += lj_energy(o->ff.lj_c[tj * 32 + tk], o->ff.lj_s6[tj * 32 + tk], rsq);
+        }
+        for (int j = 0; j < 29; j++) {
+            unsigned mask = o->isub[base + j];
+            const double *xm = o->r + 3 * (base + j);
+            for (int k = 0; k < 29; k++) {
+                if (!((mask >> k) & 1u)) continue;
+                const double *xn = o->r + 3 * (base + k);
+                double u = xm[0] - xn[0], v = xm[1] - xn[1], w = xm[2] - xn[2];
+                sb -= 0.5 * o->ff.q[3 + j] * o->ff.q[3 + k] * o->ff.xpi / sqrt(u * u + v * v + w * w);
+            }
+        }
+        for (int a = 0; a < 36; a++) an += angle_energy(o, base, a);
+        for (int d = 0; d < 18; d++) dh += dihedral_energy(o, base, d);
+        for (int d = 18; d < 30; d++) db += dihedral_energy(o, base, d);
+    }
+    e[ORC_E_INTERNAL_COUL] = ic;
+    e[ORC_E_INTERNAL_LJ] = il;
+    e[ORC_E_SUBTRACT] = sb;
+    e[ORC_E_ANGLE] = an;
+    e[ORC_E_DIHEDRAL] = dh;
+    e[ORC_E_DIHEDRAL_BE] = db;
+    double pot = 0.0;
+    for (int k = 0; k < ORC_E_POTENTIAL; k++) pot += e[k];
+    e[ORC_E_POTENTIAL] = pot;
+    double dt = o->ff.dt, kin = 0.0;
+    for (int i = 0; i < o->n; i++) {
+        const double *v = o->v + 3 * i;
+        kin += 0.5 * o->ff.mass[o->type[i]] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / (dt * dt);
+    }
+    e[ORC_E_KINETIC] = kin;
+    int df = 3 * o->n - (3 * o->nw + 28 * o->ns);
+    e[ORC_E_TEMPERATURE] = 2.0 * kin / df;
+    e[ORC_E_THERMOSTAT] = o->pnh1 != 0.0 ? 0.25 * dt / o->pnh1 * o->eta * o->eta - 0.5 * o->pnh2 * o->xi : 0.0;
+    e[ORC_E_TOTAL] = pot + kin + e[ORC_E_THERMOSTAT];
+}
 
 /* Full steps in the reference's order (src/mybox.f90:130-227): part A; on
  * every ifar-th step the rebuild and the far-field chain; the forces; part B. */

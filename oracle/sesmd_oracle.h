@@ -70,6 +70,20 @@ void orc_move_b(void *h, const double *f);
 void orc_init_accel(void *h, const double *f);
 int orc_shake_failures(void *h);
 
+/* energies (not in the reference; the potentials of the force terms above):
+ * components in this order, then their sum, kinetic, thermostat, total,
+ * near-field pair virial sum r.F, temperature (2K/df).  orc_xi: the time
+ * integral of eta accumulated by orc_move_a. */
+enum {
+    ORC_E_NEAR_COUL, ORC_E_NEAR_LJ, ORC_E_FAR, ORC_E_SUBTRACT, ORC_E_INTERNAL_COUL,
+    ORC_E_INTERNAL_LJ, ORC_E_ANGLE, ORC_E_DIHEDRAL, ORC_E_DIHEDRAL_BE, ORC_E_WALL,
+    ORC_E_GRAPHITE, ORC_E_POTENTIAL, ORC_E_KINETIC, ORC_E_THERMOSTAT, ORC_E_TOTAL,
+    ORC_E_VIRIAL, ORC_E_TEMPERATURE, ORC_E_COUNT
+};
+void orc_energies(void *h, double *e);
+double orc_xi(void *h);
+void orc_set_xi(void *h, double xi);
+
 /* full steps in the reference's cadence (rebuild + far field every ifar
  * steps): kb is the step counter, advanced by nsteps */
 void orc_steps(void *h, int nsteps, int ifar, int *kb, double *f_scratch);

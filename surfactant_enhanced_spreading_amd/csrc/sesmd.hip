@@ -141,7 +141,8 @@ struct sesmd_ctx {
     Topo *dtopo;
     // integrator
     double *partials;  // [nblocks][NH_VALS]
-    double *nh;        // [0] eta, [1..6] sums, [7] tr, [8] pnh1, [9] pnh2
+    double *nh;        // [0] eta, [1..6] sums, [7] tr, [8] pnh1, [9] pnh2, [10] xi = integral of eta dt
+    double *esum;      // energy component sums (sesmd_energies)
     int *counters;     // [0] shake failures
     int nblk_w, nblk_s;
     bool surf_fast;    // topology equals the reference's: unrolled SHAKE/RATTLE
@@ -1297,6 +1298,41 @@ __global__ __launch_bounds__(64) void k_l2l(Geo g, int nd_child, const double2 *
 // ------------------------------------------------------------ per-site terms
 
 // L2P (src/evalLjk4.f90:38-68, the same formula for every receiver class) and
+// Legendre table P[l + 1][m] = P_l^m(x) for l = -1..p, m = 0..p (P[0][0] = 1,
+// rLegendreP's l = -1 case), by the recurrence of src/rLegendreP.f90:3-48.
+template <int p>
+__device__ __forceinline__ void legendre_table(double rp1, double (&P)[MAXP + 2][MAXP + 1])
+{
+#pragma unroll
+    for (int l = 0; l <= p + 1; l++)
+#pragma unroll
+        for (int m = 0; m <= p; m++) P[l][m] = 0.0;
+    P[0][0] = 1.0;  // l = -1
+    double somx2 = sqrt((1.0 - rp1) * (1.0 + rp1));
+#pragma unroll
+    for (int m = 0; m <= p; m++) {
+        double pmm = 1.0, fact = 1.0;
+#pragma unroll
+        for (int i2 = 1; i2 <= m; i2++) {
+            pmm = -pmm * fact * somx2;
+            fact = fact + 2.0;
+        }
+        P[m + 1][m] = pmm;
+        if (m + 1 <= p) {
+            double pmmp1 = rp1 * (2 * m + 1) * pmm;
+            P[m + 2][m] = pmmp1;
+            double pm2 = pmm;
+#pragma unroll
+            for (int ll = m + 2; ll <= p; ll++) {
+                double pll = (rp1 * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pm2) / (ll - m);
+                pm2 = pmmp1;
+                pmmp1 = pll;
+                P[ll + 1][m] = pll;
+            }
+        }
+    }
+}
+
 // the 1/x^13 walls (src/eval_wall.f90:149-188), one thread per site.
 template <int PP>
 // Per-site terms, and the first write of f: f = (near field: the nine row
@@ -1337,36 +1373,8 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
         double sr = sqrt(rho2);
         double2 rp2 = make_double2(rxpb / sr, rypb / sr);
         double rp3 = 1.0 / rho2;
-        // Legendre table P[l][m], l = -1..p (P[-1][0] = 1, rLegendreP's l = -1 case)
         double P[MAXP + 2][MAXP + 1];
-#pragma unroll
-        for (int l = 0; l <= p + 1; l++)
-#pragma unroll
-            for (int m = 0; m <= p; m++) P[l][m] = 0.0;
-        P[0][0] = 1.0;  // l = -1
-        double somx2 = sqrt((1.0 - rp1) * (1.0 + rp1));
-#pragma unroll
-        for (int m = 0; m <= p; m++) {
-            double pmm = 1.0, fact = 1.0;
-#pragma unroll
-            for (int i2 = 1; i2 <= m; i2++) {
-                pmm = -pmm * fact * somx2;
-                fact = fact + 2.0;
-            }
-            P[m + 1][m] = pmm;
-            if (m + 1 <= p) {
-                double pmmp1 = rp1 * (2 * m + 1) * pmm;
-                P[m + 2][m] = pmmp1;
-                double pm2 = pmm;
-#pragma unroll
-                for (int ll = m + 2; ll <= p; ll++) {
-                    double pll = (rp1 * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pm2) / (ll - m);
-                    pm2 = pmmp1;
-                    pmmp1 = pll;
-                    P[ll + 1][m] = pll;
-                }
-            }
-        }
+        legendre_table<p>(rp1, P);
         // e^{i k phi} for k = 0..p by repeated multiplication
         double2 pw[MAXP + 1];
         pw[0] = make_double2(1.0, 0.0);
@@ -2111,7 +2119,11 @@ __device__ void nh_update_block(Geo g, const double *__restrict__ partials, int 
         double dt2 = g.dt * g.dt;
         double sum1 = (S[0] + S[2] + S[4]) / dt2;
         double sum2 = (S[1] + S[3] + S[5]) / dt2;
-        if (update_eta) nh[0] = nh[0] + nh[8] * (sum1 + nh[9] + sum2);
+        if (update_eta) {
+            double eta_old = nh[0];
+            nh[0] = nh[0] + nh[8] * (sum1 + nh[9] + sum2);
+            nh[10] += 0.5 * g.dt * (eta_old + nh[0]);  // for the thermostat energy
+        }
         for (int c = 0; c < 6; c++) nh[1 + c] = S[c];
         for (int c = 0; c < NH_VALS; c++) nh[16 + c] = tot[c];
     }
@@ -2593,6 +2605,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->partials, (size_t)(c->nblk_w + c->nblk_s + 1) * NH_VALS));
     A(dalloc(&c->fs, 3 * 29 * (size_t)g.ns + 1));
     A(dalloc(&c->nh, 16 + NH_VALS));
+    A(dalloc(&c->esum, 16));
     A(dalloc(&c->counters, 4));
     if (rc) {
         std::string e = g_err;
@@ -2666,7 +2679,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
                     c->wkey_out, c->widx_in, c->widx_out, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
-                    c->dtopo, c->partials, c->nh, c->counters, c->spos32, c->lj32, c->fs};
+                    c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->side) hipStreamSynchronize(c->side);
@@ -3331,6 +3344,285 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
         for (int c = 0; c < NH_VALS; c++) acc[c] += partials[(size_t)b * NH_VALS + c];
     block_sum_store<NH_VALS>(acc, out);
+}
+
+// ------------------------------------------------------------ energies
+//
+// Not computed by the reference: the potentials whose gradients are the force
+// terms, for energy and drift reporting.  Each kernel writes one value per
+// site and component; k_sum_cols adds them in a fixed order (deterministic).
+// Pair terms are seen from both ends and weighted 1/2.
+
+constexpr int E_SITE_COLS = 6;   // near Coulomb, near LJ, virial, far, wall, graphite (sorted sites)
+constexpr int E_INTRA_COLS = 6;  // internal Coulomb, internal LJ, subtract, angle, dihedral, Berendsen
+
+// LJ energy of the force c r^-6 (s6 r^-6 - 1/2) r / r^2
+__device__ __forceinline__ double lj_energy(double c, double s6, double rsq)
+{
+    double rsi = 1.0 / rsq, r6 = rsi * rsi * rsi;
+    return c / 12.0 * r6 * (s6 * r6 - 1.0);
+}
+
+template <int PP>
+__global__ __launch_bounds__(128) void k_energy_sites(
+    Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta, const int *__restrict__ bstart,
+    const int *__restrict__ rkind, const int *__restrict__ cbox, const double2 *__restrict__ L4,
+    const double *__restrict__ lfac, const sesmd_forcefield *__restrict__ ff, const double *__restrict__ gr,
+    const int *__restrict__ gstart, const int *__restrict__ gidx, int ngr, double *__restrict__ E)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    double4 pi = spos[k];
+    int4 me = smeta[k];
+    int i = me.x, ti = me.y & 255, rk = rkind[i];
+    double ec = 0.0, el = 0.0, vir = 0.0, far = 0.0, wl = 0.0, egr = 0.0;
+    int nb = g.nb;
+    if (rk >= 0) {
+        int b = cbox[i], bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+        double qi = pi.w * g.xpi;
+        // near field: the pair set of k_near (same ring rule, other molecules)
+        for (int z = max(bz - 1, 0); z <= min(bz + 1, nb - 1); z++)
+            for (int y = max(by - 1, 0); y <= min(by + 1, nb - 1); y++)
+                for (int x = max(bx - 1, 0); x <= min(bx + 1, nb - 1); x++) {
+                    int bb = x + nb * (y + nb * z);
+                    for (int t = bstart[bb]; t < bstart[bb + 1]; t++) {
+                        int4 mj = smeta[t];
+                        if (mj.z == me.z) continue;
+                        int cls = (mj.y >> 8) & 3;
+                        int ring = max(abs((mj.w & 255) - bx),
+                                       max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
+                        int lim = cls == 0 ? 1 : cls == 1 ? 2 : 3;
+                        if (ring > lim) continue;
+                        double4 pj = spos[t];
+                        double u = pi.x - pj.x, v = pi.y - pj.y, w = pi.z - pj.z;
+                        double rsq = u * u + v * v + w * w, r = sqrt(rsq);
+                        double qq = qi * pj.w;
+                        ec += 0.5 * qq / r;
+                        vir += 0.5 * qq / r;
+                        int tj = mj.y & 255;
+                        double c = ff->lj_c[ti * 32 + tj];
+                        if (c != 0.0 && r <= g.rc) {
+                            double s6 = ff->lj_s6[ti * 32 + tj];
+                            double rsi = 1.0 / rsq, r6 = rsi * rsi * rsi;
+                            el += 0.5 * lj_energy(c, s6, rsq);
+                            vir += 0.5 * c * r6 * (s6 * r6 - 0.5);
+                        }
+                    }
+                }
+        // far field: 1/2 q_i Phi(x_i), Phi = Re sum L_jk lfac_jk e^{ik phi} r^j P_j^|k|
+        constexpr int p = PP;
+        double rxpb = pi.x - d_cen4(g, bx), rypb = pi.y - d_cen4(g, by), rzpb = pi.z - d_cen4(g, bz);
+        double rho2 = rxpb * rxpb + rypb * rypb;
+        double rr = sqrt(rho2 + rzpb * rzpb);
+        double sr = sqrt(rho2);
+        double2 rp2 = make_double2(rxpb / sr, rypb / sr);
+        double P[MAXP + 2][MAXP + 1];
+        legendre_table<p>(rzpb / rr, P);
+        double2 pw[MAXP + 1];
+        pw[0] = make_double2(1.0, 0.0);
+#pragma unroll
+        for (int k2 = 1; k2 <= p; k2++) pw[k2] = cmul(pw[k2 - 1], rp2);
+        const double2 *L = L4 + (size_t)b * g.lp;
+        double phi = 0.0, rj = 1.0;
+#pragma unroll
+        for (int j = 0; j <= p; j++) {
+#pragma unroll
+            for (int kk = -j; kk <= j; kk++) {
+                int ak = kk < 0 ? -kk : kk;
+                double2 e = pw[ak];
+                if (kk < 0) {
+                    double d = e.x * e.x + e.y * e.y;
+                    e = make_double2(e.x / d, -e.y / d);
+                }
+                double2 l = L[j * (j + 1) + kk];
+                double sc = lfac[j * (j + 1) + kk];
+                phi += (l.x * sc * e.x - l.y * sc * e.y) * rj * P[j + 1][ak];
+            }
+            rj *= rr;
+        }
+        far = 0.5 * qi * phi;
+        // graphite: receivers filed in their own box of the bottom layer
+        if (ngr > 0 && b < nb * nb && (rk == 0 || rk == 1 || rk == 3)) {
+            double c = ff->gr_c[ti], s6 = ff->gr_s6[ti];
+            if (c != 0.0)
+                for (int q = gstart[b]; q < gstart[b + 1]; q++) {
+                    int a = gidx[q];
+                    double u = pi.x - gr[3 * a], v = pi.y - gr[3 * a + 1], w = pi.z - gr[3 * a + 2];
+                    double rsq = u * u + v * v + w * w;
+                    if (sqrt(rsq) <= g.rc) egr += lj_energy(c, s6, rsq);
+                }
+        }
+    }
+    {
+        double h = g.rhs, xs[3] = {pi.x, pi.y, pi.z};
+        for (int a = 0; a < 3; a++) {
+            double h1 = xs[a] - h, h2 = xs[a] + h;
+            double u1 = 1.0 / (h1 * h1), u2 = 1.0 / (h2 * h2);
+            double u16 = u1 * u1 * u1, u26 = u2 * u2 * u2;
+            wl += 4.0 * u16 * u16 + 4.0 * u26 * u26;
+        }
+    }
+    size_t n = g.n;
+    E[k] = ec;
+    E[n + k] = el;
+    E[2 * n + k] = vir;
+    E[3 * n + k] = far;
+    E[4 * n + k] = wl;
+    E[5 * n + k] = egr;
+}
+
+// Intramolecular energies, one thread per surfactant site (the term lists of
+// k_intra): pair terms weighted 1/2 (listed at both sites), each angle and
+// dihedral counted at its first site.
+__global__ __launch_bounds__(64) void k_energy_intra(Geo g, const double *__restrict__ r,
+                                                     const unsigned *__restrict__ isub,
+                                                     const sesmd_forcefield *__restrict__ ff,
+                                                     const Topo *__restrict__ topo, double *__restrict__ E)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int m = 29 * g.ns;
+    if (t >= m) return;
+    int s = t / 29, j = t % 29;
+    int base = 3 * g.nw + 29 * s, n = g.n;
+    double xs[3];
+    ld3(r, n, base + j, xs);
+    double ic = 0.0, il = 0.0, sb = 0.0, an = 0.0, dh = 0.0, db = 0.0;
+    for (int q = 0; q < topo->npair[j]; q++) {
+        int k = topo->pair[j][q];
+        double xk[3];
+        ld3(r, n, base + k, xk);
+        double u = xs[0] - xk[0], v = xs[1] - xk[1], w = xs[2] - xk[2];
+        double rsq = u * u + v * v + w * w;
+        int tj = 3 + j, tk = 3 + k;
+        ic += 0.5 * ff->q[tj] * ff->q[tk] * g.xpi / sqrt(rsq);
+        il += 0.5 * lj_energy(ff->lj_c[tj * 32 + tk], ff->lj_s6[tj * 32 + tk], rsq);
+    }
+    unsigned mask = isub[base + j];
+    while (mask) {
+        int k = __ffs(mask) - 1;
+        mask &= mask - 1;
+        double xk[3];
+        ld3(r, n, base + k, xk);
+        double u = xs[0] - xk[0], v = xs[1] - xk[1], w = xs[2] - xk[2];
+        sb -= 0.5 * ff->q[3 + j] * ff->q[3 + k] * g.xpi / sqrt(u * u + v * v + w * w);
+    }
+    for (int q = 0; q < topo->nang[j]; q++) {
+        if (topo->ang[j][q] % 3 != 0) continue;
+        int a = topo->ang[j][q] / 3;
+        double ri[3], rjv[3], rk[3];
+        ld3(r, n, base + ff->ang_i[a] - 1, ri);
+        ld3(r, n, base + ff->ang_j[a] - 1, rjv);
+        ld3(r, n, base + ff->ang_k[a] - 1, rk);
+        double rxji = ri[0] - rjv[0], ryji = ri[1] - rjv[1], rzji = ri[2] - rjv[2];
+        double rxki = rk[0] - rjv[0], ryki = rk[1] - rjv[1], rzki = rk[2] - rjv[2];
+        double dp = rxji * rxki + ryji * ryki + rzji * rzki;
+        double rjin = sqrt(rxji * rxji + ryji * ryji + rzji * rzji);
+        double rkin = sqrt(rxki * rxki + ryki * ryki + rzki * rzki);
+        double d = acos(dp / rjin / rkin) - ff->th[a];
+        an += 0.5 * ff->rkth[a] * d * d;
+    }
+    for (int q = 0; q < topo->ndih[j]; q++) {
+        if (topo->dih[j][q] % 4 != 0) continue;
+        int d = topo->dih[j][q] / 4;
+        double Ri[3], Rj[3], Rk[3], Rl[3];
+        ld3(r, n, base + ff->dih_i[d] - 1, Ri);
+        ld3(r, n, base + ff->dih_j[d] - 1, Rj);
+        ld3(r, n, base + ff->dih_k[d] - 1, Rk);
+        ld3(r, n, base + ff->dih_l[d] - 1, Rl);
+        double rxji = Rj[0] - Ri[0], ryji = Rj[1] - Ri[1], rzji = Rj[2] - Ri[2];
+        double rxkj = Rk[0] - Rj[0], rykj = Rk[1] - Rj[1], rzkj = Rk[2] - Rj[2];
+        double rxnk = Rl[0] - Rk[0], rynk = Rl[1] - Rk[1], rznk = Rl[2] - Rk[2];
+        double ax = rxji * rykj - ryji * rxkj, ay = rxji * rzkj - rzji * rxkj, az = ryji * rzkj - rzji * rykj;
+        double bx = rxkj * rynk - rykj * rxnk, by = rxkj * rznk - rzkj * rxnk, bz = rykj * rznk - rzkj * rynk;
+        double na = sqrt(ax * ax + ay * ay + az * az), nbn = sqrt(bx * bx + by * by + bz * bz);
+        double arg = (ax * bx + ay * by + az * bz) / na / nbn;
+        double clamp = d < 18 ? ff->clamp_opls : ff->clamp_be;
+        if (arg <= -1.0) arg = -clamp;
+        if (arg >= 1.0) arg = clamp;
+        double th = acos(arg);
+        if (d < 18)
+            dh += ff->dva[d] * (1.0 + cos(th)) + ff->dvb[d] * (1.0 - cos(2.0 * th)) + ff->dvc[d] * (1.0 + cos(3.0 * th));
+        else
+            db += ff->dva[d] * (1.0 + cos(3.0 * th));
+    }
+    E[t] = ic;
+    E[m + t] = il;
+    E[2 * m + t] = sb;
+    E[3 * m + t] = an;
+    E[4 * m + t] = dh;
+    E[5 * m + t] = db;
+}
+
+// out[c] = sum over k of E[c * n + k] for c < NC: one block, each thread a
+// fixed stride, then a fixed tree (deterministic)
+template <int NC>
+__global__ __launch_bounds__(256) void k_sum_cols(const double *__restrict__ E, int n, double *__restrict__ out)
+{
+    double acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[c] += E[(size_t)c * n + k];
+    block_sum_store<NC>(acc, out);
+}
+
+extern "C" int sesmd_energies(sesmd_ctx *c, double *e)
+{
+    if (!c || !e) return fail("null argument");
+    if (!c->built) return fail("sesmd_energies before sesmd_rebuild");
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
+    // the near-field partials are free between steps: per-site values go there
+    double *E = c->npart, *EI = c->npart + (size_t)E_SITE_COLS * g.n;
+    k_gather_pos<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->r, c->type, c->dff, c->spos);
+    (g.p == 4 ? k_energy_sites<4> : k_energy_sites<8>)<<<nblk(g.n, 128), 128, 0, st>>>(
+        g, c->spos, c->smeta, c->bstart, c->rkind, c->cbox, c->L4, c->lfac, c->dff, c->gr, c->gstart,
+        c->gidx, c->ngr, E);
+    k_sum_cols<E_SITE_COLS><<<1, 256, 0, st>>>(E, g.n, c->esum);
+    if (g.ns) {
+        k_energy_intra<<<nblk(29 * g.ns, 64), 64, 0, st>>>(g, c->r, c->isub, c->dff, c->dtopo, EI);
+        k_sum_cols<E_INTRA_COLS><<<1, 256, 0, st>>>(EI, 29 * g.ns, c->esum + E_SITE_COLS);
+    } else {
+        k_fill_d<<<1, 64, 0, st>>>(c->esum + E_SITE_COLS, E_INTRA_COLS, 0.0);
+    }
+    int nbk = nblk(g.nmol, 256);
+    if (nbk > c->nblk_w + c->nblk_s + 1) return fail("energies: partial buffer too small");
+    k_kinetic<<<nbk, 256, 0, st>>>(g, c->v, c->cls, c->partials);
+    k_sum_partials<<<1, 256, 0, st>>>(c->partials, nbk, c->nh + 16);
+    HC(hipGetLastError());
+    double es[E_SITE_COLS + E_INTRA_COLS], nh[16 + NH_VALS];
+    HC(hipMemcpyAsync(es, c->esum, sizeof es, hipMemcpyDeviceToHost, st));
+    HC(hipMemcpyAsync(nh, c->nh, sizeof nh, hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    for (int k = 0; k < SESMD_E_COUNT; k++) e[k] = 0.0;
+    e[SESMD_E_NEAR_COUL] = es[0];
+    e[SESMD_E_NEAR_LJ] = es[1];
+    e[SESMD_E_VIRIAL] = es[2];
+    e[SESMD_E_FAR] = es[3];
+    e[SESMD_E_WALL] = es[4];
+    e[SESMD_E_GRAPHITE] = es[5];
+    e[SESMD_E_INTERNAL_COUL] = es[6];
+    e[SESMD_E_INTERNAL_LJ] = es[7];
+    e[SESMD_E_SUBTRACT] = es[8];
+    e[SESMD_E_ANGLE] = es[9];
+    e[SESMD_E_DIHEDRAL] = es[10];
+    e[SESMD_E_DIHEDRAL_BE] = es[11];
+    double pot = 0.0;
+    for (int k = 0; k < SESMD_E_POTENTIAL; k++) pot += e[k];
+    e[SESMD_E_POTENTIAL] = pot;
+    // kinetic: the nine group sums of |v dt|^2 weighted by the group masses
+    const double *t = nh + 16, *m = c->ff.mass;
+    double mv2 = m[0] * (t[0] + t[2]) + m[1] * (t[1] + t[3]) + m[3] * t[4] + m[4] * t[5] + m[5] * t[6] +
+                 m[3 + 17] * t[7] + m[3 + 18] * t[8];
+    double dt = g.dt, kin = 0.5 * mv2 / (dt * dt);
+    e[SESMD_E_KINETIC] = kin;
+    int df = 3 * g.n - (3 * g.nw + 28 * g.ns);
+    e[SESMD_E_TEMPERATURE] = df > 0 ? 2.0 * kin / df : 0.0;
+    double eta = nh[0], pnh1 = nh[8], pnh2 = nh[9], xi = nh[10];
+    e[SESMD_E_THERMOSTAT] = pnh1 != 0.0 ? 0.25 * dt / pnh1 * eta * eta - 0.5 * pnh2 * xi : 0.0;
+    e[SESMD_E_TOTAL] = pot + kin + e[SESMD_E_THERMOSTAT];
+    return 0;
 }
 
 extern "C" int sesmd_kinetic_sums(sesmd_ctx *c, double *sums9)

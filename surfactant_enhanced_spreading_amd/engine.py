@@ -114,12 +114,19 @@ def lib():
         L.sesmd_stream.restype = vp
         L.sesmd_near_only.argtypes = [vp]
         L.sesmd_pair_counts.argtypes = [vp, dp, dp]
+        L.sesmd_energies.argtypes = [vp, dp]
         _lib = L
     return _lib
 
 
 def _p(a, t=ctypes.c_double):
     return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+# sesmd_energies order (SESMD_E_* in include/sesmd.h)
+ENERGY_NAMES = ('near_coul', 'near_lj', 'far', 'subtract', 'internal_coul', 'internal_lj',
+                'angle', 'dihedral', 'dihedral_be', 'wall', 'graphite', 'potential', 'kinetic',
+                'thermostat', 'total', 'virial', 'temperature')
 
 
 class SesmdError(RuntimeError):
@@ -260,6 +267,12 @@ class Engine:
 
     def stream(self):
         return self.L.sesmd_stream(self.h)
+
+    def energies(self):
+        """Energy components at the current state (sesmd_energies; FP64)."""
+        e = np.zeros(len(ENERGY_NAMES))
+        _ck(self.L.sesmd_energies(self.h, _p(e)))
+        return dict(zip(ENERGY_NAMES, e))
 
     def near_only(self):
         _ck(self.L.sesmd_near_only(self.h))
