@@ -12,6 +12,8 @@ through oracle/refharness.py (the 27-rank COMMON emulation):
   small_system.npz  inputs + bootstrap forces per component, per reference
                     kernel, classes, isub, far-field expansions, one full step
   medium_total.npz  inputs + total bootstrap force of a larger system
+  velocities.json   the first 64 values of the reference's generator
+                    (src/random.f90) from the start-up seed, as hex floats
   formats.json      SHA-256 digests of the reference's own file output
                     (make_graphite's graphite.dat, write_pdb's snapshot of the
                     file-format test system) and of the positions/velocities
@@ -183,12 +185,28 @@ def formats(ref):
         fh.write('\n')
 
 
+def velocities(ref):
+    import ctypes
+    f = ref.lib.random_
+    f.argtypes = [ctypes.POINTER(ctypes.c_double)] * 2
+    x, seq = 0.378389247, []
+    for _ in range(64):
+        a, b = ctypes.c_double(x), ctypes.c_double()
+        f(ctypes.byref(a), ctypes.byref(b))
+        x = b.value
+        seq.append(x.hex())
+    with open(os.path.join(HERE, 'velocities.json'), 'w') as fh:
+        json.dump(dict(seed=0.378389247, lcg=seq), fh, indent=1)
+        fh.write('\n')
+
+
 if __name__ == '__main__':
     ref = rh.RefLib()
-    if len(sys.argv) > 1 and sys.argv[1] == 'formats':
-        formats(ref)
+    if len(sys.argv) > 1 and sys.argv[1] in ('formats', 'velocities'):
+        {'formats': formats, 'velocities': velocities}[sys.argv[1]](ref)
         sys.exit(0)
     formats(ref)
+    velocities(ref)
     tables(ref)
     ylm(ref)
     small(ref)
