@@ -128,6 +128,8 @@ struct sesmd_ctx {
     double2 *O4, *L4, *O3, *L3, *O2, *L2;
     int *ne4, *ne3, *ne2;         // nonempty flags per box
     double2 *T4, *T3, *T2;        // M2L operators per offset
+    double *AF4, *AF3;            // the same, real-expanded in MFMA operand order (k_m2l_mfma)
+    double *m2l_part;             // k_m2l_mfma partial sums [tile][split][rows][16]
     double2 *top_part;            // top-level M2L partials per (target, source)
     double2 *M43, *M32, *S23, *S34; // M2M / L2L operators per child
     double *c4;                   // L4 centres (3 per box)
@@ -1240,6 +1242,171 @@ __global__ __launch_bounds__(64 * W) void k_m2l(Geo g, int nd, const double2 *__
     cur.y += acc.y;
     Lt[jj * (jj + 1) + k] = cur;
     if (k > 0) Lt[jj * (jj + 1) - k] = make_double2(cur.x, -cur.y);
+}
+
+// M2L on the matrix cores.  The targets of one parity share their candidate
+// offsets, so for one offset the translation of 16 targets' sources is one
+// real matrix product: [L_re; L_im] (2 lk rows) += [[T_re, -T_im], [T_im, T_re]]
+// [O_re; O_im] (2 lp columns, interleaved).  A tile is 16 targets of one
+// parity (the N = 16 columns of v_mfma_f64_16x16x4f64); the operator comes
+// pre-arranged in A-operand order (AF), the sources' expansions are the B
+// operand.  Block = (tile, split of the 216 candidates), four wavefronts taking
+// the split's candidates in turn; their sums are added in wave order and
+// stored per split; k_m2l_mfma_sum adds the splits in order (deterministic).
+// Split counts measured best on the bench workload (L4 time flat from 4 to 32).
+constexpr int M2L_SPLIT4 = 9, M2L_SPLIT3 = 27;
+typedef double m2l_d4 __attribute__((ext_vector_type(4)));
+
+template <int MT, int KS>
+__global__ __launch_bounds__(256) void k_m2l_mfma(Geo g, int nd, int tiles_per_par, int S,
+                                                  const double *__restrict__ O, const int *__restrict__ ne,
+                                                  const double *__restrict__ AF, double *__restrict__ part)
+{
+    __shared__ m2l_d4 s_acc[4][MT][64];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int tile = blockIdx.x / S, split = blockIdx.x % S;
+    int par = tile / tiles_per_par, tl = tile % tiles_per_par;
+    int px = par & 1, py = (par >> 1) & 1, pz = par >> 2;
+    int h = nd / 2, n = lane & 15, kq = lane >> 4;
+    int q = 16 * tl + n;
+    bool valid_t = q < h * h * h;
+    int tx = 2 * (q % h) + px, ty = 2 * ((q / h) % h) + py, tz = 2 * (q / (h * h)) + pz;
+    int lp2 = 2 * g.lp;
+    // two accumulators per row block (even / odd k steps): twice the
+    // independent chains for the matrix core
+    m2l_d4 acc[MT][2];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) acc[mt][0] = acc[mt][1] = (m2l_d4){0.0, 0.0, 0.0, 0.0};
+    int i1 = (split + 1) * 216 / S;
+    // the wave's candidates: i0 + w, i0 + w + 4, ... less the 27 near offsets (wave-uniform)
+    auto next = [&](int i) {
+        for (; i < i1; i += 4) {
+            int dx = i % 6 - 2 - px, dy = (i / 6) % 6 - 2 - py, dz = i / 36 - 2 - pz;
+            if (!(iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1)) return i;
+        }
+        return i1;
+    };
+    // p = 4: operands of candidate i into set k, the next candidate's loaded
+    // while the current one's products run; larger orders (whose operands
+    // would not fit the registers twice) load per k step instead
+    constexpr bool PIPE = MT * KS <= 32;
+    constexpr int NA = PIPE ? MT * KS : 1, NB = PIPE ? KS : 1;
+    double a[2][NA], b[2][NB];
+    auto source = [&](int i, const double *&src, const double *&af, bool &valid) {
+        int dx = i % 6 - 2 - px, dy = (i / 6) % 6 - 2 - py, dz = i / 36 - 2 - pz;
+        int sx = tx + dx, sy = ty + dy, sz = tz + dz;
+        valid = valid_t && sx >= 0 && sx < nd && sy >= 0 && sy < nd && sz >= 0 && sz < nd;
+        int S_ = valid ? sx + nd * (sy + nd * sz) : 0;
+        valid = valid && ne[S_];
+        src = O + (size_t)S_ * lp2;
+        int off = (dx + 3) + 7 * ((dy + 3) + 7 * (dz + 3));
+        af = AF + (size_t)off * MT * KS * 64 + lane;
+    };
+    auto load = [&](int i, int k) {
+        const double *src, *af;
+        bool valid;
+        source(i, src, af, valid);
+#pragma unroll
+        for (int e = 0; e < MT * KS; e++) a[k][e] = af[e * 64];
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            int kk = 4 * ks + kq;
+            double v = src[min(kk, lp2 - 1)];  // always inside the source's row
+            b[k][ks] = (valid && kk < lp2) ? v : 0.0;
+        }
+    };
+    auto product = [&](int k) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+                acc[mt][ks & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[k][mt * KS + ks], b[k][ks],
+                                                                       acc[mt][ks & 1], 0, 0, 0);
+    };
+    if constexpr (PIPE) {
+        int i = next(split * 216 / S + w);
+        if (i < i1) load(i, 0);
+        while (i < i1) {
+            int j = next(i + 4);
+            if (j < i1) load(j, 1);
+            product(0);
+            if (j >= i1) break;
+            i = next(j + 4);
+            if (i < i1) load(i, 0);
+            product(1);
+        }
+    } else {
+        for (int i = next(split * 216 / S + w); i < i1; i = next(i + 4)) {
+            const double *src, *af;
+            bool valid;
+            source(i, src, af, valid);
+            for (int ks = 0; ks < KS; ks++) {
+                int kk = 4 * ks + kq;
+                double bv = src[min(kk, lp2 - 1)];
+                bv = (valid && kk < lp2) ? bv : 0.0;
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++)
+                    acc[mt][ks & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[(mt * KS + ks) * 64], bv,
+                                                                           acc[mt][ks & 1], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+        m2l_d4 t = acc[mt][0], u = acc[mt][1];
+        s_acc[w][mt][lane] = (m2l_d4){t.x + u.x, t.y + u.y, t.z + u.z, t.w + u.w};
+    }
+    __syncthreads();
+    if (w != 0) return;
+    // output element (16 mt + 4 r + lane / 16, lane % 16) of fragment mt, register r
+    double *dst = part + (size_t)(tile * S + split) * MT * 16 * 16;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+        m2l_d4 t = s_acc[0][mt][lane];
+        for (int q2 = 1; q2 < 4; q2++) {
+            m2l_d4 u = s_acc[q2][mt][lane];
+            t.x += u.x;
+            t.y += u.y;
+            t.z += u.z;
+            t.w += u.w;
+        }
+        int row = 16 * mt + kq;
+        dst[(row + 0) * 16 + n] = t.x;
+        dst[(row + 4) * 16 + n] = t.y;
+        dst[(row + 8) * 16 + n] = t.z;
+        dst[(row + 12) * 16 + n] = t.w;
+    }
+}
+
+// Adds the splits of k_m2l_mfma in order onto L (the L2L part already there):
+// one thread per (tile, column, k >= 0 row).
+__global__ void k_m2l_mfma_sum(Geo g, int nd, int tiles_per_par, int S, int MT, int ntiles,
+                               const double *__restrict__ part, double2 *__restrict__ L)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int lk = g.lk;
+    if (t >= ntiles * 16 * lk) return;
+    int row = t % lk, n = (t / lk) % 16, tile = t / (lk * 16);
+    int par = tile / tiles_per_par, tl = tile % tiles_per_par;
+    int h = nd / 2, q = 16 * tl + n;
+    if (q >= h * h * h) return;
+    int tx = 2 * (q % h) + (par & 1), ty = 2 * ((q / h) % h) + ((par >> 1) & 1),
+        tz = 2 * (q / (h * h)) + (par >> 2);
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < S; s++) {
+        const double *P = part + (size_t)(tile * S + s) * MT * 16 * 16;
+        re += P[(2 * row) * 16 + n];
+        im += P[(2 * row + 1) * 16 + n];
+    }
+    int j = 0;
+    while ((j + 1) * (j + 2) / 2 <= row) j++;
+    int k = row - j * (j + 1) / 2;
+    double2 *Lt = L + (size_t)(tx + nd * (ty + nd * tz)) * g.lp;
+    double2 cur = Lt[j * (j + 1) + k];
+    cur.x += re;
+    cur.y += im;
+    Lt[j * (j + 1) + k] = cur;
+    if (k > 0) Lt[j * (j + 1) - k] = make_double2(cur.x, -cur.y);
 }
 
 // top level (computeLjk2): every L2 box that is not adjacent to the target
@@ -2408,9 +2575,30 @@ static int upload_ops(sesmd_ctx *c)
                 buf.push_back(make_double2(z.real(), z.imag()));
             }
     };
+    // MFMA operand order of a real-expanded operator: rows 2r + {re, im} of
+    // the output, columns 2c + {re, im} of the input, padded to MT x 16 rows
+    // and KS x 4 columns; fragment (mt, ks), lane l holds element
+    // (16 mt + l % 16, 4 ks + l / 16) -- the A operand of v_mfma_f64_16x16x4f64.
+    const int MT = (2 * lk + 15) / 16, KS = (2 * lp + 3) / 4;
+    std::vector<double> af;
+    auto append_frag = [&]() {
+        for (int mt = 0; mt < MT; mt++)
+            for (int ks = 0; ks < KS; ks++)
+                for (int l = 0; l < 64; l++) {
+                    int mm = 16 * mt + l % 16, kk = 4 * ks + l / 16;
+                    double v = 0.0;
+                    if (mm < 2 * lk && kk < 2 * lp) {
+                        const hostm::C &z = T[(size_t)(mm / 2) * lp + kk / 2];
+                        int ro = mm & 1, ci = kk & 1;
+                        v = ro == ci ? z.real() : (ro == 0 ? -z.imag() : z.imag());
+                    }
+                    af.push_back(v);
+                }
+    };
     // M2L at L4 and L3: offsets in [-3, 3]^3 (spacing rs and 2 rs)
     for (int lvl = 0; lvl < 2; lvl++) {
         buf.clear();
+        af.clear();
         double sp = lvl == 0 ? g.rs : 2.0 * g.rs;
         for (int dz = -3; dz <= 3; dz++)
             for (int dy = -3; dy <= 3; dy++)
@@ -2421,8 +2609,11 @@ static int upload_ops(sesmd_ctx *c)
                         hostm::m2l_op(fact, p, dx * sp, dy * sp, dz * sp, T.data());
                     }
                     append();
+                    append_frag();
                 }
         CK(put(lvl == 0 ? c->T4 : c->T3, buf.size()));
+        HC(hipMemcpy(lvl == 0 ? c->AF4 : c->AF3, af.data(), sizeof(double) * af.size(),
+                     hipMemcpyHostToDevice));
     }
     // M2L at the top level (spacing 4 rs), offsets in [-(nb2-1), nb2-1]^3
     {
@@ -2571,6 +2762,14 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->ne3, g.n33));
     A(dalloc(&c->ne2, g.n2));
     A(dalloc(&c->T4, (size_t)343 * g.lk * g.lp));
+    {
+        size_t mt = (2 * g.lk + 15) / 16, ks = (2 * g.lp + 3) / 4;
+        A(dalloc(&c->AF4, (size_t)343 * mt * ks * 64));
+        A(dalloc(&c->AF3, (size_t)343 * mt * ks * 64));
+        size_t h4 = g.nb / 2, h3 = g.n3 / 2;
+        size_t t4 = 8 * ((h4 * h4 * h4 + 15) / 16), t3 = 8 * ((h3 * h3 * h3 + 15) / 16);
+        A(dalloc(&c->m2l_part, std::max(t4 * M2L_SPLIT4, t3 * M2L_SPLIT3) * mt * 16 * 16));
+    }
     A(dalloc(&c->T3, (size_t)343 * g.lk * g.lp));
     int w2 = 2 * nb2 - 1;
     A(dalloc(&c->T2, (size_t)w2 * w2 * w2 * g.lk * g.lp));
@@ -2679,6 +2878,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
                     c->wkey_out, c->widx_in, c->widx_out, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
+                    c->AF4, c->AF3, c->m2l_part,
                     c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -2826,14 +3026,30 @@ extern "C" int sesmd_rebuild(sesmd_ctx *c)
 }
 
 // M2L over the interaction lists of one level (computeLjk3 / computeLjk4);
-// nd is even at every level (each level halves the one below it)
+// nd is even at every level (each level halves the one below it).  Matrix
+// cores for p = 4 and p = 8; the vector kernel k_m2l stays for other orders.
 static void launch_m2l(sesmd_ctx *c, int nd, const double2 *O, const int *ne, const double2 *T, double2 *L)
 {
     const Geo &g = c->g;
     int h = nd / 2;
+    int tpp = (h * h * h + 15) / 16, ntiles = 8 * tpp;
+    bool l4 = nd == g.nb;
+    int S = l4 ? M2L_SPLIT4 : M2L_SPLIT3;
+    const double *AF = l4 ? c->AF4 : c->AF3;
+    const double *Od = (const double *)O;
+    if (g.p == 4 || g.p == 8) {
+        if (g.p == 4)
+            k_m2l_mfma<2, 13><<<ntiles * S, 256, 0, c->stream>>>(g, nd, tpp, S, Od, ne, AF, c->m2l_part);
+        else
+            k_m2l_mfma<6, 41><<<ntiles * S, 256, 0, c->stream>>>(g, nd, tpp, S, Od, ne, AF, c->m2l_part);
+        int MT = (2 * g.lk + 15) / 16;
+        int nthr = ntiles * 16 * g.lk;
+        k_m2l_mfma_sum<<<nblk(nthr, 256), 256, 0, c->stream>>>(g, nd, tpp, S, MT, ntiles, c->m2l_part, L);
+        return;
+    }
     if (g.lk <= 16) {
-        int nblk = 8 * ((h + 1) / 2) * ((h + 1) / 2) * h;
-        k_m2l<16, 16><<<nblk, 64 * 16, 0, c->stream>>>(g, nd, O, ne, T, L);
+        int nblk_ = 8 * ((h + 1) / 2) * ((h + 1) / 2) * h;
+        k_m2l<16, 16><<<nblk_, 64 * 16, 0, c->stream>>>(g, nd, O, ne, T, L);
     } else {
         k_m2l<64, 8><<<8 * h * h * h, 64 * 8, 0, c->stream>>>(g, nd, O, ne, T, L);
     }
