@@ -67,6 +67,8 @@ struct Geo {
     double rs, rsd, rhs, rc, rc2, xpi, dt, dtsq2;
 };
 
+constexpr int INTRA_MAXC = 60;  // intramolecular contributions per surfactant site, at most 32 + 12 + 16
+
 // per-site term lists of the surfactant topology (built on the host)
 struct Topo {
     int npair[29];
@@ -75,6 +77,13 @@ struct Topo {
     int ang[29][12];    // angle index * 3 + role (0 = i, 1 = j, 2 = k)
     int ndih[29];
     int dih[29][16];    // dihedral index * 4 + role
+    int pslot[29][32];  // term slot of pair[j][q]: 2 p for the pair's first site, 2 p + 1 its second
+    // k_intra_terms: the contribution of term slot t (2 p / 2 p + 1 for pair p,
+    // 2 ni + 3 a + role for angle a, 2 ni + 108 + 4 d + role for dihedral d)
+    // goes to position dst[t] = j * INTRA_MAXC + q of the molecule's table:
+    // site j's q-th contribution, pairs then angles then dihedrals
+    int cnt[29];
+    int dst[2 * 320 + 108 + 120];
 };
 
 // The reference surfactant's 28 bonds (src/init_arrays.f90:229-264), 0-based
@@ -101,6 +110,7 @@ struct sesmd_ctx {
     hipStream_t side;      // surfactant work of a step, beside the main stream
     hipEvent_t ev[4];      // fork/join points of a step
     double *fs;            // intramolecular forces of a step, 29 ns sites (x, y, z blocks)
+    double *fsl;           // k_intra_terms contributions [molecule][site][INTRA_MAXC][xyz]
     // device state
     double *r, *v, *a, *f;  // each 3*N: x block, y block, z block
     int *type, *mol, *first;
@@ -1465,6 +1475,14 @@ __global__ __launch_bounds__(64) void k_l2l(Geo g, int nd_child, const double2 *
 // ------------------------------------------------------------ per-site terms
 
 // L2P (src/evalLjk4.f90:38-68, the same formula for every receiver class) and
+// site i of a SoA (x block, y block, z block of n) array
+__device__ __forceinline__ void ld3(const double *r, int n, int i, double *x)
+{
+    x[0] = r[i];
+    x[1] = r[n + i];
+    x[2] = r[2 * n + i];
+}
+
 // Legendre table P[l + 1][m] = P_l^m(x) for l = -1..p, m = 0..p (P[0][0] = 1,
 // rLegendreP's l = -1 case), by the recurrence of src/rLegendreP.f90:3-48.
 template <int p>
@@ -1510,8 +1528,11 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
                              const int *__restrict__ rkind, const double2 *__restrict__ L4,
                              const double *__restrict__ lfac, const sesmd_forcefield *__restrict__ ff,
                              const double *__restrict__ part, const double4 *__restrict__ spos,
-                             const int4 *__restrict__ smeta, double *__restrict__ f)
+                             const int4 *__restrict__ smeta, double *__restrict__ f,
+                             const double *__restrict__ gr, const int *__restrict__ gstart,
+                             const int *__restrict__ gidx, int ngr)
 {
+    // ngr > 0: the graphite term too (the step pipeline's form)
     // one thread per sorted position k (coalesced reads of the sorted copies)
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= g.n) return;
@@ -1591,6 +1612,34 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
         fy += 48.0 / p13(y - h) + 48.0 / p13(h + y);
         fz += 48.0 / p13(z - h) + 48.0 / p13(h + z);
     }
+    // graphite (src/eval_graphite.f90:13-132): receivers filed in their own box
+    // of the bottom layer, against that column's carbons
+    if (ngr > 0 && (what & SESMD_GRAPHITE)) {
+        int b = cbox[i], rk = rkind[i];
+        if (b < g.nb * g.nb && (rk == 0 || rk == 1 || rk == 3)) {
+            int t = type[i];
+            double c = ff->gr_c[t], s6 = ff->gr_s6[t];
+            if (c != 0.0) {
+                double gx = 0.0, gy = 0.0, gz = 0.0;
+                for (int q = gstart[b]; q < gstart[b + 1]; q++) {
+                    int a = gidx[q];
+                    double xij = x - gr[3 * a], yij = y - gr[3 * a + 1], zij = z - gr[3 * a + 2];
+                    double rijsq = xij * xij + yij * yij + zij * zij;
+                    if (sqrt(rijsq) <= g.rc) {
+                        double rsi = 1.0 / rijsq, r6 = rsi * rsi * rsi;
+                        double flj = c * r6 * (s6 * r6 - 0.5) * rsi;
+                        gx += xij * flj;
+                        gy += yij * flj;
+                        gz += zij * flj;
+                    }
+                }
+                fx += gx;
+                fy += gy;
+                fz += gz;
+            }
+        }
+    }
+
     f[i] = nx + fx;
     f[g.n + i] = ny + fy;
     f[2 * g.n + i] = nz + fz;
@@ -1598,12 +1647,6 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
 
 // ------------------------------------------------------------ intramolecular
 
-__device__ __forceinline__ void ld3(const double *r, int n, int i, double *x)
-{
-    x[0] = r[i];
-    x[1] = r[n + i];
-    x[2] = r[2 * n + i];
-}
 
 // one dihedral's forces on its four sites (src/eval_dihedral.f90:116-271)
 __device__ void dihedral_forces(const sesmd_forcefield *ff, int d, const double *Ri,
@@ -1806,6 +1849,85 @@ __global__ __launch_bounds__(64) void k_intra(Geo g, unsigned what, const double
         f[base + j] += fx;
         f[n + base + j] += fy;
         f[2 * n + base + j] += fz;
+    }
+}
+
+// The same intramolecular terms one thread per (term, molecule), for the step
+// pipeline: every internal pair, angle and dihedral is evaluated once, and
+// each of its sites' contributions goes to its own place in the molecule's
+// table (Topo::dst) -- fsl[((s * 29 + j) * INTRA_MAXC + q) * 3 + d] -- which
+// part B then adds per site in order (no atomics).  Terms outside `what`
+// write zeros.
+__device__ __forceinline__ void intra_terms(int t, Geo g, unsigned what, const double *__restrict__ r,
+                                            const sesmd_forcefield *__restrict__ ff,
+                                            const Topo *__restrict__ topo, double *__restrict__ fsl)
+{
+    int ns = g.ns, ni = ff->n_internal;
+    if (t >= (ni + 66) * ns) return;
+    int term = t / ns, s = t % ns;
+    int base = 3 * g.nw + 29 * s, n = g.n;
+    double *mol = fsl + (size_t)s * 29 * INTRA_MAXC * 3;
+    auto put = [&](int slot, double x, double y, double z) {
+        double *o = mol + (size_t)topo->dst[slot] * 3;
+        o[0] = x;
+        o[1] = y;
+        o[2] = z;
+    };
+    if (term < ni) {
+        int j = ff->int_j[term] - 1, k = ff->int_k[term] - 1;
+        double xs[3], xk[3];
+        ld3(r, n, base + j, xs);
+        ld3(r, n, base + k, xk);
+        double x = xs[0] - xk[0], y = xs[1] - xk[1], z = xs[2] - xk[2];
+        double ft = 0.0;
+        if (what & SESMD_INTERNAL) {
+            double rsq = x * x + y * y + z * z;
+            double rctr = sqrt(rsq);
+            int tj = 3 + j, tk = 3 + k;
+            double fel = ff->q[tj] * ff->q[tk] * g.xpi / (rctr * rctr * rctr);
+            double rsi = 1.0 / rsq, r6 = rsi * rsi * rsi;
+            double flj = ff->lj_c[tj * 32 + tk] * r6 * (ff->lj_s6[tj * 32 + tk] * r6 - 0.5) * rsi;
+            ft = fel + flj;
+        }
+        put(2 * term, x * ft, y * ft, z * ft);
+        put(2 * term + 1, -(x * ft), -(y * ft), -(z * ft));
+    } else if (term < ni + 36) {
+        int a = term - ni, sl = 2 * ni + 3 * a;
+        double ri[3], rj[3], rk[3];
+        ld3(r, n, base + ff->ang_i[a] - 1, ri);
+        ld3(r, n, base + ff->ang_j[a] - 1, rj);
+        ld3(r, n, base + ff->ang_k[a] - 1, rk);
+        double rxji = ri[0] - rj[0], ryji = ri[1] - rj[1], rzji = ri[2] - rj[2];
+        double rxki = rk[0] - rj[0], ryki = rk[1] - rj[1], rzki = rk[2] - rj[2];
+        double dp = rxji * rxki + ryji * ryki + rzji * rzki;
+        double rjin = sqrt(rxji * rxji + ryji * ryji + rzji * rzji);
+        double rkin = sqrt(rxki * rxki + ryki * ryki + rzki * rzki);
+        double th = acos(dp / rjin / rkin);
+        double costh = cos(th);
+        double r1 = (what & SESMD_VALENCE) ? ff->rkth[a] * (th - ff->th[a]) / sin(th) : 0.0;
+        double fjx = rxki / rjin / rkin - costh * rxji / (rjin * rjin);
+        double fjy = ryki / rjin / rkin - costh * ryji / (rjin * rjin);
+        double fjz = rzki / rjin / rkin - costh * rzji / (rjin * rjin);
+        double fkx = rxji / rjin / rkin - costh * rxki / (rkin * rkin);
+        double fky = ryji / rjin / rkin - costh * ryki / (rkin * rkin);
+        double fkz = rzji / rjin / rkin - costh * rzki / (rkin * rkin);
+        put(sl, r1 * fjx, r1 * fjy, r1 * fjz);
+        put(sl + 1, -(r1 * (fjx + fkx)), -(r1 * (fjy + fky)), -(r1 * (fjz + fkz)));
+        put(sl + 2, r1 * fkx, r1 * fky, r1 * fkz);
+    } else {
+        int d = term - ni - 36, sl = 2 * ni + 108 + 4 * d;
+        bool on = d < 18 ? (what & SESMD_DIHEDRAL) : (what & SESMD_DIHEDRAL_BE);
+        double F[4][3] = {};
+        if (on) {
+            double Ri[3], Rj[3], Rk[3], Rl[3];
+            ld3(r, n, base + ff->dih_i[d] - 1, Ri);
+            ld3(r, n, base + ff->dih_j[d] - 1, Rj);
+            ld3(r, n, base + ff->dih_k[d] - 1, Rk);
+            ld3(r, n, base + ff->dih_l[d] - 1, Rl);
+            dihedral_forces(ff, d, Ri, Rj, Rk, Rl, F);
+        }
+#pragma unroll
+        for (int role = 0; role < 4; role++) put(sl + role, F[role][0], F[role][1], F[role][2]);
     }
 }
 
@@ -2206,9 +2328,20 @@ __device__ __forceinline__ void surf_a(
 // site i's new acceleration and velocity, lane c < 28 owns bond c in the
 // pipelined RATTLE (reference order and tolerance).  The two divisions of the
 // correction are folded into one factor per bond (within 2 ulp).
+// Intramolecular force on a surfactant site in part B: fs (k_intra's per-site
+// totals), or the site's k_intra_terms contributions in order plus the
+// eval_subtract_fma terms from the molecule's positions in LDS.
+struct IntraIn {
+    const double *fs = nullptr;
+    const double *fsl = nullptr;
+    const Topo *topo = nullptr;
+    const unsigned *isub = nullptr;
+    const sesmd_forcefield *ff = nullptr;
+};
+
 __device__ __forceinline__ void surf_b(
     int blk, Geo g, const double *__restrict__ r, double *__restrict__ v, double *__restrict__ a,
-    const double *__restrict__ f, const double *__restrict__ fs, SurfConst sc,
+    const double *__restrict__ f, IntraIn in, SurfConst sc,
     const double *__restrict__ nh, int *__restrict__ counters)
 {
     __shared__ SurfMolLds L;
@@ -2217,13 +2350,50 @@ __device__ __forceinline__ void surf_b(
     if (s >= g.ns) return;  // wave-uniform; no block-wide barrier follows
     int n = g.n, base = 3 * g.nw + 29 * s;
     bool site = lane < 29;
+    if (site)
+#pragma unroll
+        for (int d = 0; d < 3; d++) L.r[w][lane][d] = r[d * n + base + lane];
+    __asm__ volatile("" ::: "memory");
     if (site) {
+        double fi[3] = {0.0, 0.0, 0.0};
+        if (in.fsl) {
+            const double *c = in.fsl + ((size_t)s * 29 + lane) * INTRA_MAXC * 3;
+            int cnt = in.topo->cnt[lane];
+            // in order, the loads of a chunk of 8 contributions issued together
+            for (int q0 = 0; q0 < cnt; q0 += 8) {
+                double t[8][3];
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (q0 + k < cnt)
+#pragma unroll
+                        for (int d = 0; d < 3; d++) t[k][d] = c[3 * (q0 + k) + d];
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (q0 + k < cnt)
+#pragma unroll
+                        for (int d = 0; d < 3; d++) fi[d] += t[k][d];
+            }
+            unsigned mask = in.isub[base + lane];
+            while (mask) {
+                int k = __ffs(mask) - 1;
+                mask &= mask - 1;
+                double xmn = L.r[w][lane][0] - L.r[w][k][0], ymn = L.r[w][lane][1] - L.r[w][k][1],
+                       zmn = L.r[w][lane][2] - L.r[w][k][2];
+                double rmn = sqrt(xmn * xmn + ymn * ymn + zmn * zmn);
+                double fel = in.ff->q[3 + lane] * in.ff->q[3 + k] * g.xpi / (rmn * rmn * rmn);
+                fi[0] -= xmn * fel;
+                fi[1] -= ymn * fel;
+                fi[2] -= zmn * fel;
+            }
+        } else if (in.fs) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) fi[d] = in.fs[d * 29 * g.ns + 29 * s + lane];
+        }
         double escale = 2.0 / (2.0 + nh[0] * g.dt);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            L.r[w][lane][d] = r[d * n + base + lane];
             double fv = f[d * n + base + lane];
-            if (fs) fv += fs[d * 29 * g.ns + 29 * s + lane];
+            if (in.fsl || in.fs) fv += fi[d];
             double ac = fv * g.dtsq2 / sc.m[lane];
             a[d * n + base + lane] = ac;
             L.p[w][lane][d] = escale * (v[d * n + base + lane] + ac);
@@ -2301,6 +2471,22 @@ __global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restri
                                                    double *__restrict__ nh, int update_eta)
 {
     nh_update_block(g, partials, nblocks, ff, nh, update_eta);
+}
+
+// Step pipeline: block 0 updates eta from part A's kinetic sums (k_nh_update),
+// the other blocks evaluate the intramolecular terms into their slots; part B
+// adds each surfactant site's slots in order (surf_b).
+__global__ __launch_bounds__(128) void k_nh_intra(Geo g, const double *__restrict__ partials, int nblocks,
+                                                  const sesmd_forcefield *__restrict__ ff,
+                                                  double *__restrict__ nh, const double *__restrict__ r,
+                                                  const Topo *__restrict__ topo, double *__restrict__ fsl)
+{
+    if (blockIdx.x == 0) {
+        nh_update_block(g, partials, nblocks, ff, nh, 1);
+        return;
+    }
+    intra_terms((blockIdx.x - 1) * blockDim.x + threadIdx.x, g,
+                SESMD_INTERNAL | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE, r, ff, topo, fsl);
 }
 
 // Part B for waters (src/move_w1_b.f90:23-123): a = F dt^2/2m, velocity with
@@ -2415,20 +2601,21 @@ __global__ __launch_bounds__(256) void k_move_b_surf_fast(Geo g, const double *_
                                                               const double *__restrict__ nh,
                                                               int *__restrict__ counters)
 {
-    surf_b(blockIdx.x, g, r, v, a, f, fs, sc, nh, counters);
+    IntraIn in;
+    in.fs = fs;
+    surf_b(blockIdx.x, g, r, v, a, f, in, sc, nh, counters);
 }
 
 __global__ __launch_bounds__(256) void k_move_b_fused(Geo g, const double *__restrict__ r,
                                                       double *__restrict__ v, double *__restrict__ a,
-                                                      const double *__restrict__ f,
-                                                      const double *__restrict__ fs,
+                                                      const double *__restrict__ f, IntraIn in,
                                                       const sesmd_forcefield *__restrict__ ff,
                                                       SurfConst sc, const double *__restrict__ nh,
                                                       int *__restrict__ counters,
                                                       const int *__restrict__ cls, int nblk_s)
 {
     if ((int)blockIdx.x < nblk_s)
-        surf_b(blockIdx.x, g, r, v, a, f, fs, sc, nh, counters);
+        surf_b(blockIdx.x, g, r, v, a, f, in, sc, nh, counters);
     else
         water_b(blockIdx.x - nblk_s, g, r, v, a, f, ff, nh, counters, cls, 7u);
 }
@@ -2517,6 +2704,8 @@ static void build_topology(const sesmd_forcefield &ff, Topo &t)
     memset(&t, 0, sizeof t);
     for (int p = 0; p < ff.n_internal; p++) {
         int j = ff.int_j[p] - 1, k = ff.int_k[p] - 1;
+        t.pslot[j][t.npair[j]] = 2 * p;
+        t.pslot[k][t.npair[k]] = 2 * p + 1;
         t.pair[j][t.npair[j]++] = k;
         t.pair[k][t.npair[k]++] = j;
     }
@@ -2527,6 +2716,14 @@ static void build_topology(const sesmd_forcefield &ff, Topo &t)
     for (int d = 0; d < 30; d++) {
         int s[4] = {ff.dih_i[d] - 1, ff.dih_j[d] - 1, ff.dih_k[d] - 1, ff.dih_l[d] - 1};
         for (int role = 0; role < 4; role++) t.dih[s[role]][t.ndih[s[role]]++] = 4 * d + role;
+    }
+    int ni = ff.n_internal;
+    for (int j = 0; j < 29; j++) {
+        int q = 0;
+        for (int k = 0; k < t.npair[j]; k++) t.dst[t.pslot[j][k]] = j * INTRA_MAXC + q++;
+        for (int k = 0; k < t.nang[j]; k++) t.dst[2 * ni + t.ang[j][k]] = j * INTRA_MAXC + q++;
+        for (int k = 0; k < t.ndih[j]; k++) t.dst[2 * ni + 108 + t.dih[j][k]] = j * INTRA_MAXC + q++;
+        t.cnt[j] = q;
     }
 }
 
@@ -2803,6 +3000,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     c->nblk_s = g.ns > 0 ? (int)nblk(g.ns, c->surf_fast ? SURF_MPB : 64) : 0;
     A(dalloc(&c->partials, (size_t)(c->nblk_w + c->nblk_s + 1) * NH_VALS));
     A(dalloc(&c->fs, 3 * 29 * (size_t)g.ns + 1));
+    A(dalloc(&c->fsl, (size_t)29 * INTRA_MAXC * 3 * g.ns + 1));  // k_intra_terms contributions
     A(dalloc(&c->nh, 16 + NH_VALS));
     A(dalloc(&c->esum, 16));
     A(dalloc(&c->counters, 4));
@@ -2879,7 +3077,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
                     c->AF4, c->AF3, c->m2l_part,
-                    c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs};
+                    c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs, c->fsl};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->side) hipStreamSynchronize(c->side);
@@ -3105,7 +3303,8 @@ static void launch_near(sesmd_ctx *c, uint64_t mask)
 
 // gathered: the sorted copies of the positions are already current (written
 // by the fused part A of a regular step)
-static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask, bool gathered = false)
+// fold: graphite evaluated inside k_site_terms (the step pipeline's form)
+static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask, bool gathered = false, bool fold = false)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
@@ -3115,14 +3314,14 @@ static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask, bool gather
         k_gather_pos32<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->key_out, c->r, c->type, c->dff,
                                                        c->spos32);
     if (what & SESMD_NEAR) launch_near(c, mask);
+    const unsigned intra = SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE;
     (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<nblk(g.n, 128), 128, 0, st>>>(
         g, what, c->r, c->type, c->cbox, c->rkind, c->L4, c->lfac, c->dff, c->npart, c->spos, c->smeta,
-        c->f);
-    if (g.ns && (what & (SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL |
-                         SESMD_DIHEDRAL_BE)))
+        c->f, c->gr, c->gstart, c->gidx, fold ? c->ngr : 0);
+    if (g.ns && (what & intra))
         k_intra<<<nblk(29 * g.ns, 64), 64, 0, st>>>(g, what, c->r, c->isub, c->dff, c->dtopo, c->f,
                                                     nullptr);
-    if ((what & SESMD_GRAPHITE) && c->ngr > 0)
+    if (!fold && (what & SESMD_GRAPHITE) && c->ngr > 0)
         k_graphite<<<g.nb * g.nb, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->gr,
                                                c->gstart, c->gidx, c->dff, c->f);
     HC(hipGetLastError());
@@ -3240,7 +3439,7 @@ extern "C" int sesmd_init_accel(sesmd_ctx *c)
 static int launch_step(sesmd_ctx *c, bool rebuild)
 {
     const Geo &g = c->g;
-    hipStream_t st = c->stream, ss = c->side;
+    hipStream_t st = c->stream;
     bool gathered = false;
     if (c->surf_fast) {
         SortedOut so;
@@ -3254,7 +3453,13 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
         k_move_a_fused<<<c->nblk_s + c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->cls, c->dff,
                                                               c->sconst, c->nh, c->partials, c->counters,
                                                               c->nblk_s, so);
-        k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1);
+        if (g.ns) {  // eta update and the intramolecular terms in one launch
+            int nt = (c->ff.n_internal + 66) * g.ns;
+            k_nh_intra<<<1 + nblk(nt, 128), 128, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff,
+                                                          c->nh, c->r, c->dtopo, c->fsl);
+        } else {
+            k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1);
+        }
     } else {
         if (c->nblk_w)
             k_move_a_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->cls, c->dff, c->nh,
@@ -3269,20 +3474,21 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
         CK(launch_rebuild(c));
         CK(launch_fmm(c));
     }
-    if (g.ns) {
-        HC(hipEventRecord(c->ev[0], st));
-        HC(hipStreamWaitEvent(ss, c->ev[0], 0));
-        k_intra<<<nblk(29 * g.ns, 64), 64, 0, ss>>>(g, SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE |
-                                                          SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE,
-                                                    c->r, c->isub, c->dff, c->dtopo, c->f, c->fs);
-        HC(hipEventRecord(c->ev[1], ss));
-    }
-    CK(launch_forces(c, SESMD_ALL & ~(SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE |
-                                       SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE),
-                     ~0ull, gathered));
-    if (g.ns) HC(hipStreamWaitEvent(st, c->ev[1], 0));
+    // one stream, no cross-queue fork and join: with the reference topology
+    // the intramolecular terms were evaluated with the eta update and part B
+    // adds them per site (with the subtraction); graphite runs inside
+    // k_site_terms
+    const unsigned intra = SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE;
+    CK(launch_forces(c, c->surf_fast ? SESMD_ALL & ~intra : SESMD_ALL, ~0ull, gathered, true));
     if (c->surf_fast) {
-        k_move_b_fused<<<c->nblk_s + c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->fs, c->dff,
+        IntraIn in;
+        if (g.ns) {
+            in.fsl = c->fsl;
+            in.topo = c->dtopo;
+            in.isub = c->isub;
+            in.ff = c->dff;
+        }
+        k_move_b_fused<<<c->nblk_s + c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, in, c->dff,
                                                               c->sconst, c->nh, c->counters, c->cls,
                                                               c->nblk_s);
     } else {
@@ -3290,7 +3496,7 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
             k_move_b_water<<<c->nblk_w, 256, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->counters,
                                                        c->cls, 7u);
         if (c->nblk_s)
-            k_move_b_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->fs, c->dff, c->nh,
+            k_move_b_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, nullptr, c->dff, c->nh,
                                                     c->counters);
     }
     HC(hipGetLastError());
