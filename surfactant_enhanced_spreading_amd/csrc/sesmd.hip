@@ -59,7 +59,8 @@ extern "C" const char *sesmd_last_error(void) { return g_err.c_str(); }
 constexpr int MAXP = 8;                       // highest supported expansion order
 constexpr int MAXLP = (MAXP + 1) * (MAXP + 1);
 constexpr int MAXLK = (MAXP + 1) * (MAXP + 2) / 2;  // rows with k >= 0
-constexpr int NEAR_TILE = 64;                 // receivers per near-field work item
+constexpr int NEAR_TILE = 64;                 // lanes of a near-field wavefront
+constexpr int NEAR_CHUNK = 128;               // receivers per near-field work item (two per lane)
 constexpr int NH_VALS = 18;                   // kinetic partial sums per block
 
 struct Geo {
@@ -78,7 +79,7 @@ struct Topo {
     int ndih[29];
     int dih[29][16];    // dihedral index * 4 + role
     int pslot[29][32];  // term slot of pair[j][q]: 2 p for the pair's first site, 2 p + 1 its second
-    // k_intra_terms: the contribution of term slot t (2 p / 2 p + 1 for pair p,
+    // k_nh_intra: the contribution of term slot t (2 p / 2 p + 1 for pair p,
     // 2 ni + 3 a + role for angle a, 2 ni + 108 + 4 d + role for dihedral d)
     // goes to position dst[t] = j * INTRA_MAXC + q of the molecule's table:
     // site j's q-th contribution, pairs then angles then dihedrals
@@ -110,7 +111,6 @@ struct sesmd_ctx {
     hipStream_t side;      // surfactant work of a step, beside the main stream
     hipEvent_t ev[4];      // fork/join points of a step
     double *fs;            // intramolecular forces of a step, 29 ns sites (x, y, z blocks)
-    double *fsl;           // k_intra_terms contributions [molecule][site][INTRA_MAXC][xyz]
     // device state
     double *r, *v, *a, *f;  // each 3*N: x block, y block, z block
     int *type, *mol, *first;
@@ -512,8 +512,7 @@ __global__ void k_bstart(Geo g, const int *__restrict__ keys, int *__restrict__ 
     if ((kk & 3) == 0) bstart[kk >> 2] = lo;
 }
 
-// near-field work list: one item per NEAR_TILE receivers of a box; single block
-// Near-field work items: per box, chunks of NEAR_TILE receivers, in box order,
+// Near-field work items (one block): per box, chunks of NEAR_CHUNK receivers, in box order,
 // each with a cost estimate (receivers x sites in the 3^3 block) as a sort key;
 // the items are then ordered by decreasing cost (k_work_gather), so the longest
 // workgroups start first and the short ones fill in behind them.
@@ -528,7 +527,7 @@ __global__ void k_worklist(Geo g, const int *__restrict__ bstart, int2 *__restri
     for (int b0 = 0; b0 < g.nb3; b0 += blockDim.x) {
         int b = b0 + threadIdx.x;
         int nch = 0;
-        if (b < g.nb3) nch = (bstart[b + 1] - bstart[b] + NEAR_TILE - 1) / NEAR_TILE;
+        if (b < g.nb3) nch = (bstart[b + 1] - bstart[b] + NEAR_CHUNK - 1) / NEAR_CHUNK;
         s_cnt[threadIdx.x] = nch;
         __syncthreads();
         // inclusive scan (Hillis-Steele)
@@ -550,8 +549,8 @@ __global__ void k_worklist(Geo g, const int *__restrict__ bstart, int2 *__restri
         }
         for (int c = 0; c < nch; c++)
             if (excl + c < max_work) {
-                int k0 = bstart[b] + c * NEAR_TILE;
-                int nr = min(NEAR_TILE, bstart[b + 1] - k0);
+                int k0 = bstart[b] + c * NEAR_CHUNK;
+                int nr = min(NEAR_CHUNK, bstart[b + 1] - k0);
                 work[excl + c] = make_int2(b, k0);
                 wkey[excl + c] = 1u + (unsigned)min(nr * src, 1 << 30);
                 widx[excl + c] = excl + c;
@@ -645,11 +644,13 @@ struct Recv {
     double x, y, z, q;       // receiver position, charge * xpi
     double qO, qH;           // q * (water O charge), q * (water H charge)
     double cO, s6O;          // LJ against water oxygen
-    const double *ljc, *ljs; // LJ row of the receiver's type
-    int mol, rk;
+    double rc2O;             // rc^2 where the receiver takes LJ from water O, else -1
+    int ljrow, mol, rk;      // LJ table row of the receiver's type (32 * type)
 };
 
-// Sources t in [s0, s1) of one LJ class for one receiver per lane.
+// Sources t in [s0, s1) of one LJ class for NR receivers per lane (lane l
+// holds receivers l and 64 + l of its chunk; their sums are independent, so
+// each receiver adds the same terms in the same order either way).
 // CAT 0: water O (LJ with the receiver's O parameters), 1: surfactant sites (LJ
 // from the type table), 2: water H (Coulomb only).  CHECK: test the storage
 // ring of every source (only rows holding a site that some neighbour box does
@@ -657,58 +658,90 @@ struct Recv {
 // CEN: the segment lies in the receivers' own box (every source may share a
 // molecule with a receiver); elsewhere only sources of split molecules can.
 // HQ: both water hydrogens carry one charge, so q_i q_j is per lane for CAT 2.
-template <int CAT, bool CHECK, bool MASKED, bool CEN, bool HQ>
+// The LJ term is evaluated only when some lane holds a pair inside rc (a
+// wave-uniform branch): most LJ-class sources lie beyond rc of every receiver
+// of the chunk.  Where a pair is outside rc or its LJ constant is zero the
+// term is not added, as before the test.
+template <int CAT, bool CHECK, bool MASKED, bool CEN, bool HQ, int NR>
 __device__ __forceinline__ void near_pair(const Geo &g, const double4 &pj, const int4 &mj, int bx, int by,
-                                          int bz, const Recv &R, unsigned long long mask, double &ax,
-                                          double &ay, double &az)
+                                          int bz, const Recv (&R)[NR], const double *__restrict__ ljc,
+                                          const double *__restrict__ ljs, unsigned long long mask,
+                                          double (&ax)[NR], double (&ay)[NR], double (&az)[NR])
 {
-    bool ok = mj.z != R.mol;
+    int sk = 0;
     if (CHECK || MASKED) {
         int cls = (mj.y >> 8) & 3;
         int ring = max(abs((mj.w & 255) - bx),
                        max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
-        int sk = cls == 0 ? (ring <= 1 ? ring : -1)
-                          : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
+        sk = cls == 0 ? (ring <= 1 ? ring : -1)
+                      : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
         if (sk < 0) return;  // outside every list of the reference (uniform)
-        if (MASKED) ok = ok && ((mask >> (max(R.rk, 0) * 9 + sk)) & 1ull);
     }
-    double dx = R.x - pj.x, dy = R.y - pj.y, dz = R.z - pj.z;
-    double r2 = dx * dx + dy * dy + dz * dz;
-    double rinv = rsqrt64(r2);
-    double rinv2 = rinv * rinv;
-    double qq = CAT == 0 ? R.qO : (CAT == 2 && HQ) ? R.qH : R.q * pj.w;
-    double ft = qq * (rinv2 * rinv);
+    double dx[NR], dy[NR], dz[NR], r2[NR], rinv2[NR], ft[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        dx[r] = R[r].x - pj.x;
+        dy[r] = R[r].y - pj.y;
+        dz[r] = R[r].z - pj.z;
+        r2[r] = dx[r] * dx[r] + dy[r] * dy[r] + dz[r] * dz[r];
+        double rinv = rsqrt64(r2[r]);
+        rinv2[r] = rinv * rinv;
+        double qq = CAT == 0 ? R[r].qO : (CAT == 2 && HQ) ? R[r].qH : R[r].q * pj.w;
+        ft[r] = qq * (rinv2[r] * rinv);
+    }
     if (CAT != 2) {
-        double c, s6;
-        if (CAT == 0) {
-            c = R.cO;
-            s6 = R.s6O;
-        } else {
-            int tj = mj.y & 255;
-            c = R.ljc[tj];
-            s6 = R.ljs[tj];
+        bool in[NR], any = false;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            in[r] = r2[r] <= (CAT == 0 ? R[r].rc2O : g.rc2);
+            any = any || in[r];
         }
-        double r6 = rinv2 * rinv2 * rinv2;
-        double flj = c * r6 * (s6 * r6 - 0.5) * rinv2;
-        ft = (c != 0.0 && r2 <= g.rc2) ? ft + flj : ft;
+        if (__builtin_amdgcn_ballot_w64(any)) {
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                double c, s6;
+                if (CAT == 0) {
+                    c = R[r].cO;
+                    s6 = R[r].s6O;
+                } else {
+                    int tj = R[r].ljrow + (mj.y & 255);
+                    c = ljc[tj];
+                    s6 = ljs[tj];
+                }
+                double r6 = rinv2[r] * rinv2[r] * rinv2[r];
+                double flj = c * r6 * (s6 * r6 - 0.5) * rinv2[r];
+                ft[r] = (c != 0.0 && in[r]) ? ft[r] + flj : ft[r];
+            }
+        }
     }
-    if (CHECK || MASKED || CEN)
-        ft = ok ? ft : 0.0;
-    else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
+    if (CHECK || MASKED || CEN) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            bool ok = mj.z != R[r].mol;
+            if (MASKED) ok = ok && ((mask >> (max(R[r].rk, 0) * 9 + sk)) & 1ull);
+            ft[r] = ok ? ft[r] : 0.0;
+        }
+    } else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
         __asm__ volatile("");
-        ft = mj.z != R.mol ? ft : 0.0;
+#pragma unroll
+        for (int r = 0; r < NR; r++) ft[r] = mj.z != R[r].mol ? ft[r] : 0.0;
     }
-    ax = fma(dx, ft, ax);
-    ay = fma(dy, ft, ay);
-    az = fma(dz, ft, az);
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        ax[r] = fma(dx[r], ft[r], ax[r]);
+        ay[r] = fma(dy[r], ft[r], ay[r]);
+        az[r] = fma(dz[r], ft[r], az[r]);
+    }
 }
 
-template <int CAT, bool CHECK, bool MASKED, bool CEN, bool HQ>
+template <int CAT, bool CHECK, bool MASKED, bool CEN, bool HQ, int NR>
 __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__restrict__ spos,
                                              const int4 *__restrict__ smeta, int s0, int s1,
-                                             int bx, int by, int bz, const Recv &R,
-                                             unsigned long long mask, double &ax, double &ay,
-                                             double &az)
+                                             int bx, int by, int bz, const Recv (&R)[NR],
+                                             const double *__restrict__ ljc,
+                                             const double *__restrict__ ljs,
+                                             unsigned long long mask, double (&ax)[NR],
+                                             double (&ay)[NR], double (&az)[NR])
 {
     if (s0 >= s1) return;
     // Sources two at a time, the next two loaded while these are used: scalar
@@ -731,39 +764,46 @@ __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__rest
         pA = ps[0];
         pB = ps[1];
         __asm__ volatile("" ::: "memory");  // keeps these loads ahead of the pair arithmetic
-        near_pair<CAT, CHECK, MASKED, CEN, HQ>(g, pa, ma, bx, by, bz, R, mask, ax, ay, az);
-        if (t > 1) near_pair<CAT, CHECK, MASKED, CEN, HQ>(g, pb, mb, bx, by, bz, R, mask, ax, ay, az);
+        near_pair<CAT, CHECK, MASKED, CEN, HQ, NR>(g, pa, ma, bx, by, bz, R, ljc, ljs, mask, ax, ay, az);
+        if (t > 1)
+            near_pair<CAT, CHECK, MASKED, CEN, HQ, NR>(g, pb, mb, bx, by, bz, R, ljc, ljs, mask, ax, ay, az);
     }
 }
 
-template <bool CHECK, bool MASKED, bool CEN, bool HQ>
+template <bool CHECK, bool MASKED, bool CEN, bool HQ, int NR>
 __device__ __forceinline__ void near_box(const Geo &g, const double4 *__restrict__ spos,
                                          const int4 *__restrict__ smeta, const int *__restrict__ bs4,
-                                         int bb, int bx, int by, int bz, const Recv &R,
-                                         unsigned long long mask, double &ax, double &ay, double &az)
+                                         int bb, int bx, int by, int bz, const Recv (&R)[NR],
+                                         const double *__restrict__ ljc, const double *__restrict__ ljs,
+                                         unsigned long long mask, double (&ax)[NR], double (&ay)[NR],
+                                         double (&az)[NR])
 {
     // one source box: the classes O, S, H
     int k0 = __builtin_amdgcn_readfirstlane(bs4[4 * bb]);
     int k1 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 1]);
     int k2 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 2]);
     int k3 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 3]);
-    near_segment<0, CHECK, MASKED, CEN, HQ>(g, spos, smeta, k0, k1, bx, by, bz, R, mask, ax, ay, az);
-    near_segment<1, CHECK, MASKED, CEN, HQ>(g, spos, smeta, k1, k2, bx, by, bz, R, mask, ax, ay, az);
-    near_segment<2, CHECK, MASKED, CEN, HQ>(g, spos, smeta, k2, k3, bx, by, bz, R, mask, ax, ay, az);
+    near_segment<0, CHECK, MASKED, CEN, HQ, NR>(g, spos, smeta, k0, k1, bx, by, bz, R, ljc, ljs, mask, ax, ay, az);
+    near_segment<1, CHECK, MASKED, CEN, HQ, NR>(g, spos, smeta, k1, k2, bx, by, bz, R, ljc, ljs, mask, ax, ay, az);
+    near_segment<2, CHECK, MASKED, CEN, HQ, NR>(g, spos, smeta, k2, k3, bx, by, bz, R, ljc, ljs, mask, ax, ay, az);
 }
 
-template <bool CHECK, bool MASKED, bool HQ>
+template <bool CHECK, bool MASKED, bool HQ, int NR>
 __device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict__ spos,
                                          const int4 *__restrict__ smeta, const int *__restrict__ bs4,
-                                         int r0, int r1, int bc, int bx, int by, int bz, const Recv &R,
-                                         unsigned long long mask, double &ax, double &ay, double &az)
+                                         int r0, int r1, int bc, int bx, int by, int bz,
+                                         const Recv (&R)[NR], const double *__restrict__ ljc,
+                                         const double *__restrict__ ljs, unsigned long long mask,
+                                         double (&ax)[NR], double (&ay)[NR], double (&az)[NR])
 {
     // boxes r0..r1 (consecutive indices); bc: the receivers' own box
     for (int bb = r0; bb <= r1; bb++) {
         if (bb == bc)
-            near_box<CHECK, MASKED, true, HQ>(g, spos, smeta, bs4, bb, bx, by, bz, R, mask, ax, ay, az);
+            near_box<CHECK, MASKED, true, HQ, NR>(g, spos, smeta, bs4, bb, bx, by, bz, R, ljc, ljs, mask, ax,
+                                                  ay, az);
         else
-            near_box<CHECK, MASKED, false, HQ>(g, spos, smeta, bs4, bb, bx, by, bz, R, mask, ax, ay, az);
+            near_box<CHECK, MASKED, false, HQ, NR>(g, spos, smeta, bs4, bb, bx, by, bz, R, ljc, ljs, mask, ax,
+                                                   ay, az);
     }
 }
 
@@ -771,16 +811,16 @@ __device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict
 // in FP32 on coordinates relative to the source box's centre, the forces of
 // one (box, class) segment summed in FP32 and then added to the FP64 totals.
 struct Recv32 {
-    float q, qO, qH, cO, s6O, rc2;
-    const float *ljc, *ljs;
+    float q, qO, qH, cO, s6O, rc2O;
 };
 
-template <int CAT, bool CHECK, bool CEN, bool HQ>
+template <int CAT, bool CHECK, bool CEN, bool HQ, int NR>
 __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, int bx, int by, int bz,
-                                            float xr, float yr, float zr, const Recv &R,
-                                            const Recv32 &Q, float &fx, float &fy, float &fz)
+                                            const float (&xr)[NR], const float (&yr)[NR],
+                                            const float (&zr)[NR], const Recv (&R)[NR],
+                                            const Recv32 (&Q)[NR], const float *__restrict__ lj32,
+                                            float rc2, float (&fx)[NR], float (&fy)[NR], float (&fz)[NR])
 {
-    bool ok = mj.z != R.mol;
     if (CHECK) {
         int cls = (mj.y >> 8) & 3;
         int ring = max(abs((mj.w & 255) - bx),
@@ -789,46 +829,72 @@ __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, in
                           : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
         if (sk < 0) return;
     }
-    float dx = xr - pj.x, dy = yr - pj.y, dz = zr - pj.z;
-    float r2 = dx * dx + dy * dy + dz * dz;
-    float rinv = __builtin_amdgcn_rsqf(r2);
-    float rinv2 = rinv * rinv;
-    float qq = CAT == 0 ? Q.qO : (CAT == 2 && HQ) ? Q.qH : Q.q * pj.w;
-    float ft = qq * (rinv2 * rinv);
+    float dx[NR], dy[NR], dz[NR], r2[NR], rinv2[NR], ft[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        dx[r] = xr[r] - pj.x;
+        dy[r] = yr[r] - pj.y;
+        dz[r] = zr[r] - pj.z;
+        r2[r] = dx[r] * dx[r] + dy[r] * dy[r] + dz[r] * dz[r];
+        float rinv = __builtin_amdgcn_rsqf(r2[r]);
+        rinv2[r] = rinv * rinv;
+        float qq = CAT == 0 ? Q[r].qO : (CAT == 2 && HQ) ? Q[r].qH : Q[r].q * pj.w;
+        ft[r] = qq * (rinv2[r] * rinv);
+    }
     if (CAT != 2) {
-        float c, s6;
-        if (CAT == 0) {
-            c = Q.cO;
-            s6 = Q.s6O;
-        } else {
-            int tj = mj.y & 255;
-            c = Q.ljc[tj];
-            s6 = Q.ljs[tj];
+        bool in[NR], any = false;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            in[r] = r2[r] <= (CAT == 0 ? Q[r].rc2O : rc2);
+            any = any || in[r];
         }
-        float r6 = rinv2 * rinv2 * rinv2;
-        float flj = c * r6 * (s6 * r6 - 0.5f) * rinv2;
-        ft = (c != 0.0f && r2 <= Q.rc2) ? ft + flj : ft;
+        if (__builtin_amdgcn_ballot_w64(any)) {
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                float c, s6;
+                if (CAT == 0) {
+                    c = Q[r].cO;
+                    s6 = Q[r].s6O;
+                } else {
+                    int tj = R[r].ljrow + (mj.y & 255);
+                    c = lj32[tj];
+                    s6 = lj32[1024 + tj];
+                }
+                float r6 = rinv2[r] * rinv2[r] * rinv2[r];
+                float flj = c * r6 * (s6 * r6 - 0.5f) * rinv2[r];
+                ft[r] = (c != 0.0f && in[r]) ? ft[r] + flj : ft[r];
+            }
+        }
     }
-    if (CHECK || CEN)
-        ft = ok ? ft : 0.0f;
-    else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
+    if (CHECK || CEN) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) ft[r] = mj.z != R[r].mol ? ft[r] : 0.0f;
+    } else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
         __asm__ volatile("");
-        ft = mj.z != R.mol ? ft : 0.0f;
+#pragma unroll
+        for (int r = 0; r < NR; r++) ft[r] = mj.z != R[r].mol ? ft[r] : 0.0f;
     }
-    fx = fmaf(dx, ft, fx);
-    fy = fmaf(dy, ft, fy);
-    fz = fmaf(dz, ft, fz);
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        fx[r] = fmaf(dx[r], ft[r], fx[r]);
+        fy[r] = fmaf(dy[r], ft[r], fy[r]);
+        fz[r] = fmaf(dz[r], ft[r], fz[r]);
+    }
 }
 
-template <int CAT, bool CHECK, bool CEN, bool HQ>
+template <int CAT, bool CHECK, bool CEN, bool HQ, int NR>
 __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32,
                                                const int4 *__restrict__ smeta, int s0, int s1,
-                                               int bx, int by, int bz, float xr, float yr, float zr,
-                                               const Recv &R, const Recv32 &Q, double &ax,
-                                               double &ay, double &az)
+                                               int bx, int by, int bz, const float (&xr)[NR],
+                                               const float (&yr)[NR], const float (&zr)[NR],
+                                               const Recv (&R)[NR], const Recv32 (&Q)[NR],
+                                               const float *__restrict__ lj32, float rc2,
+                                               double (&ax)[NR], double (&ay)[NR], double (&az)[NR])
 {
     if (s0 >= s1) return;
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    float fx[NR], fy[NR], fz[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) fx[r] = fy[r] = fz[r] = 0.0f;
     const float4 *ps = spos32 + s0;
     const int4 *pm = smeta + s0;
     int4 mA = pm[0], mB = pm[1];
@@ -843,93 +909,99 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
         pA = ps[0];
         pB = ps[1];
         __asm__ volatile("" ::: "memory");  // keeps these loads ahead of the pair arithmetic
-        near_pair32<CAT, CHECK, CEN, HQ>(pa, ma, bx, by, bz, xr, yr, zr, R, Q, fx, fy, fz);
-        if (t > 1) near_pair32<CAT, CHECK, CEN, HQ>(pb, mb, bx, by, bz, xr, yr, zr, R, Q, fx, fy, fz);
+        near_pair32<CAT, CHECK, CEN, HQ, NR>(pa, ma, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
+        if (t > 1)
+            near_pair32<CAT, CHECK, CEN, HQ, NR>(pb, mb, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
     }
-    ax += (double)fx;
-    ay += (double)fy;
-    az += (double)fz;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        ax[r] += (double)fx[r];
+        ay[r] += (double)fy[r];
+        az[r] += (double)fz[r];
+    }
 }
 
-template <bool CHECK, bool HQ>
+template <bool CHECK, bool HQ, int NR>
 __device__ __forceinline__ void near_row32(const Geo &g, const float4 *__restrict__ spos32,
                                            const int4 *__restrict__ smeta, const int *__restrict__ bs4,
                                            int r0, int r1, int bc, int bx, int by, int bz,
-                                           const Recv &R, const Recv32 &Q, double &ax, double &ay,
-                                           double &az)
+                                           const Recv (&R)[NR], const Recv32 (&Q)[NR],
+                                           const float *__restrict__ lj32, double (&ax)[NR],
+                                           double (&ay)[NR], double (&az)[NR])
 {
     int nb = g.nb;
-    float yr = (float)(R.y - d_cen4(g, (r0 / nb) % nb)), zr = (float)(R.z - d_cen4(g, r0 / (nb * nb)));
+    float rc2 = (float)g.rc2;
+    double cy = d_cen4(g, (r0 / nb) % nb), cz = d_cen4(g, r0 / (nb * nb));
+    float yr[NR], zr[NR], xr[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        yr[r] = (float)(R[r].y - cy);
+        zr[r] = (float)(R[r].z - cz);
+    }
     for (int bb = r0; bb <= r1; bb++) {
-        float xr = (float)(R.x - d_cen4(g, bb % nb));
+        double cx = d_cen4(g, bb % nb);
+#pragma unroll
+        for (int r = 0; r < NR; r++) xr[r] = (float)(R[r].x - cx);
         int k0 = __builtin_amdgcn_readfirstlane(bs4[4 * bb]);
         int k1 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 1]);
         int k2 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 2]);
         int k3 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 3]);
         if (bb == bc) {
-            near_segment32<0, CHECK, true, HQ>(spos32, smeta, k0, k1, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
-            near_segment32<1, CHECK, true, HQ>(spos32, smeta, k1, k2, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
-            near_segment32<2, CHECK, true, HQ>(spos32, smeta, k2, k3, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
+            near_segment32<0, CHECK, true, HQ, NR>(spos32, smeta, k0, k1, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<1, CHECK, true, HQ, NR>(spos32, smeta, k1, k2, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<2, CHECK, true, HQ, NR>(spos32, smeta, k2, k3, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
         } else {
-            near_segment32<0, CHECK, false, HQ>(spos32, smeta, k0, k1, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
-            near_segment32<1, CHECK, false, HQ>(spos32, smeta, k1, k2, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
-            near_segment32<2, CHECK, false, HQ>(spos32, smeta, k2, k3, bx, by, bz, xr, yr, zr, R, Q, ax, ay, az);
+            near_segment32<0, CHECK, false, HQ, NR>(spos32, smeta, k0, k1, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<1, CHECK, false, HQ, NR>(spos32, smeta, k1, k2, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<2, CHECK, false, HQ, NR>(spos32, smeta, k2, k3, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
         }
     }
 }
 
-template <bool MASKED, bool HQ, bool MIX>
-__global__ __launch_bounds__(NEAR_TILE) void k_near(
-    Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
-    const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
-    const int *__restrict__ rkind, const int2 *__restrict__ work, const int *__restrict__ nwork,
-    const sesmd_forcefield *__restrict__ ff, unsigned long long mask, double *__restrict__ f,
-    const float4 *__restrict__ spos32, const float *__restrict__ lj32, double *__restrict__ part)
+// One wave's share of a work item: receivers k0 + lane (+ 64 for NR = 2),
+// below kend, against one row.
+template <bool MASKED, bool HQ, bool MIX, int NR>
+__device__ __forceinline__ void near_item(
+    const Geo &g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
+    const int *__restrict__ bs4, const int *__restrict__ bflag, const int *__restrict__ rkind,
+    const sesmd_forcefield *__restrict__ ff, unsigned long long mask, const float4 *__restrict__ spos32,
+    const float *__restrict__ lj32, double *__restrict__ part, int b, int k0, int kend, int row, int lane)
 {
-    // One 64-lane wavefront per (work item, row): NEAR_TILE receivers of box
-    // b against one (dz, dy) row of the 3^3 neighbourhood -- the three
-    // x-neighbours of a row are consecutive boxes.  Every lane walks the same
-    // source sequence, so each source record is a wave-uniform scalar load and
-    // the per-source integer work runs on the scalar unit; the vector unit only
-    // does the pair arithmetic.  Within a box the sources are ordered by LJ
-    // class, so the loops carry no per-source type branch.  Each wave writes
-    // its partial forces; k_near_sum adds the nine rows in order (no atomics,
-    // deterministic), and no wave waits for the other rows of its item.
-    int item = blockIdx.x / NEAR_ROWS;  // items come longest first (see k_worklist)
-    if (item >= *nwork) return;
-    int row = blockIdx.x % NEAR_ROWS, lane = threadIdx.x;
-    int2 w = work[item];
-    int b = w.x;
-    int kend = bstart[b + 1];
-    int k = w.y + lane;
-    bool active = k < kend;
-    Recv R;
-    R.x = R.y = R.z = R.q = 0.0;
-    R.mol = -1;
-    R.rk = -1;
-    int ti = 0, site = 0;
-    if (active) {
-        double4 pi = spos[k];
-        int4 me = smeta[k];
-        R.x = pi.x;
-        R.y = pi.y;
-        R.z = pi.z;
-        R.q = pi.w * g.xpi;
-        site = me.x;
-        ti = me.y & 255;
-        R.mol = me.z;
-        R.rk = rkind[site];
-        if (R.rk < 0) active = false;
+    Recv R[NR];
+    int ti[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        int k = k0 + NEAR_TILE * r + lane;
+        bool active = k < kend;
+        R[r].x = R[r].y = R[r].z = 1e18;  // an empty lane: far from every source
+        R[r].q = 0.0;
+        R[r].mol = -1;
+        R[r].rk = -1;
+        ti[r] = 0;
+        if (active) {
+            double4 pi = spos[k];
+            int4 me = smeta[k];
+            R[r].x = pi.x;
+            R[r].y = pi.y;
+            R[r].z = pi.z;
+            R[r].q = pi.w * g.xpi;
+            ti[r] = me.y & 255;
+            R[r].mol = me.z;
+            R[r].rk = rkind[me.x];
+            if (R[r].rk < 0) active = false;
+        }
+        R[r].qO = R[r].q * ff->q[0];
+        R[r].qH = R[r].q * ff->q[1];
+        R[r].cO = ff->lj_c[ti[r] * 32];
+        R[r].s6O = ff->lj_s6[ti[r] * 32];
+        R[r].rc2O = active && R[r].cO != 0.0 ? g.rc2 : -1.0;
+        R[r].ljrow = ti[r] * 32;
     }
-    R.qO = R.q * ff->q[0];
-    R.qH = R.q * ff->q[1];
-    R.cO = ff->lj_c[ti * 32];
-    R.s6O = ff->lj_s6[ti * 32];
-    R.ljc = ff->lj_c + ti * 32;
-    R.ljs = ff->lj_s6 + ti * 32;
     int nb = g.nb;
     int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
-    double ax = 0.0, ay = 0.0, az = 0.0;
+    double ax[NR], ay[NR], az[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) ax[r] = ay[r] = az[r] = 0.0;
     int z = bz + row / 3 - 1, y = by + row % 3 - 1;
     if (z >= 0 && z < nb && y >= 0 && y < nb) {
         int r0 = max(bx - 1, 0) + nb * (y + nb * z), r1 = min(bx + 1, nb - 1) + nb * (y + nb * z);
@@ -937,31 +1009,75 @@ __global__ __launch_bounds__(NEAR_TILE) void k_near(
         if (!MASKED)
             for (int bb = r0; bb <= r1; bb++) check = check || bflag[bb];
         if (MIX) {
-            Recv32 Q;
-            Q.q = (float)R.q;
-            Q.qO = (float)R.qO;
-            Q.qH = (float)R.qH;
-            Q.cO = lj32[ti * 32];
-            Q.s6O = lj32[1024 + ti * 32];
-            Q.rc2 = (float)g.rc2;
-            Q.ljc = lj32 + ti * 32;
-            Q.ljs = lj32 + 1024 + ti * 32;
+            Recv32 Q[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                Q[r].q = (float)R[r].q;
+                Q[r].qO = (float)R[r].qO;
+                Q[r].qH = (float)R[r].qH;
+                Q[r].cO = lj32[ti[r] * 32];
+                Q[r].s6O = lj32[1024 + ti[r] * 32];
+                Q[r].rc2O = (float)R[r].rc2O;
+            }
             if (check)
-                near_row32<true, HQ>(g, spos32, smeta, bs4, r0, r1, b, bx, by, bz, R, Q, ax, ay, az);
+                near_row32<true, HQ, NR>(g, spos32, smeta, bs4, r0, r1, b, bx, by, bz, R, Q, lj32, ax, ay, az);
             else
-                near_row32<false, HQ>(g, spos32, smeta, bs4, r0, r1, b, bx, by, bz, R, Q, ax, ay, az);
+                near_row32<false, HQ, NR>(g, spos32, smeta, bs4, r0, r1, b, bx, by, bz, R, Q, lj32, ax, ay, az);
         } else if (MASKED)
-            near_row<true, true, HQ>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, mask, ax, ay, az);
+            near_row<true, true, HQ, NR>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, ff->lj_c, ff->lj_s6, mask, ax,
+                              ay, az);
         else if (check)
-            near_row<true, false, HQ>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, mask, ax, ay, az);
+            near_row<true, false, HQ, NR>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, ff->lj_c, ff->lj_s6, mask, ax,
+                              ay, az);
         else
-            near_row<false, false, HQ>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, mask, ax, ay, az);
+            near_row<false, false, HQ, NR>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, ff->lj_c, ff->lj_s6, mask, ax,
+                              ay, az);
     }
-    if (k < kend) {
-        part[(size_t)(3 * row) * g.n + k] = ax;
-        part[(size_t)(3 * row + 1) * g.n + k] = ay;
-        part[(size_t)(3 * row + 2) * g.n + k] = az;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        int k = k0 + NEAR_TILE * r + lane;
+        if (k < kend) {
+            part[(size_t)(3 * row) * g.n + k] = ax[r];
+            part[(size_t)(3 * row + 1) * g.n + k] = ay[r];
+            part[(size_t)(3 * row + 2) * g.n + k] = az[r];
+        }
     }
+}
+
+#ifndef NEAR_MINW
+#define NEAR_MINW 1
+#endif
+template <bool MASKED, bool HQ, bool MIX>
+__global__ __launch_bounds__(NEAR_TILE, NEAR_MINW) void k_near(
+    Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
+    const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
+    const int *__restrict__ rkind, const int2 *__restrict__ work, const int *__restrict__ nwork,
+    const sesmd_forcefield *__restrict__ ff, unsigned long long mask, double *__restrict__ f,
+    const float4 *__restrict__ spos32, const float *__restrict__ lj32, double *__restrict__ part)
+{
+    // One 64-lane wavefront per (work item, row): the NEAR_CHUNK receivers of
+    // an item (box b) against one (dz, dy) row of the 3^3 neighbourhood -- the
+    // three x-neighbours of a row are consecutive boxes.  A chunk of more than
+    // 64 receivers gives each lane two of them, so every source record loaded
+    // serves two pairs per lane.  Every lane walks the same source sequence,
+    // so each source record is a wave-uniform scalar load and the per-source
+    // integer work runs on the scalar unit; the vector unit only does the pair
+    // arithmetic.  Within a box the sources are ordered by LJ class, so the
+    // loops carry no per-source type branch.  Each wave writes its partial
+    // forces; the site-terms kernel adds the nine rows in order (no atomics,
+    // deterministic), and no wave waits for the other rows of its item.
+    int item = blockIdx.x / NEAR_ROWS;  // items come longest first (see k_worklist)
+    if (item >= *nwork) return;
+    int row = blockIdx.x % NEAR_ROWS, lane = threadIdx.x;
+    int2 w = work[item];
+    int b = w.x, k0 = w.y;
+    int kend = min(bstart[b + 1], k0 + NEAR_CHUNK);
+    if (kend - k0 > NEAR_TILE)
+        near_item<MASKED, HQ, MIX, 2>(g, spos, smeta, bs4, bflag, rkind, ff, mask, spos32, lj32, part, b,
+                                      k0, kend, row, lane);
+    else
+        near_item<MASKED, HQ, MIX, 1>(g, spos, smeta, bs4, bflag, rkind, ff, mask, spos32, lj32, part, b,
+                                      k0, kend, row, lane);
 }
 
 
@@ -1852,21 +1968,18 @@ __global__ __launch_bounds__(64) void k_intra(Geo g, unsigned what, const double
     }
 }
 
-// The same intramolecular terms one thread per (term, molecule), for the step
-// pipeline: every internal pair, angle and dihedral is evaluated once, and
+// The same intramolecular terms one thread per term, for the step pipeline:
+// every internal pair, angle and dihedral of molecule s is evaluated once, and
 // each of its sites' contributions goes to its own place in the molecule's
-// table (Topo::dst) -- fsl[((s * 29 + j) * INTRA_MAXC + q) * 3 + d] -- which
-// part B then adds per site in order (no atomics).  Terms outside `what`
-// write zeros.
-__device__ __forceinline__ void intra_terms(int t, Geo g, unsigned what, const double *__restrict__ r,
-                                            const sesmd_forcefield *__restrict__ ff,
-                                            const Topo *__restrict__ topo, double *__restrict__ fsl)
+// table (Topo::dst) -- mol[(j * INTRA_MAXC + q) * 3 + d] -- which is then
+// added per site in order (no atomics).  Terms outside `what` write zeros.
+__device__ __forceinline__ void intra_term(int term, int s, Geo g, unsigned what,
+                                           const double *__restrict__ r,
+                                           const sesmd_forcefield *__restrict__ ff,
+                                           const Topo *__restrict__ topo, double *mol)
 {
-    int ns = g.ns, ni = ff->n_internal;
-    if (t >= (ni + 66) * ns) return;
-    int term = t / ns, s = t % ns;
+    int ni = ff->n_internal;
     int base = 3 * g.nw + 29 * s, n = g.n;
-    double *mol = fsl + (size_t)s * 29 * INTRA_MAXC * 3;
     auto put = [&](int slot, double x, double y, double z) {
         double *o = mol + (size_t)topo->dst[slot] * 3;
         o[0] = x;
@@ -2328,13 +2441,11 @@ __device__ __forceinline__ void surf_a(
 // site i's new acceleration and velocity, lane c < 28 owns bond c in the
 // pipelined RATTLE (reference order and tolerance).  The two divisions of the
 // correction are folded into one factor per bond (within 2 ulp).
-// Intramolecular force on a surfactant site in part B: fs (k_intra's per-site
-// totals), or the site's k_intra_terms contributions in order plus the
+// Intramolecular force on a surfactant site in part B: fs, the per-site
+// totals (of k_intra, or of k_nh_intra's terms), then with isub set the
 // eval_subtract_fma terms from the molecule's positions in LDS.
 struct IntraIn {
     const double *fs = nullptr;
-    const double *fsl = nullptr;
-    const Topo *topo = nullptr;
     const unsigned *isub = nullptr;
     const sesmd_forcefield *ff = nullptr;
 };
@@ -2356,23 +2467,11 @@ __device__ __forceinline__ void surf_b(
     __asm__ volatile("" ::: "memory");
     if (site) {
         double fi[3] = {0.0, 0.0, 0.0};
-        if (in.fsl) {
-            const double *c = in.fsl + ((size_t)s * 29 + lane) * INTRA_MAXC * 3;
-            int cnt = in.topo->cnt[lane];
-            // in order, the loads of a chunk of 8 contributions issued together
-            for (int q0 = 0; q0 < cnt; q0 += 8) {
-                double t[8][3];
+        if (in.fs) {
 #pragma unroll
-                for (int k = 0; k < 8; k++)
-                    if (q0 + k < cnt)
-#pragma unroll
-                        for (int d = 0; d < 3; d++) t[k][d] = c[3 * (q0 + k) + d];
-#pragma unroll
-                for (int k = 0; k < 8; k++)
-                    if (q0 + k < cnt)
-#pragma unroll
-                        for (int d = 0; d < 3; d++) fi[d] += t[k][d];
-            }
+            for (int d = 0; d < 3; d++) fi[d] = in.fs[d * 29 * g.ns + 29 * s + lane];
+        }
+        if (in.isub) {
             unsigned mask = in.isub[base + lane];
             while (mask) {
                 int k = __ffs(mask) - 1;
@@ -2385,15 +2484,12 @@ __device__ __forceinline__ void surf_b(
                 fi[1] -= ymn * fel;
                 fi[2] -= zmn * fel;
             }
-        } else if (in.fs) {
-#pragma unroll
-            for (int d = 0; d < 3; d++) fi[d] = in.fs[d * 29 * g.ns + 29 * s + lane];
         }
         double escale = 2.0 / (2.0 + nh[0] * g.dt);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             double fv = f[d * n + base + lane];
-            if (in.fsl || in.fs) fv += fi[d];
+            if (in.fs) fv += fi[d];
             double ac = fv * g.dtsq2 / sc.m[lane];
             a[d * n + base + lane] = ac;
             L.p[w][lane][d] = escale * (v[d * n + base + lane] + ac);
@@ -2473,20 +2569,32 @@ __global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restri
     nh_update_block(g, partials, nblocks, ff, nh, update_eta);
 }
 
-// Step pipeline: block 0 updates eta from part A's kinetic sums (k_nh_update),
-// the other blocks evaluate the intramolecular terms into their slots; part B
-// adds each surfactant site's slots in order (surf_b).
-__global__ __launch_bounds__(128) void k_nh_intra(Geo g, const double *__restrict__ partials, int nblocks,
+// Step pipeline: block 0 updates eta from part A's kinetic sums (k_nh_update);
+// block 1 + s evaluates molecule s's intramolecular terms, one thread per term,
+// into its table in LDS, then adds each site's contributions in table order
+// (internal pairs, angles, dihedrals) into fs[d][29 ns]; part B adds the
+// eval_subtract_fma terms with the lists of the step's own rebuild (surf_b).
+__global__ __launch_bounds__(256) void k_nh_intra(Geo g, const double *__restrict__ partials, int nblocks,
                                                   const sesmd_forcefield *__restrict__ ff,
                                                   double *__restrict__ nh, const double *__restrict__ r,
-                                                  const Topo *__restrict__ topo, double *__restrict__ fsl)
+                                                  const Topo *__restrict__ topo, double *__restrict__ fs)
 {
     if (blockIdx.x == 0) {
         nh_update_block(g, partials, nblocks, ff, nh, 1);
         return;
     }
-    intra_terms((blockIdx.x - 1) * blockDim.x + threadIdx.x, g,
-                SESMD_INTERNAL | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE, r, ff, topo, fsl);
+    __shared__ double tab[29 * INTRA_MAXC * 3];
+    int s = blockIdx.x - 1, nterm = ff->n_internal + 66;
+    for (int term = threadIdx.x; term < nterm; term += blockDim.x)
+        intra_term(term, s, g, SESMD_INTERNAL | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE, r, ff,
+                   topo, tab);
+    __syncthreads();
+    for (int t = threadIdx.x; t < 29 * 3; t += blockDim.x) {
+        int j = t / 3, d = t - 3 * j, cnt = topo->cnt[j];
+        double acc = 0.0;
+        for (int q = 0; q < cnt; q++) acc += tab[(j * INTRA_MAXC + q) * 3 + d];
+        fs[(size_t)d * 29 * g.ns + 29 * s + j] = acc;
+    }
 }
 
 // Part B for waters (src/move_w1_b.f90:23-123): a = F dt^2/2m, velocity with
@@ -2514,6 +2622,14 @@ __device__ __forceinline__ void water_b(int blk, Geo g, const double *__restrict
             vx[i][d] = escale * (v[d * n + base + i] + ac);
         }
     }
+    // gab = -(r_ab . v_ab / dt) / ((1/m_a + 1/m_b) d^2): the two divisions
+    // folded into one factor per constraint (within 2 ulp), as for surfactants
+    double kc[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        int ia = c, ib = c + 1 > 2 ? 0 : c + 1;
+        kc[c] = -1.0 / (g.dt * ((rm[ia] + rm[ib]) * ff->dsqw[c]));
+    }
     int it = 0;
     bool done = false;
     while (!done && it <= 100) {
@@ -2522,8 +2638,7 @@ __device__ __forceinline__ void water_b(int blk, Geo g, const double *__restrict
             int ia = c, ib = c + 1 > 2 ? 0 : c + 1;
             double vxab = vx[ia][0] - vx[ib][0], vyab = vx[ia][1] - vx[ib][1], vzab = vx[ia][2] - vx[ib][2];
             double rxab = rx[ia][0] - rx[ib][0], ryab = rx[ia][1] - rx[ib][1], rzab = rx[ia][2] - rx[ib][2];
-            double rvab = (rxab * vxab + ryab * vyab + rzab * vzab) / g.dt;
-            double gab = -rvab / ((rm[ia] + rm[ib]) * ff->dsqw[c]);
+            double gab = (rxab * vxab + ryab * vyab + rzab * vzab) * kc[c];
             if (fabs(gab) > 0.0001) {
                 double dd[3] = {rxab * gab, ryab * gab, rzab * gab};
                 for (int d = 0; d < 3; d++) {
@@ -2933,7 +3048,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->bs4, 4 * (size_t)g.nb3 + 1));
     A(dalloc(&c->bflag, g.nb3));
     A(dalloc(&c->nvalid, 1));
-    c->max_work = g.nb3 + (int)(n / NEAR_TILE) + 1;
+    c->max_work = g.nb3 + (int)(n / NEAR_CHUNK) + 1;
     A(dalloc(&c->work, c->max_work));
     A(dalloc(&c->work_raw, c->max_work));
     A(dalloc(&c->wkey_in, c->max_work));
@@ -3000,7 +3115,6 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     c->nblk_s = g.ns > 0 ? (int)nblk(g.ns, c->surf_fast ? SURF_MPB : 64) : 0;
     A(dalloc(&c->partials, (size_t)(c->nblk_w + c->nblk_s + 1) * NH_VALS));
     A(dalloc(&c->fs, 3 * 29 * (size_t)g.ns + 1));
-    A(dalloc(&c->fsl, (size_t)29 * INTRA_MAXC * 3 * g.ns + 1));  // k_intra_terms contributions
     A(dalloc(&c->nh, 16 + NH_VALS));
     A(dalloc(&c->esum, 16));
     A(dalloc(&c->counters, 4));
@@ -3077,7 +3191,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
                     c->AF4, c->AF3, c->m2l_part,
-                    c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs, c->fsl};
+                    c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->side) hipStreamSynchronize(c->side);
@@ -3454,9 +3568,8 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
                                                               c->sconst, c->nh, c->partials, c->counters,
                                                               c->nblk_s, so);
         if (g.ns) {  // eta update and the intramolecular terms in one launch
-            int nt = (c->ff.n_internal + 66) * g.ns;
-            k_nh_intra<<<1 + nblk(nt, 128), 128, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff,
-                                                          c->nh, c->r, c->dtopo, c->fsl);
+            k_nh_intra<<<1 + g.ns, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, c->r,
+                                                 c->dtopo, c->fs);
         } else {
             k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1);
         }
@@ -3483,8 +3596,7 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
     if (c->surf_fast) {
         IntraIn in;
         if (g.ns) {
-            in.fsl = c->fsl;
-            in.topo = c->dtopo;
+            in.fs = c->fs;
             in.isub = c->isub;
             in.ff = c->dff;
         }
