@@ -122,10 +122,14 @@ struct sesmd_ctx {
     int *bstart, *nvalid;
     int *bs4;    // first sorted position of every (box, LJ class) key
     int *bflag;  // box holds a site not listed from every neighbour box
-    int2 *work;
+    int2 *work;                      // box-centred near-field items (masked evaluations)
     int *nwork;
     int max_work;
-    int2 *work_raw;                  // items in box order, before the cost sort
+    bool boxlist;                    // `work` matches the last rebuild
+    int2 *rwork;                     // row-centred near-field items (the full near field)
+    int *nrwork;
+    int max_rwork;
+    int2 *work_raw;                  // items in list order, before the cost sort
     unsigned *wkey_in, *wkey_out;    // 1 + estimated cost per item (0: padding)
     int *widx_in, *widx_out;
     double4 *spos;
@@ -646,6 +650,7 @@ struct Recv {
     double cO, s6O;          // LJ against water oxygen
     double rc2O;             // rc^2 where the receiver takes LJ from water O, else -1
     int ljrow, mol, rk;      // LJ table row of the receiver's type (32 * type)
+    int by, bz;              // the receiver's box (y, z); every receiver of a wave has box x = bx
 };
 
 // Sources t in [s0, s1) of one LJ class for NR receivers per lane (lane l
@@ -653,8 +658,9 @@ struct Recv {
 // each receiver adds the same terms in the same order either way).
 // CAT 0: water O (LJ with the receiver's O parameters), 1: surfactant sites (LJ
 // from the type table), 2: water H (Coulomb only).  CHECK: test the storage
-// ring of every source (only rows holding a site that some neighbour box does
-// not list); MASKED: keep the (receiver kind, source kind) pairs of `mask`.
+// ring of every source against each receiver's box (only rows holding a site
+// that some neighbour box does not list); MASKED: keep the (receiver kind,
+// source kind) pairs of `mask`.
 // CEN: the segment lies in the receivers' own box (every source may share a
 // molecule with a receiver); elsewhere only sources of split molecules can.
 // HQ: both water hydrogens carry one charge, so q_i q_j is per lane for CAT 2.
@@ -663,19 +669,23 @@ struct Recv {
 // of the chunk.  Where a pair is outside rc or its LJ constant is zero the
 // term is not added, as before the test.
 template <int CAT, bool CHECK, bool MASKED, bool CEN, bool HQ, int NR>
-__device__ __forceinline__ void near_pair(const Geo &g, const double4 &pj, const int4 &mj, int bx, int by,
-                                          int bz, const Recv (&R)[NR], const double *__restrict__ ljc,
+__device__ __forceinline__ void near_pair(const Geo &g, const double4 &pj, const int4 &mj, int bx,
+                                          const Recv (&R)[NR], const double *__restrict__ ljc,
                                           const double *__restrict__ ljs, unsigned long long mask,
                                           double (&ax)[NR], double (&ay)[NR], double (&az)[NR])
 {
-    int sk = 0;
+    int sk[NR];
     if (CHECK || MASKED) {
-        int cls = (mj.y >> 8) & 3;
-        int ring = max(abs((mj.w & 255) - bx),
-                       max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
-        sk = cls == 0 ? (ring <= 1 ? ring : -1)
-                      : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
-        if (sk < 0) return;  // outside every list of the reference (uniform)
+        int cls = (mj.y >> 8) & 3, sx = mj.w & 255, sy = (mj.w >> 8) & 255, sz = (mj.w >> 16) & 255;
+        int lim = cls == 0 ? 1 : cls == 1 ? 2 : 3, first = cls == 0 ? 0 : cls == 1 ? 2 : 5;
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            int ring = max(abs(sx - bx), max(abs(sy - R[r].by), abs(sz - R[r].bz)));
+            sk[r] = ring <= lim ? first + ring : -1;  // -1: outside every list of the reference
+            any = any || sk[r] >= 0;
+        }
+        if (!__builtin_amdgcn_ballot_w64(any)) return;
     }
     double dx[NR], dy[NR], dz[NR], r2[NR], rinv2[NR], ft[NR];
 #pragma unroll
@@ -718,7 +728,8 @@ __device__ __forceinline__ void near_pair(const Geo &g, const double4 &pj, const
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             bool ok = mj.z != R[r].mol;
-            if (MASKED) ok = ok && ((mask >> (max(R[r].rk, 0) * 9 + sk)) & 1ull);
+            if (CHECK || MASKED) ok = ok && sk[r] >= 0;
+            if (MASKED) ok = ok && ((mask >> (max(R[r].rk, 0) * 9 + max(sk[r], 0))) & 1ull);
             ft[r] = ok ? ft[r] : 0.0;
         }
     } else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
@@ -737,7 +748,7 @@ __device__ __forceinline__ void near_pair(const Geo &g, const double4 &pj, const
 template <int CAT, bool CHECK, bool MASKED, bool CEN, bool HQ, int NR>
 __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__restrict__ spos,
                                              const int4 *__restrict__ smeta, int s0, int s1,
-                                             int bx, int by, int bz, const Recv (&R)[NR],
+                                             int bx, const Recv (&R)[NR],
                                              const double *__restrict__ ljc,
                                              const double *__restrict__ ljs,
                                              unsigned long long mask, double (&ax)[NR],
@@ -764,16 +775,16 @@ __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__rest
         pA = ps[0];
         pB = ps[1];
         __asm__ volatile("" ::: "memory");  // keeps these loads ahead of the pair arithmetic
-        near_pair<CAT, CHECK, MASKED, CEN, HQ, NR>(g, pa, ma, bx, by, bz, R, ljc, ljs, mask, ax, ay, az);
+        near_pair<CAT, CHECK, MASKED, CEN, HQ, NR>(g, pa, ma, bx, R, ljc, ljs, mask, ax, ay, az);
         if (t > 1)
-            near_pair<CAT, CHECK, MASKED, CEN, HQ, NR>(g, pb, mb, bx, by, bz, R, ljc, ljs, mask, ax, ay, az);
+            near_pair<CAT, CHECK, MASKED, CEN, HQ, NR>(g, pb, mb, bx, R, ljc, ljs, mask, ax, ay, az);
     }
 }
 
 template <bool CHECK, bool MASKED, bool CEN, bool HQ, int NR>
 __device__ __forceinline__ void near_box(const Geo &g, const double4 *__restrict__ spos,
                                          const int4 *__restrict__ smeta, const int *__restrict__ bs4,
-                                         int bb, int bx, int by, int bz, const Recv (&R)[NR],
+                                         int bb, int bx, const Recv (&R)[NR],
                                          const double *__restrict__ ljc, const double *__restrict__ ljs,
                                          unsigned long long mask, double (&ax)[NR], double (&ay)[NR],
                                          double (&az)[NR])
@@ -783,15 +794,15 @@ __device__ __forceinline__ void near_box(const Geo &g, const double4 *__restrict
     int k1 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 1]);
     int k2 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 2]);
     int k3 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 3]);
-    near_segment<0, CHECK, MASKED, CEN, HQ, NR>(g, spos, smeta, k0, k1, bx, by, bz, R, ljc, ljs, mask, ax, ay, az);
-    near_segment<1, CHECK, MASKED, CEN, HQ, NR>(g, spos, smeta, k1, k2, bx, by, bz, R, ljc, ljs, mask, ax, ay, az);
-    near_segment<2, CHECK, MASKED, CEN, HQ, NR>(g, spos, smeta, k2, k3, bx, by, bz, R, ljc, ljs, mask, ax, ay, az);
+    near_segment<0, CHECK, MASKED, CEN, HQ, NR>(g, spos, smeta, k0, k1, bx, R, ljc, ljs, mask, ax, ay, az);
+    near_segment<1, CHECK, MASKED, CEN, HQ, NR>(g, spos, smeta, k1, k2, bx, R, ljc, ljs, mask, ax, ay, az);
+    near_segment<2, CHECK, MASKED, CEN, HQ, NR>(g, spos, smeta, k2, k3, bx, R, ljc, ljs, mask, ax, ay, az);
 }
 
 template <bool CHECK, bool MASKED, bool HQ, int NR>
 __device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict__ spos,
                                          const int4 *__restrict__ smeta, const int *__restrict__ bs4,
-                                         int r0, int r1, int bc, int bx, int by, int bz,
+                                         int r0, int r1, int bc, int bx,
                                          const Recv (&R)[NR], const double *__restrict__ ljc,
                                          const double *__restrict__ ljs, unsigned long long mask,
                                          double (&ax)[NR], double (&ay)[NR], double (&az)[NR])
@@ -799,10 +810,10 @@ __device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict
     // boxes r0..r1 (consecutive indices); bc: the receivers' own box
     for (int bb = r0; bb <= r1; bb++) {
         if (bb == bc)
-            near_box<CHECK, MASKED, true, HQ, NR>(g, spos, smeta, bs4, bb, bx, by, bz, R, ljc, ljs, mask, ax,
+            near_box<CHECK, MASKED, true, HQ, NR>(g, spos, smeta, bs4, bb, bx, R, ljc, ljs, mask, ax,
                                                   ay, az);
         else
-            near_box<CHECK, MASKED, false, HQ, NR>(g, spos, smeta, bs4, bb, bx, by, bz, R, ljc, ljs, mask, ax,
+            near_box<CHECK, MASKED, false, HQ, NR>(g, spos, smeta, bs4, bb, bx, R, ljc, ljs, mask, ax,
                                                    ay, az);
     }
 }
@@ -815,19 +826,23 @@ struct Recv32 {
 };
 
 template <int CAT, bool CHECK, bool CEN, bool HQ, int NR>
-__device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, int bx, int by, int bz,
+__device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, int bx,
                                             const float (&xr)[NR], const float (&yr)[NR],
                                             const float (&zr)[NR], const Recv (&R)[NR],
                                             const Recv32 (&Q)[NR], const float *__restrict__ lj32,
                                             float rc2, float (&fx)[NR], float (&fy)[NR], float (&fz)[NR])
 {
+    bool okc[NR];
     if (CHECK) {
-        int cls = (mj.y >> 8) & 3;
-        int ring = max(abs((mj.w & 255) - bx),
-                       max(abs(((mj.w >> 8) & 255) - by), abs(((mj.w >> 16) & 255) - bz)));
-        int sk = cls == 0 ? (ring <= 1 ? ring : -1)
-                          : cls == 1 ? (ring <= 2 ? 2 + ring : -1) : (ring <= 3 ? 5 + ring : -1);
-        if (sk < 0) return;
+        int cls = (mj.y >> 8) & 3, sx = mj.w & 255, sy = (mj.w >> 8) & 255, sz = (mj.w >> 16) & 255;
+        int lim = cls == 0 ? 1 : cls == 1 ? 2 : 3;
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            okc[r] = max(abs(sx - bx), max(abs(sy - R[r].by), abs(sz - R[r].bz))) <= lim;
+            any = any || okc[r];
+        }
+        if (!__builtin_amdgcn_ballot_w64(any)) return;
     }
     float dx[NR], dy[NR], dz[NR], r2[NR], rinv2[NR], ft[NR];
 #pragma unroll
@@ -868,7 +883,7 @@ __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, in
     }
     if (CHECK || CEN) {
 #pragma unroll
-        for (int r = 0; r < NR; r++) ft[r] = mj.z != R[r].mol ? ft[r] : 0.0f;
+        for (int r = 0; r < NR; r++) ft[r] = (mj.z != R[r].mol && (!CHECK || okc[r])) ? ft[r] : 0.0f;
     } else if ((mj.y >> 10) & 1) {  // wave-uniform: a branch, not a select
         __asm__ volatile("");
 #pragma unroll
@@ -885,7 +900,7 @@ __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, in
 template <int CAT, bool CHECK, bool CEN, bool HQ, int NR>
 __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32,
                                                const int4 *__restrict__ smeta, int s0, int s1,
-                                               int bx, int by, int bz, const float (&xr)[NR],
+                                               int bx, const float (&xr)[NR],
                                                const float (&yr)[NR], const float (&zr)[NR],
                                                const Recv (&R)[NR], const Recv32 (&Q)[NR],
                                                const float *__restrict__ lj32, float rc2,
@@ -909,9 +924,9 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
         pA = ps[0];
         pB = ps[1];
         __asm__ volatile("" ::: "memory");  // keeps these loads ahead of the pair arithmetic
-        near_pair32<CAT, CHECK, CEN, HQ, NR>(pa, ma, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
+        near_pair32<CAT, CHECK, CEN, HQ, NR>(pa, ma, bx, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
         if (t > 1)
-            near_pair32<CAT, CHECK, CEN, HQ, NR>(pb, mb, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
+            near_pair32<CAT, CHECK, CEN, HQ, NR>(pb, mb, bx, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
     }
 #pragma unroll
     for (int r = 0; r < NR; r++) {
@@ -924,7 +939,7 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
 template <bool CHECK, bool HQ, int NR>
 __device__ __forceinline__ void near_row32(const Geo &g, const float4 *__restrict__ spos32,
                                            const int4 *__restrict__ smeta, const int *__restrict__ bs4,
-                                           int r0, int r1, int bc, int bx, int by, int bz,
+                                           int r0, int r1, int bc, int bx,
                                            const Recv (&R)[NR], const Recv32 (&Q)[NR],
                                            const float *__restrict__ lj32, double (&ax)[NR],
                                            double (&ay)[NR], double (&az)[NR])
@@ -947,19 +962,90 @@ __device__ __forceinline__ void near_row32(const Geo &g, const float4 *__restric
         int k2 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 2]);
         int k3 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 3]);
         if (bb == bc) {
-            near_segment32<0, CHECK, true, HQ, NR>(spos32, smeta, k0, k1, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
-            near_segment32<1, CHECK, true, HQ, NR>(spos32, smeta, k1, k2, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
-            near_segment32<2, CHECK, true, HQ, NR>(spos32, smeta, k2, k3, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<0, CHECK, true, HQ, NR>(spos32, smeta, k0, k1, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<1, CHECK, true, HQ, NR>(spos32, smeta, k1, k2, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<2, CHECK, true, HQ, NR>(spos32, smeta, k2, k3, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
         } else {
-            near_segment32<0, CHECK, false, HQ, NR>(spos32, smeta, k0, k1, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
-            near_segment32<1, CHECK, false, HQ, NR>(spos32, smeta, k1, k2, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
-            near_segment32<2, CHECK, false, HQ, NR>(spos32, smeta, k2, k3, bx, by, bz, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<0, CHECK, false, HQ, NR>(spos32, smeta, k0, k1, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<1, CHECK, false, HQ, NR>(spos32, smeta, k1, k2, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<2, CHECK, false, HQ, NR>(spos32, smeta, k2, k3, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
         }
     }
 }
 
-// One wave's share of a work item: receivers k0 + lane (+ 64 for NR = 2),
-// below kend, against one row.
+// One receiver: sorted record k (active) or an empty lane.
+__device__ __forceinline__ void load_recv(const Geo &g, const double4 *__restrict__ spos,
+                                          const int4 *__restrict__ smeta, const int *__restrict__ rkind,
+                                          const sesmd_forcefield *__restrict__ ff, int k, bool active,
+                                          int by, int bz, Recv &R, int &ti)
+{
+    R.x = R.y = R.z = 1e18;  // an empty lane: far from every source
+    R.q = 0.0;
+    R.mol = -1;
+    R.rk = -1;
+    R.by = by;
+    R.bz = bz;
+    ti = 0;
+    if (active) {
+        double4 pi = spos[k];
+        int4 me = smeta[k];
+        R.x = pi.x;
+        R.y = pi.y;
+        R.z = pi.z;
+        R.q = pi.w * g.xpi;
+        ti = me.y & 255;
+        R.mol = me.z;
+        R.rk = rkind[me.x];
+        if (R.rk < 0) active = false;
+    }
+    R.qO = R.q * ff->q[0];
+    R.qH = R.q * ff->q[1];
+    R.cO = ff->lj_c[ti * 32];
+    R.s6O = ff->lj_s6[ti * 32];
+    R.rc2O = active && R.cO != 0.0 ? g.rc2 : -1.0;
+    R.ljrow = ti * 32;
+}
+
+// The receivers of a wave against source boxes r0..r1 (one row, consecutive
+// indices, every receiver in box column bx); bc: the source box that may hold
+// receivers' own molecules unsplit (-1: none).
+template <bool MASKED, bool HQ, bool MIX, int NR>
+__device__ __forceinline__ void near_sources(const Geo &g, const double4 *__restrict__ spos,
+                                             const int4 *__restrict__ smeta, const int *__restrict__ bs4,
+                                             bool check, const sesmd_forcefield *__restrict__ ff,
+                                             unsigned long long mask, const float4 *__restrict__ spos32,
+                                             const float *__restrict__ lj32, int r0, int r1, int bc, int bx,
+                                             const Recv (&R)[NR], const int (&ti)[NR], double (&ax)[NR],
+                                             double (&ay)[NR], double (&az)[NR])
+{
+    if (MIX) {
+        Recv32 Q[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            Q[r].q = (float)R[r].q;
+            Q[r].qO = (float)R[r].qO;
+            Q[r].qH = (float)R[r].qH;
+            Q[r].cO = lj32[ti[r] * 32];
+            Q[r].s6O = lj32[1024 + ti[r] * 32];
+            Q[r].rc2O = (float)R[r].rc2O;
+        }
+        if (check)
+            near_row32<true, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, ax, ay, az);
+        else
+            near_row32<false, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, ax, ay, az);
+    } else if (MASKED)
+        near_row<true, true, HQ, NR>(g, spos, smeta, bs4, r0, r1, bc, bx, R, ff->lj_c, ff->lj_s6, mask, ax, ay,
+                                     az);
+    else if (check)
+        near_row<true, false, HQ, NR>(g, spos, smeta, bs4, r0, r1, bc, bx, R, ff->lj_c, ff->lj_s6, mask, ax, ay,
+                                      az);
+    else
+        near_row<false, false, HQ, NR>(g, spos, smeta, bs4, r0, r1, bc, bx, R, ff->lj_c, ff->lj_s6, mask, ax,
+                                       ay, az);
+}
+
+// One wave's share of a box-centred work item: receivers k0 + lane (+ 64 for
+// NR = 2), below kend, all in box b, against one row.
 template <bool MASKED, bool HQ, bool MIX, int NR>
 __device__ __forceinline__ void near_item(
     const Geo &g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
@@ -967,38 +1053,15 @@ __device__ __forceinline__ void near_item(
     const sesmd_forcefield *__restrict__ ff, unsigned long long mask, const float4 *__restrict__ spos32,
     const float *__restrict__ lj32, double *__restrict__ part, int b, int k0, int kend, int row, int lane)
 {
+    int nb = g.nb;
+    int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
     Recv R[NR];
     int ti[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         int k = k0 + NEAR_TILE * r + lane;
-        bool active = k < kend;
-        R[r].x = R[r].y = R[r].z = 1e18;  // an empty lane: far from every source
-        R[r].q = 0.0;
-        R[r].mol = -1;
-        R[r].rk = -1;
-        ti[r] = 0;
-        if (active) {
-            double4 pi = spos[k];
-            int4 me = smeta[k];
-            R[r].x = pi.x;
-            R[r].y = pi.y;
-            R[r].z = pi.z;
-            R[r].q = pi.w * g.xpi;
-            ti[r] = me.y & 255;
-            R[r].mol = me.z;
-            R[r].rk = rkind[me.x];
-            if (R[r].rk < 0) active = false;
-        }
-        R[r].qO = R[r].q * ff->q[0];
-        R[r].qH = R[r].q * ff->q[1];
-        R[r].cO = ff->lj_c[ti[r] * 32];
-        R[r].s6O = ff->lj_s6[ti[r] * 32];
-        R[r].rc2O = active && R[r].cO != 0.0 ? g.rc2 : -1.0;
-        R[r].ljrow = ti[r] * 32;
+        load_recv(g, spos, smeta, rkind, ff, k, k < kend, by, bz, R[r], ti[r]);
     }
-    int nb = g.nb;
-    int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
     double ax[NR], ay[NR], az[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) ax[r] = ay[r] = az[r] = 0.0;
@@ -1008,30 +1071,8 @@ __device__ __forceinline__ void near_item(
         bool check = MASKED;
         if (!MASKED)
             for (int bb = r0; bb <= r1; bb++) check = check || bflag[bb];
-        if (MIX) {
-            Recv32 Q[NR];
-#pragma unroll
-            for (int r = 0; r < NR; r++) {
-                Q[r].q = (float)R[r].q;
-                Q[r].qO = (float)R[r].qO;
-                Q[r].qH = (float)R[r].qH;
-                Q[r].cO = lj32[ti[r] * 32];
-                Q[r].s6O = lj32[1024 + ti[r] * 32];
-                Q[r].rc2O = (float)R[r].rc2O;
-            }
-            if (check)
-                near_row32<true, HQ, NR>(g, spos32, smeta, bs4, r0, r1, b, bx, by, bz, R, Q, lj32, ax, ay, az);
-            else
-                near_row32<false, HQ, NR>(g, spos32, smeta, bs4, r0, r1, b, bx, by, bz, R, Q, lj32, ax, ay, az);
-        } else if (MASKED)
-            near_row<true, true, HQ, NR>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, ff->lj_c, ff->lj_s6, mask, ax,
-                              ay, az);
-        else if (check)
-            near_row<true, false, HQ, NR>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, ff->lj_c, ff->lj_s6, mask, ax,
-                              ay, az);
-        else
-            near_row<false, false, HQ, NR>(g, spos, smeta, bs4, r0, r1, b, bx, by, bz, R, ff->lj_c, ff->lj_s6, mask, ax,
-                              ay, az);
+        near_sources<MASKED, HQ, MIX, NR>(g, spos, smeta, bs4, check, ff, mask, spos32, lj32, r0, r1, b, bx, R,
+                                          ti, ax, ay, az);
     }
 #pragma unroll
     for (int r = 0; r < NR; r++) {
@@ -1065,7 +1106,9 @@ __global__ __launch_bounds__(NEAR_TILE, NEAR_MINW) void k_near(
     // arithmetic.  Within a box the sources are ordered by LJ class, so the
     // loops carry no per-source type branch.  Each wave writes its partial
     // forces; the site-terms kernel adds the nine rows in order (no atomics,
-    // deterministic), and no wave waits for the other rows of its item.
+    // deterministic), and no wave waits for the other rows of its item.  This
+    // box-centred form serves masked evaluations (subsets of the reference's
+    // 39 kernels); the full near field of a step runs k_near_rows.
     int item = blockIdx.x / NEAR_ROWS;  // items come longest first (see k_worklist)
     if (item >= *nwork) return;
     int row = blockIdx.x % NEAR_ROWS, lane = threadIdx.x;
@@ -1078,6 +1121,171 @@ __global__ __launch_bounds__(NEAR_TILE, NEAR_MINW) void k_near(
     else
         near_item<MASKED, HQ, MIX, 1>(g, spos, smeta, bs4, bflag, rkind, ff, mask, spos32, lj32, part, b,
                                       k0, kend, row, lane);
+}
+
+// The receiver list of source row (x0, y, z): the boxes (x0, y + dy, z + dz),
+// slot 3 (dz + 1) + (dy + 1), each box's sorted sites in order; slots outside
+// the grid are empty.  Slot 4 is the row's own centre box.
+struct RowRecv {
+    int start[9], pre[10];  // first sorted site of each slot's box, list position of each slot
+};
+
+__device__ __forceinline__ void row_receivers(const Geo &g, const int *__restrict__ bstart, int x0, int y,
+                                              int z, RowRecv &L)
+{
+    int nb = g.nb;
+    L.pre[0] = 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        int yy = y + q % 3 - 1, zz = z + q / 3 - 1, cnt = 0;
+        L.start[q] = 0;
+        if (yy >= 0 && yy < nb && zz >= 0 && zz < nb) {
+            int b = x0 + nb * (yy + nb * zz);
+            L.start[q] = bstart[b];
+            cnt = bstart[b + 1] - L.start[q];
+        }
+        L.pre[q + 1] = L.pre[q] + cnt;
+    }
+}
+
+// Row-centred work list (one block): one item per NEAR_CHUNK receivers of
+// each row's receiver list, with a cost estimate (receivers x row sources) as
+// the sort key; ordered by decreasing cost like k_worklist's items.
+__global__ void k_rowlist(Geo g, const int *__restrict__ bstart, int2 *__restrict__ work,
+                          unsigned *__restrict__ wkey, int *__restrict__ widx, int *__restrict__ nwork,
+                          int max_work)
+{
+    __shared__ int s_cnt[1024];
+    __shared__ int s_base;
+    if (threadIdx.x == 0) s_base = 0;
+    __syncthreads();
+    int nb = g.nb;
+    for (int q0 = 0; q0 < g.nb3; q0 += blockDim.x) {
+        int q = q0 + threadIdx.x;
+        int nch = 0, total = 0, src = 0;
+        if (q < g.nb3) {
+            int x0 = q % nb, y = (q / nb) % nb, z = q / (nb * nb);
+            RowRecv L;
+            row_receivers(g, bstart, x0, y, z, L);
+            total = L.pre[9];
+            nch = (total + NEAR_CHUNK - 1) / NEAR_CHUNK;
+            int r0 = max(x0 - 1, 0) + nb * (y + nb * z), r1 = min(x0 + 1, nb - 1) + nb * (y + nb * z);
+            src = bstart[r1 + 1] - bstart[r0];
+        }
+        s_cnt[threadIdx.x] = nch;
+        __syncthreads();
+        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+            int v = threadIdx.x >= (unsigned)off ? s_cnt[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_cnt[threadIdx.x] += v;
+            __syncthreads();
+        }
+        int excl = s_base + s_cnt[threadIdx.x] - nch;
+        for (int c = 0; c < nch; c++)
+            if (excl + c < max_work) {
+                int c0 = c * NEAR_CHUNK, nr = min(NEAR_CHUNK, total - c0);
+                work[excl + c] = make_int2(q, c0);
+                wkey[excl + c] = 1u + (unsigned)min(nr * src, 1 << 30);
+                widx[excl + c] = excl + c;
+            }
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) s_base += s_cnt[threadIdx.x];
+        __syncthreads();
+    }
+    for (int i = min(s_base, max_work) + threadIdx.x; i < max_work; i += blockDim.x) {
+        wkey[i] = 0u;
+        widx[i] = i;
+    }
+    if (threadIdx.x == 0) *nwork = min(s_base, max_work);
+}
+
+// One row-centred item: positions c0 + lane (+ 64 for NR = 2) of the row's
+// receiver list, below cend, against the row's sources; each receiver's
+// partial goes to the row slot its box sees this row in.
+template <bool HQ, bool MIX, int NR>
+__device__ __forceinline__ void near_rows_item(
+    const Geo &g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
+    const int *__restrict__ bs4, const int *__restrict__ bflag, const int *__restrict__ rkind,
+    const sesmd_forcefield *__restrict__ ff, const float4 *__restrict__ spos32,
+    const float *__restrict__ lj32, double *__restrict__ part, const RowRecv &L, int x0, int y, int z,
+    int c0, int cend, int lane)
+{
+    int nb = g.nb;
+    Recv R[NR];
+    int ti[NR], k[NR], slot[NR];
+    bool cen = false;  // some receiver of this wave lives in box (x0, y, z)
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        int p = c0 + NEAR_TILE * r + lane;
+        // the list slot holding position p: the last slot starting at or
+        // before p (empty slots start where the next one does); selects, so
+        // L stays in scalar registers
+        int m = 0, st = L.start[0], pr = 0;
+#pragma unroll
+        for (int q = 1; q < 9; q++)
+            if (p >= L.pre[q]) {
+                m = q;
+                st = L.start[q];
+                pr = L.pre[q];
+            }
+        bool on = p < cend;
+        int ddy = m % 3 - 1, ddz = m / 3 - 1;
+        k[r] = st + p - pr;
+        slot[r] = 8 - m;  // the row's (dz, dy) seen from the receiver's box: 3 (1 - ddz) + (1 - ddy)
+        load_recv(g, spos, smeta, rkind, ff, k[r], on, y + ddy, z + ddz, R[r], ti[r]);
+        cen = cen || (on && m == 4);
+    }
+    bool anycen = __builtin_amdgcn_ballot_w64(cen) != 0;
+    double ax[NR], ay[NR], az[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) ax[r] = ay[r] = az[r] = 0.0;
+    int r0 = max(x0 - 1, 0) + nb * (y + nb * z), r1 = min(x0 + 1, nb - 1) + nb * (y + nb * z);
+    bool check = false;
+    for (int bb = r0; bb <= r1; bb++) check = check || bflag[bb];
+    int bc = anycen ? x0 + nb * (y + nb * z) : -1;
+    near_sources<false, HQ, MIX, NR>(g, spos, smeta, bs4, check, ff, 0ull, spos32, lj32, r0, r1, bc, x0, R, ti,
+                                     ax, ay, az);
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        if (c0 + NEAR_TILE * r + lane < cend) {
+            part[(size_t)(3 * slot[r]) * g.n + k[r]] = ax[r];
+            part[(size_t)(3 * slot[r] + 1) * g.n + k[r]] = ay[r];
+            part[(size_t)(3 * slot[r] + 2) * g.n + k[r]] = az[r];
+        }
+    }
+}
+
+template <bool HQ, bool MIX>
+__global__ __launch_bounds__(NEAR_TILE, NEAR_MINW) void k_near_rows(
+    Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
+    const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
+    const int *__restrict__ rkind, const int2 *__restrict__ work, const int *__restrict__ nwork,
+    const sesmd_forcefield *__restrict__ ff, const float4 *__restrict__ spos32,
+    const float *__restrict__ lj32, double *__restrict__ part)
+{
+    // The full near field, row-centred: one 64-lane wavefront per work item,
+    // the item being up to NEAR_CHUNK receivers of the nine boxes that all see
+    // one source row (three x-consecutive boxes) -- the receivers of several
+    // boxes fill the lanes, where a box-centred item leaves the lanes beyond
+    // its box's last site empty.  Every (receiver, row) pair of the
+    // box-centred form is evaluated once, with the same sources in the same
+    // order, into the same row slot; slots of rows outside the grid are not
+    // written (the site-terms kernel skips them).
+    int item = blockIdx.x;  // items come longest first (see k_rowlist)
+    if (item >= *nwork) return;
+    int lane = threadIdx.x, nb = g.nb;
+    int2 w = work[item];
+    int q = w.x, c0 = w.y;
+    int x0 = q % nb, y = (q / nb) % nb, z = q / (nb * nb);
+    RowRecv L;
+    row_receivers(g, bstart, x0, y, z, L);
+    int cend = min(L.pre[9], c0 + NEAR_CHUNK);
+    if (cend - c0 > NEAR_TILE)
+        near_rows_item<HQ, MIX, 2>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, c0,
+                                   cend, lane);
+    else
+        near_rows_item<HQ, MIX, 1>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, c0,
+                                   cend, lane);
 }
 
 
@@ -1657,8 +1865,12 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
     double x = pk.x, y = pk.y, z = pk.z;
     double nx = 0.0, ny = 0.0, nz = 0.0;
     if ((what & SESMD_NEAR) && rkind[i] >= 0) {
+        // the rows (dz, dy) of the receiver's box that lie inside the grid
+        int b = cbox[i], nb = g.nb, by = (b / nb) % nb, bz = b / (nb * nb);
 #pragma unroll
         for (int q = 0; q < NEAR_ROWS; q++) {
+            int y = by + q % 3 - 1, z = bz + q / 3 - 1;
+            if (y < 0 || y >= nb || z < 0 || z >= nb) continue;
             nx += part[(size_t)(3 * q) * g.n + k];
             ny += part[(size_t)(3 * q + 1) * g.n + k];
             nz += part[(size_t)(3 * q + 2) * g.n + k];
@@ -3049,13 +3261,19 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->bflag, g.nb3));
     A(dalloc(&c->nvalid, 1));
     c->max_work = g.nb3 + (int)(n / NEAR_CHUNK) + 1;
+    // a row's list holds the receivers of up to 9 boxes: sum over rows of
+    // ceil(receivers / NEAR_CHUNK) <= 9 n / NEAR_CHUNK + (rows with receivers)
+    c->max_rwork = g.nb3 + (int)(9 * n / NEAR_CHUNK) + 1;
+    int max_keys = std::max(c->max_work, c->max_rwork);
     A(dalloc(&c->work, c->max_work));
-    A(dalloc(&c->work_raw, c->max_work));
-    A(dalloc(&c->wkey_in, c->max_work));
-    A(dalloc(&c->wkey_out, c->max_work));
-    A(dalloc(&c->widx_in, c->max_work));
-    A(dalloc(&c->widx_out, c->max_work));
+    A(dalloc(&c->rwork, c->max_rwork));
+    A(dalloc(&c->work_raw, max_keys));
+    A(dalloc(&c->wkey_in, max_keys));
+    A(dalloc(&c->wkey_out, max_keys));
+    A(dalloc(&c->widx_in, max_keys));
+    A(dalloc(&c->widx_out, max_keys));
     A(dalloc(&c->nwork, 1));
+    A(dalloc(&c->nrwork, 1));
     A(dalloc(&c->spos, n + SRC_PAD));
     A(dalloc(&c->npart, 27 * n));
     A(dalloc(&c->iperm, n));
@@ -3167,7 +3385,8 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
                                        (int)n, 0, endbit, c->stream);
     size_t wb = 0;
     hipcub::DeviceRadixSort::SortPairsDescending(nullptr, wb, c->wkey_in, c->wkey_out, c->widx_in,
-                                                 c->widx_out, c->max_work, 0, 32, c->stream);
+                                                 c->widx_out, std::max(c->max_work, c->max_rwork), 0, 32,
+                                                 c->stream);
     bytes = std::max(bytes, wb);
     c->cub_bytes = bytes;
     if (hipMalloc(&c->cub_tmp, bytes ? bytes : 16) != hipSuccess) {
@@ -3186,7 +3405,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
     if (c->graph_run_len) hipGraphExecDestroy(c->graph_run);
     void *ptrs[] = {c->r, c->v, c->a, c->f, c->type, c->mol, c->first, c->cls, c->anc, c->cbox,
                     c->rkind, c->isub, c->key_in, c->key_out, c->val_in, c->perm, c->cub_tmp,
-                    c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->work_raw, c->npart, c->iperm, c->wkey_in,
+                    c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->rwork, c->nrwork, c->work_raw, c->npart, c->iperm, c->wkey_in,
                     c->wkey_out, c->widx_in, c->widx_out, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
@@ -3316,12 +3535,13 @@ static int launch_rebuild(sesmd_ctx *c)
     k_bstart<<<nblk(4 * g.nb3 + 1, 256), 256, 0, st>>>(g, c->key_out, c->bs4, c->bstart);
     k_invperm<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->iperm);
     k_fill_i<<<nblk(g.nb3, 256), 256, 0, st>>>(c->bflag, g.nb3, 0);
-    k_worklist<<<1, 1024, 0, st>>>(g, c->bstart, c->work_raw, c->wkey_in, c->widx_in, c->nwork,
-                                   c->max_work);
+    k_rowlist<<<1, 1024, 0, st>>>(g, c->bstart, c->work_raw, c->wkey_in, c->widx_in, c->nrwork,
+                                  c->max_rwork);
     size_t wbytes = c->cub_bytes;
     HC(hipcub::DeviceRadixSort::SortPairsDescending(c->cub_tmp, wbytes, c->wkey_in, c->wkey_out, c->widx_in,
-                                                    c->widx_out, c->max_work, 0, 32, st));
-    k_work_gather<<<nblk(c->max_work, 256), 256, 0, st>>>(c->work_raw, c->widx_out, c->nwork, c->work);
+                                                    c->widx_out, c->max_rwork, 0, 32, st));
+    k_work_gather<<<nblk(c->max_rwork, 256), 256, 0, st>>>(c->work_raw, c->widx_out, c->nrwork, c->rwork);
+    c->boxlist = false;  // built on the first masked evaluation after this rebuild
     k_gather_meta<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->type, c->mol, c->first, c->cls, c->anc,
                                                   c->cbox, c->smeta, c->bflag);
     HC(hipGetLastError());
@@ -3396,21 +3616,46 @@ extern "C" int sesmd_fmm(sesmd_ctx *c)
     return 0;
 }
 
-// The near-field kernel for `mask` (~0: all 39 reference kernels).
+// The box-centred work list for masked evaluations, once per rebuild.
+static void ensure_boxlist(sesmd_ctx *c)
+{
+    if (c->boxlist) return;
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
+    k_worklist<<<1, 1024, 0, st>>>(g, c->bstart, c->work_raw, c->wkey_in, c->widx_in, c->nwork,
+                                   c->max_work);
+    size_t wbytes = c->cub_bytes;
+    hipcub::DeviceRadixSort::SortPairsDescending(c->cub_tmp, wbytes, c->wkey_in, c->wkey_out, c->widx_in,
+                                                 c->widx_out, c->max_work, 0, 32, st);
+    k_work_gather<<<nblk(c->max_work, 256), 256, 0, st>>>(c->work_raw, c->widx_out, c->nwork, c->work);
+    c->boxlist = true;
+}
+
+// The near-field kernel for `mask` (~0: all 39 reference kernels, the
+// row-centred form; otherwise the box-centred form with the mask).
 static void launch_near(sesmd_ctx *c, uint64_t mask)
 {
     const Geo &g = c->g;
     bool hq = c->ff.q[1] == c->ff.q[2];
-    bool mix = c->precision == SESMD_MIXED && mask == ~0ull;
+    if (mask == ~0ull) {
+        bool mix = c->precision == SESMD_MIXED;
+        dim3 grid(c->max_rwork), block(NEAR_TILE);
+#define ROWS_ARGS \
+    g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind, c->rwork, c->nrwork, c->dff, c->spos32, \
+        c->lj32, c->npart
+        if (mix && hq) k_near_rows<true, true><<<grid, block, 0, c->stream>>>(ROWS_ARGS);
+        else if (mix) k_near_rows<false, true><<<grid, block, 0, c->stream>>>(ROWS_ARGS);
+        else if (hq) k_near_rows<true, false><<<grid, block, 0, c->stream>>>(ROWS_ARGS);
+        else k_near_rows<false, false><<<grid, block, 0, c->stream>>>(ROWS_ARGS);
+#undef ROWS_ARGS
+        return;
+    }
+    ensure_boxlist(c);
     dim3 grid(c->max_work * NEAR_ROWS), block(NEAR_TILE);
 #define NEAR_ARGS \
     g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind, c->work, c->nwork, c->dff, mask, c->f, \
         c->spos32, c->lj32, c->npart
-    if (mix && hq) k_near<false, true, true><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
-    else if (mix) k_near<false, false, true><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
-    else if (mask == ~0ull && hq) k_near<false, true, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
-    else if (mask == ~0ull) k_near<false, false, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
-    else if (hq) k_near<true, true, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
+    if (hq) k_near<true, true, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
     else k_near<true, false, false><<<grid, block, 0, c->stream>>>(NEAR_ARGS);
 #undef NEAR_ARGS
 }
