@@ -163,11 +163,11 @@ def rms_force_error(sys_):
     return float(np.sqrt(((f[1] - f[0]) ** 2).sum() / (f[0] ** 2).sum()))
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_near_counters_bench_fp64.md')
+TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_near_rows_counters_bench_fp64.md')
 
 
 def near_traffic():
-    """HBM bytes per k_near launch on this workload, from the committed
+    """HBM bytes per k_near_rows launch on this workload, from the committed
     FETCH_SIZE / WRITE_SIZE counter runs (each scaled by what the counter reads
     for a plain copy of known size); None if that summary is absent."""
     try:
@@ -233,7 +233,7 @@ def main():
     achieved = flop / (r64['near_ms'] * 1e-3) / 1e12
     roofline = dict(bound='fp64-valu', achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS,
                     unit='TFLOP/s', frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=near_traffic(),
-                    kernel='k_near', kernel_ms=round(r64['near_ms'], 4),
+                    kernel='k_near_rows', kernel_ms=round(r64['near_ms'], 4),
                     pairs=pairs, lj_pairs=lj_pairs)
 
     out = dict(metric=METRIC, value=round(steps_s, 2), unit='steps/s', n_gpus=1,
