@@ -647,8 +647,7 @@ constexpr int SRC_PAD = 4;    // spare records after the sorted source arrays (r
 struct Recv {
     double x, y, z, q;       // receiver position, charge * xpi
     double qO, qH;           // q * (water O charge), q * (water H charge)
-    double cO, s6O;          // LJ against water oxygen
-    double rc2O;             // rc^2 where the receiver takes LJ from water O, else -1
+    bool ljO;                // an active receiver with a nonzero LJ constant against water O
     int ljrow, mol, rk;      // LJ table row of the receiver's type (32 * type)
     int by, bz;              // the receiver's box (y, z); every receiver of a wave has box x = bx
 };
@@ -703,21 +702,16 @@ __device__ __forceinline__ void near_pair(const Geo &g, const double4 &pj, const
         bool in[NR], any = false;
 #pragma unroll
         for (int r = 0; r < NR; r++) {
-            in[r] = r2[r] <= (CAT == 0 ? R[r].rc2O : g.rc2);
+            in[r] = r2[r] <= g.rc2 && (CAT != 0 || R[r].ljO);
             any = any || in[r];
         }
         if (__builtin_amdgcn_ballot_w64(any)) {
 #pragma unroll
             for (int r = 0; r < NR; r++) {
-                double c, s6;
-                if (CAT == 0) {
-                    c = R[r].cO;
-                    s6 = R[r].s6O;
-                } else {
-                    int tj = R[r].ljrow + (mj.y & 255);
-                    c = ljc[tj];
-                    s6 = ljs[tj];
-                }
+                // the constants are read here, only when the term is needed
+                // (registers for them would cost the kernel a wavefront per SIMD)
+                int tj = R[r].ljrow + (CAT == 0 ? 0 : (mj.y & 255));
+                double c = ljc[tj], s6 = ljs[tj];
                 double r6 = rinv2[r] * rinv2[r] * rinv2[r];
                 double flj = c * r6 * (s6 * r6 - 0.5) * rinv2[r];
                 ft[r] = (c != 0.0 && in[r]) ? ft[r] + flj : ft[r];
@@ -822,7 +816,7 @@ __device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict
 // in FP32 on coordinates relative to the source box's centre, the forces of
 // one (box, class) segment summed in FP32 and then added to the FP64 totals.
 struct Recv32 {
-    float q, qO, qH, cO, s6O, rc2O;
+    float q, qO, qH, cO, s6O;
 };
 
 template <int CAT, bool CHECK, bool CEN, bool HQ, int NR>
@@ -860,7 +854,7 @@ __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, in
         bool in[NR], any = false;
 #pragma unroll
         for (int r = 0; r < NR; r++) {
-            in[r] = r2[r] <= (CAT == 0 ? Q[r].rc2O : rc2);
+            in[r] = r2[r] <= rc2 && (CAT != 0 || R[r].ljO);
             any = any || in[r];
         }
         if (__builtin_amdgcn_ballot_w64(any)) {
@@ -1000,9 +994,7 @@ __device__ __forceinline__ void load_recv(const Geo &g, const double4 *__restric
     }
     R.qO = R.q * ff->q[0];
     R.qH = R.q * ff->q[1];
-    R.cO = ff->lj_c[ti * 32];
-    R.s6O = ff->lj_s6[ti * 32];
-    R.rc2O = active && R.cO != 0.0 ? g.rc2 : -1.0;
+    R.ljO = active && ff->lj_c[ti * 32] != 0.0;
     R.ljrow = ti * 32;
 }
 
@@ -1027,7 +1019,6 @@ __device__ __forceinline__ void near_sources(const Geo &g, const double4 *__rest
             Q[r].qH = (float)R[r].qH;
             Q[r].cO = lj32[ti[r] * 32];
             Q[r].s6O = lj32[1024 + ti[r] * 32];
-            Q[r].rc2O = (float)R[r].rc2O;
         }
         if (check)
             near_row32<true, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, ax, ay, az);
@@ -1085,11 +1076,8 @@ __device__ __forceinline__ void near_item(
     }
 }
 
-#ifndef NEAR_MINW
-#define NEAR_MINW 1
-#endif
 template <bool MASKED, bool HQ, bool MIX>
-__global__ __launch_bounds__(NEAR_TILE, NEAR_MINW) void k_near(
+__global__ __launch_bounds__(NEAR_TILE) void k_near(
     Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
     const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
     const int *__restrict__ rkind, const int2 *__restrict__ work, const int *__restrict__ nwork,
@@ -1255,8 +1243,9 @@ __device__ __forceinline__ void near_rows_item(
     }
 }
 
+// FP64: at most 96 VGPRs, five wavefronts per SIMD (measured 2.5% faster than four)
 template <bool HQ, bool MIX>
-__global__ __launch_bounds__(NEAR_TILE, NEAR_MINW) void k_near_rows(
+__global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
     Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
     const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
     const int *__restrict__ rkind, const int2 *__restrict__ work, const int *__restrict__ nwork,
