@@ -121,7 +121,8 @@ struct sesmd_ctx {
     size_t cub_bytes;
     int *bstart, *nvalid;
     int *bs4;    // first sorted position of every (box, LJ class) key
-    int *bflag;  // box holds a site not listed from every neighbour box
+    int *bflag;  // box holds a site not listed from every neighbour box (inside kcnt)
+    int *kcnt;   // box sort: counts per key, then arrival counters per key
     int2 *work;                      // box-centred near-field items (masked evaluations)
     int *nwork;
     int max_work;
@@ -129,7 +130,7 @@ struct sesmd_ctx {
     int2 *rwork;                     // row-centred near-field items (the full near field)
     int *nrwork;
     int max_rwork;
-    int2 *work_raw;                  // items in list order, before the cost sort
+    int2 *work_raw;                  // box-centred items in box order, before the cost sort
     unsigned *wkey_in, *wkey_out;    // 1 + estimated cost per item (0: padding)
     int *widx_in, *widx_out;
     double4 *spos;
@@ -165,9 +166,10 @@ struct sesmd_ctx {
     SurfConst sconst;
     // graphs of one regular step and of the ifar - 1 regular steps between
     // two rebuilds
-    hipGraphExec_t graph, graph_run;
+    hipGraphExec_t graph, graph_run, graph_cycle;
     bool graph_ok;
     int graph_run_len;
+    int graph_cycle_len;  // steps in graph_cycle: a rebuild step, then ifar - 1 regular steps
     bool built;
     bool classes_fixed;  // classes/storage boxes set by the caller (sesmd_set_classes)
 };
@@ -421,7 +423,7 @@ __global__ void k_classify(Geo g, const double *__restrict__ r, const int *__res
 __global__ void k_site_box(Geo g, const double *__restrict__ r, const int *__restrict__ mol,
                            const int *__restrict__ type, const int *__restrict__ cls,
                            const int *__restrict__ anc, int *__restrict__ cbox,
-                           int *__restrict__ rkind, int *__restrict__ key, int *__restrict__ val)
+                           int *__restrict__ rkind, int *__restrict__ key)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
@@ -456,7 +458,6 @@ __global__ void k_site_box(Geo g, const double *__restrict__ r, const int *__res
     // near-field loops need no per-source type test
     int t = type[i];
     key[i] = 4 * b + (t == 0 ? 0 : (t >= 3 ? 1 : 2));
-    val[i] = i;
 }
 
 // buildinternal, src/buildinternal.f90:11-63: the same-molecule partners of a
@@ -501,19 +502,63 @@ __global__ void k_invperm(Geo g, const int *__restrict__ perm, int *__restrict__
     if (k < g.n) iperm[perm[k]] = k;
 }
 
-__global__ void k_bstart(Geo g, const int *__restrict__ keys, int *__restrict__ bs4,
-                         int *__restrict__ bstart)
+// The box sort (stable by site index within each key 4 box + LJ class), a
+// counting sort in four launches with no library call and no memset, so a
+// rebuild can be captured in a graph: count per key, scan (key starts bs4,
+// box starts bstart), scatter (positions within a key in arrival order), then
+// order each key's sites by index (a rank sort per key: every key holds at
+// most a few hundred sites).
+__global__ void k_key_count(Geo g, const int *__restrict__ key, int *__restrict__ cnt)
 {
-    int kk = blockIdx.x * blockDim.x + threadIdx.x;
-    if (kk > 4 * g.nb3) return;
-    int lo = 0, hi = g.n;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (keys[mid] < kk) lo = mid + 1;
-        else hi = mid;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < g.n) atomicAdd(&cnt[key[i]], 1);
+}
+
+__global__ __launch_bounds__(1024) void k_key_scan(Geo g, const int *__restrict__ cnt, int *__restrict__ bs4,
+                                                   int *__restrict__ bstart)
+{
+    __shared__ int s[1024];
+    int K = 4 * g.nb3, t = threadIdx.x;
+    int per = (K + (int)blockDim.x - 1) / (int)blockDim.x;
+    int a = min(t * per, K), b = min(a + per, K), sum = 0;
+    for (int i = a; i < b; i++) sum += cnt[i];
+    s[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        int v = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
     }
-    bs4[kk] = lo;
-    if ((kk & 3) == 0) bstart[kk >> 2] = lo;
+    int run = s[t] - sum;
+    for (int i = a; i < b; i++) {
+        bs4[i] = run;
+        if ((i & 3) == 0) bstart[i >> 2] = run;
+        run += cnt[i];
+    }
+    if (t == (int)blockDim.x - 1) {
+        bs4[K] = s[t];
+        bstart[g.nb3] = s[t];
+    }
+}
+
+__global__ void k_key_scatter(Geo g, const int *__restrict__ key, const int *__restrict__ bs4,
+                              int *__restrict__ fill, int *__restrict__ tmp)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < g.n) tmp[bs4[key[i]] + atomicAdd(&fill[key[i]], 1)] = i;
+}
+
+__global__ __launch_bounds__(64) void k_key_order(Geo g, const int *__restrict__ bs4, const int *__restrict__ tmp,
+                                                  int *__restrict__ perm, int *__restrict__ key_out)
+{
+    int kk = blockIdx.x, s0 = bs4[kk], len = bs4[kk + 1] - s0;
+    for (int j = threadIdx.x; j < len; j += blockDim.x) {
+        int v = tmp[s0 + j], rank = 0;
+        for (int t = 0; t < len; t++) rank += tmp[s0 + t] < v ? 1 : 0;
+        perm[s0 + rank] = v;
+        key_out[s0 + rank] = kk;
+    }
 }
 
 // Near-field work items (one block): per box, chunks of NEAR_CHUNK receivers, in box order,
@@ -755,6 +800,9 @@ __device__ __forceinline__ void near_segment(const Geo &g, const double4 *__rest
     // past s1 (the sorted arrays carry SRC_PAD spare records) and use one
     // incremented pointer per array, which keeps the scalar unit's address
     // arithmetic to a few instructions per pair of sources.
+    // (A loop body without the branch on the second source lets the two
+    // sources' arithmetic interleave, as the FP32 form does; in FP64 the extra
+    // live registers cost more than the interleaving gains.)
     const double4 *ps = spos + s0;
     const int4 *pm = smeta + s0;
     int4 mA = pm[0], mB = pm[1];
@@ -904,11 +952,14 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
     float fx[NR], fy[NR], fz[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) fx[r] = fy[r] = fz[r] = 0.0f;
+    // No branch between the loop body's two sources, so their pair arithmetic
+    // interleaves; an odd last source runs after the loop.
     const float4 *ps = spos32 + s0;
     const int4 *pm = smeta + s0;
     int4 mA = pm[0], mB = pm[1];
     float4 pA = ps[0], pB = ps[1];
-    for (int t = s1 - s0; t > 0; t -= 2) {
+    int t = s1 - s0;
+    for (; t > 1; t -= 2) {
         const int4 ma = mA, mb = mB;
         const float4 pa = pA, pb = pB;
         ps += 2;
@@ -919,9 +970,9 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
         pB = ps[1];
         __asm__ volatile("" ::: "memory");  // keeps these loads ahead of the pair arithmetic
         near_pair32<CAT, CHECK, CEN, HQ, NR>(pa, ma, bx, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
-        if (t > 1)
-            near_pair32<CAT, CHECK, CEN, HQ, NR>(pb, mb, bx, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
+        near_pair32<CAT, CHECK, CEN, HQ, NR>(pb, mb, bx, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
     }
+    if (t == 1) near_pair32<CAT, CHECK, CEN, HQ, NR>(pA, mA, bx, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         ax[r] += (double)fx[r];
@@ -1136,55 +1187,62 @@ __device__ __forceinline__ void row_receivers(const Geo &g, const int *__restric
     }
 }
 
-// Row-centred work list (one block): one item per NEAR_CHUNK receivers of
-// each row's receiver list, with a cost estimate (receivers x row sources) as
-// the sort key; ordered by decreasing cost like k_worklist's items.
-__global__ void k_rowlist(Geo g, const int *__restrict__ bstart, int2 *__restrict__ work,
-                          unsigned *__restrict__ wkey, int *__restrict__ widx, int *__restrict__ nwork,
-                          int max_work)
+// Row-centred work list (one block): one item per NEAR_CHUNK receivers of each
+// row's receiver list, longest first.  Items are placed by a bucket order on
+// their cost estimate (receivers x row sources; four buckets per octave), so
+// no sort launch is needed; the order within a bucket is arrival order, which
+// changes no result (every item writes its own receivers' row slots).
+constexpr int RL_BUCKETS = 128;
+
+__device__ __forceinline__ int cost_bucket(unsigned cost)
 {
-    __shared__ int s_cnt[1024];
-    __shared__ int s_base;
-    if (threadIdx.x == 0) s_base = 0;
+    cost |= 1u;
+    int e = 31 - __clz(cost);
+    int frac = e >= 2 ? (cost >> (e - 2)) & 3 : (cost << (2 - e)) & 3;
+    return min(4 * e + frac, RL_BUCKETS - 1);
+}
+
+__device__ __forceinline__ void row_item_info(const Geo &g, const int *__restrict__ bstart, int q, int &total,
+                                              int &src)
+{
+    int nb = g.nb, x0 = q % nb, y = (q / nb) % nb, z = q / (nb * nb);
+    RowRecv L;
+    row_receivers(g, bstart, x0, y, z, L);
+    total = L.pre[9];
+    int r0 = max(x0 - 1, 0) + nb * (y + nb * z), r1 = min(x0 + 1, nb - 1) + nb * (y + nb * z);
+    src = bstart[r1 + 1] - bstart[r0];
+}
+
+__global__ __launch_bounds__(1024) void k_rowlist(Geo g, const int *__restrict__ bstart, int2 *__restrict__ work,
+                                                  int *__restrict__ nwork, int max_work)
+{
+    __shared__ int s_hist[RL_BUCKETS], s_off[RL_BUCKETS];
+    for (int i = threadIdx.x; i < RL_BUCKETS; i += blockDim.x) s_hist[i] = 0;
     __syncthreads();
-    int nb = g.nb;
-    for (int q0 = 0; q0 < g.nb3; q0 += blockDim.x) {
-        int q = q0 + threadIdx.x;
-        int nch = 0, total = 0, src = 0;
-        if (q < g.nb3) {
-            int x0 = q % nb, y = (q / nb) % nb, z = q / (nb * nb);
-            RowRecv L;
-            row_receivers(g, bstart, x0, y, z, L);
-            total = L.pre[9];
-            nch = (total + NEAR_CHUNK - 1) / NEAR_CHUNK;
-            int r0 = max(x0 - 1, 0) + nb * (y + nb * z), r1 = min(x0 + 1, nb - 1) + nb * (y + nb * z);
-            src = bstart[r1 + 1] - bstart[r0];
-        }
-        s_cnt[threadIdx.x] = nch;
-        __syncthreads();
-        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-            int v = threadIdx.x >= (unsigned)off ? s_cnt[threadIdx.x - off] : 0;
-            __syncthreads();
-            s_cnt[threadIdx.x] += v;
-            __syncthreads();
-        }
-        int excl = s_base + s_cnt[threadIdx.x] - nch;
-        for (int c = 0; c < nch; c++)
-            if (excl + c < max_work) {
-                int c0 = c * NEAR_CHUNK, nr = min(NEAR_CHUNK, total - c0);
-                work[excl + c] = make_int2(q, c0);
-                wkey[excl + c] = 1u + (unsigned)min(nr * src, 1 << 30);
-                widx[excl + c] = excl + c;
-            }
-        __syncthreads();
-        if (threadIdx.x == blockDim.x - 1) s_base += s_cnt[threadIdx.x];
-        __syncthreads();
+    for (int q = threadIdx.x; q < g.nb3; q += blockDim.x) {
+        int total, src;
+        row_item_info(g, bstart, q, total, src);
+        for (int c0 = 0; c0 < total; c0 += NEAR_CHUNK)
+            atomicAdd(&s_hist[cost_bucket((unsigned)(min(NEAR_CHUNK, total - c0) * src))], 1);
     }
-    for (int i = min(s_base, max_work) + threadIdx.x; i < max_work; i += blockDim.x) {
-        wkey[i] = 0u;
-        widx[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int b = RL_BUCKETS - 1; b >= 0; b--) {
+            s_off[b] = run;
+            run += s_hist[b];
+        }
+        *nwork = min(run, max_work);
     }
-    if (threadIdx.x == 0) *nwork = min(s_base, max_work);
+    __syncthreads();
+    for (int q = threadIdx.x; q < g.nb3; q += blockDim.x) {
+        int total, src;
+        row_item_info(g, bstart, q, total, src);
+        for (int c0 = 0; c0 < total; c0 += NEAR_CHUNK) {
+            int pos = atomicAdd(&s_off[cost_bucket((unsigned)(min(NEAR_CHUNK, total - c0) * src))], 1);
+            if (pos < max_work) work[pos] = make_int2(q, c0);
+        }
+    }
 }
 
 // One row-centred item: positions c0 + lane (+ 64 for NR = 2) of the row's
@@ -3247,20 +3305,21 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->perm, n));
     A(dalloc(&c->bstart, g.nb3 + 1));
     A(dalloc(&c->bs4, 4 * (size_t)g.nb3 + 1));
-    A(dalloc(&c->bflag, g.nb3));
+    // key counts (4 nb3), arrival counters (4 nb3) and bflag (nb3): one allocation, one fill per rebuild
+    A(dalloc(&c->kcnt, 8 * (size_t)g.nb3 + g.nb3));
+    c->bflag = c->kcnt + 8 * (size_t)g.nb3;
     A(dalloc(&c->nvalid, 1));
     c->max_work = g.nb3 + (int)(n / NEAR_CHUNK) + 1;
     // a row's list holds the receivers of up to 9 boxes: sum over rows of
     // ceil(receivers / NEAR_CHUNK) <= 9 n / NEAR_CHUNK + (rows with receivers)
     c->max_rwork = g.nb3 + (int)(9 * n / NEAR_CHUNK) + 1;
-    int max_keys = std::max(c->max_work, c->max_rwork);
     A(dalloc(&c->work, c->max_work));
     A(dalloc(&c->rwork, c->max_rwork));
-    A(dalloc(&c->work_raw, max_keys));
-    A(dalloc(&c->wkey_in, max_keys));
-    A(dalloc(&c->wkey_out, max_keys));
-    A(dalloc(&c->widx_in, max_keys));
-    A(dalloc(&c->widx_out, max_keys));
+    A(dalloc(&c->work_raw, c->max_work));
+    A(dalloc(&c->wkey_in, c->max_work));
+    A(dalloc(&c->wkey_out, c->max_work));
+    A(dalloc(&c->widx_in, c->max_work));
+    A(dalloc(&c->widx_out, c->max_work));
     A(dalloc(&c->nwork, 1));
     A(dalloc(&c->nrwork, 1));
     A(dalloc(&c->spos, n + SRC_PAD));
@@ -3367,16 +3426,10 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
         return 1;
     }
     // radix-sort workspace
+    // (the box-centred list of masked evaluations only; the rebuild itself calls no library sort)
     size_t bytes = 0;
-    int endbit = 1;
-    while ((1 << endbit) <= 4 * g.nb3) endbit++;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->key_in, c->key_out, c->val_in, c->perm,
-                                       (int)n, 0, endbit, c->stream);
-    size_t wb = 0;
-    hipcub::DeviceRadixSort::SortPairsDescending(nullptr, wb, c->wkey_in, c->wkey_out, c->widx_in,
-                                                 c->widx_out, std::max(c->max_work, c->max_rwork), 0, 32,
-                                                 c->stream);
-    bytes = std::max(bytes, wb);
+    hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, c->wkey_in, c->wkey_out, c->widx_in,
+                                                 c->widx_out, c->max_work, 0, 32, c->stream);
     c->cub_bytes = bytes;
     if (hipMalloc(&c->cub_tmp, bytes ? bytes : 16) != hipSuccess) {
         sesmd_destroy(c);
@@ -3392,9 +3445,10 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->graph_ok) hipGraphExecDestroy(c->graph);
     if (c->graph_run_len) hipGraphExecDestroy(c->graph_run);
+    if (c->graph_cycle_len) hipGraphExecDestroy(c->graph_cycle);
     void *ptrs[] = {c->r, c->v, c->a, c->f, c->type, c->mol, c->first, c->cls, c->anc, c->cbox,
                     c->rkind, c->isub, c->key_in, c->key_out, c->val_in, c->perm, c->cub_tmp,
-                    c->bstart, c->bs4, c->bflag, c->nvalid, c->work, c->nwork, c->rwork, c->nrwork, c->work_raw, c->npart, c->iperm, c->wkey_in,
+                    c->bstart, c->bs4, c->kcnt, c->nvalid, c->work, c->nwork, c->rwork, c->nrwork, c->work_raw, c->npart, c->iperm, c->wkey_in,
                     c->wkey_out, c->widx_in, c->widx_out, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
@@ -3462,6 +3516,18 @@ extern "C" int sesmd_set_nh(sesmd_ctx *c, double tr, double pnh1, double pnh2)
     return 0;
 }
 
+// Captured step graphs bake in host-side choices (classification on or off,
+// graphite arrays): drop them when those change.
+static void drop_graphs(sesmd_ctx *c)
+{
+    if (c->graph_ok) hipGraphExecDestroy(c->graph);
+    c->graph_ok = false;
+    if (c->graph_run_len) hipGraphExecDestroy(c->graph_run);
+    c->graph_run_len = 0;
+    if (c->graph_cycle_len) hipGraphExecDestroy(c->graph_cycle);
+    c->graph_cycle_len = 0;
+}
+
 extern "C" int sesmd_set_graphite(sesmd_ctx *c, int n, const double *xyz)
 {
     if (!c) return fail("null context");
@@ -3495,15 +3561,7 @@ extern "C" int sesmd_set_graphite(sesmd_ctx *c, int n, const double *xyz)
     if (n) HC(hipMemcpy(c->gr, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
     HC(hipMemcpy(c->gstart, start.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
     c->ngr = n;
-    c->graph_ok = false;
-    if (c->graph) {
-        hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
-    if (c->graph_run_len) {
-        hipGraphExecDestroy(c->graph_run);
-        c->graph_run_len = 0;
-    }
+    drop_graphs(c);
     return 0;
 }
 
@@ -3514,22 +3572,17 @@ static int launch_rebuild(sesmd_ctx *c)
     if (!c->classes_fixed)
         k_classify<<<nblk(g.nmol, 256), 256, 0, st>>>(g, c->r, c->first, c->cls, c->anc);
     k_site_box<<<nblk(g.n, 256), 256, 0, st>>>(g, c->r, c->mol, c->type, c->cls, c->anc, c->cbox,
-                                               c->rkind, c->key_in, c->val_in);
+                                               c->rkind, c->key_in);
     if (g.ns) k_isub<<<nblk(29 * g.ns, 128), 128, 0, st>>>(g, c->r, c->anc, c->isub);
-    int endbit = 1;
-    while ((1 << endbit) <= 4 * g.nb3) endbit++;
-    size_t bytes = c->cub_bytes;
-    HC(hipcub::DeviceRadixSort::SortPairs(c->cub_tmp, bytes, c->key_in, c->key_out, c->val_in,
-                                          c->perm, g.n, 0, endbit, st));
-    k_bstart<<<nblk(4 * g.nb3 + 1, 256), 256, 0, st>>>(g, c->key_out, c->bs4, c->bstart);
+    // counting sort by key (kcnt: counts then arrival counters), and bflag, cleared by one fill
+    int K = 4 * g.nb3;
+    k_fill_i<<<nblk(2 * K + g.nb3, 256), 256, 0, st>>>(c->kcnt, 2 * K + g.nb3, 0);
+    k_key_count<<<nblk(g.n, 256), 256, 0, st>>>(g, c->key_in, c->kcnt);
+    k_key_scan<<<1, 1024, 0, st>>>(g, c->kcnt, c->bs4, c->bstart);
+    k_key_scatter<<<nblk(g.n, 256), 256, 0, st>>>(g, c->key_in, c->bs4, c->kcnt + K, c->val_in);
+    k_key_order<<<K, 64, 0, st>>>(g, c->bs4, c->val_in, c->perm, c->key_out);
     k_invperm<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->iperm);
-    k_fill_i<<<nblk(g.nb3, 256), 256, 0, st>>>(c->bflag, g.nb3, 0);
-    k_rowlist<<<1, 1024, 0, st>>>(g, c->bstart, c->work_raw, c->wkey_in, c->widx_in, c->nrwork,
-                                  c->max_rwork);
-    size_t wbytes = c->cub_bytes;
-    HC(hipcub::DeviceRadixSort::SortPairsDescending(c->cub_tmp, wbytes, c->wkey_in, c->wkey_out, c->widx_in,
-                                                    c->widx_out, c->max_rwork, 0, 32, st));
-    k_work_gather<<<nblk(c->max_rwork, 256), 256, 0, st>>>(c->work_raw, c->widx_out, c->nrwork, c->rwork);
+    k_rowlist<<<1, 1024, 0, st>>>(g, c->bstart, c->rwork, c->nrwork, c->max_rwork);
     c->boxlist = false;  // built on the first masked evaluation after this rebuild
     k_gather_meta<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->type, c->mol, c->first, c->cls, c->anc,
                                                   c->cbox, c->smeta, c->bflag);
@@ -3849,12 +3902,14 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
     return 0;
 }
 
-static int capture_steps(sesmd_ctx *c, int nsteps, hipGraphExec_t *out)
+// Captures nsteps steps, the first a rebuild step if `rebuild`.  Every launch of
+// a step is a kernel (the rebuild's sort included), with no memset node.
+static int capture_steps(sesmd_ctx *c, int nsteps, hipGraphExec_t *out, bool rebuild = false)
 {
     hipGraph_t gr;
     HC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int r = 0;
-    for (int k = 0; k < nsteps && !r; k++) r = launch_step(c, false);
+    for (int k = 0; k < nsteps && !r; k++) r = launch_step(c, rebuild && k == 0);
     hipError_t e = hipStreamEndCapture(c->stream, &gr);
     if (r) return 1;
     HC(e);
@@ -3871,6 +3926,20 @@ extern "C" int sesmd_steps(sesmd_ctx *c, int nsteps, int ifar, int *kb)
     if (ifar < 1) return fail("ifar must be >= 1");
     int run = ifar - 1;  // regular steps between two rebuild steps
     for (int s = 0; s < nsteps;) {
+        if ((*kb + 1) % ifar == 0 && nsteps - s >= ifar) {
+            // a whole cycle, the rebuild step and its run of regular steps, as one graph
+            if (c->graph_cycle_len != ifar) {
+                if (c->graph_cycle_len) HC(hipGraphExecDestroy(c->graph_cycle));
+                c->graph_cycle_len = 0;
+                CK(capture_steps(c, ifar, &c->graph_cycle, true));
+                c->graph_cycle_len = ifar;
+            }
+            HC(hipGraphLaunch(c->graph_cycle, c->stream));
+            c->boxlist = false;  // the replayed rebuild made a new box sort
+            *kb += ifar;
+            s += ifar;
+            continue;
+        }
         if ((*kb + 1) % ifar == 0) {
             (*kb)++;
             s++;
@@ -3978,6 +4047,7 @@ extern "C" int sesmd_set_classes(sesmd_ctx *c, const int *cls, const int *anc)
     if (!c) return fail("null context");
     const Geo &g = c->g;
     if (!cls || !anc) {
+        if (c->classes_fixed) drop_graphs(c);
         c->classes_fixed = false;
         return 0;
     }
@@ -3988,6 +4058,7 @@ extern "C" int sesmd_set_classes(sesmd_ctx *c, const int *cls, const int *anc)
     }
     HC(hipMemcpy(c->cls, cls, sizeof(int) * g.nmol, hipMemcpyHostToDevice));
     HC(hipMemcpy(c->anc, anc, sizeof(int) * g.nmol, hipMemcpyHostToDevice));
+    if (!c->classes_fixed) drop_graphs(c);  // a rebuild no longer classifies
     c->classes_fixed = true;
     return 0;
 }
