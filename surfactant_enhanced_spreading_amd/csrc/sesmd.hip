@@ -417,13 +417,40 @@ __global__ void k_classify(Geo g, const double *__restrict__ r, const int *__res
     }
 }
 
+// Reserves one slot per active lane in counter[key] with one atomic per
+// distinct key in the wavefront (keys of neighbouring lanes mostly agree, and
+// contended global atomics serialise): the lanes are grouped by key first
+// (register work only), then every group's first lane adds its group's size,
+// all in one atomic instruction.  Returns the counter's value before the
+// group's addition plus the lane's rank in its group.
+__device__ __forceinline__ int wave_reserve(int *counter, int key)
+{
+    int lane = threadIdx.x & 63, leader_of = lane, rank = 0, size = 0;
+    unsigned long long todo = __builtin_amdgcn_ballot_w64(true);
+    while (todo) {
+        int leader = __builtin_ctzll(todo);
+        int kl = __builtin_amdgcn_readlane(key, leader);
+        unsigned long long same = __builtin_amdgcn_ballot_w64(key == kl) & todo;
+        if ((same >> lane) & 1ull) {
+            leader_of = leader;
+            rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(same >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((unsigned)same, 0u));
+        }
+        if (lane == leader) size = (int)__builtin_popcountll(same);
+        todo &= ~same;
+    }
+    int base = 0;
+    if (lane == leader_of) base = atomicAdd(&counter[key], size);
+    return __shfl(base, leader_of, 64) + rank;
+}
+
 // Site boxes: W1 sites live in their molecule's box; W2 and surfactant sites in
 // the box whose half-open cube [c - rs/2, c + rs/2) holds them (isCentral,
 // src/ibox.f90:101-124, as the list builders use it).  Receiver kinds follow.
 __global__ void k_site_box(Geo g, const double *__restrict__ r, const int *__restrict__ mol,
                            const int *__restrict__ type, const int *__restrict__ cls,
                            const int *__restrict__ anc, int *__restrict__ cbox,
-                           int *__restrict__ rkind, int *__restrict__ key)
+                           int *__restrict__ rkind, int *__restrict__ key, int *__restrict__ kcount)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
@@ -457,50 +484,50 @@ __global__ void k_site_box(Geo g, const double *__restrict__ r, const int *__res
     // sort key: box, then LJ class (water O, surfactant site, water H), so the
     // near-field loops need no per-source type test
     int t = type[i];
-    key[i] = 4 * b + (t == 0 ? 0 : (t >= 3 ? 1 : 2));
+    int k = 4 * b + (t == 0 ? 0 : (t >= 3 ? 1 : 2));
+    key[i] = k;
+    wave_reserve(kcount, k);  // the box sort's key counts (cleared before this launch)
 }
 
 // buildinternal, src/buildinternal.f90:11-63: the same-molecule partners of a
 // surfactant site that lie outside the 3^3 block around the site's box (box by
 // ibox1000 of the molecule's rank, centre by rcen1000, test by isnearest).
-__global__ void k_isub(Geo g, const double *__restrict__ r, const int *__restrict__ anc,
-                       unsigned *__restrict__ isub)
+// One 64-lane block per molecule: lane j < 29 tests site j against the
+// molecule's other sites, read from LDS.
+__global__ __launch_bounds__(64) void k_isub(Geo g, const double *__restrict__ r, const int *__restrict__ anc,
+                                             unsigned *__restrict__ isub)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 29 * g.ns) return;
-    int s = t / 29, j = t % 29;
+    __shared__ double xs[3][29];
+    int s = blockIdx.x, j = threadIdx.x;
     int m = g.nw + s;
     int base = 3 * g.nw + 29 * s;
+    if (j < 29)
+        for (int d = 0; d < 3; d++) xs[d][j] = r[d * g.n + base + j];
+    __syncthreads();
+    if (j >= 29) return;
     int a = anc[m];
     int q[3] = {(a % g.nb) / 4, ((a / g.nb) % g.nb) / 4, (a / (g.nb * g.nb)) / 4};
     double cen[3];
     for (int d = 0; d < 3; d++) {
         double c2 = d_rcen2(g, q[d]);
-        double xm = r[d * g.n + base + j];
+        double xm = xs[d][j];
         int li = (int)((xm - c2 + 5.0 * g.rs) / g.rs) + 1;
         cen[d] = 10.0 * g.rs * li / 10.0 - 10.0 * g.rs / 2.0 - 10.0 * g.rs / 20.0 + c2;
     }
     double rs15 = 1.5 * g.rs;
     unsigned mask = 0;
     for (int k = 0; k < 29; k++) {
-        if (k == j) continue;
         bool near = true;
+#pragma unroll
         for (int d = 0; d < 3; d++) {
-            double xn = r[d * g.n + base + k];
+            double xn = xs[d][k];
             near = near && (xn < cen[d] + rs15 && xn >= cen[d] - rs15);
         }
-        if (!near) mask |= 1u << k;
+        if (k != j && !near) mask |= 1u << k;
     }
     isub[base + j] = mask;
 }
 
-// first sorted position of every (box, class) key and of every box (binary
-// search in the sorted keys)
-__global__ void k_invperm(Geo g, const int *__restrict__ perm, int *__restrict__ iperm)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < g.n) iperm[perm[k]] = k;
-}
 
 // The box sort (stable by site index within each key 4 box + LJ class), a
 // counting sort in four launches with no library call and no memset, so a
@@ -508,56 +535,66 @@ __global__ void k_invperm(Geo g, const int *__restrict__ perm, int *__restrict__
 // box starts bstart), scatter (positions within a key in arrival order), then
 // order each key's sites by index (a rank sort per key: every key holds at
 // most a few hundred sites).
-__global__ void k_key_count(Geo g, const int *__restrict__ key, int *__restrict__ cnt)
+// One block: exclusive scan of the key counts (bs4, K + 1 entries) and the box
+// starts (bstart, every fourth).  Thread t scans `per` consecutive keys; the
+// per-thread totals are scanned with wavefront prefix sums.
+__device__ __forceinline__ void key_scan_block(const Geo &g, const int *__restrict__ cnt, int *__restrict__ bs4,
+                                               int *__restrict__ bstart)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < g.n) atomicAdd(&cnt[key[i]], 1);
-}
-
-__global__ __launch_bounds__(1024) void k_key_scan(Geo g, const int *__restrict__ cnt, int *__restrict__ bs4,
-                                                   int *__restrict__ bstart)
-{
-    __shared__ int s[1024];
-    int K = 4 * g.nb3, t = threadIdx.x;
+    __shared__ int wsum[16];
+    int K = 4 * g.nb3, t = threadIdx.x, lane = t & 63, w = t >> 6;
     int per = (K + (int)blockDim.x - 1) / (int)blockDim.x;
     int a = min(t * per, K), b = min(a + per, K), sum = 0;
     for (int i = a; i < b; i++) sum += cnt[i];
-    s[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-        int v = t >= off ? s[t - off] : 0;
-        __syncthreads();
-        s[t] += v;
-        __syncthreads();
+    int incl = sum;  // inclusive prefix within the wavefront
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
     }
-    int run = s[t] - sum;
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int j = 0; j < (int)(blockDim.x >> 6); j++) {
+        base += j < w ? wsum[j] : 0;
+        total += wsum[j];
+    }
+    int run = base + incl - sum;
     for (int i = a; i < b; i++) {
         bs4[i] = run;
         if ((i & 3) == 0) bstart[i >> 2] = run;
         run += cnt[i];
     }
-    if (t == (int)blockDim.x - 1) {
-        bs4[K] = s[t];
-        bstart[g.nb3] = s[t];
+    if (t == 0) {
+        bs4[K] = total;
+        bstart[g.nb3] = total;
     }
 }
 
+// Positions within a key in arrival order (k_key_order orders them).
 __global__ void k_key_scatter(Geo g, const int *__restrict__ key, const int *__restrict__ bs4,
                               int *__restrict__ fill, int *__restrict__ tmp)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < g.n) tmp[bs4[key[i]] + atomicAdd(&fill[key[i]], 1)] = i;
+    if (i >= g.n) return;
+    int k = key[i];
+    tmp[bs4[k] + wave_reserve(fill, k)] = i;
 }
 
+// Each key's sites in index order, by a rank sort (distinct indices; a key
+// holds at most a few hundred sites), one wavefront per key 4 box + class; also
+// the inverse map site -> sorted position.
 __global__ __launch_bounds__(64) void k_key_order(Geo g, const int *__restrict__ bs4, const int *__restrict__ tmp,
-                                                  int *__restrict__ perm, int *__restrict__ key_out)
+                                                  int *__restrict__ perm, int *__restrict__ key_out,
+                                                  int *__restrict__ iperm)
 {
     int kk = blockIdx.x, s0 = bs4[kk], len = bs4[kk + 1] - s0;
-    for (int j = threadIdx.x; j < len; j += blockDim.x) {
+    for (int j = threadIdx.x; j < len; j += 64) {
         int v = tmp[s0 + j], rank = 0;
         for (int t = 0; t < len; t++) rank += tmp[s0 + t] < v ? 1 : 0;
         perm[s0 + rank] = v;
         key_out[s0 + rank] = kk;
+        iperm[v] = s0 + rank;
     }
 }
 
@@ -1187,11 +1224,11 @@ __device__ __forceinline__ void row_receivers(const Geo &g, const int *__restric
     }
 }
 
-// Row-centred work list (one block): one item per NEAR_CHUNK receivers of each
-// row's receiver list, longest first.  Items are placed by a bucket order on
-// their cost estimate (receivers x row sources; four buckets per octave), so
-// no sort launch is needed; the order within a bucket is arrival order, which
-// changes no result (every item writes its own receivers' row slots).
+// Row-centred work list: one item per NEAR_CHUNK receivers of each row's
+// receiver list, longest first.  Items are placed by a bucket order on their
+// cost estimate (receivers x row sources; four buckets per octave), so no sort
+// launch is needed; the order within a bucket is arrival order, which changes
+// no result (every item writes its own receivers' row slots).
 constexpr int RL_BUCKETS = 128;
 
 __device__ __forceinline__ int cost_bucket(unsigned cost)
@@ -1213,8 +1250,11 @@ __device__ __forceinline__ void row_item_info(const Geo &g, const int *__restric
     src = bstart[r1 + 1] - bstart[r0];
 }
 
-__global__ __launch_bounds__(1024) void k_rowlist(Geo g, const int *__restrict__ bstart, int2 *__restrict__ work,
-                                                  int *__restrict__ nwork, int max_work)
+// The row list in one block of 1024 threads (called after the box sort's
+// scan, in the same launch): tally the items per cost bucket in LDS, scan the
+// buckets from the costliest down, place the items.
+__device__ __forceinline__ void row_list_block(const Geo &g, const int *__restrict__ bstart,
+                                               int2 *__restrict__ work, int *__restrict__ nwork, int max_work)
 {
     __shared__ int s_hist[RL_BUCKETS], s_off[RL_BUCKETS];
     for (int i = threadIdx.x; i < RL_BUCKETS; i += blockDim.x) s_hist[i] = 0;
@@ -1243,6 +1283,17 @@ __global__ __launch_bounds__(1024) void k_rowlist(Geo g, const int *__restrict__
             if (pos < max_work) work[pos] = make_int2(q, c0);
         }
     }
+}
+
+// The box sort's scan and, from its box starts, the row list: one block.
+__global__ __launch_bounds__(1024) void k_key_scan_rows(Geo g, const int *__restrict__ cnt, int *__restrict__ bs4,
+                                                        int *__restrict__ bstart, int2 *__restrict__ work,
+                                                        int *__restrict__ nwork, int max_work)
+{
+    key_scan_block(g, cnt, bs4, bstart);
+    __threadfence_block();  // this block's bstart stores before its loads below
+    __syncthreads();
+    row_list_block(g, bstart, work, nwork, max_work);
 }
 
 // One row-centred item: positions c0 + lane (+ 64 for NR = 2) of the row's
@@ -1318,7 +1369,7 @@ __global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
     // box-centred form is evaluated once, with the same sources in the same
     // order, into the same row slot; slots of rows outside the grid are not
     // written (the site-terms kernel skips them).
-    int item = blockIdx.x;  // items come longest first (see k_rowlist)
+    int item = blockIdx.x;  // items come longest first (see row_list_block)
     if (item >= *nwork) return;
     int lane = threadIdx.x, nb = g.nb;
     int2 w = work[item];
@@ -1395,9 +1446,10 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
                                             const sesmd_forcefield *__restrict__ ff,
                                             double2 *__restrict__ O4, int *__restrict__ ne4)
 {
-    __shared__ double s_re[64][MAXLK], s_im[64][MAXLK];
-    int b = blockIdx.x;
     constexpr int p = PP, lk = (PP + 1) * (PP + 2) / 2, lp = (PP + 1) * (PP + 1);
+    // sized by the order: 15 KB at p = 4, so every box's block is resident at once
+    __shared__ double s_re[64][lk], s_im[64][lk];
+    int b = blockIdx.x;
     int nb = g.nb;
     double cx = d_cen4(g, b % nb), cy = d_cen4(g, (b / nb) % nb), cz = d_cen4(g, b / (nb * nb));
     double are[MAXLK], aim[MAXLK];
@@ -1773,11 +1825,23 @@ __global__ void k_m2l_mfma_sum(Geo g, int nd, int tiles_per_par, int S, int MT, 
     if (q >= h * h * h) return;
     int tx = 2 * (q % h) + (par & 1), ty = 2 * ((q / h) % h) + ((par >> 1) & 1),
         tz = 2 * (q / (h * h)) + (par >> 2);
+    // the splits in order, the loads of a chunk of 9 issued together
     double re = 0.0, im = 0.0;
-    for (int s = 0; s < S; s++) {
-        const double *P = part + (size_t)(tile * S + s) * MT * 16 * 16;
-        re += P[(2 * row) * 16 + n];
-        im += P[(2 * row + 1) * 16 + n];
+    for (int s0 = 0; s0 < S; s0 += 9) {
+        double vr[9], vi[9];
+#pragma unroll
+        for (int q2 = 0; q2 < 9; q2++)
+            if (s0 + q2 < S) {
+                const double *P = part + (size_t)(tile * S + s0 + q2) * MT * 16 * 16;
+                vr[q2] = P[(2 * row) * 16 + n];
+                vi[q2] = P[(2 * row + 1) * 16 + n];
+            }
+#pragma unroll
+        for (int q2 = 0; q2 < 9; q2++)
+            if (s0 + q2 < S) {
+                re += vr[q2];
+                im += vi[q2];
+            }
     }
     int j = 0;
     while ((j + 1) * (j + 2) / 2 <= row) j++;
@@ -1819,11 +1883,19 @@ __global__ __launch_bounds__(64) void k_m2l_top_sum(Geo g, const double2 *__rest
 {
     int Tb = blockIdx.x, row = threadIdx.x;
     if (row >= g.lk) return;
+    // the sources in index order, the loads of a chunk of 9 issued together
     double2 acc = make_double2(0.0, 0.0);
-    for (int S = 0; S < g.n2; S++) {
-        double2 v = part[((size_t)Tb * g.n2 + S) * g.lk + row];
-        acc.x += v.x;
-        acc.y += v.y;
+    for (int S0 = 0; S0 < g.n2; S0 += 9) {
+        double2 v[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+            if (S0 + q < g.n2) v[q] = part[((size_t)Tb * g.n2 + S0 + q) * g.lk + row];
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+            if (S0 + q < g.n2) {
+                acc.x += v[q].x;
+                acc.y += v[q].y;
+            }
     }
     store_rows(g, L2 + (size_t)Tb * g.lp, row, acc);
 }
@@ -3571,18 +3643,15 @@ static int launch_rebuild(sesmd_ctx *c)
     hipStream_t st = c->stream;
     if (!c->classes_fixed)
         k_classify<<<nblk(g.nmol, 256), 256, 0, st>>>(g, c->r, c->first, c->cls, c->anc);
-    k_site_box<<<nblk(g.n, 256), 256, 0, st>>>(g, c->r, c->mol, c->type, c->cls, c->anc, c->cbox,
-                                               c->rkind, c->key_in);
-    if (g.ns) k_isub<<<nblk(29 * g.ns, 128), 128, 0, st>>>(g, c->r, c->anc, c->isub);
-    // counting sort by key (kcnt: counts then arrival counters), and bflag, cleared by one fill
+    // counting sort by key (kcnt: counts, then arrival counters; then bflag), cleared by one fill
     int K = 4 * g.nb3;
     k_fill_i<<<nblk(2 * K + g.nb3, 256), 256, 0, st>>>(c->kcnt, 2 * K + g.nb3, 0);
-    k_key_count<<<nblk(g.n, 256), 256, 0, st>>>(g, c->key_in, c->kcnt);
-    k_key_scan<<<1, 1024, 0, st>>>(g, c->kcnt, c->bs4, c->bstart);
+    k_site_box<<<nblk(g.n, 256), 256, 0, st>>>(g, c->r, c->mol, c->type, c->cls, c->anc, c->cbox,
+                                               c->rkind, c->key_in, c->kcnt);
+    if (g.ns) k_isub<<<g.ns, 64, 0, st>>>(g, c->r, c->anc, c->isub);
+    k_key_scan_rows<<<1, 1024, 0, st>>>(g, c->kcnt, c->bs4, c->bstart, c->rwork, c->nrwork, c->max_rwork);
     k_key_scatter<<<nblk(g.n, 256), 256, 0, st>>>(g, c->key_in, c->bs4, c->kcnt + K, c->val_in);
-    k_key_order<<<K, 64, 0, st>>>(g, c->bs4, c->val_in, c->perm, c->key_out);
-    k_invperm<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->iperm);
-    k_rowlist<<<1, 1024, 0, st>>>(g, c->bstart, c->rwork, c->nrwork, c->max_rwork);
+    k_key_order<<<K, 64, 0, st>>>(g, c->bs4, c->val_in, c->perm, c->key_out, c->iperm);
     c->boxlist = false;  // built on the first masked evaluation after this rebuild
     k_gather_meta<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->type, c->mol, c->first, c->cls, c->anc,
                                                   c->cbox, c->smeta, c->bflag);
