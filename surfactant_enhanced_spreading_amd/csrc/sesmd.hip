@@ -63,9 +63,12 @@ constexpr int NEAR_TILE = 64;                 // lanes of a near-field wavefront
 constexpr int NEAR_CHUNK = 128;               // receivers per near-field work item (two per lane)
 constexpr int NH_VALS = 18;                   // kinetic partial sums per block
 
+constexpr int MAXNB = 32;  // L4 boxes per axis (nb2 <= 8)
+
 struct Geo {
     int n, nw, ns, nmol, nb2, nb, nb3, n3, n33, n2, p, lp, lk;
     double rs, rsd, rhs, rc, rc2, xpi, dt, dtsq2;
+    double cen4[MAXNB];  // L4 box centre per axis index (d_cen4_formula, evaluated on the device)
 };
 
 constexpr int INTRA_MAXC = 60;  // intramolecular contributions per surfactant site, at most 32 + 12 + 16
@@ -352,10 +355,18 @@ __device__ __forceinline__ double d_rcen2(const Geo &g, int ix)
     return g.rsd * (ix + 1) / (double)g.nb2 - g.rhs - g.rsd / (double)g.nb2 / 2.0;
 }
 // L4 centre along one axis for global box coordinate gx (rcen4 of the rank)
-__device__ __forceinline__ double d_cen4(const Geo &g, int gx)
+__device__ __forceinline__ double d_cen4_formula(const Geo &g, int gx)
 {
     int i = gx % 4 + 1;
     return 4.0 * g.rs * i / 4.0 - 2.0 * g.rs - 4.0 * g.rs / 8.0 + d_rcen2(g, gx / 4);
+}
+// the same values from the table the context filled at creation
+__device__ __forceinline__ double d_cen4(const Geo &g, int gx) { return g.cen4[gx]; }
+
+__global__ void k_cen4_table(Geo g, double *out)
+{
+    int gx = threadIdx.x;
+    if (gx < g.nb) out[gx] = d_cen4_formula(g, gx);
 }
 
 // ------------------------------------------------------------ fill kernels
@@ -659,7 +670,8 @@ __global__ void k_work_gather(const int2 *__restrict__ raw, const int *__restric
     if (i < *nwork) work[i] = raw[order[i]];
 }
 
-// static metadata of the sorted sites: (site, type | cls << 8, mol, packed
+// static metadata of the sorted sites: (site, type | cls << 8 | split << 10 |
+// (receiver kind + 1) << 11, mol, packed
 // storage box) with the storage box as x | y << 8 | z << 16 and bit 24 set when
 // the site is listed by the reference from every box of the 3^3 block around
 // its own box (its storage ring stays within the class's list rings: W1 <= 1,
@@ -667,8 +679,8 @@ __global__ void k_work_gather(const int2 *__restrict__ raw, const int *__restric
 __global__ void k_gather_meta(Geo g, const int *__restrict__ perm, const int *__restrict__ type,
                               const int *__restrict__ mol, const int *__restrict__ first,
                               const int *__restrict__ cls, const int *__restrict__ anc,
-                              const int *__restrict__ cbox, int4 *__restrict__ smeta,
-                              int *__restrict__ bflag)
+                              const int *__restrict__ cbox, const int *__restrict__ rkind,
+                              int4 *__restrict__ smeta, int *__restrict__ bflag)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= g.n) return;
@@ -684,7 +696,7 @@ __global__ void k_gather_meta(Geo g, const int *__restrict__ perm, const int *__
     int split = 0;
     if (c != 0)
         for (int s = first[m]; s < first[m + 1]; s++) split |= cbox[s] != cbox[i];
-    smeta[k] = make_int4(i, type[i] | (c << 8) | (split << 10), m,
+    smeta[k] = make_int4(i, type[i] | (c << 8) | (split << 10) | ((rkind[i] + 1) << 11), m,
                          ax | (ay << 8) | (az << 16) | (always << 24));
     if (!always) atomicOr(&bflag[cbox[i]], 1);
 }
@@ -1967,8 +1979,7 @@ template <int PP>
 // partials of k_near, in row order) + L2P + wall.  The pair kernels before it
 // leave their results in `part`; the kernels after it add to f.
 __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const double *__restrict__ r,
-                             const int *__restrict__ type, const int *__restrict__ cbox,
-                             const int *__restrict__ rkind, const double2 *__restrict__ L4,
+                             const int *__restrict__ key, const double2 *__restrict__ L4,
                              const double *__restrict__ lfac, const sesmd_forcefield *__restrict__ ff,
                              const double *__restrict__ part, const double4 *__restrict__ spos,
                              const int4 *__restrict__ smeta, double *__restrict__ f,
@@ -1976,16 +1987,19 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
                              const int *__restrict__ gidx, int ngr)
 {
     // ngr > 0: the graphite term too (the step pipeline's form)
-    // one thread per sorted position k (coalesced reads of the sorted copies)
+    // one thread per sorted position k (coalesced reads of the sorted copies;
+    // the site's box, receiver kind and type come from them, not from per-site
+    // arrays indexed by the site, so no load waits for another)
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= g.n) return;
-    int i = smeta[k].x;
+    int4 me = smeta[k];
+    int i = me.x, rk = ((me.y >> 11) & 7) - 1, bk = key[k] >> 2;
     double4 pk = spos[k];
     double x = pk.x, y = pk.y, z = pk.z;
     double nx = 0.0, ny = 0.0, nz = 0.0;
-    if ((what & SESMD_NEAR) && rkind[i] >= 0) {
+    if ((what & SESMD_NEAR) && rk >= 0) {
         // the rows (dz, dy) of the receiver's box that lie inside the grid
-        int b = cbox[i], nb = g.nb, by = (b / nb) % nb, bz = b / (nb * nb);
+        int b = bk, nb = g.nb, by = (b / nb) % nb, bz = b / (nb * nb);
 #pragma unroll
         for (int q = 0; q < NEAR_ROWS; q++) {
             int y = by + q % 3 - 1, z = bz + q / 3 - 1;
@@ -1996,8 +2010,8 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
         }
     }
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    if ((what & SESMD_FAR) && rkind[i] >= 0) {
-        int b = cbox[i], nb = g.nb;
+    if ((what & SESMD_FAR) && rk >= 0) {
+        int b = bk, nb = g.nb;
         constexpr int p = PP;
         double rxpb = x - d_cen4(g, b % nb), rypb = y - d_cen4(g, (b / nb) % nb),
                rzpb = z - d_cen4(g, b / (nb * nb));
@@ -2005,16 +2019,22 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
         double rho2 = rxpb * rxpb + rypb * rypb;
         double rr = sqrt(rho2 + rzpb * rzpb);
         double rp1 = rzpb / rr;
-        double sr = sqrt(rho2);
-        double2 rp2 = make_double2(rxpb / sr, rypb / sr);
+        double sr = sqrt(rho2), rsr = 1.0 / sr;
+        double2 rp2 = make_double2(rxpb * rsr, rypb * rsr);
         double rp3 = 1.0 / rho2;
         double P[MAXP + 2][MAXP + 1];
         legendre_table<p>(rp1, P);
-        // e^{i k phi} for k = 0..p by repeated multiplication
-        double2 pw[MAXP + 1];
-        pw[0] = make_double2(1.0, 0.0);
+        // e^{i k phi} and e^{-i k phi} for k = 0..p by repeated multiplication
+        // (the inverse of e^{i phi}: its conjugate over |e|^2)
+        double2 pw[MAXP + 1], pwi[MAXP + 1];
+        double rd = 1.0 / (rp2.x * rp2.x + rp2.y * rp2.y);
+        double2 rp2i = make_double2(rp2.x * rd, -rp2.y * rd);
+        pw[0] = pwi[0] = make_double2(1.0, 0.0);
 #pragma unroll
-        for (int k2 = 1; k2 <= p; k2++) pw[k2] = cmul(pw[k2 - 1], rp2);
+        for (int k2 = 1; k2 <= p; k2++) {
+            pw[k2] = cmul(pw[k2 - 1], rp2);
+            pwi[k2] = cmul(pwi[k2 - 1], rp2i);
+        }
         const double2 *L = L4 + (size_t)b * g.lp;
         double rrj = 1.0 / rr;  // rr^(j-1)
         double ax = 0.0, ay = 0.0, az = 0.0;
@@ -2023,11 +2043,7 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
 #pragma unroll
             for (int k = -j; k <= j; k++) {
                 int ak = k < 0 ? -k : k;
-                double2 e = pw[ak];
-                if (k < 0) {  // 1 / e^{i|k|phi}
-                    double d = e.x * e.x + e.y * e.y;
-                    e = make_double2(e.x / d, -e.y / d);
-                }
+                double2 e = k < 0 ? pwi[ak] : pw[ak];  // e^{i k phi}
                 double2 l = L[j * (j + 1) + k];
                 double sc = lfac[j * (j + 1) + k] * rrj;
                 double2 rq0 = cmul(make_double2(l.x * sc, l.y * sc), e);
@@ -2062,9 +2078,9 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
     // graphite (src/eval_graphite.f90:13-132): receivers filed in their own box
     // of the bottom layer, against that column's carbons
     if (ngr > 0 && (what & SESMD_GRAPHITE)) {
-        int b = cbox[i], rk = rkind[i];
+        int b = bk;
         if (b < g.nb * g.nb && (rk == 0 || rk == 1 || rk == 3)) {
-            int t = type[i];
+            int t = me.y & 255;
             double c = ff->gr_c[t], s6 = ff->gr_s6[t];
             if (c != 0.0) {
                 double gx = 0.0, gy = 0.0, gz = 0.0;
@@ -2916,9 +2932,11 @@ __global__ __launch_bounds__(256) void k_nh_intra(Geo g, const double *__restric
     }
     __shared__ double tab[29 * INTRA_MAXC * 3];
     int s = blockIdx.x - 1, nterm = ff->n_internal + 66;
-    for (int term = threadIdx.x; term < nterm; term += blockDim.x)
-        intra_term(term, s, g, SESMD_INTERNAL | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE, r, ff,
-                   topo, tab);
+    // last term first: the dihedrals and angles (the long ones) take the first
+    // pass of the block's threads, the internal pairs the second
+    for (int t = threadIdx.x; t < nterm; t += blockDim.x)
+        intra_term(nterm - 1 - t, s, g, SESMD_INTERNAL | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE, r,
+                   ff, topo, tab);
     __syncthreads();
     for (int t = threadIdx.x; t < 29 * 3; t += blockDim.x) {
         int j = t / 3, d = t - 3 * j, cnt = topo->cnt[j];
@@ -3359,6 +3377,17 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     for (hipEvent_t &e : c->ev)
         A(hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ? fail("event create failed") : 0);
     size_t n = g.n;
+    {  // box centres, computed on the device with the classification's own formula
+        double *tab = nullptr;
+        if (!rc && hipMalloc(&tab, sizeof(double) * MAXNB) == hipSuccess) {
+            k_cen4_table<<<1, MAXNB>>>(g, tab);
+            A(hipMemcpy(g.cen4, tab, sizeof(double) * g.nb, hipMemcpyDeviceToHost) != hipSuccess
+                  ? fail("box centre table failed") : 0);
+            hipFree(tab);
+        } else {
+            A(fail("box centre table alloc failed"));
+        }
+    }
     A(dalloc(&c->r, 3 * n));
     A(dalloc(&c->v, 3 * n));
     A(dalloc(&c->a, 3 * n));
@@ -3654,7 +3683,7 @@ static int launch_rebuild(sesmd_ctx *c)
     k_key_order<<<K, 64, 0, st>>>(g, c->bs4, c->val_in, c->perm, c->key_out, c->iperm);
     c->boxlist = false;  // built on the first masked evaluation after this rebuild
     k_gather_meta<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->type, c->mol, c->first, c->cls, c->anc,
-                                                  c->cbox, c->smeta, c->bflag);
+                                                  c->cbox, c->rkind, c->smeta, c->bflag);
     HC(hipGetLastError());
     c->built = true;
     return 0;
@@ -3786,8 +3815,8 @@ static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask, bool gather
     if (what & SESMD_NEAR) launch_near(c, mask);
     const unsigned intra = SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE;
     (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<nblk(g.n, 128), 128, 0, st>>>(
-        g, what, c->r, c->type, c->cbox, c->rkind, c->L4, c->lfac, c->dff, c->npart, c->spos, c->smeta,
-        c->f, c->gr, c->gstart, c->gidx, fold ? c->ngr : 0);
+        g, what, c->r, c->key_out, c->L4, c->lfac, c->dff, c->npart, c->spos, c->smeta, c->f, c->gr,
+        c->gstart, c->gidx, fold ? c->ngr : 0);
     if (g.ns && (what & intra))
         k_intra<<<nblk(29 * g.ns, 64), 64, 0, st>>>(g, what, c->r, c->isub, c->dff, c->dtopo, c->f,
                                                     nullptr);
