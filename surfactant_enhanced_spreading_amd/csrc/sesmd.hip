@@ -2672,23 +2672,31 @@ struct SurfMolLds {
 // Runs the pipelined sweeps for one molecule (wave-uniform call).  `upd`
 // applies bond c's correction to P[ia], P[ib] and returns whether it corrected;
 // returns whether the iteration converged within `smax` sweeps.
+// The loop runs one period of four steps per iteration: with off = 4 st + ph,
+// bond c's step T = 4 it + j is its turn when j == ph, in sweep it - st.
 template <typename Upd>
 __device__ __forceinline__ bool pipelined_sweeps(bool bond, int off, int smax, Upd upd)
 {
+    static_assert(kEnd % 4 == 3, "the convergence test falls on a period's last step");
     unsigned hist = 0;  // bit (s & 7): this bond corrected in sweep s
-    for (int T = 0;; T++) {
-        int u = T - off;
-        if (bond && u >= 0 && (u & 3) == 0 && (u >> 2) < smax) {
-            int bit = (u >> 2) & 7;
-            bool corr = upd();
-            hist = (hist & ~(1u << bit)) | ((unsigned)corr << bit);
+    int ph = off & 3, st = off >> 2;
+    for (int it = 0;; it++) {
+        int s = it - st;
+        bool on = bond && s >= 0 && s < smax;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (on && ph == j) {
+                bool corr = upd();
+                int bit = s & 7;
+                hist = (hist & ~(1u << bit)) | ((unsigned)corr << bit);
+            }
+            // the next step's LDS reads must see this step's writes (one
+            // wavefront: its LDS operations complete in order)
+            __asm__ volatile("" ::: "memory");
         }
-        // the next step's LDS reads must see this step's writes (one
-        // wavefront: its LDS operations complete in order)
-        __asm__ volatile("" ::: "memory");
-        int ue = T - kEnd;
-        if (ue >= 0 && (ue & 3) == 0) {
-            int se = ue >> 2;
+        // step 4 it + 3 = kEnd + 4 se ends sweep se
+        int se = it - (kEnd >> 2);
+        if (se >= 0) {
             bool any = __ballot(bond && ((hist >> (se & 7)) & 1u)) != 0;
             if (!any) return true;
             if (se + 1 >= smax) return false;
