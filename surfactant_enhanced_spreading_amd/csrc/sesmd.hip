@@ -1709,8 +1709,11 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(Geo g, int nd, int tiles_per_p
 {
     __shared__ m2l_d4 s_acc[4][MT][64];
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int tile = blockIdx.x / S, split = blockIdx.x % S;
-    int par = tile / tiles_per_par, tl = tile % tiles_per_par;
+    // consecutive blocks go to the eight XCDs in turn, so block % 8 picks the
+    // parity: each XCD's L2 then holds the operators of one parity's 216
+    // offsets (2.9 MB at p = 4) instead of all 343
+    int par = blockIdx.x & 7, rest = blockIdx.x >> 3;
+    int tl = rest / S, split = rest % S, tile = par * tiles_per_par + tl;
     int px = par & 1, py = (par >> 1) & 1, pz = par >> 2;
     int h = nd / 2, n = lane & 15, kq = lane >> 4;
     int q = 16 * tl + n;
