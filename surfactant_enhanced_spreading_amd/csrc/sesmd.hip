@@ -110,9 +110,7 @@ struct sesmd_ctx {
     Geo g;
     int precision;
     sesmd_forcefield ff;
-    hipStream_t stream;
-    hipStream_t side;      // surfactant work of a step, beside the main stream
-    hipEvent_t ev[4];      // fork/join points of a step
+    hipStream_t stream;    // every launch of the context, in order
     double *fs;            // intramolecular forces of a step, 29 ns sites (x, y, z blocks)
     // device state
     double *r, *v, *a, *f;  // each 3*N: x block, y block, z block
@@ -3382,11 +3380,8 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     auto A = [&](int r2) {
         if (r2) rc = 1;
     };
-    A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-              hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess
-          ? fail("stream create failed") : 0);
-    for (hipEvent_t &e : c->ev)
-        A(hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ? fail("event create failed") : 0);
+    A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ? fail("stream create failed")
+                                                                                 : 0);
     size_t n = g.n;
     {  // box centres, computed on the device with the classification's own formula
         double *tab = nullptr;
@@ -3568,10 +3563,6 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
                     c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs};
     for (void *p : ptrs)
         if (p) hipFree(p);
-    if (c->side) hipStreamSynchronize(c->side);
-    for (hipEvent_t e : c->ev)
-        if (e) hipEventDestroy(e);
-    if (c->side) hipStreamDestroy(c->side);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
