@@ -942,8 +942,10 @@ __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, in
         r2[r] = dx[r] * dx[r] + dy[r] * dy[r] + dz[r] * dz[r];
         float rinv = __builtin_amdgcn_rsqf(r2[r]);
         rinv2[r] = rinv * rinv;
-        float qq = CAT == 0 ? Q[r].qO : (CAT == 2 && HQ) ? Q[r].qH : Q[r].q * pj.w;
-        ft[r] = qq * (rinv2[r] * rinv);
+        // hydrogen sources of one charge: q_i q_H scales the segment's sum once
+        // (near_segment32), not every pair
+        float qq = CAT == 0 ? Q[r].qO : Q[r].q * pj.w;
+        ft[r] = (CAT == 2 && HQ) ? rinv2[r] * rinv : qq * (rinv2[r] * rinv);
     }
     if (CAT != 2) {
         bool in[NR], any = false;
@@ -1022,9 +1024,10 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
     if (t == 1) near_pair32<CAT, CHECK, CEN, HQ, NR>(pA, mA, bx, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-        ax[r] += (double)fx[r];
-        ay[r] += (double)fy[r];
-        az[r] += (double)fz[r];
+        float sc = (CAT == 2 && HQ) ? Q[r].qH : 1.0f;
+        ax[r] += (double)(sc * fx[r]);
+        ay[r] += (double)(sc * fy[r]);
+        az[r] += (double)(sc * fz[r]);
     }
 }
 
