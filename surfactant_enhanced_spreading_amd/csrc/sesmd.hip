@@ -2115,7 +2115,12 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
 // ------------------------------------------------------------ intramolecular
 
 
-// one dihedral's forces on its four sites (src/eval_dihedral.f90:116-271)
+// one dihedral's forces on its four sites (src/eval_dihedral.f90:116-271).
+// FAST (the step pipeline): sin 2t / sin t = 2 cos t and sin 3t / sin t =
+// 4 cos^2 t - 1 replace the trigonometric calls, and the divisions by |a| |b|,
+// |a|^2, |b|^2 become products with reciprocals (a few ulp from the reference's
+// own evaluation, which loses up to ~1e-11 of r1 itself near trans, t -> pi).
+template <bool FAST>
 __device__ void dihedral_forces(const sesmd_forcefield *ff, int d, const double *Ri,
                                 const double *Rj, const double *Rk, const double *Rl, double F[4][3])
 {
@@ -2125,22 +2130,35 @@ __device__ void dihedral_forces(const sesmd_forcefield *ff, int d, const double 
     double ax = rxji * rykj - ryji * rxkj, ay = rxji * rzkj - rzji * rxkj, az = ryji * rzkj - rzji * rykj;
     double bx = rxkj * rynk - rykj * rxnk, by = rxkj * rznk - rzkj * rxnk, bz = rykj * rznk - rzkj * rynk;
     double na = sqrt(ax * ax + ay * ay + az * az), nbn = sqrt(bx * bx + by * by + bz * bz);
-    double arg = (ax * bx + ay * by + az * bz) / na / nbn;
+    double rna = FAST ? 1.0 / na : 0.0, rnb = FAST ? 1.0 / nbn : 0.0;
+    double arg = FAST ? (ax * bx + ay * by + az * bz) * rna * rnb : (ax * bx + ay * by + az * bz) / na / nbn;
     double clamp = d < 18 ? ff->clamp_opls : ff->clamp_be;
     if (arg <= -1.0) arg = -clamp;
     if (arg >= 1.0) arg = clamp;
-    double th = acos(arg), costh = cos(th), sinth = sin(th);
-    double r1 = d < 18 ? -ff->dva[d] + 2.0 * ff->dvb[d] * sin(2.0 * th) / sinth -
-                             3.0 * ff->dvc[d] * sin(3.0 * th) / sinth
-                       : -3.0 * ff->dva[d] * sin(3.0 * th) / sinth;
+    double costh, r1;
+    if (FAST) {
+        costh = arg;
+        double s2 = 2.0 * arg, s3 = 4.0 * arg * arg - 1.0;  // sin 2t / sin t, sin 3t / sin t
+        r1 = d < 18 ? -ff->dva[d] + 2.0 * ff->dvb[d] * s2 - 3.0 * ff->dvc[d] * s3 : -3.0 * ff->dva[d] * s3;
+    } else {
+        double th = acos(arg), sinth = sin(th);
+        costh = cos(th);
+        r1 = d < 18 ? -ff->dva[d] + 2.0 * ff->dvb[d] * sin(2.0 * th) / sinth -
+                          3.0 * ff->dvc[d] * sin(3.0 * th) / sinth
+                    : -3.0 * ff->dva[d] * sin(3.0 * th) / sinth;
+    }
     double c1 = na * na, c2 = nbn * nbn, ab = na * nbn;
+    double rab = rna * rnb, rc1 = rna * rna, rc2 = rnb * rnb;
+    auto dab = [&](double x) { return FAST ? x * rab : x / ab; };
+    auto dc1 = [&](double x) { return FAST ? x * rc1 : x / c1; };
+    auto dc2 = [&](double x) { return FAST ? x * rc2 : x / c2; };
     // site i
-    double fix = (rxkj * (-rykj * rynk - rzkj * rznk) + rxnk * (rykj * rykj + rzkj * rzkj)) / ab;
-    double fiy = (rykj * (-rxkj * rxnk - rzkj * rznk) + rynk * (rxkj * rxkj + rzkj * rzkj)) / ab;
-    double fiz = (rzkj * (-rxkj * rxnk - rykj * rynk) + rznk * (rxkj * rxkj + rykj * rykj)) / ab;
-    double fix2 = (2.0 * rxji * (-rykj * rykj - rzkj * rzkj) + 2.0 * rxkj * (ryji * rykj + rzji * rzkj)) / c1;
-    double fiy2 = (2.0 * ryji * (-rxkj * rxkj - rzkj * rzkj) + 2.0 * rykj * (rxji * rxkj + rzji * rzkj)) / c1;
-    double fiz2 = (2.0 * rzji * (-rxkj * rxkj - rykj * rykj) + 2.0 * rzkj * (rxji * rxkj + ryji * rykj)) / c1;
+    double fix = dab((rxkj * (-rykj * rynk - rzkj * rznk) + rxnk * (rykj * rykj + rzkj * rzkj)));
+    double fiy = dab((rykj * (-rxkj * rxnk - rzkj * rznk) + rynk * (rxkj * rxkj + rzkj * rzkj)));
+    double fiz = dab((rzkj * (-rxkj * rxnk - rykj * rynk) + rznk * (rxkj * rxkj + rykj * rykj)));
+    double fix2 = dc1((2.0 * rxji * (-rykj * rykj - rzkj * rzkj) + 2.0 * rxkj * (ryji * rykj + rzji * rzkj)));
+    double fiy2 = dc1((2.0 * ryji * (-rxkj * rxkj - rzkj * rzkj) + 2.0 * rykj * (rxji * rxkj + rzji * rzkj)));
+    double fiz2 = dc1((2.0 * rzji * (-rxkj * rxkj - rykj * rykj) + 2.0 * rzkj * (rxji * rxkj + ryji * rykj)));
     F[0][0] = r1 * (fix - 0.5 * costh * fix2);
     F[0][1] = r1 * (fiy - 0.5 * costh * fiy2);
     F[0][2] = r1 * (fiz - 0.5 * costh * fiz2);
@@ -2148,53 +2166,53 @@ __device__ void dihedral_forces(const sesmd_forcefield *ff, int d, const double 
     double fjx = rxji * (-rykj * rynk - rzkj * rznk) + rxkj * (rykj * rynk + rzkj * rznk);
     double fjy = ryji * (-rxkj * rxnk - rzkj * rznk) + rykj * (rxkj * rxnk + rzkj * rznk);
     double fjz = rzji * (-rxkj * rxnk - rykj * rynk) + rzkj * (rxkj * rxnk + rykj * rynk);
-    fjx = (fjx + rxnk * (-ryji * rykj - rzji * rzkj - rykj * rykj - rzkj * rzkj) +
-           2.0 * rxkj * (ryji * rynk + rzji * rznk)) / ab;
-    fjy = (fjy + rynk * (-rxji * rxkj - rzji * rzkj - rxkj * rxkj - rzkj * rzkj) +
-           2.0 * rykj * (rxji * rxnk + rzji * rznk)) / ab;
-    fjz = (fjz + rznk * (-rxji * rxkj - ryji * rykj - rxkj * rxkj - rykj * rykj) +
-           2.0 * rzkj * (rxji * rxnk + ryji * rynk)) / ab;
-    double fjx3 = (2.0 * rxji * (rykj * rykj + rzkj * rzkj + ryji * rykj + rzji * rzkj) +
-                   2.0 * rxkj * (-ryji * ryji - rzji * rzji - ryji * rykj - rzji * rzkj)) / c1;
-    double fjy3 = (2.0 * ryji * (rxkj * rxkj + rzkj * rzkj + rxji * rxkj + rzji * rzkj) +
-                   2.0 * rykj * (-rxji * rxji - rzji * rzji - rxji * rxkj - rzji * rzkj)) / c1;
-    double fjz3 = (2.0 * rzji * (rxkj * rxkj + rykj * rykj + rxji * rxkj + ryji * rykj) +
-                   2.0 * rzkj * (-rxji * rxji - ryji * ryji - rxji * rxkj - ryji * rykj)) / c1;
-    double fjx5 = (2.0 * rxnk * (rykj * rynk + rzkj * rznk) + 2.0 * rxkj * (-rynk * rynk - rznk * rznk)) / c2;
-    double fjy5 = (2.0 * rynk * (rxkj * rxnk + rzkj * rznk) + 2.0 * rykj * (-rxnk * rxnk - rznk * rznk)) / c2;
-    double fjz5 = (2.0 * rznk * (rxkj * rxnk + rykj * rynk) + 2.0 * rzkj * (-rxnk * rxnk - rynk * rynk)) / c2;
+    fjx = dab((fjx + rxnk * (-ryji * rykj - rzji * rzkj - rykj * rykj - rzkj * rzkj) +
+           2.0 * rxkj * (ryji * rynk + rzji * rznk)));
+    fjy = dab((fjy + rynk * (-rxji * rxkj - rzji * rzkj - rxkj * rxkj - rzkj * rzkj) +
+           2.0 * rykj * (rxji * rxnk + rzji * rznk)));
+    fjz = dab((fjz + rznk * (-rxji * rxkj - ryji * rykj - rxkj * rxkj - rykj * rykj) +
+           2.0 * rzkj * (rxji * rxnk + ryji * rynk)));
+    double fjx3 = dc1((2.0 * rxji * (rykj * rykj + rzkj * rzkj + ryji * rykj + rzji * rzkj) +
+                   2.0 * rxkj * (-ryji * ryji - rzji * rzji - ryji * rykj - rzji * rzkj)));
+    double fjy3 = dc1((2.0 * ryji * (rxkj * rxkj + rzkj * rzkj + rxji * rxkj + rzji * rzkj) +
+                   2.0 * rykj * (-rxji * rxji - rzji * rzji - rxji * rxkj - rzji * rzkj)));
+    double fjz3 = dc1((2.0 * rzji * (rxkj * rxkj + rykj * rykj + rxji * rxkj + ryji * rykj) +
+                   2.0 * rzkj * (-rxji * rxji - ryji * ryji - rxji * rxkj - ryji * rykj)));
+    double fjx5 = dc2((2.0 * rxnk * (rykj * rynk + rzkj * rznk) + 2.0 * rxkj * (-rynk * rynk - rznk * rznk)));
+    double fjy5 = dc2((2.0 * rynk * (rxkj * rxnk + rzkj * rznk) + 2.0 * rykj * (-rxnk * rxnk - rznk * rznk)));
+    double fjz5 = dc2((2.0 * rznk * (rxkj * rxnk + rykj * rynk) + 2.0 * rzkj * (-rxnk * rxnk - rynk * rynk)));
     F[1][0] = r1 * (fjx - 0.5 * costh * (fjx3 + fjx5));
     F[1][1] = r1 * (fjy - 0.5 * costh * (fjy3 + fjy5));
     F[1][2] = r1 * (fjz - 0.5 * costh * (fjz3 + fjz5));
     // site k
-    double fkx = (rxji * (rykj * rykj + rzkj * rzkj + rykj * rynk + rzkj * rznk) +
+    double fkx = dab((rxji * (rykj * rykj + rzkj * rzkj + rykj * rynk + rzkj * rznk) +
                   rxkj * (-ryji * rykj - rzji * rzkj) + rxnk * (ryji * rykj + rzji * rzkj) +
-                  2.0 * rxkj * (-ryji * rynk - rzji * rznk)) / ab;
-    double fky = (ryji * (rxkj * rxkj + rzkj * rzkj + rxkj * rxnk + rzkj * rznk) +
+                  2.0 * rxkj * (-ryji * rynk - rzji * rznk)));
+    double fky = dab((ryji * (rxkj * rxkj + rzkj * rzkj + rxkj * rxnk + rzkj * rznk) +
                   rykj * (-rxji * rxkj - rzji * rzkj) + rynk * (rxji * rxkj + rzji * rzkj) +
-                  2.0 * rykj * (-rxji * rxnk - rzji * rznk)) / ab;
-    double fkz = (rzji * (rxkj * rxkj + rykj * rykj + rxkj * rxnk + rykj * rynk) +
+                  2.0 * rykj * (-rxji * rxnk - rzji * rznk)));
+    double fkz = dab((rzji * (rxkj * rxkj + rykj * rykj + rxkj * rxnk + rykj * rynk) +
                   rzkj * (-rxji * rxkj - ryji * rykj) + rznk * (rxji * rxkj + ryji * rykj) +
-                  2.0 * rzkj * (-rxji * rxnk - ryji * rynk)) / ab;
-    double fkx4 = (2.0 * rxji * (-ryji * rykj - rzji * rzkj) + 2.0 * rxkj * (ryji * ryji + rzji * rzji)) / c1;
-    double fky4 = (2.0 * ryji * (-rxji * rxkj - rzji * rzkj) + 2.0 * rykj * (rxji * rxji + rzji * rzji)) / c1;
-    double fkz4 = (2.0 * rzji * (-rxji * rxkj - ryji * rykj) + 2.0 * rzkj * (rxji * rxji + ryji * ryji)) / c1;
-    double fkx5 = (2.0 * rxnk * (-rykj * rykj - rzkj * rzkj - rykj * rynk - rzkj * rznk) +
-                   2.0 * rxkj * (rynk * rynk + rznk * rznk + rykj * rynk + rzkj * rznk)) / c2;
-    double fky5 = (2.0 * rynk * (-rxkj * rxkj - rzkj * rzkj - rxkj * rxnk - rzkj * rznk) +
-                   2.0 * rykj * (rxnk * rxnk + rznk * rznk + rxkj * rxnk + rzkj * rznk)) / c2;
-    double fkz5 = (2.0 * rznk * (-rxkj * rxkj - rykj * rykj - rxkj * rxnk - rykj * rynk) +
-                   2.0 * rzkj * (rxnk * rxnk + rynk * rynk + rxkj * rxnk + rykj * rynk)) / c2;
+                  2.0 * rzkj * (-rxji * rxnk - ryji * rynk)));
+    double fkx4 = dc1((2.0 * rxji * (-ryji * rykj - rzji * rzkj) + 2.0 * rxkj * (ryji * ryji + rzji * rzji)));
+    double fky4 = dc1((2.0 * ryji * (-rxji * rxkj - rzji * rzkj) + 2.0 * rykj * (rxji * rxji + rzji * rzji)));
+    double fkz4 = dc1((2.0 * rzji * (-rxji * rxkj - ryji * rykj) + 2.0 * rzkj * (rxji * rxji + ryji * ryji)));
+    double fkx5 = dc2((2.0 * rxnk * (-rykj * rykj - rzkj * rzkj - rykj * rynk - rzkj * rznk) +
+                   2.0 * rxkj * (rynk * rynk + rznk * rznk + rykj * rynk + rzkj * rznk)));
+    double fky5 = dc2((2.0 * rynk * (-rxkj * rxkj - rzkj * rzkj - rxkj * rxnk - rzkj * rznk) +
+                   2.0 * rykj * (rxnk * rxnk + rznk * rznk + rxkj * rxnk + rzkj * rznk)));
+    double fkz5 = dc2((2.0 * rznk * (-rxkj * rxkj - rykj * rykj - rxkj * rxnk - rykj * rynk) +
+                   2.0 * rzkj * (rxnk * rxnk + rynk * rynk + rxkj * rxnk + rykj * rynk)));
     F[2][0] = r1 * (fkx - 0.5 * costh * (fkx4 + fkx5));
     F[2][1] = r1 * (fky - 0.5 * costh * (fky4 + fky5));
     F[2][2] = r1 * (fkz - 0.5 * costh * (fkz4 + fkz5));
     // site l
-    double fnx = (rxji * (-rykj * rykj - rzkj * rzkj) + rxkj * (ryji * rykj + rzji * rzkj)) / ab;
-    double fny = (ryji * (-rxkj * rxkj - rzkj * rzkj) + rykj * (rxji * rxkj + rzji * rzkj)) / ab;
-    double fnz = (rzji * (-rxkj * rxkj - rykj * rykj) + rzkj * (rxji * rxkj + ryji * rykj)) / ab;
-    double fnx2 = (2.0 * rxnk * (rykj * rykj + rzkj * rzkj) + 2.0 * rxkj * (-rykj * rynk - rzkj * rznk)) / c2;
-    double fny2 = (2.0 * rynk * (rxkj * rxkj + rzkj * rzkj) + 2.0 * rykj * (-rxkj * rxnk - rzkj * rznk)) / c2;
-    double fnz2 = (2.0 * rznk * (rxkj * rxkj + rykj * rykj) + 2.0 * rzkj * (-rxkj * rxnk - rykj * rynk)) / c2;
+    double fnx = dab((rxji * (-rykj * rykj - rzkj * rzkj) + rxkj * (ryji * rykj + rzji * rzkj)));
+    double fny = dab((ryji * (-rxkj * rxkj - rzkj * rzkj) + rykj * (rxji * rxkj + rzji * rzkj)));
+    double fnz = dab((rzji * (-rxkj * rxkj - rykj * rykj) + rzkj * (rxji * rxkj + ryji * rykj)));
+    double fnx2 = dc2((2.0 * rxnk * (rykj * rykj + rzkj * rzkj) + 2.0 * rxkj * (-rykj * rynk - rzkj * rznk)));
+    double fny2 = dc2((2.0 * rynk * (rxkj * rxkj + rzkj * rzkj) + 2.0 * rykj * (-rxkj * rxnk - rzkj * rznk)));
+    double fnz2 = dc2((2.0 * rznk * (rxkj * rxkj + rykj * rykj) + 2.0 * rzkj * (-rxkj * rxnk - rykj * rynk)));
     F[3][0] = r1 * (fnx - 0.5 * costh * fnx2);
     F[3][1] = r1 * (fny - 0.5 * costh * fny2);
     F[3][2] = r1 * (fnz - 0.5 * costh * fnz2);
@@ -2297,7 +2315,7 @@ __global__ __launch_bounds__(64) void k_intra(Geo g, unsigned what, const double
             ld3(r, n, base + ff->dih_k[d] - 1, Rk);
             ld3(r, n, base + ff->dih_l[d] - 1, Rl);
             double F[4][3];
-            dihedral_forces(ff, d, Ri, Rj, Rk, Rl, F);
+            dihedral_forces<false>(ff, d, Ri, Rj, Rk, Rl, F);
             // select the site's row without indexing F at run time (keeps F in registers)
             double gx = role == 0 ? F[0][0] : role == 1 ? F[1][0] : role == 2 ? F[2][0] : F[3][0];
             double gy = role == 0 ? F[0][1] : role == 1 ? F[1][1] : role == 2 ? F[2][1] : F[3][1];
@@ -2324,13 +2342,17 @@ __global__ __launch_bounds__(64) void k_intra(Geo g, unsigned what, const double
 // each of its sites' contributions goes to its own place in the molecule's
 // table (Topo::dst) -- mol[(j * INTRA_MAXC + q) * 3 + d] -- which is then
 // added per site in order (no atomics).  Terms outside `what` write zeros.
-__device__ __forceinline__ void intra_term(int term, int s, Geo g, unsigned what,
-                                           const double *__restrict__ r,
+__device__ __forceinline__ void intra_term(int term, const Geo &g, unsigned what, const double *pm,
                                            const sesmd_forcefield *__restrict__ ff,
                                            const Topo *__restrict__ topo, double *mol)
 {
+    // pm: the molecule's positions in LDS, pm[d * 29 + site]
     int ni = ff->n_internal;
-    int base = 3 * g.nw + 29 * s, n = g.n;
+    auto ld = [&](int j, double *x) {
+        x[0] = pm[j];
+        x[1] = pm[29 + j];
+        x[2] = pm[58 + j];
+    };
     auto put = [&](int slot, double x, double y, double z) {
         double *o = mol + (size_t)topo->dst[slot] * 3;
         o[0] = x;
@@ -2340,8 +2362,8 @@ __device__ __forceinline__ void intra_term(int term, int s, Geo g, unsigned what
     if (term < ni) {
         int j = ff->int_j[term] - 1, k = ff->int_k[term] - 1;
         double xs[3], xk[3];
-        ld3(r, n, base + j, xs);
-        ld3(r, n, base + k, xk);
+        ld(j, xs);
+        ld(k, xk);
         double x = xs[0] - xk[0], y = xs[1] - xk[1], z = xs[2] - xk[2];
         double ft = 0.0;
         if (what & SESMD_INTERNAL) {
@@ -2358,23 +2380,25 @@ __device__ __forceinline__ void intra_term(int term, int s, Geo g, unsigned what
     } else if (term < ni + 36) {
         int a = term - ni, sl = 2 * ni + 3 * a;
         double ri[3], rj[3], rk[3];
-        ld3(r, n, base + ff->ang_i[a] - 1, ri);
-        ld3(r, n, base + ff->ang_j[a] - 1, rj);
-        ld3(r, n, base + ff->ang_k[a] - 1, rk);
+        ld(ff->ang_i[a] - 1, ri);
+        ld(ff->ang_j[a] - 1, rj);
+        ld(ff->ang_k[a] - 1, rk);
         double rxji = ri[0] - rj[0], ryji = ri[1] - rj[1], rzji = ri[2] - rj[2];
         double rxki = rk[0] - rj[0], ryki = rk[1] - rj[1], rzki = rk[2] - rj[2];
+        // as k_intra's angle term, with cos t = c and sin t = sqrt((1 - c)(1 + c))
+        // for the calls, and reciprocals for the divisions (a few ulp apart)
         double dp = rxji * rxki + ryji * ryki + rzji * rzki;
-        double rjin = sqrt(rxji * rxji + ryji * ryji + rzji * rzji);
-        double rkin = sqrt(rxki * rxki + ryki * ryki + rzki * rzki);
-        double th = acos(dp / rjin / rkin);
-        double costh = cos(th);
-        double r1 = (what & SESMD_VALENCE) ? ff->rkth[a] * (th - ff->th[a]) / sin(th) : 0.0;
-        double fjx = rxki / rjin / rkin - costh * rxji / (rjin * rjin);
-        double fjy = ryki / rjin / rkin - costh * ryji / (rjin * rjin);
-        double fjz = rzki / rjin / rkin - costh * rzji / (rjin * rjin);
-        double fkx = rxji / rjin / rkin - costh * rxki / (rkin * rkin);
-        double fky = ryji / rjin / rkin - costh * ryki / (rkin * rkin);
-        double fkz = rzji / rjin / rkin - costh * rzki / (rkin * rkin);
+        double rji = 1.0 / sqrt(rxji * rxji + ryji * ryji + rzji * rzji);
+        double rki = 1.0 / sqrt(rxki * rxki + ryki * ryki + rzki * rzki);
+        double rjk = rji * rki;
+        double costh = dp * rjk;
+        double th = acos(costh);
+        double r1 = (what & SESMD_VALENCE)
+                        ? ff->rkth[a] * (th - ff->th[a]) / sqrt((1.0 - costh) * (1.0 + costh))
+                        : 0.0;
+        double cj = costh * rji * rji, ck = costh * rki * rki;
+        double fjx = rxki * rjk - cj * rxji, fjy = ryki * rjk - cj * ryji, fjz = rzki * rjk - cj * rzji;
+        double fkx = rxji * rjk - ck * rxki, fky = ryji * rjk - ck * ryki, fkz = rzji * rjk - ck * rzki;
         put(sl, r1 * fjx, r1 * fjy, r1 * fjz);
         put(sl + 1, -(r1 * (fjx + fkx)), -(r1 * (fjy + fky)), -(r1 * (fjz + fkz)));
         put(sl + 2, r1 * fkx, r1 * fky, r1 * fkz);
@@ -2384,11 +2408,11 @@ __device__ __forceinline__ void intra_term(int term, int s, Geo g, unsigned what
         double F[4][3] = {};
         if (on) {
             double Ri[3], Rj[3], Rk[3], Rl[3];
-            ld3(r, n, base + ff->dih_i[d] - 1, Ri);
-            ld3(r, n, base + ff->dih_j[d] - 1, Rj);
-            ld3(r, n, base + ff->dih_k[d] - 1, Rk);
-            ld3(r, n, base + ff->dih_l[d] - 1, Rl);
-            dihedral_forces(ff, d, Ri, Rj, Rk, Rl, F);
+            ld(ff->dih_i[d] - 1, Ri);
+            ld(ff->dih_j[d] - 1, Rj);
+            ld(ff->dih_k[d] - 1, Rk);
+            ld(ff->dih_l[d] - 1, Rl);
+            dihedral_forces<true>(ff, d, Ri, Rj, Rk, Rl, F);
         }
 #pragma unroll
         for (int role = 0; role < 4; role++) put(sl + role, F[role][0], F[role][1], F[role][2]);
@@ -2943,11 +2967,17 @@ __global__ __launch_bounds__(256) void k_nh_intra(Geo g, const double *__restric
         return;
     }
     __shared__ double tab[29 * INTRA_MAXC * 3];
+    __shared__ double pm[3 * 29];
     int s = blockIdx.x - 1, nterm = ff->n_internal + 66;
+    if (threadIdx.x < 3 * 29) {
+        int d = threadIdx.x / 29, j = threadIdx.x - 29 * d;
+        pm[threadIdx.x] = r[(size_t)d * g.n + 3 * g.nw + 29 * s + j];
+    }
+    __syncthreads();
     // last term first: the dihedrals and angles (the long ones) take the first
     // pass of the block's threads, the internal pairs the second
     for (int t = threadIdx.x; t < nterm; t += blockDim.x)
-        intra_term(nterm - 1 - t, s, g, SESMD_INTERNAL | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE, r,
+        intra_term(nterm - 1 - t, g, SESMD_INTERNAL | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE, pm,
                    ff, topo, tab);
     __syncthreads();
     for (int t = threadIdx.x; t < 29 * 3; t += blockDim.x) {
