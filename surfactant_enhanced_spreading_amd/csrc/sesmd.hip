@@ -1476,8 +1476,16 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
         any = 1;
         double dx = r[i] - cx, dy = r[g.n + i] - cy, dz = r[2 * g.n + i] - cz;
         double rho = sqrt(dx * dx + dy * dy + dz * dz);
-        double al = acos(dz / rho), be = atan2(dy, dx);
-        double ca = cos(al);
+        // cos(alpha) = dz / rho; e^{-i beta} = (dx, -dy) / rho_xy, its powers by
+        // multiplication (e^{-i 0} where rho_xy = 0, as atan2(0, 0) = 0 gives)
+        double ca = dz / rho;
+        double rxy = sqrt(dx * dx + dy * dy);
+        double2 e1 = rxy > 0.0 ? make_double2(dx / rxy, -dy / rxy) : make_double2(1.0, 0.0);
+        double2 em[MAXP + 1];
+        em[0] = make_double2(1.0, 0.0);
+#pragma unroll
+        for (int m = 1; m <= p; m++) em[m] = make_double2(em[m - 1].x * e1.x - em[m - 1].y * e1.y,
+                                                          em[m - 1].x * e1.y + em[m - 1].y * e1.x);
         double q = ff->q[type[i]];
         // P_n^m(cos alpha) by the recurrence of rLegendreP, columns m = 0..p
         double P[MAXP + 1][MAXP + 1];
@@ -1510,7 +1518,7 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
 #pragma unroll
             for (int m = 0; m <= n; m++) {
                 // Y_n^{-m}: phase exp(-i m beta) rounded to single precision
-                float cr = (float)cos(-m * be), ci = (float)sin(-m * be);
+                float cr = (float)em[m].x, ci = (float)em[m].y;
                 double sc = sqrt(ff->rifact[n - m] / ff->rifact[n + m]) * P[n][m];
                 double w = q * rhon;
                 int c = n * (n + 1) / 2 + m;
