@@ -598,9 +598,17 @@ __global__ __launch_bounds__(64) void k_key_order(Geo g, const int *__restrict__
                                                   int *__restrict__ iperm)
 {
     int kk = blockIdx.x, s0 = bs4[kk], len = bs4[kk + 1] - s0;
+    const int *seg = tmp + s0;
     for (int j = threadIdx.x; j < len; j += 64) {
-        int v = tmp[s0 + j], rank = 0;
-        for (int t = 0; t < len; t++) rank += tmp[s0 + t] < v ? 1 : 0;
+        int v = seg[j], rank = 0, t = 0;
+        for (; t + 8 <= len; t += 8) {  // eight loads in flight per round
+            int w[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) w[u] = seg[t + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) rank += w[u] < v ? 1 : 0;
+        }
+        for (; t < len; t++) rank += seg[t] < v ? 1 : 0;
         perm[s0 + rank] = v;
         key_out[s0 + rank] = kk;
         iperm[v] = s0 + rank;
@@ -1243,6 +1251,7 @@ __device__ __forceinline__ void row_receivers(const Geo &g, const int *__restric
 // launch is needed; the order within a bucket is arrival order, which changes
 // no result (every item writes its own receivers' row slots).
 constexpr int RL_BUCKETS = 128;
+constexpr int RL_ROWS_LDS = 4096;  // rows whose counts the row list keeps in LDS (16^3)
 
 __device__ __forceinline__ int cost_bucket(unsigned cost)
 {
@@ -1270,11 +1279,13 @@ __device__ __forceinline__ void row_list_block(const Geo &g, const int *__restri
                                                int2 *__restrict__ work, int *__restrict__ nwork, int max_work)
 {
     __shared__ int s_hist[RL_BUCKETS], s_off[RL_BUCKETS];
+    __shared__ int2 s_row[RL_ROWS_LDS];  // each row's (receivers, sources), kept for the second pass
     for (int i = threadIdx.x; i < RL_BUCKETS; i += blockDim.x) s_hist[i] = 0;
     __syncthreads();
     for (int q = threadIdx.x; q < g.nb3; q += blockDim.x) {
         int total, src;
         row_item_info(g, bstart, q, total, src);
+        if (q < RL_ROWS_LDS) s_row[q] = make_int2(total, src);
         for (int c0 = 0; c0 < total; c0 += NEAR_CHUNK)
             atomicAdd(&s_hist[cost_bucket((unsigned)(min(NEAR_CHUNK, total - c0) * src))], 1);
     }
@@ -1290,7 +1301,12 @@ __device__ __forceinline__ void row_list_block(const Geo &g, const int *__restri
     __syncthreads();
     for (int q = threadIdx.x; q < g.nb3; q += blockDim.x) {
         int total, src;
-        row_item_info(g, bstart, q, total, src);
+        if (q < RL_ROWS_LDS) {
+            total = s_row[q].x;
+            src = s_row[q].y;
+        } else {
+            row_item_info(g, bstart, q, total, src);
+        }
         for (int c0 = 0; c0 < total; c0 += NEAR_CHUNK) {
             int pos = atomicAdd(&s_off[cost_bucket((unsigned)(min(NEAR_CHUNK, total - c0) * src))], 1);
             if (pos < max_work) work[pos] = make_int2(q, c0);
