@@ -319,13 +319,14 @@ static double rcen2(const Geo &g, int ix)
 
 // ------------------------------------------------------------ device helpers
 
-__device__ __forceinline__ double rsqrt64(double x)
+__device__ __forceinline__ double rcube64(double x)
 {
-    // v_rsq_f64 (relative error below 2^-22) refined by one Newton step,
-    // y (3 - x y^2) / 2: the error squares, to ~1e-14 relative
+    // x^(-3/2) from v_rsq_f64's y = x^(-1/2) (1 + e), |e| < 2^-22, by one
+    // Newton-type step on the cube itself: x y^2 = 1 + 2e + e^2, so
+    // y^3 (5/2 - 3/2 x y^2) = x^(-3/2) (1 - 15/2 e^2 + O(e^3)), ~4e-13 relative
     double y = __builtin_amdgcn_rsq(x);
-    double hx = 0.5 * x;
-    return y * fma(-hx * y, y, 1.5);
+    double y2 = y * y;
+    return (y2 * y) * fma(x * y2, -1.5, 2.5);
 }
 
 __device__ __forceinline__ int iabs_d(int a) { return a < 0 ? -a : a; }
@@ -786,17 +787,16 @@ __device__ __forceinline__ void near_pair(const Geo &g, const double4 &pj, const
         }
         if (!__builtin_amdgcn_ballot_w64(any)) return;
     }
-    double dx[NR], dy[NR], dz[NR], r2[NR], rinv2[NR], ft[NR];
+    double dx[NR], dy[NR], dz[NR], r2[NR], rinv3[NR], ft[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         dx[r] = R[r].x - pj.x;
         dy[r] = R[r].y - pj.y;
         dz[r] = R[r].z - pj.z;
         r2[r] = dx[r] * dx[r] + dy[r] * dy[r] + dz[r] * dz[r];
-        double rinv = rsqrt64(r2[r]);
-        rinv2[r] = rinv * rinv;
+        rinv3[r] = rcube64(r2[r]);
         double qq = CAT == 0 ? R[r].qO : (CAT == 2 && HQ) ? R[r].qH : R[r].q * pj.w;
-        ft[r] = qq * (rinv2[r] * rinv);
+        ft[r] = qq * rinv3[r];
     }
     if (CAT != 2) {
         bool in[NR], any = false;
@@ -812,8 +812,9 @@ __device__ __forceinline__ void near_pair(const Geo &g, const double4 &pj, const
                 // (registers for them would cost the kernel a wavefront per SIMD)
                 int tj = R[r].ljrow + (CAT == 0 ? 0 : (mj.y & 255));
                 double c = ljc[tj], s6 = ljs[tj];
-                double r6 = rinv2[r] * rinv2[r] * rinv2[r];
-                double flj = c * r6 * (s6 * r6 - 0.5) * rinv2[r];
+                double r6 = rinv3[r] * rinv3[r];
+                double rinv2 = r6 * (r2[r] * r2[r]);  // r^-6 r^4
+                double flj = c * r6 * (s6 * r6 - 0.5) * rinv2;
                 ft[r] = (c != 0.0 && in[r]) ? ft[r] + flj : ft[r];
             }
         }
