@@ -322,11 +322,14 @@ static double rcen2(const Geo &g, int ix)
 __device__ __forceinline__ double rcube64(double x)
 {
     // x^(-3/2) from v_rsq_f64's y = x^(-1/2) (1 + e), |e| < 2^-22, by one
-    // Newton-type step on the cube itself: x y^2 = 1 + 2e + e^2, so
-    // y^3 (5/2 - 3/2 x y^2) = x^(-3/2) (1 - 15/2 e^2 + O(e^3)), ~4e-13 relative
+    // Newton-type step on the cube itself: with u = 1 - x y^2 = -2e - e^2,
+    // y^3 (1 + 3/2 u) = x^(-3/2) (1 - 15/2 e^2 + O(e^3)), ~4e-13 relative.
+    // Written with the inline constants 1 and 1/2 only (a VOP3 operand
+    // cannot be a literal here).
     double y = __builtin_amdgcn_rsq(x);
-    double y2 = y * y;
-    return (y2 * y) * fma(x * y2, -1.5, 2.5);
+    double y2 = y * y, y3 = y2 * y;
+    double u = fma(-x, y2, 1.0);
+    return fma(y3, fma(u, 0.5, u), y3);
 }
 
 __device__ __forceinline__ int iabs_d(int a) { return a < 0 ? -a : a; }
