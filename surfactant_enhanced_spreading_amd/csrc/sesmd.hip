@@ -3990,14 +3990,14 @@ extern "C" int sesmd_init_accel(sesmd_ctx *c)
     return 0;
 }
 
-// One full step (src/mybox.f90:130-227).  Part A and part B each run the
-// surfactant and water molecules in one launch (the few, latency-bound
-// surfactant blocks first); the intramolecular surfactant forces run on a
-// second stream beside the near field, L2P and graphite, into their own
-// buffer fs, which part B adds:
-//   main:  A(surf+water) - nh(eta) - [rebuild, far field] -+- fill, gather, near,
-//                                                          |  L2P + wall, graphite -+- B(surf+water)
-//   side:                                                  +- intra -> fs ----------+
+// One full step (src/mybox.f90:130-227), every launch on the context's one
+// stream.  Part A and part B each run the surfactant and water molecules in
+// one launch (the few, latency-bound surfactant blocks first); the eta update
+// and the intramolecular surfactant forces share one launch, writing the
+// latter into their own buffer fs, which part B adds:
+//   A(surf+water) - eta + intra -> fs - [rebuild, far field] - near - site terms - B(surf+water)
+// (A second stream for the eta/intra launch, beside the forces, measured
+// slower: the join before part B costs more than the launch it hides.)
 static int launch_step(sesmd_ctx *c, bool rebuild)
 {
     const Geo &g = c->g;
