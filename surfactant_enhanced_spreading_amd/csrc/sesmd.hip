@@ -756,6 +756,22 @@ struct Recv {
     int by, bz;              // the receiver's box (y, z); every receiver of a wave has box x = bx
 };
 
+// Each lane's LJ constants (c, s6) of receiver slot r against water O, staged
+// in LDS when the receivers are loaded (stage_recv_ljo): the water-O LJ term,
+// the most frequent one, then reads them there instead of gathering them from
+// the table in global memory at every source.  Only the lane that writes an
+// entry reads it, and a near-field block is one wavefront.
+__device__ __forceinline__ double2 &recv_ljo(int r)
+{
+    __shared__ double2 s_ljo[2][64];
+    return s_ljo[r][threadIdx.x & 63];
+}
+
+__device__ __forceinline__ void stage_recv_ljo(const sesmd_forcefield *__restrict__ ff, int r, int ti)
+{
+    recv_ljo(r) = make_double2(ff->lj_c[ti * 32], ff->lj_s6[ti * 32]);
+}
+
 // Sources t in [s0, s1) of one LJ class for NR receivers per lane (lane l
 // holds receivers l and 64 + l of its chunk; their sums are independent, so
 // each receiver adds the same terms in the same order either way).
@@ -813,8 +829,16 @@ __device__ __forceinline__ void near_pair(const Geo &g, const double4 &pj, const
             for (int r = 0; r < NR; r++) {
                 // the constants are read here, only when the term is needed
                 // (registers for them would cost the kernel a wavefront per SIMD)
-                int tj = R[r].ljrow + (CAT == 0 ? 0 : (mj.y & 255));
-                double c = ljc[tj], s6 = ljs[tj];
+                double c, s6;
+                if (CAT == 0) {
+                    double2 cs = recv_ljo(r);
+                    c = cs.x;
+                    s6 = cs.y;
+                } else {
+                    int tj = R[r].ljrow + (mj.y & 255);
+                    c = ljc[tj];
+                    s6 = ljs[tj];
+                }
                 double r6 = rinv3[r] * rinv3[r];
                 double rinv2 = r6 * (r2[r] * r2[r]);  // r^-6 r^4
                 double flj = c * r6 * (s6 * r6 - 0.5) * rinv2;
@@ -1165,6 +1189,7 @@ __device__ __forceinline__ void near_item(
     for (int r = 0; r < NR; r++) {
         int k = k0 + NEAR_TILE * r + lane;
         load_recv(g, spos, smeta, rkind, ff, k, k < kend, by, bz, R[r], ti[r]);
+        if (!MIX) stage_recv_ljo(ff, r, ti[r]);
     }
     double ax[NR], ay[NR], az[NR];
 #pragma unroll
@@ -1363,6 +1388,7 @@ __device__ __forceinline__ void near_rows_item(
         k[r] = st + p - pr;
         slot[r] = 8 - m;  // the row's (dz, dy) seen from the receiver's box: 3 (1 - ddz) + (1 - ddy)
         load_recv(g, spos, smeta, rkind, ff, k[r], on, y + ddy, z + ddz, R[r], ti[r]);
+        if (!MIX) stage_recv_ljo(ff, r, ti[r]);
         cen = cen || (on && m == 4);
     }
     bool anycen = __builtin_amdgcn_ballot_w64(cen) != 0;
