@@ -761,6 +761,7 @@ struct Recv {
 // the most frequent one, then reads them there instead of gathering them from
 // the table in global memory at every source.  Only the lane that writes an
 // entry reads it, and a near-field block is one wavefront.
+static_assert(NEAR_TILE == 64, "the staged LJ constants are indexed by lane: one wavefront per block");
 __device__ __forceinline__ double2 &recv_ljo(int r)
 {
     __shared__ double2 s_ljo[2][64];
@@ -1388,9 +1389,11 @@ __device__ __forceinline__ void near_rows_item(
         k[r] = st + p - pr;
         slot[r] = 8 - m;  // the row's (dz, dy) seen from the receiver's box: 3 (1 - ddz) + (1 - ddy)
         load_recv(g, spos, smeta, rkind, ff, k[r], on, y + ddy, z + ddz, R[r], ti[r]);
-        if (!MIX) stage_recv_ljo(ff, r, ti[r]);
         cen = cen || (on && m == 4);
     }
+    if (!MIX)
+#pragma unroll
+        for (int r = 0; r < NR; r++) stage_recv_ljo(ff, r, ti[r]);
     bool anycen = __builtin_amdgcn_ballot_w64(cen) != 0;
     double ax[NR], ay[NR], az[NR];
 #pragma unroll
@@ -1434,8 +1437,15 @@ __global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
     int2 w = work[item];
     int q = w.x, c0 = w.y;
     int x0 = q % nb, y = (q / nb) % nb, z = q / (nb * nb);
-    RowRecv L;
-    row_receivers(g, bstart, x0, y, z, L);
+    // the row's receiver list is the same for every lane: one copy in LDS
+    // (the per-lane slot search below indexes it)
+    __shared__ RowRecv L;
+    {
+        RowRecv Lr;
+        row_receivers(g, bstart, x0, y, z, Lr);
+        if (lane == 0) L = Lr;
+        __syncthreads();
+    }
     int cend = min(L.pre[9], c0 + NEAR_CHUNK);
     if (cend - c0 > NEAR_TILE)
         near_rows_item<HQ, MIX, 2>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, c0,
