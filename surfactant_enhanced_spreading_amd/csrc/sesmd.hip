@@ -2065,16 +2065,26 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
     double4 pk = spos[k];
     double x = pk.x, y = pk.y, z = pk.z;
     double nx = 0.0, ny = 0.0, nz = 0.0;
-    if ((what & SESMD_NEAR) && rk >= 0) {
-        // the rows (dz, dy) of the receiver's box that lie inside the grid
-        int b = bk, nb = g.nb, by = (b / nb) % nb, bz = b / (nb * nb);
+    if (what & SESMD_NEAR) {
+        // the nine row partials are loaded with the sorted records, not after
+        // them (one wavefront per SIMD here: nothing hides a second round of
+        // loads); the slots of rows outside the grid hold stale values, and
+        // only the rows (dz, dy) of the receiver's box inside the grid are added
+        double pr[NEAR_ROWS][3];
 #pragma unroll
-        for (int q = 0; q < NEAR_ROWS; q++) {
-            int y = by + q % 3 - 1, z = bz + q / 3 - 1;
-            if (y < 0 || y >= nb || z < 0 || z >= nb) continue;
-            nx += part[(size_t)(3 * q) * g.n + k];
-            ny += part[(size_t)(3 * q + 1) * g.n + k];
-            nz += part[(size_t)(3 * q + 2) * g.n + k];
+        for (int q = 0; q < NEAR_ROWS; q++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) pr[q][d] = part[(size_t)(3 * q + d) * g.n + k];
+        if (rk >= 0) {
+            int b = bk, nb = g.nb, by = (b / nb) % nb, bz = b / (nb * nb);
+#pragma unroll
+            for (int q = 0; q < NEAR_ROWS; q++) {
+                int y = by + q % 3 - 1, z = bz + q / 3 - 1;
+                if (y < 0 || y >= nb || z < 0 || z >= nb) continue;
+                nx += pr[q][0];
+                ny += pr[q][1];
+                nz += pr[q][2];
+            }
         }
     }
     double fx = 0.0, fy = 0.0, fz = 0.0;
