@@ -217,8 +217,8 @@ def main():
         copy_calibration(args.copy_calibrate)
         return
     if args.gpus != 1:
-        print('bench.py measures one GPU: this build runs the whole system on one MI355X '
-              'and has no multi-GPU path (see DESIGN.md, "How it would shard")', file=sys.stderr)
+        print('bench.py measures one GPU: the whole system runs on one MI355X '
+              'and there is no multi-GPU path', file=sys.stderr)
         sys.exit(2)
 
     from surfactant_enhanced_spreading_amd import builder, engine
