@@ -1,15 +1,17 @@
 #!/usr/bin/env python3
 """Throughput of the per-timestep force + integration path on one MI355X.
 
-Workload: one droplet of 20,000 rigid SPC/E waters carrying 200 29-site
-M(D'E4OH)M surfactants (65,800 sites) in the reference's geometry (144 A cube,
-12^3 L4 boxes of 12 A, p = 4, far field and lists refreshed every 10 steps,
-2 fs, Nose-Hoover at 298 K).  A step is the reference's full step
-(src/mybox.f90:130-227); the timed region covers K consecutive steps, rebuild
-steps included, with all inputs resident in HBM.  `value` is the FP64 path
-(the drop-in's precision); the "mixed" object reports the mixed-precision mode
-(FP32 pair arithmetic) on the same workload, its speedup and its RMS force
-error against FP64.
+Headline workload: one droplet of 20,000 rigid SPC/E waters carrying 200
+29-site M(D'E4OH)M surfactants (65,800 sites) in a 16^3-box domain (nb2 = 4:
+4,096 L4 boxes of 12 A, a 192 A cube), expansion order p = 8, far field and
+lists refreshed every 10 steps, 2 fs, Nose-Hoover at 298 K.  A step is the
+reference's full step (src/mybox.f90:130-227); the timed region covers K
+consecutive steps, rebuild steps included, with all inputs resident in HBM.
+`value` is the FP64 path (the drop-in's precision); the "mixed" object reports
+the mixed-precision mode (FP32 pair arithmetic) on the same workload, its
+speedup and its RMS force error against FP64, at the start and after the
+mixed drift run.  "reference_geometry" repeats the timing in the reference's
+own geometry (12^3 boxes, p = 4).
 
 Prints one JSON line.  The library and torch ship different HSA runtimes, so
 timing uses HIP events of the library's own runtime (/opt/rocm), not torch.
@@ -61,14 +63,29 @@ class HipTimer:
         assert self.hip.hipDeviceSynchronize() == 0
 
 
-def cpu_baseline(sys_, seconds=20.0):
+# The two workloads: the headline (20k + 200 droplet, 16^3 boxes, p = 8) and
+# the same droplet in the reference's own geometry (12^3 boxes, p = 4).
+CONFIGS = {
+    'c2': dict(nb2=4, p=8, workload="20k SPC/E water + 200 M(D'E4OH)M surfactant droplet, "
+                                    "16^3 L4 boxes of 12 A (4,096 boxes), p=8, ifar=10, 2 fs"),
+    'ref': dict(nb2=3, p=4, workload="the same droplet in the reference geometry "
+                                     "(12^3 L4 boxes of 12 A, p=4, ifar=10, 2 fs)"),
+}
+
+
+def make_system(name):
+    from surfactant_enhanced_spreading_amd import builder
+    return builder.config1_system() if name == 'c2' else builder.bench_system()
+
+
+def cpu_baseline(sys_, nb2, p, seconds=20.0):
     """The C restatement (oracle) on the host cores: one rebuild step (classes,
     far field, forces) then regular steps until ~`seconds` have passed."""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle as orc
     threads = min(16, os.cpu_count() or 1)
     orc.lib().orc_set_threads(threads)
-    o = orc.Oracle(sys_)
+    o = orc.Oracle(sys_, nb2=nb2, p=p)
     o.rebuild()
     o.fmm()
     o.init_accel(o.forces())
@@ -83,14 +100,17 @@ def cpu_baseline(sys_, seconds=20.0):
             break
     return dict(value=n / el, unit='steps/s', cores=threads, kind='port',
                 sample='%d full steps (1 rebuild+far-field step, %d regular) of the same '
-                       '65,800-site system, C restatement with OpenMP' % (n, n - 1))
+                       '%d-site system at p=%d, C restatement with OpenMP'
+                       % (n, n - 1, sys_.n_sites, p))
 
 
-def run(sys_, prec, steps, warmup, timer):
+def run(sys_, cfg, prec, steps, warmup, timer):
     """Bootstrap, warm up, then time `steps` full steps and the near-field
-    kernel alone; returns the figures and the engine (still open)."""
+    kernel alone; returns the figures and the engine (still open).  The first
+    steps() call captures the step graphs before it runs any step, so no
+    capture falls inside the timed region."""
     from surfactant_enhanced_spreading_amd import engine
-    e = engine.Engine(sys_, precision=prec)
+    e = engine.Engine(sys_, nb2=cfg['nb2'], p=cfg['p'], precision=prec)
     e.bootstrap()
     stream = e.stream()
     kb = e.steps(warmup, 10, 0)
@@ -117,14 +137,66 @@ def run(sys_, prec, steps, warmup, timer):
     return dict(ms_step=ms_step, steps_s=1000.0 / ms_step, wall=wall, near_ms=near_ms), e
 
 
-def energy_drift(sys_, prec, nsteps, every=500, settle=100, ifar=10):
+def m2l_translations(O4, nb):
+    """(target, source) L4 translations with a non-empty source: for every L4
+    box, the children of its parent's 27 neighbours less its own 27 neighbours
+    (src/computeLjk4.f90:58-170), counted where the source holds charge."""
+    import numpy as np
+    ne = (np.abs(O4).sum(axis=1) != 0).reshape(nb, nb, nb)  # [z][y][x]
+    total = 0
+    idx = np.arange(nb)
+    for dz in range(-3, 4):
+        for dy in range(-3, 4):
+            for dx in range(-3, 4):
+                if max(abs(dx), abs(dy), abs(dz)) <= 1:
+                    continue
+                # targets t whose parent neighbours the source's parent
+                ok = []
+                for d in (dx, dy, dz):
+                    t = idx
+                    s = t + d
+                    ok.append((s >= 0) & (s < nb) & (np.abs(s // 2 - t // 2) <= 1))
+                okx, oky, okz = ok
+                m = okz[:, None, None] & oky[None, :, None] & okx[None, None, :]
+                zz, yy, xx = np.nonzero(m)
+                total += int(ne[zz + dz, yy + dy, xx + dx].sum())
+    return total
+
+
+def m2l_timing(e, timer, reps=20):
+    """The L4 M2L (computeLjk4) alone, `reps` launches on the context's stream
+    after a rebuild and far field at the current state; achieved FP64 rate of
+    the algorithmic count 8 ((p+1)(p+2)/2) (p+1)^2 per translation."""
+    from surfactant_enhanced_spreading_amd import engine
+    e.rebuild()
+    e.fmm()
+    T = m2l_translations(e.expansions()['O4'], e.nb)
+    stream = e.stream()
+    e0, e1 = timer.event(), timer.event()
+    timer.sync()
+    timer.record(e0, stream)
+    for _ in range(reps):
+        e.fmm_stage(engine.M2L_4, wait=False)
+    timer.record(e1, stream)
+    ms = timer.elapsed_ms(e0, e1) / reps
+    timer.sync()
+    p = e.p
+    flop = 8.0 * ((p + 1) * (p + 2) // 2) * (p + 1) ** 2 * T
+    ach = flop / (ms * 1e-3) / 1e12
+    return dict(kernel='M2L at L4 (k_m2l_count, k_m2l_scan, k_m2l_fill, k_m2l_gemm, k_m2l_accum)', translations=T, ms=round(ms, 4),
+                achieved_tflops=round(ach, 3), peak_tflops=FP64_PEAK_TFLOPS,
+                frac_fp64_mfma_peak=round(ach / FP64_PEAK_TFLOPS, 4))
+
+
+def energy_drift(sys_, cfg, prec, nsteps, every=500, settle=100, ifar=10, keep=False):
     """Extended (Nose-Hoover) energy over `nsteps` steps after `settle` steps
     (the first steps project the start-up velocities onto the constraints):
     drift and largest excursion relative to the kinetic energy, energies
     evaluated in FP64 (sesmd_energies) every `every` steps.  ifar: steps
-    between list / far-field refreshes (the reference's 10)."""
+    between list / far-field refreshes (the reference's 10).  keep: also
+    return the final state."""
     from surfactant_enhanced_spreading_amd import engine
-    e = engine.Engine(sys_, precision=prec)
+    e = engine.Engine(sys_, nb2=cfg['nb2'], p=cfg['p'], precision=prec)
     try:
         e.bootstrap()
         kb = e.steps(settle, ifar, 0)
@@ -137,25 +209,31 @@ def energy_drift(sys_, prec, nsteps, every=500, settle=100, ifar=10):
             done += k
             hs.append(e.energies()['total'])
         e1 = e.energies()
+        final = e.state() if keep else None
     finally:
         e.close()
     k0 = e0['kinetic']
     ns = nsteps * 2.0e-6  # 2 fs steps
-    return dict(steps=nsteps, ifar=ifar, kinetic0=round(k0, 3), total0=round(hs[0], 4), total_end=round(hs[-1], 4),
-                drift_rel_kinetic=(hs[-1] - hs[0]) / k0,
-                drift_rel_kinetic_per_ns=(hs[-1] - hs[0]) / k0 / ns,
-                max_excursion_rel_kinetic=max(abs(h - hs[0]) for h in hs) / k0,
-                temperature0=round(e0['temperature'], 4), temperature_end=round(e1['temperature'], 4))
+    out = dict(steps=nsteps, ifar=ifar, kinetic0=round(k0, 3), total0=round(hs[0], 4),
+               total_end=round(hs[-1], 4),
+               drift_rel_kinetic=(hs[-1] - hs[0]) / k0,
+               drift_rel_kinetic_per_ns=(hs[-1] - hs[0]) / k0 / ns,
+               max_excursion_rel_kinetic=max(abs(h - hs[0]) for h in hs) / k0,
+               temperature0=round(e0['temperature'], 4), temperature_end=round(e1['temperature'], 4))
+    return (out, final) if keep else out
 
 
-def rms_force_error(sys_):
+def rms_force_error(sys_, cfg, state=None):
     """||F_mixed - F_fp64|| / ||F_fp64|| over all sites and components, total
-    force at the start-up configuration."""
+    force after a fresh rebuild and far field, at the system's configuration
+    or at `state` (positions, velocities, accelerations, eta)."""
     import numpy as np
     from surfactant_enhanced_spreading_amd import engine
     f = []
     for prec in (engine.FP64, engine.MIXED):
-        e = engine.Engine(sys_, precision=prec)
+        e = engine.Engine(sys_, nb2=cfg['nb2'], p=cfg['p'], precision=prec)
+        if state is not None:
+            e.set_state(*state)
         e.rebuild()
         e.fmm()
         f.append(e.forces())
@@ -163,13 +241,14 @@ def rms_force_error(sys_):
     return float(np.sqrt(((f[1] - f[0]) ** 2).sum() / (f[0] ** 2).sum()))
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_near_rows_counters_bench_fp64.md')
+TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_near_rows_counters_c2_fp64.md')
 
 
 def near_traffic():
-    """HBM bytes per k_near_rows launch on this workload, from the committed
-    FETCH_SIZE / WRITE_SIZE counter runs (each scaled by what the counter reads
-    for a plain copy of known size); None if that summary is absent."""
+    """HBM bytes per k_near_rows launch on the headline workload, from the
+    committed FETCH_SIZE / WRITE_SIZE counter runs (each scaled by what the
+    counter reads for a plain copy of known size); None if that summary is
+    absent."""
     try:
         with open(TRAFFIC_PROFILE) as fh:
             for line in fh:
@@ -205,11 +284,16 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=500)
     ap.add_argument('--warmup', type=int, default=30)
+    ap.add_argument('--config', choices=sorted(CONFIGS), default='c2',
+                    help='c2: the headline workload; ref: only the reference-geometry workload')
     ap.add_argument('--no-mixed', action='store_true', help='skip the mixed-precision leg')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--no-secondary', action='store_true', help='skip the reference-geometry leg')
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
     ap.add_argument('--drift-steps', type=int, default=10000,
                     help='steps of the energy-drift runs (FP64 and mixed); 0 skips them')
+    ap.add_argument('--ifar-steps', type=int, default=2000,
+                    help='steps of each run of the drift-vs-ifar series (ifar 1, 2, 5, 10); 0 skips it')
     ap.add_argument('--copy-calibrate', type=int, default=0, metavar='BYTES',
                     help='only run a device copy of BYTES (counter calibration) and exit')
     args = ap.parse_args()
@@ -221,53 +305,79 @@ def main():
               'and there is no multi-GPU path', file=sys.stderr)
         sys.exit(2)
 
-    from surfactant_enhanced_spreading_amd import builder, engine
-    sys_ = builder.bench_system()
+    from surfactant_enhanced_spreading_amd import engine
+    name = args.config
+    cfg = CONFIGS[name]
+    sys_ = make_system(name)
     timer = HipTimer()
-    r64, e = run(sys_, engine.FP64, args.steps, args.warmup, timer)
+    print('bench: %s, %d sites, nb2=%d, p=%d' % (name, sys_.n_sites, cfg['nb2'], cfg['p']),
+          file=sys.stderr, flush=True)
+    r64, e = run(sys_, cfg, engine.FP64, args.steps, args.warmup, timer)
     pairs, lj_pairs = e.pair_counts()
     shake_failures, temperature = e.shake_failures(), e.temperature()
+    m2l = m2l_timing(e, timer)
     e.close()
     steps_s = r64['steps_s']
     flop = 21.0 * pairs + 9.0 * lj_pairs
     achieved = flop / (r64['near_ms'] * 1e-3) / 1e12
     roofline = dict(bound='fp64-valu', achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS,
-                    unit='TFLOP/s', frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=near_traffic(),
+                    unit='TFLOP/s', frac=round(achieved / FP64_PEAK_TFLOPS, 4),
+                    traffic=near_traffic() if name == 'c2' else None,
                     kernel='k_near_rows', kernel_ms=round(r64['near_ms'], 4),
                     pairs=pairs, lj_pairs=lj_pairs)
-
+    nb = 4 * cfg['nb2']
     out = dict(metric=METRIC, value=round(steps_s, 2), unit='steps/s', n_gpus=1,
                steps=args.steps, warmup=args.warmup, ms_per_step=round(r64['ms_step'], 4),
                higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f64',
                data='synthetic',
-               config=dict(workload='20k SPC/E water + 200 M(D\'E4OH)M surfactant droplet, '
-                                    'reference geometry (12^3 L4 boxes, p=4, ifar=10, 2 fs)',
-                           n_water=sys_.n_water, n_surf=sys_.n_surf, sites=sys_.n_sites,
-                           boxes=12 ** 3, p=4, ifar=10, dt_fs=2.0, precision='fp64'),
+               config=dict(workload=cfg['workload'], n_water=sys_.n_water, n_surf=sys_.n_surf,
+                           sites=sys_.n_sites, boxes=nb ** 3, p=cfg['p'], ifar=10, dt_fs=2.0,
+                           precision='fp64'),
                ns_per_day=round(steps_s * 2.0 * 0.0864, 2),  # dt = 2 fs (src/mybox.cmns:43)
                wall_s=round(r64['wall'], 3), shake_failures=shake_failures,
-               temperature=round(temperature, 4), roofline=roofline)
+               temperature=round(temperature, 4), roofline=roofline, m2l=m2l)
+    print('bench: fp64 %.1f steps/s' % steps_s, file=sys.stderr, flush=True)
     if not args.no_mixed:
-        rm, e = run(sys_, engine.MIXED, args.steps, args.warmup, timer)
+        rm, e = run(sys_, cfg, engine.MIXED, args.steps, args.warmup, timer)
         e.close()
         ach = flop / (rm['near_ms'] * 1e-3) / 1e12
         out['mixed'] = dict(steps_per_s=round(rm['steps_s'], 2), ms_per_step=round(rm['ms_step'], 4),
                             ns_per_day=round(rm['steps_s'] * 2.0 * 0.0864, 2),
                             speedup_vs_fp64=round(rm['steps_s'] / steps_s, 3),
-                            rms_force_err=rms_force_error(sys_), dtype='f32 pair / f64',
+                            rms_force_err=rms_force_error(sys_, cfg), dtype='f32 pair / f64',
                             near_kernel_ms=round(rm['near_ms'], 4),
                             near_achieved_tflops=round(ach, 3),
                             near_frac_fp32_peak=round(ach / FP32_PEAK_TFLOPS, 4))
+        print('bench: mixed %.1f steps/s' % rm['steps_s'], file=sys.stderr, flush=True)
+    if not args.no_secondary and name == 'c2':
+        rcfg = CONFIGS['ref']
+        rsys = make_system('ref')
+        sec = dict(workload=rcfg['workload'], boxes=(4 * rcfg['nb2']) ** 3, p=rcfg['p'])
+        for prec, key in ((engine.FP64, 'fp64'), (engine.MIXED, 'mixed')):
+            if prec == engine.MIXED and args.no_mixed:
+                continue
+            rr, e = run(rsys, rcfg, prec, args.steps, args.warmup, timer)
+            e.close()
+            sec[key] = dict(steps_per_s=round(rr['steps_s'], 2), ms_per_step=round(rr['ms_step'], 4),
+                            near_kernel_ms=round(rr['near_ms'], 4))
+        out['reference_geometry'] = sec
+        print('bench: reference geometry done', file=sys.stderr, flush=True)
     if args.drift_steps > 0:
-        out['energy_drift'] = dict(fp64=energy_drift(sys_, engine.FP64, args.drift_steps))
+        out['energy_drift'] = dict(fp64=energy_drift(sys_, cfg, engine.FP64, args.drift_steps))
         if not args.no_mixed:
-            out['energy_drift']['mixed'] = energy_drift(sys_, engine.MIXED, args.drift_steps)
-        # the same dynamics with lists and far field refreshed every step: the
-        # drift above is the reference's 10-step refresh, not the integrator
-        out['energy_drift']['fp64_refresh_every_step'] = energy_drift(
-            sys_, engine.FP64, min(args.drift_steps, 2000), ifar=1)
+            dm, final = energy_drift(sys_, cfg, engine.MIXED, args.drift_steps, keep=True)
+            out['energy_drift']['mixed'] = dm
+            out['mixed']['rms_force_err_after_drift'] = rms_force_error(sys_, cfg, final)
+        print('bench: drift done', file=sys.stderr, flush=True)
+    if args.ifar_steps > 0:
+        # the same dynamics with the lists and far field refreshed every 1, 2,
+        # 5 and 10 steps, all runs of equal length: how the drift depends on
+        # the refresh interval
+        out['drift_vs_ifar'] = [energy_drift(sys_, cfg, engine.FP64, args.ifar_steps, ifar=k)
+                                for k in (1, 2, 5, 10)]
+        print('bench: ifar series done', file=sys.stderr, flush=True)
     if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline(sys_, args.cpu_seconds)
+        out['cpu_baseline'] = cpu_baseline(sys_, cfg['nb2'], cfg['p'], args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

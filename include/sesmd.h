@@ -116,7 +116,10 @@ int sesmd_init_accel(sesmd_ctx *ctx);
 
 /* nsteps full steps in the order of src/mybox.f90:130-227 (part A; rebuild +
  * far field when kb % ifar == 0; forces; part B), all on the GPU, the regular
- * steps replayed from a captured HIP graph.  *kb is the step counter. */
+ * steps replayed from a captured HIP graph.  *kb is the step counter.  The
+ * first call for an ifar captures every graph it replays (the whole cycle of
+ * ifar steps, the run of ifar - 1 regular steps, one regular step) before it
+ * runs any step. */
 int sesmd_steps(sesmd_ctx *ctx, int nsteps, int ifar, int *kb);
 
 /* Inspection: classes (per molecule 0 W1, 1 W2, 2 S), storage box (global L4
@@ -163,6 +166,9 @@ enum {
     SESMD_M2L_4 = 7    /* computeLjk4 (src/computeLjk4.f90:58-170)  */
 };
 int sesmd_fmm_stage(sesmd_ctx *ctx, int stage);
+/* The same stage enqueued on sesmd_stream(ctx) without waiting for it (for
+ * timing one far-field kernel with events on that stream). */
+int sesmd_fmm_stage_enqueue(sesmd_ctx *ctx, int stage);
 /* One integrator half for the molecule classes in cls_mask (bit 0 W1, 1 W2,
  * 2 S) without the eta update: part 0 = A (move_*_a without the all-reduce),
  * part 1 = B (move_*_b).  For part A, sums18 (may be NULL) receives the

@@ -122,6 +122,31 @@ void nh_water1_(double *out);                       /* src/nh_water1.f90:1-43 */
 void nh_water2_(double *out);                       /* src/nh_water2.f90 */
 void nh_surfactant_(double *out);                   /* src/nh_surfactant.f90:1-201 */
 
+/* ---- box maps (src/ibox.f90:1-220): integer functions of positions in
+ * reduced units; int() truncates toward zero.  ibox2/3/4: global L2/L3/L4
+ * box (1-based) from /cmn03/ /cmn04/ and ibdx3.. /cmn01/, ibdx4.. /cmn02/;
+ * ibox64/ibox1000: box in the rank's 4^3 / 10^3 grid around rcen2*tmp
+ * (/cmn17/); ibox64mod: the 4^3 grid around a given centre; ib64to216/512/1000,
+ * ib8to216, ib512to64: storage index maps.  isnearest / iscentral are Fortran
+ * LOGICAL(4) functions (1 = .true., 0 = .false.): the half-open tests
+ * [c - 1.5 rs, c + 1.5 rs) and [c - rs/2, c + rs/2) on every axis. ---- */
+int32_t ibox2_(const double *x, const double *y, const double *z);      /* src/ibox.f90:127-139 */
+int32_t ibox3_(const double *x, const double *y, const double *z);      /* src/ibox.f90:142-154 */
+int32_t ibox4_(const double *x, const double *y, const double *z);      /* src/ibox.f90:157-169 */
+int32_t ibox64_(const double *x, const double *y, const double *z);     /* src/ibox.f90:172-186 */
+int32_t ibox64mod_(const double *x, const double *y, const double *z,
+                   const double *cx, const double *cy, const double *cz); /* src/ibox.f90:189-203 */
+int32_t ibox1000_(const double *x, const double *y, const double *z);   /* src/ibox.f90:206-220 */
+int32_t ib64to216_(const int32_t *iaq);   /* src/ibox.f90:31-43 */
+int32_t ib64to512_(const int32_t *iaq);   /* src/ibox.f90:16-28 */
+int32_t ib64to1000_(const int32_t *iaq);  /* src/ibox.f90:1-13  */
+int32_t ib8to216_(const int32_t *iaq);    /* src/ibox.f90:46-58 */
+int32_t ib512to64_(const int32_t *iaq);   /* src/ibox.f90:61-72 */
+int32_t isnearest_(const double *x, const double *y, const double *z,
+                   const double *cx, const double *cy, const double *cz); /* src/ibox.f90:75-98  */
+int32_t iscentral_(const double *x, const double *y, const double *z,
+                   const double *cx, const double *cy, const double *cz); /* src/ibox.f90:101-124 */
+
 /* ---- library calls with no Fortran counterpart ---- */
 /* Sum n doubles over all ranks of communicator *comm (the Fortran handle
  * move_w2_a receives), in place; the reference's mpi_allreduce calls of

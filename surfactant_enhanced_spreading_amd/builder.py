@@ -292,3 +292,11 @@ def bench_system(n_water=20000, n_surf=200, seed=20261204):
     n_surf surfactants, centred in the reference's 144 A cube above the sheet."""
     d = droplet(n_water, n_surf, center_A=(0.0, 0.0, 4.0), seed=seed)
     return assemble([d], graphite=graphite_sheet(), seed=seed)
+
+
+def config1_system(n_water=20000, n_surf=200, seed=20261204):
+    """The 20k-water + 200-surfactant droplet in the 16^3-box domain (nb2 = 4:
+    4,096 L4 boxes of 12 A, a 192 A cube): the droplet of bench_system above a
+    sheet spanning the domain's 192 x 192 A floor."""
+    d = droplet(n_water, n_surf, center_A=(0.0, 0.0, 4.0), seed=seed)
+    return assemble([d], graphite=graphite_sheet(half_side_A=96.0), seed=seed)

@@ -36,6 +36,9 @@
 
 // COMMON blocks (src/mybox.cmns:207-259), resolved against the Fortran side.
 extern "C" {
+extern char cmn01_[] __attribute__((weak));
+extern char cmn02_[] __attribute__((weak));
+extern char cmn03_[] __attribute__((weak));
 extern char cmn04_[] __attribute__((weak));
 extern char cmn05_[] __attribute__((weak));
 extern char cmn06_[] __attribute__((weak));
@@ -489,10 +492,20 @@ bool build_lists(int c, int ring, ListView *central, ListView *near26, bool cent
                     }
         }
     }
+    // an overflowing list keeps its first cap entries and advertises only
+    // those (ict = cap), so a reference routine reading it stays in bounds
+    int over = -1;
     for (int b = 0; b < 64; b++) {
-        if ((central && central->ict[b] > central->cap) || near26->ict[b] > near26->cap)
-            return fail("list of own box %d exceeds its capacity", b + 1);
+        if (central && central->ict[b] > central->cap) {
+            central->ict[b] = central->cap;
+            if (over < 0) over = b;
+        }
+        if (near26->ict[b] > near26->cap) {
+            near26->ict[b] = near26->cap;
+            if (over < 0) over = b;
+        }
     }
+    if (over >= 0) return fail("list of own box %d exceeds its capacity (truncated to it)", over + 1);
     return true;
 }
 
@@ -1037,5 +1050,125 @@ void init_accel_(const int32_t *)
 void nh_water1_(double *out) { nh_sum(0, out); }
 void nh_water2_(double *out) { nh_sum(1, out); }
 void nh_surfactant_(double *out) { nh_sum(2, out); }
+
+// ---- box maps, src/ibox.f90:1-220 ----
+// Plain host arithmetic in the reference's evaluation order (left to right,
+// products and quotients rounded separately, int() truncating toward zero),
+// over the geometry in COMMON /cmn01/ (ibdx3..), /cmn02/ (ibdx4..), /cmn03/
+// (rsdx..), /cmn04/ (rhsx..) and /cmn17/ (rcen2xtmp..).  Without COMMON the
+// functions that read it count an error and return 0 / .false.
+
+static const double kRS = 10.0 / 3.166 + 2.0 / 3.166;  // rs = rc + delta (src/mybox.cmns:38-40)
+
+static bool have_geometry(const char *who, bool l2_boxes)
+{
+    if (!cmn03_ || !cmn04_ || (l2_boxes && (!cmn01_ || !cmn02_)) || (!l2_boxes && !cmn17_)) {
+        fail("%s: COMMON blocks not found: link the reference's Fortran objects", who);
+        return false;
+    }
+    return true;
+}
+
+// 1 + int((x + rhs) * n / rsd) per axis, axes weighted 1, nx, nx ny
+static int32_t global_box(const double *x, const double *y, const double *z, const int32_t *n)
+{
+    const double *rsd = (const double *)cmn03_, *rhs = (const double *)cmn04_;
+    int32_t b = 1 + (int32_t)((*x + rhs[0]) * n[0] / rsd[0]);
+    b = b + (int32_t)((*y + rhs[1]) * n[1] / rsd[1]) * n[0];
+    b = b + (int32_t)((*z + rhs[2]) * n[2] / rsd[2]) * n[0] * n[1];
+    return b;
+}
+
+// storage box of a point in a grid of g^3 boxes of side rs whose first box
+// starts `off` boxes below centre (cx, cy, cz)
+static int32_t grid_box(double x, double y, double z, double cx, double cy, double cz, double off, int g)
+{
+    double ax = x - cx + off * kRS, ay = y - cy + off * kRS, az = z - cz + off * kRS;
+    return 1 + (int32_t)(ax / kRS) + (int32_t)(ay / kRS) * g + (int32_t)(az / kRS) * g * g;
+}
+
+// half-open test [c - w, c + w) on each axis
+static int32_t within(const double *x, const double *y, const double *z, const double *cx,
+                      const double *cy, const double *cz, double w)
+{
+    bool bx = *x < *cx + w && *x >= *cx - w;
+    bool by = *y < *cy + w && *y >= *cy - w;
+    bool bz = *z < *cz + w && *z >= *cz - w;
+    return (bx && by && bz) ? 1 : 0;
+}
+
+int32_t ibox2_(const double *x, const double *y, const double *z)
+{
+    if (!have_geometry("ibox2", false)) return 0;
+    const int32_t n[3] = {NB2, NB2, NB2};  // ibdx2 = ibdy2 = ibdz2 = 3 (PARAMETERs)
+    return global_box(x, y, z, n);
+}
+
+int32_t ibox3_(const double *x, const double *y, const double *z)
+{
+    if (!have_geometry("ibox3", true)) return 0;
+    return global_box(x, y, z, (const int32_t *)cmn01_);
+}
+
+int32_t ibox4_(const double *x, const double *y, const double *z)
+{
+    if (!have_geometry("ibox4", true)) return 0;
+    return global_box(x, y, z, (const int32_t *)cmn02_);
+}
+
+int32_t ibox64_(const double *x, const double *y, const double *z)
+{
+    if (!have_geometry("ibox64", false)) return 0;
+    const double *c = (const double *)cmn17_;  // rcen2xtmp, rcen2ytmp, rcen2ztmp
+    return grid_box(*x, *y, *z, c[0], c[1], c[2], 2.0, 4);
+}
+
+int32_t ibox64mod_(const double *x, const double *y, const double *z, const double *cx,
+                   const double *cy, const double *cz)
+{
+    return grid_box(*x, *y, *z, *cx, *cy, *cz, 2.0, 4);
+}
+
+int32_t ibox1000_(const double *x, const double *y, const double *z)
+{
+    if (!have_geometry("ibox1000", false)) return 0;
+    const double *c = (const double *)cmn17_;
+    return grid_box(*x, *y, *z, c[0], c[1], c[2], 5.0, 10);
+}
+
+// own box iaq = 1..64 (a 4^3 block, x fastest) -> index in a g^3 storage grid
+// whose own block starts at corner (h, h, h); 8 -> 216 uses the 2^3 block of
+// one L3 box
+static int32_t own_to_grid(int32_t iaq, int32_t per, int32_t g, int32_t base)
+{
+    int32_t k = (iaq - 1) / (per * per);
+    int32_t m = (iaq - per * per * k - 1) / per;
+    int32_t r = iaq - per * per * k - per * m;
+    return base + g * g * k + g * m + r;
+}
+
+int32_t ib64to216_(const int32_t *iaq) { return own_to_grid(*iaq, 4, 6, 43); }
+int32_t ib64to512_(const int32_t *iaq) { return own_to_grid(*iaq, 4, 8, 146); }
+int32_t ib64to1000_(const int32_t *iaq) { return own_to_grid(*iaq, 4, 10, 333); }
+int32_t ib8to216_(const int32_t *iaq) { return own_to_grid(*iaq, 2, 6, 86); }
+
+int32_t ib512to64_(const int32_t *iaq)
+{
+    int32_t k = (*iaq - 1) / 64;
+    int32_t m = (*iaq - 64 * k - 1) / 8;
+    return *iaq - 64 * k - 8 * m - 2 + 16 * (k - 2) + 4 * (m - 2);
+}
+
+int32_t isnearest_(const double *x, const double *y, const double *z, const double *cx,
+                   const double *cy, const double *cz)
+{
+    return within(x, y, z, cx, cy, cz, 1.5 * kRS);
+}
+
+int32_t iscentral_(const double *x, const double *y, const double *z, const double *cx,
+                   const double *cy, const double *cz)
+{
+    return within(x, y, z, cx, cy, cz, kRS / 2.0);
+}
 
 }  // extern "C"

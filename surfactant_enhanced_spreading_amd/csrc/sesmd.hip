@@ -144,8 +144,12 @@ struct sesmd_ctx {
     double2 *O4, *L4, *O3, *L3, *O2, *L2;
     int *ne4, *ne3, *ne2;         // nonempty flags per box
     double2 *T4, *T3, *T2;        // M2L operators per offset
-    double *AF4, *AF3;            // the same, real-expanded in MFMA operand order (k_m2l_mfma)
-    double *m2l_part;             // k_m2l_mfma partial sums [tile][split][rows][16]
+    double *AF4, *AF3;            // the same, real-expanded in MFMA operand order (k_m2l_gemm)
+    double *m2l_res;              // M2L results, one row (M2L_RES doubles) per list entry
+    int *m2l_cnt, *m2l_base, *m2l_bbase;  // per (class, candidate) list: entries, first entry, first block
+    int2 *m2l_list;               // list entries (target within its class, source box)
+    int *m2l_idx;                 // per target box and candidate: its result row, -1 none
+    int m2l_max_blocks;           // k_m2l_gemm grid (blocks of the densest possible level)
     double2 *top_part;            // top-level M2L partials per (target, source)
     double2 *M43, *M32, *S23, *S34; // M2M / L2L operators per child
     double *c4;                   // L4 centres (3 per box)
@@ -1754,179 +1758,314 @@ __global__ __launch_bounds__(64 * W) void k_m2l(Geo g, int nd, const double2 *__
     if (k > 0) Lt[jj * (jj + 1) - k] = make_double2(cur.x, -cur.y);
 }
 
-// M2L on the matrix cores.  The targets of one parity share their candidate
-// offsets, so for one offset the translation of 16 targets' sources is one
-// real matrix product: [L_re; L_im] (2 lk rows) += [[T_re, -T_im], [T_im, T_re]]
-// [O_re; O_im] (2 lp columns, interleaved).  A tile is 16 targets of one
-// parity (the N = 16 columns of v_mfma_f64_16x16x4f64); the operator comes
-// pre-arranged in A-operand order (AF), the sources' expansions are the B
-// operand.  Block = (tile, split of the 216 candidates), four wavefronts taking
-// the split's candidates in turn; their sums are added in wave order and
-// stored per split; k_m2l_mfma_sum adds the splits in order (deterministic).
-// Split counts measured best on the bench workload (L4 time flat from 4 to 32).
-constexpr int M2L_SPLIT4 = 9, M2L_SPLIT3 = 27;
+// M2L on the matrix cores (computeLjk3 / computeLjk4).  Every target of one
+// parity class per axis has the same 216 candidate offsets (the children of
+// its parent's neighbours; 189 of them are not near), and for one offset the
+// translation is the same operator for every target:
+//   [L_re; L_im] += [[T_re, -T_im], [T_im, T_re]] . [O_re; O_im]
+// (2 lk output rows, 2 lp input columns, re / im interleaved).  So the work of
+// a level is 8 x 216 matrix products, one per (class, offset), whose columns
+// are the targets of the class that have a source with charge at that offset
+// -- only those: a droplet fills few of the boxes, and a column per target of
+// the class would mostly multiply zeros.
+//
+// Per rebuild, from the occupied boxes (ne): k_m2l_count counts each list's
+// targets, k_m2l_scan gives each list its first position and first block,
+// k_m2l_fill writes the lists (targets in box order) and, per target and
+// candidate, the position of its result (-1: none).  k_m2l_gemm takes up to
+// M2L_TPB x 16 consecutive entries of one list per block: the operator slab is
+// the A operand of v_mfma_f64_16x16x4f64 (2 lk rows padded to MT x 16, 2 lp
+// columns padded to KS x 4), the 16 entries of a tile the N = 16 columns, the
+// sources' expansions the B operand.  Each result column goes to its own row
+// of `res`; k_m2l_accum adds a target's results in candidate order, so the
+// sums do not depend on scheduling.
+constexpr int M2L_TPB = 8;        // tiles of 16 list entries per block
+constexpr int M2L_LISTS = 8 * 216;
+constexpr int M2L_RES = 96;       // doubles per result row (2 lk <= 90, padded)
 typedef double m2l_d4 __attribute__((ext_vector_type(4)));
 
-template <int MT, int KS>
-__global__ __launch_bounds__(256) void k_m2l_mfma(Geo g, int nd, int tiles_per_par, int S,
-                                                  const double *__restrict__ O, const int *__restrict__ ne,
-                                                  const double *__restrict__ AF, double *__restrict__ part)
+// candidate ci (0..215, x fastest over the 6^3 children of the parent's
+// neighbours) of parity class par -> offset; false for the 27 near ones
+__device__ __forceinline__ bool m2l_offset(int par, int ci, int &dx, int &dy, int &dz)
 {
-    __shared__ m2l_d4 s_acc[4][MT][64];
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // consecutive blocks go to the eight XCDs in turn, so block % 8 picks the
-    // parity: each XCD's L2 then holds the operators of one parity's 216
-    // offsets (2.9 MB at p = 4) instead of all 343
-    int par = blockIdx.x & 7, rest = blockIdx.x >> 3;
-    int tl = rest / S, split = rest % S, tile = par * tiles_per_par + tl;
-    int px = par & 1, py = (par >> 1) & 1, pz = par >> 2;
-    int h = nd / 2, n = lane & 15, kq = lane >> 4;
-    int q = 16 * tl + n;
-    bool valid_t = q < h * h * h;
-    int tx = 2 * (q % h) + px, ty = 2 * ((q / h) % h) + py, tz = 2 * (q / (h * h)) + pz;
-    int lp2 = 2 * g.lp;
-    // two accumulators per row block (even / odd k steps): twice the
-    // independent chains for the matrix core
-    m2l_d4 acc[MT][2];
-#pragma unroll
-    for (int mt = 0; mt < MT; mt++) acc[mt][0] = acc[mt][1] = (m2l_d4){0.0, 0.0, 0.0, 0.0};
-    int i1 = (split + 1) * 216 / S;
-    // the wave's candidates: i0 + w, i0 + w + 4, ... less the 27 near offsets (wave-uniform)
-    auto next = [&](int i) {
-        for (; i < i1; i += 4) {
-            int dx = i % 6 - 2 - px, dy = (i / 6) % 6 - 2 - py, dz = i / 36 - 2 - pz;
-            if (!(iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1)) return i;
-        }
-        return i1;
-    };
-    // p = 4: operands of candidate i into set k, the next candidate's loaded
-    // while the current one's products run; larger orders (whose operands
-    // would not fit the registers twice) load per k step instead
-    constexpr bool PIPE = MT * KS <= 32;
-    constexpr int NA = PIPE ? MT * KS : 1, NB = PIPE ? KS : 1;
-    double a[2][NA], b[2][NB];
-    auto source = [&](int i, const double *&src, const double *&af, bool &valid) {
-        int dx = i % 6 - 2 - px, dy = (i / 6) % 6 - 2 - py, dz = i / 36 - 2 - pz;
-        int sx = tx + dx, sy = ty + dy, sz = tz + dz;
-        valid = valid_t && sx >= 0 && sx < nd && sy >= 0 && sy < nd && sz >= 0 && sz < nd;
-        int S_ = valid ? sx + nd * (sy + nd * sz) : 0;
-        valid = valid && ne[S_];
-        src = O + (size_t)S_ * lp2;
-        int off = (dx + 3) + 7 * ((dy + 3) + 7 * (dz + 3));
-        af = AF + (size_t)off * MT * KS * 64 + lane;
-    };
-    auto load = [&](int i, int k) {
-        const double *src, *af;
-        bool valid;
-        source(i, src, af, valid);
-#pragma unroll
-        for (int e = 0; e < MT * KS; e++) a[k][e] = af[e * 64];
-#pragma unroll
-        for (int ks = 0; ks < KS; ks++) {
-            int kk = 4 * ks + kq;
-            double v = src[min(kk, lp2 - 1)];  // always inside the source's row
-            b[k][ks] = (valid && kk < lp2) ? v : 0.0;
-        }
-    };
-    auto product = [&](int k) {
-#pragma unroll
-        for (int ks = 0; ks < KS; ks++)
-#pragma unroll
-            for (int mt = 0; mt < MT; mt++)
-                acc[mt][ks & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[k][mt * KS + ks], b[k][ks],
-                                                                       acc[mt][ks & 1], 0, 0, 0);
-    };
-    if constexpr (PIPE) {
-        int i = next(split * 216 / S + w);
-        if (i < i1) load(i, 0);
-        while (i < i1) {
-            int j = next(i + 4);
-            if (j < i1) load(j, 1);
-            product(0);
-            if (j >= i1) break;
-            i = next(j + 4);
-            if (i < i1) load(i, 0);
-            product(1);
-        }
-    } else {
-        for (int i = next(split * 216 / S + w); i < i1; i = next(i + 4)) {
-            const double *src, *af;
-            bool valid;
-            source(i, src, af, valid);
-            for (int ks = 0; ks < KS; ks++) {
-                int kk = 4 * ks + kq;
-                double bv = src[min(kk, lp2 - 1)];
-                bv = (valid && kk < lp2) ? bv : 0.0;
-#pragma unroll
-                for (int mt = 0; mt < MT; mt++)
-                    acc[mt][ks & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[(mt * KS + ks) * 64], bv,
-                                                                           acc[mt][ks & 1], 0, 0, 0);
-            }
-        }
-    }
-#pragma unroll
-    for (int mt = 0; mt < MT; mt++) {
-        m2l_d4 t = acc[mt][0], u = acc[mt][1];
-        s_acc[w][mt][lane] = (m2l_d4){t.x + u.x, t.y + u.y, t.z + u.z, t.w + u.w};
-    }
+    dx = ci % 6 - 2 - (par & 1);
+    dy = (ci / 6) % 6 - 2 - ((par >> 1) & 1);
+    dz = ci / 36 - 2 - (par >> 2);
+    return !(iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1);
+}
+
+// target q (0..h^3 - 1) of class par -> box coordinates; source box for the
+// offset, or -1 (outside the grid / without charge)
+__device__ __forceinline__ int m2l_src(int nd, const int *__restrict__ ne, int par, int q, int dx, int dy, int dz,
+                                       int *tb)
+{
+    int h = nd / 2;
+    int tx = 2 * (q % h) + (par & 1), ty = 2 * ((q / h) % h) + ((par >> 1) & 1), tz = 2 * (q / (h * h)) + (par >> 2);
+    *tb = tx + nd * (ty + nd * tz);
+    int sx = tx + dx, sy = ty + dy, sz = tz + dz;
+    if (sx < 0 || sx >= nd || sy < 0 || sy >= nd || sz < 0 || sz >= nd) return -1;
+    int si = sx + nd * (sy + nd * sz);
+    return ne[si] ? si : -1;
+}
+
+// One block per list (class, candidate): its target count.
+__global__ __launch_bounds__(256) void k_m2l_count(int nd, const int *__restrict__ ne, int *__restrict__ cnt)
+{
+    int L = blockIdx.x, par = L / 216, ci = L % 216, h = nd / 2, h3 = h * h * h;
+    int dx, dy, dz, tb;
+    bool far = m2l_offset(par, ci, dx, dy, dz);
+    __shared__ int s_c;
+    if (threadIdx.x == 0) s_c = 0;
     __syncthreads();
-    if (w != 0) return;
-    // output element (16 mt + 4 r + lane / 16, lane % 16) of fragment mt, register r
-    double *dst = part + (size_t)(tile * S + split) * MT * 16 * 16;
-#pragma unroll
-    for (int mt = 0; mt < MT; mt++) {
-        m2l_d4 t = s_acc[0][mt][lane];
-        for (int q2 = 1; q2 < 4; q2++) {
-            m2l_d4 u = s_acc[q2][mt][lane];
-            t.x += u.x;
-            t.y += u.y;
-            t.z += u.z;
-            t.w += u.w;
+    int c = 0;
+    if (far)
+        for (int q = threadIdx.x; q < h3; q += blockDim.x) c += m2l_src(nd, ne, par, q, dx, dy, dz, &tb) >= 0;
+    if (c) atomicAdd(&s_c, c);  // integer: the total does not depend on order
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[L] = s_c;
+}
+
+// One block: first position (base) and first block (bbase) of every list;
+// bbase[M2L_LISTS] = blocks of the level.
+__global__ __launch_bounds__(1024) void k_m2l_scan(const int *__restrict__ cnt, int *__restrict__ base,
+                                                   int *__restrict__ bbase)
+{
+    __shared__ int s_a[1024], s_b[1024];
+    constexpr int PER = (M2L_LISTS + 1023) / 1024;
+    int t = threadIdx.x, a = 0, bsum = 0;
+    for (int e = 0; e < PER; e++) {
+        int L = t * PER + e;
+        if (L < M2L_LISTS) {
+            a += cnt[L];
+            bsum += (cnt[L] + M2L_TPB * 16 - 1) / (M2L_TPB * 16);
         }
-        int row = 16 * mt + kq;
-        dst[(row + 0) * 16 + n] = t.x;
-        dst[(row + 4) * 16 + n] = t.y;
-        dst[(row + 8) * 16 + n] = t.z;
-        dst[(row + 12) * 16 + n] = t.w;
+    }
+    s_a[t] = a;
+    s_b[t] = bsum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d *= 2) {  // inclusive scan of the per-thread sums
+        int va = t >= d ? s_a[t - d] : 0, vb = t >= d ? s_b[t - d] : 0;
+        __syncthreads();
+        s_a[t] += va;
+        s_b[t] += vb;
+        __syncthreads();
+    }
+    int ra = s_a[t] - a, rb = s_b[t] - bsum;  // exclusive
+    for (int e = 0; e < PER; e++) {
+        int L = t * PER + e;
+        if (L < M2L_LISTS) {
+            base[L] = ra;
+            bbase[L] = rb;
+            ra += cnt[L];
+            rb += (cnt[L] + M2L_TPB * 16 - 1) / (M2L_TPB * 16);
+        }
+    }
+    if (t == 1023) bbase[M2L_LISTS] = s_b[1023];
+}
+
+// One block per list: its entries (target q, source box) in q order, and for
+// every target of the class the position of its result for this candidate
+// (idx[target box][ci], -1 none).
+__global__ __launch_bounds__(256) void k_m2l_fill(int nd, const int *__restrict__ ne, const int *__restrict__ base,
+                                                  int2 *__restrict__ list, int *__restrict__ idx)
+{
+    int L = blockIdx.x, par = L / 216, ci = L % 216, h = nd / 2, h3 = h * h * h;
+    int dx, dy, dz;
+    bool far = m2l_offset(par, ci, dx, dy, dz);
+    __shared__ int s_w[4];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int run = base[L];
+    for (int q0 = 0; q0 < h3; q0 += 256) {
+        int q = q0 + (int)threadIdx.x, tb = -1, si = -1;
+        if (q < h3) {
+            si = m2l_src(nd, ne, par, q, dx, dy, dz, &tb);
+            if (!far) si = -1;
+        }
+        unsigned long long m = __builtin_amdgcn_ballot_w64(si >= 0);
+        int before = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (lane == 0) s_w[w] = __builtin_popcountll(m);
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int k = 0; k < 4; k++) {
+            off += k < w ? s_w[k] : 0;
+            tot += s_w[k];
+        }
+        int pos = run + off + before;
+        if (q < h3) {
+            if (si >= 0) list[pos] = make_int2(q, si);
+            idx[(size_t)tb * 216 + ci] = si >= 0 ? pos : -1;
+        }
+        run += tot;
+        __syncthreads();  // s_w is rewritten by the next round
     }
 }
 
-// Adds the splits of k_m2l_mfma in order onto L (the L2L part already there):
-// one thread per (tile, column, k >= 0 row).
-__global__ void k_m2l_mfma_sum(Geo g, int nd, int tiles_per_par, int S, int MT, int ntiles,
-                               const double *__restrict__ part, double2 *__restrict__ L)
+// One block per M2L_TPB x 16 consecutive entries of one list.  MT x WG
+// wavefronts: wave w computes the 16-row slab w % MT of the operator times the
+// expansions of tiles (w / MT) TPB / WG .. + TPB / WG - 1, so the block
+// computes the (MT 16) x (TPB 16) product as slabs side by side.  A wave reads
+// only its own slab of the operator, straight from global memory into
+// registers (one fragment per k step serves TPB / WG tiles); the expansions --
+// the B operand the waves share -- go through LDS in chunks of C k steps,
+// double-buffered, loaded two chunks ahead: while chunk s's products run, chunk
+// s + 1's data goes into the other buffer and chunk s + 2's loads are in flight.
+template <int MT, int KS, int TPB, int WG>
+__global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__restrict__ cnt,
+                                                           const int *__restrict__ base,
+                                                           const int *__restrict__ bbase,
+                                                           const int2 *__restrict__ list,
+                                                           const double *__restrict__ O,
+                                                           const double *__restrict__ AF,
+                                                           double *__restrict__ res)
+{
+    constexpr int NT = 64 * MT * WG;            // threads
+    constexpr int TW = TPB / WG;                // tiles per wave
+    constexpr int NCH = (KS + 7) / 8;           // chunks
+    constexpr int C = (KS + NCH - 1) / NCH;     // k steps per chunk
+    constexpr int BCH = C * TPB * 64;           // doubles of one chunk of expansions
+    constexpr int PER = (BCH + NT - 1) / NT;    // chunk loads per thread
+    static_assert(TPB % WG == 0, "tiles per wave");
+    __shared__ double s_b[2][BCH];
+    __shared__ int s_src[TPB * 16];
+    int B = blockIdx.x;
+    if (B >= bbase[M2L_LISTS]) return;
+    // the list of block B: the last list whose first block is <= B
+    int lo = 0, hi = M2L_LISTS - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (bbase[mid] <= B) lo = mid;
+        else hi = mid - 1;
+    }
+    int L = lo, par = L / 216, ci = L % 216;
+    int e0 = (B - bbase[L]) * TPB * 16, ne_ = min(TPB * 16, cnt[L] - e0);
+    int first = base[L] + e0;
+    int dx, dy, dz;
+    m2l_offset(par, ci, dx, dy, dz);
+    int off = (dx + 3) + 7 * ((dy + 3) + 7 * (dz + 3));  // operator index in [-3, 3]^3
+    for (int j = threadIdx.x; j < TPB * 16; j += NT) s_src[j] = j < ne_ ? list[first + j].y : -1;
+    __syncthreads();
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int mt = w % MT, tw0 = (w / MT) * TW;
+    int lp2 = 2 * g.lp;
+    int ntile = (ne_ + 15) / 16;  // tiles with entries (block-uniform)
+    m2l_d4 acc[TW];
+#pragma unroll
+    for (int t = 0; t < TW; t++) acc[t] = (m2l_d4){0.0, 0.0, 0.0, 0.0};
+    struct Stage {
+        double a[C];    // this wave's operator fragments, one per k step
+        double b[PER];  // this thread's share of the chunk's expansions
+    };
+    Stage P0, P1;
+    double acur[C];
+    const double *abase = AF + (size_t)off * KS * MT * 64 + mt * 64 + lane;
+    // element idx = thread + NT e of a chunk: k step idx / (64 TPB), tile
+    // (idx / 64) % TPB, lane idx % 64 -- expansion entry 4 k + lane / 16 of the
+    // source of entry 16 tile + lane % 16.  Every load is in bounds (missing
+    // entries read box 0 and are replaced by 0), so none waits on a branch.
+    auto load = [&](int ch, Stage &P) {
+        int cs = min(C, KS - ch * C);
+        const double *src = abase + (size_t)ch * C * MT * 64;
+#pragma unroll
+        for (int kc = 0; kc < C; kc++) {
+            double v = src[(size_t)kc * MT * 64];  // the operator table carries one spare offset
+            P.a[kc] = kc < cs ? v : 0.0;
+        }
+        int si[PER];
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            int idx = (int)threadIdx.x + NT * e;
+            si[e] = idx < BCH ? s_src[16 * ((idx / 64) % TPB) + idx % 16] : -1;
+        }
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            int idx = (int)threadIdx.x + NT * e;
+            int kk = 4 * (ch * C + idx / (TPB * 64)) + (idx % 64) / 16;
+            bool ok = si[e] >= 0 && kk < lp2;
+            double v = O[(size_t)max(si[e], 0) * lp2 + min(kk, lp2 - 1)];
+            P.b[e] = ok ? v : 0.0;
+        }
+    };
+    auto stage_in = [&](int buf, const Stage &P) {
+#pragma unroll
+        for (int e = 0; e < PER; e++) {
+            int idx = (int)threadIdx.x + NT * e;
+            if (idx < BCH) s_b[buf][idx] = P.b[e];
+        }
+#pragma unroll
+        for (int kc = 0; kc < C; kc++) acur[kc] = P.a[kc];
+    };
+    auto compute = [&](int ch) {
+        int cs = min(C, KS - ch * C);
+        // the B operands of k step kc + 1 are read from LDS while the
+        // products of step kc run (TW independent accumulator chains)
+        const double *sb = &s_b[ch & 1][tw0 * 64 + lane];
+        double bq[2][TW];
+#pragma unroll
+        for (int t = 0; t < TW; t++) bq[0][t] = sb[t * 64];
+#pragma unroll
+        for (int kc = 0; kc < C; kc++) {
+            if (kc >= cs) break;
+            if (kc + 1 < cs)
+#pragma unroll
+                for (int t = 0; t < TW; t++) bq[(kc + 1) & 1][t] = sb[((kc + 1) * TPB + t) * 64];
+#pragma unroll
+            for (int t = 0; t < TW; t++)
+                if (tw0 + t < ntile)  // block-uniform
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[kc], bq[kc & 1][t], acc[t], 0, 0, 0);
+        }
+    };
+    auto step = [&](int ch, Stage &ahead, const Stage &ready) {
+        if (ch + 2 < NCH) load(ch + 2, ahead);
+        compute(ch);
+        if (ch + 1 < NCH) stage_in((ch + 1) & 1, ready);
+        __syncthreads();
+    };
+    load(0, P0);
+    stage_in(0, P0);
+    if (NCH > 1) load(1, P1);
+    __syncthreads();
+#pragma unroll 1
+    for (int ch = 0; ch < NCH; ch += 2) {
+        step(ch, P0, P1);
+        if (ch + 1 < NCH) step(ch + 1, P1, P0);
+    }
+    // output element (16 mt + 4 r + lane / 16, lane % 16) of the wave's slab,
+    // register r: row 16 mt + 4 r + lane / 16 of entry 16 tile + lane % 16
+    int n = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < TW; t++) {
+        int j = 16 * (tw0 + t) + n;
+        if (j >= ne_) continue;
+        double *dst = res + (size_t)(first + j) * M2L_RES + 16 * mt + kq;
+        m2l_d4 v = acc[t];
+        dst[0] = v.x;
+        dst[4] = v.y;
+        dst[8] = v.z;
+        dst[12] = v.w;
+    }
+}
+
+// Adds each target's results in candidate order onto L (the L2L part already
+// there): one thread per (target, k >= 0 row).
+__global__ void k_m2l_accum(Geo g, int nd, const int *__restrict__ idx, const double *__restrict__ res,
+                            double2 *__restrict__ L)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int lk = g.lk;
-    if (t >= ntiles * 16 * lk) return;
-    int row = t % lk, n = (t / lk) % 16, tile = t / (lk * 16);
-    int par = tile / tiles_per_par, tl = tile % tiles_per_par;
-    int h = nd / 2, q = 16 * tl + n;
-    if (q >= h * h * h) return;
-    int tx = 2 * (q % h) + (par & 1), ty = 2 * ((q / h) % h) + ((par >> 1) & 1),
-        tz = 2 * (q / (h * h)) + (par >> 2);
-    // the splits in order, the loads of a chunk of 9 issued together
+    if (t >= nd * nd * nd * lk) return;
+    int row = t % lk, b = t / lk;
+    const int *ix = idx + (size_t)b * 216;
     double re = 0.0, im = 0.0;
-    for (int s0 = 0; s0 < S; s0 += 9) {
-        double vr[9], vi[9];
-#pragma unroll
-        for (int q2 = 0; q2 < 9; q2++)
-            if (s0 + q2 < S) {
-                const double *P = part + (size_t)(tile * S + s0 + q2) * MT * 16 * 16;
-                vr[q2] = P[(2 * row) * 16 + n];
-                vi[q2] = P[(2 * row + 1) * 16 + n];
-            }
-#pragma unroll
-        for (int q2 = 0; q2 < 9; q2++)
-            if (s0 + q2 < S) {
-                re += vr[q2];
-                im += vi[q2];
-            }
+    for (int ci = 0; ci < 216; ci++) {
+        int pos = ix[ci];
+        if (pos < 0) continue;
+        re += res[(size_t)pos * M2L_RES + 2 * row];
+        im += res[(size_t)pos * M2L_RES + 2 * row + 1];
     }
     int j = 0;
     while ((j + 1) * (j + 2) / 2 <= row) j++;
     int k = row - j * (j + 1) / 2;
-    double2 *Lt = L + (size_t)(tx + nd * (ty + nd * tz)) * g.lp;
+    double2 *Lt = L + (size_t)b * g.lp;
     double2 cur = Lt[j * (j + 1) + k];
     cur.x += re;
     cur.y += im;
@@ -3354,13 +3493,15 @@ static int upload_ops(sesmd_ctx *c)
     };
     // MFMA operand order of a real-expanded operator: rows 2r + {re, im} of
     // the output, columns 2c + {re, im} of the input, padded to MT x 16 rows
-    // and KS x 4 columns; fragment (mt, ks), lane l holds element
-    // (16 mt + l % 16, 4 ks + l / 16) -- the A operand of v_mfma_f64_16x16x4f64.
+    // and KS x 4 columns; fragments ordered (ks, mt), so the fragments of
+    // consecutive k steps are contiguous; lane l of fragment (ks, mt) holds
+    // element (16 mt + l % 16, 4 ks + l / 16) -- the A operand of
+    // v_mfma_f64_16x16x4f64.
     const int MT = (2 * lk + 15) / 16, KS = (2 * lp + 3) / 4;
     std::vector<double> af;
     auto append_frag = [&]() {
-        for (int mt = 0; mt < MT; mt++)
-            for (int ks = 0; ks < KS; ks++)
+        for (int ks = 0; ks < KS; ks++)
+            for (int mt = 0; mt < MT; mt++)
                 for (int l = 0; l < 64; l++) {
                     int mm = 16 * mt + l % 16, kk = 4 * ks + l / 16;
                     double v = 0.0;
@@ -3389,6 +3530,7 @@ static int upload_ops(sesmd_ctx *c)
                     append_frag();
                 }
         CK(put(lvl == 0 ? c->T4 : c->T3, buf.size()));
+        af.resize(af.size() + (size_t)MT * KS * 64, 0.0);  // the spare offset
         HC(hipMemcpy(lvl == 0 ? c->AF4 : c->AF3, af.data(), sizeof(double) * af.size(),
                      hipMemcpyHostToDevice));
     }
@@ -3556,11 +3698,20 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->T4, (size_t)343 * g.lk * g.lp));
     {
         size_t mt = (2 * g.lk + 15) / 16, ks = (2 * g.lp + 3) / 4;
-        A(dalloc(&c->AF4, (size_t)343 * mt * ks * 64));
-        A(dalloc(&c->AF3, (size_t)343 * mt * ks * 64));
-        size_t h4 = g.nb / 2, h3 = g.n3 / 2;
-        size_t t4 = 8 * ((h4 * h4 * h4 + 15) / 16), t3 = 8 * ((h3 * h3 * h3 + 15) / 16);
-        A(dalloc(&c->m2l_part, std::max(t4 * M2L_SPLIT4, t3 * M2L_SPLIT3) * mt * 16 * 16));
+        // one spare offset of zeros at the end: k_m2l_gemm's operator loads
+        // run unconditionally up to a chunk past an offset's last k step
+        A(dalloc(&c->AF4, (size_t)344 * mt * ks * 64));
+        A(dalloc(&c->AF3, (size_t)344 * mt * ks * 64));
+        // M2L lists: every (target, candidate) pair of the L4 level at most
+        // (8 classes x 216 candidates x (nb/2)^3 targets), the L3 level less
+        size_t pairs = (size_t)216 * g.nb3;
+        A(dalloc(&c->m2l_res, pairs * M2L_RES));
+        A(dalloc(&c->m2l_list, pairs));
+        A(dalloc(&c->m2l_idx, pairs));
+        A(dalloc(&c->m2l_cnt, M2L_LISTS));
+        A(dalloc(&c->m2l_base, M2L_LISTS));
+        A(dalloc(&c->m2l_bbase, M2L_LISTS + 1));
+        c->m2l_max_blocks = (int)(pairs / (M2L_TPB * 16)) + M2L_LISTS;
     }
     A(dalloc(&c->T3, (size_t)343 * g.lk * g.lp));
     int w2 = 2 * nb2 - 1;
@@ -3666,7 +3817,8 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
                     c->wkey_out, c->widx_in, c->widx_out, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
-                    c->AF4, c->AF3, c->m2l_part,
+                    c->AF4, c->AF3, c->m2l_res, c->m2l_list, c->m2l_idx, c->m2l_cnt, c->m2l_base,
+                    c->m2l_bbase,
                     c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -3813,19 +3965,20 @@ static void launch_m2l(sesmd_ctx *c, int nd, const double2 *O, const int *ne, co
 {
     const Geo &g = c->g;
     int h = nd / 2;
-    int tpp = (h * h * h + 15) / 16, ntiles = 8 * tpp;
-    bool l4 = nd == g.nb;
-    int S = l4 ? M2L_SPLIT4 : M2L_SPLIT3;
-    const double *AF = l4 ? c->AF4 : c->AF3;
-    const double *Od = (const double *)O;
     if (g.p == 4 || g.p == 8) {
-        if (g.p == 4)
-            k_m2l_mfma<2, 13><<<ntiles * S, 256, 0, c->stream>>>(g, nd, tpp, S, Od, ne, AF, c->m2l_part);
-        else
-            k_m2l_mfma<6, 41><<<ntiles * S, 256, 0, c->stream>>>(g, nd, tpp, S, Od, ne, AF, c->m2l_part);
-        int MT = (2 * g.lk + 15) / 16;
-        int nthr = ntiles * 16 * g.lk;
-        k_m2l_mfma_sum<<<nblk(nthr, 256), 256, 0, c->stream>>>(g, nd, tpp, S, MT, ntiles, c->m2l_part, L);
+        hipStream_t st = c->stream;
+        const double *AF = nd == g.nb ? c->AF4 : c->AF3;
+        const double *Od = (const double *)O;
+        k_m2l_count<<<M2L_LISTS, 256, 0, st>>>(nd, ne, c->m2l_cnt);
+        k_m2l_scan<<<1, 1024, 0, st>>>(c->m2l_cnt, c->m2l_base, c->m2l_bbase);
+        k_m2l_fill<<<M2L_LISTS, 256, 0, st>>>(nd, ne, c->m2l_base, c->m2l_list, c->m2l_idx);
+        if (g.p == 4)  // two slabs of rows: two tile halves per block, four waves
+            k_m2l_gemm<2, 13, M2L_TPB, 2><<<c->m2l_max_blocks, 256, 0, st>>>(
+                g, c->m2l_cnt, c->m2l_base, c->m2l_bbase, c->m2l_list, Od, AF, c->m2l_res);
+        else  // six slabs: six waves
+            k_m2l_gemm<6, 41, M2L_TPB, 1><<<c->m2l_max_blocks, 384, 0, st>>>(
+                g, c->m2l_cnt, c->m2l_base, c->m2l_bbase, c->m2l_list, Od, AF, c->m2l_res);
+        k_m2l_accum<<<nblk((size_t)nd * nd * nd * g.lk, 256), 256, 0, st>>>(g, nd, c->m2l_idx, c->m2l_res, L);
         return;
     }
     if (g.lk <= 16) {
@@ -4126,21 +4279,43 @@ static int capture_steps(sesmd_ctx *c, int nsteps, hipGraphExec_t *out, bool reb
     return 0;
 }
 
+// Captures (without running) every graph sesmd_steps replays for this ifar:
+// the whole cycle, the run of ifar - 1 regular steps and the single regular
+// step.  Capture and instantiation cost milliseconds, so they happen once, at
+// the first sesmd_steps call for an ifar, before any of its steps run.
+static int ensure_graphs(sesmd_ctx *c, int ifar)
+{
+    int run = ifar - 1;
+    if (c->graph_cycle_len != ifar) {
+        if (c->graph_cycle_len) HC(hipGraphExecDestroy(c->graph_cycle));
+        c->graph_cycle_len = 0;
+        CK(capture_steps(c, ifar, &c->graph_cycle, true));
+        c->graph_cycle_len = ifar;
+    }
+    if (run > 1 && c->graph_run_len != run) {
+        if (c->graph_run_len) HC(hipGraphExecDestroy(c->graph_run));
+        c->graph_run_len = 0;
+        CK(capture_steps(c, run, &c->graph_run));
+        c->graph_run_len = run;
+    }
+    if (!c->graph_ok) {
+        CK(capture_steps(c, 1, &c->graph));
+        c->graph_ok = true;
+    }
+    return 0;
+}
+
 extern "C" int sesmd_steps(sesmd_ctx *c, int nsteps, int ifar, int *kb)
 {
     if (!c || !kb) return fail("null argument");
     if (!c->built) return fail("sesmd_steps before sesmd_rebuild");
     if (ifar < 1) return fail("ifar must be >= 1");
+    if (nsteps < 0) return fail("nsteps must be >= 0");
     int run = ifar - 1;  // regular steps between two rebuild steps
+    if (nsteps > 0) CK(ensure_graphs(c, ifar));
     for (int s = 0; s < nsteps;) {
         if ((*kb + 1) % ifar == 0 && nsteps - s >= ifar) {
             // a whole cycle, the rebuild step and its run of regular steps, as one graph
-            if (c->graph_cycle_len != ifar) {
-                if (c->graph_cycle_len) HC(hipGraphExecDestroy(c->graph_cycle));
-                c->graph_cycle_len = 0;
-                CK(capture_steps(c, ifar, &c->graph_cycle, true));
-                c->graph_cycle_len = ifar;
-            }
             HC(hipGraphLaunch(c->graph_cycle, c->stream));
             c->boxlist = false;  // the replayed rebuild made a new box sort
             *kb += ifar;
@@ -4308,9 +4483,9 @@ extern "C" int sesmd_set_expansions(sesmd_ctx *c, const double *O4, const double
     return 0;
 }
 
-extern "C" int sesmd_fmm_stage(sesmd_ctx *c, int stage)
+// Enqueues one far-field stage on the context's stream without waiting.
+static int enqueue_fmm_stage(sesmd_ctx *c, int stage)
 {
-    if (!c) return fail("null context");
     const Geo &g = c->g;
     hipStream_t st = c->stream;
     switch (stage) {
@@ -4336,8 +4511,21 @@ extern "C" int sesmd_fmm_stage(sesmd_ctx *c, int stage)
     default: return fail("unknown far-field stage %d", stage);
     }
     HC(hipGetLastError());
+    return 0;
+}
+
+extern "C" int sesmd_fmm_stage(sesmd_ctx *c, int stage)
+{
+    if (!c) return fail("null context");
+    CK(enqueue_fmm_stage(c, stage));
     HC(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+extern "C" int sesmd_fmm_stage_enqueue(sesmd_ctx *c, int stage)
+{
+    if (!c) return fail("null context");
+    return enqueue_fmm_stage(c, stage);
 }
 
 extern "C" int sesmd_move_species(sesmd_ctx *c, int part, unsigned cls_mask, double *sums18)

@@ -24,6 +24,7 @@ NEAR, FAR, INTERNAL, SUBTRACT, VALENCE, DIHEDRAL, DIHEDRAL_BE, WALL, GRAPHITE = 
 ALL = 511
 ALL_MASK = (1 << 64) - 1
 FP64, MIXED = 0, 1
+P2M, M2M_43, M2M_32, M2L_2, L2L_23, M2L_3, L2L_34, M2L_4 = range(8)
 
 
 class ForceField(ctypes.Structure):
@@ -113,6 +114,8 @@ def lib():
         L.sesmd_stream.argtypes = [vp]
         L.sesmd_stream.restype = vp
         L.sesmd_near_only.argtypes = [vp]
+        L.sesmd_fmm_stage.argtypes = [vp, ctypes.c_int]
+        L.sesmd_fmm_stage_enqueue.argtypes = [vp, ctypes.c_int]
         L.sesmd_pair_counts.argtypes = [vp, dp, dp]
         L.sesmd_energies.argtypes = [vp, dp]
         _lib = L
@@ -276,6 +279,14 @@ class Engine:
 
     def near_only(self):
         _ck(self.L.sesmd_near_only(self.h))
+
+    def fmm_stage(self, stage, wait=True):
+        """One far-field stage (SESMD_P2M .. SESMD_M2L_4); wait=False only
+        enqueues it on the context's stream."""
+        if wait:
+            _ck(self.L.sesmd_fmm_stage(self.h, stage))
+        else:
+            _ck(self.L.sesmd_fmm_stage_enqueue(self.h, stage))
 
     def temperature(self):
         """Kinetic temperature sum m v^2 / dt^2 / df (src/compute_temperature.f90:240-244)."""

@@ -1,0 +1,121 @@
+"""The HIP engine against the oracle on the benchmarked systems at full size.
+
+Two workloads, the ones bench.py times:
+  c2  -- 20,000 waters + 200 surfactants (65,800 sites) in the 16^3-box domain
+         (nb2 = 4, 4,096 L4 boxes), expansion order p = 8;
+  ref -- the same droplet in the reference's geometry (12^3 boxes, p = 4).
+
+Bar as in test_gpu_parity.py: classes, storage boxes, central boxes, receiver
+kinds and subtraction masks bit-exact; expansions 1e-12 of their largest
+coefficient; per-site relative force error (floored at 1e-3 of the RMS force)
+<= 1e-10 for the total force and each component family.  Fourteen
+graph-replayed steps with two rebuilds: positions within 1e-12 of their
+largest coordinate, velocities and accelerations within a per-site 1e-9 --
+fourteen force evaluations at the 1e-10 bar, each fed through SHAKE/RATTLE,
+over 65,800 sites (measured: worst site 1.4e-10 in the reference geometry,
+positions 3e-13 apart).
+
+All checks run on the benchmarked configuration with every site displaced by
+~0.04 A (conftest._thermal).  The builder places each surfactant as an
+all-trans chain at its angle equilibria: there the dihedral cosines sit at the
+clamp of +-(1 - 1e-11) and the angle forces k (theta - theta0) / sin(theta) are
+differences of equal numbers, so last-bit differences between the host's and
+the device's acos reach ~1e-4 of a site's force (measured: bootstrap
+accelerations 7e-5 apart on the undisplaced configuration, the same at p = 4
+and p = 8 and in both domains).  A thermal displacement takes every term off
+those singular points, as in the smaller parity systems.
+"""
+import numpy as np
+import pytest
+
+from conftest import _thermal, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+STEP_TOL = 1e-9
+CASES = {'c2': (4, 8), 'ref': (3, 4)}
+
+
+def _system(name):
+    from surfactant_enhanced_spreading_amd import builder
+    return builder.config1_system() if name == 'c2' else builder.bench_system()
+
+
+@pytest.fixture(scope='module', params=sorted(CASES))
+def full(request):
+    import oracle as orc
+    from surfactant_enhanced_spreading_amd import engine
+    nb2, p = CASES[request.param]
+    s = _thermal(_system(request.param), 11)
+    print('full-size system', request.param, s.n_sites, 'sites, nb2', nb2, 'p', p, flush=True)
+    o = orc.Oracle(s, nb2=nb2, p=p)
+    o.rebuild()
+    o.fmm()
+    e = engine.Engine(s, nb2=nb2, p=p)
+    e.rebuild()
+    e.fmm()
+    yield request.param, s, o, e
+    e.close()
+
+
+def test_full_rebuild_bit_exact(full):
+    name, s, o, e = full
+    a, b = o.classes(), e.classes()
+    for k in ('cls', 'anc', 'cbox', 'rkind', 'isub'):
+        assert np.array_equal(a[k], b[k]), (name, k)
+    # every molecule filed once; W1 and W2 both occur in a droplet this size
+    assert set(np.unique(a['cls'])) == {0, 1, 2}
+
+
+def test_full_expansions(full):
+    name, s, o, e = full
+    A, B = o.expansions(), e.expansions()
+    for k in ('O4', 'O3', 'O2', 'L2', 'L3', 'L4'):
+        scale = np.abs(A[k]).max()
+        assert scale > 0, (name, k)
+        err = np.abs(A[k] - B[k]).max() / scale
+        assert err <= 1e-12, (name, k, err)
+
+
+@pytest.mark.parametrize('what', ['near', 'far', 'bonded', 'all'])
+def test_full_forces(full, what):
+    name, s, o, e = full
+    bit = {'near': 1, 'far': 2, 'bonded': 4 | 8 | 16 | 32 | 64 | 128 | 256, 'all': 511}[what]
+    fo = o.forces(bit)
+    fe = e.forces(bit)
+    err = rel_err(fe, fo)
+    print(name, what, 'rel err', err, flush=True)
+    assert np.abs(fo).max() > 0
+    assert err <= TOL, (name, what, err)
+
+
+def test_full_steps(full):
+    """Steps 9..22 through sesmd_steps (a one-step graph, a whole cycle graph
+    with its rebuild and far field, an eager rebuild step, two more one-step
+    graphs) against the oracle's own step loop."""
+    import oracle as orc
+    from surfactant_enhanced_spreading_amd import engine
+    name, s, _, _ = full
+    nb2, p = CASES[name]
+    o = orc.Oracle(s, nb2=nb2, p=p)
+    o.rebuild()
+    o.fmm()
+    o.init_accel(o.forces())
+    e = engine.Engine(s, nb2=nb2, p=p)
+    try:
+        e.bootstrap()
+        ko = o.steps(14, 10, 8)
+        ke = e.steps(14, 10, 8)
+        assert ko == ke == 22
+        ro, vo, ao, eo = o.state()
+        re_, ve, ae, ee = e.state()
+        print(name, 'r', np.abs(re_ - ro).max(), 'v', rel_err(ve, vo), 'a', rel_err(ae, ao),
+              'eta', ee, eo, flush=True)
+        assert np.abs(re_ - ro).max() <= 1e-12 * np.abs(ro).max()
+        assert rel_err(ve, vo) <= STEP_TOL
+        assert rel_err(ae, ao) <= STEP_TOL
+        assert abs(ee - eo) <= 1e-10 * abs(eo)
+        assert e.shake_failures() == o.shake_failures()
+    finally:
+        e.close()
