@@ -2046,22 +2046,30 @@ __global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__r
 }
 
 // Adds each target's results in candidate order onto L (the L2L part already
-// there): one thread per (target, k >= 0 row).
-__global__ void k_m2l_accum(Geo g, int nd, const int *__restrict__ idx, const double *__restrict__ res,
-                            double2 *__restrict__ L)
+// there): one wavefront per target box, lane r its k >= 0 row r.  The 216
+// result positions of the target are loaded at once (four per lane) and
+// taken lane by lane; each result row is read by consecutive lanes.
+__global__ __launch_bounds__(64) void k_m2l_accum(Geo g, int nd, const int *__restrict__ idx,
+                                                  const double *__restrict__ res, double2 *__restrict__ L)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int lk = g.lk;
-    if (t >= nd * nd * nd * lk) return;
-    int row = t % lk, b = t / lk;
+    int b = blockIdx.x, row = threadIdx.x, lk = g.lk;
     const int *ix = idx + (size_t)b * 216;
+    int pv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) pv[u] = 64 * u + row < 216 ? ix[64 * u + row] : -1;
+    int rr = min(row, lk - 1);
     double re = 0.0, im = 0.0;
-    for (int ci = 0; ci < 216; ci++) {
-        int pos = ix[ci];
-        if (pos < 0) continue;
-        re += res[(size_t)pos * M2L_RES + 2 * row];
-        im += res[(size_t)pos * M2L_RES + 2 * row + 1];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        for (int l = 0; l < 64 && 64 * u + l < 216; l++) {
+            int pos = __builtin_amdgcn_readlane(pv[u], l);
+            if (pos < 0) continue;  // wave-uniform
+            const double *r = res + (size_t)pos * M2L_RES + 2 * rr;
+            re += r[0];
+            im += r[1];
+        }
     }
+    if (row >= lk) return;
     int j = 0;
     while ((j + 1) * (j + 2) / 2 <= row) j++;
     int k = row - j * (j + 1) / 2;
@@ -3978,7 +3986,7 @@ static void launch_m2l(sesmd_ctx *c, int nd, const double2 *O, const int *ne, co
         else  // six slabs: six waves
             k_m2l_gemm<6, 41, M2L_TPB, 1><<<c->m2l_max_blocks, 384, 0, st>>>(
                 g, c->m2l_cnt, c->m2l_base, c->m2l_bbase, c->m2l_list, Od, AF, c->m2l_res);
-        k_m2l_accum<<<nblk((size_t)nd * nd * nd * g.lk, 256), 256, 0, st>>>(g, nd, c->m2l_idx, c->m2l_res, L);
+        k_m2l_accum<<<nd * nd * nd, 64, 0, st>>>(g, nd, c->m2l_idx, c->m2l_res, L);
         return;
     }
     if (g.lk <= 16) {
