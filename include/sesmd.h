@@ -141,6 +141,17 @@ int sesmd_near_only(sesmd_ctx *ctx);
 /* pair count of the current near field (unordered intermolecular pairs in the
  * 3^3 blocks, and those within rc with an LJ term), for roofline accounting */
 int sesmd_pair_counts(sesmd_ctx *ctx, double *pairs, double *lj_pairs);
+/* The same counts without allocating anything.  No counterpart in the
+ * reference: this is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  counts[0] = pairs,
+ * counts[1] = lj_pairs, equal to sesmd_pair_counts's; counts must be memory
+ * the GPU can write (device or pinned host memory).  The work is only
+ * enqueued on sesmd_stream(ctx): synchronise that stream before reading.
+ * The partial sums live in a static device array of 1 MiB that each GPU holds
+ * until the process ends, which limits the system to 8,388,608 sites (larger
+ * ones are rejected); calls that overlap on one GPU share that array and
+ * race. */
+int sesmd_pair_counts_noalloc(sesmd_ctx *ctx, double *counts);
 
 /* ---- finer-grained calls, used by the Fortran drop-in shim (sesmd_fortran.h)
  * to run one reference routine at a time ---- */

@@ -25,6 +25,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -108,6 +109,7 @@ struct SurfConst {
 
 struct sesmd_ctx {
     Geo g;
+    int device;            // the HIP device the context was created on
     int precision;
     sesmd_forcefield ff;
     hipStream_t stream;    // every launch of the context, in order
@@ -1460,28 +1462,28 @@ __global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
 }
 
 
-// pair counts for roofline accounting (unordered pairs, counted from both sides / 2)
-__global__ void k_pair_count(Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
-                             const int *__restrict__ bstart, const int *__restrict__ rkind,
-                             const sesmd_forcefield *__restrict__ ff,
-                             unsigned long long *__restrict__ cnt)
+// Pairs of sorted site k for roofline accounting: np = intermolecular
+// sources in the 3^3 block that some reference list holds (the near field's
+// pair set), nl = those within rc with an LJ term.  Each unordered pair is
+// counted from both sides.
+__device__ void site_pair_count(const Geo &g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
+                                const int *__restrict__ bstart, const int *__restrict__ rkind,
+                                const sesmd_forcefield *__restrict__ ff, int k, unsigned long long &np,
+                                unsigned long long &nl)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= g.n) return;
+    np = nl = 0;
     int4 me = smeta[k];
     if (rkind[me.x] < 0) return;
     double4 pi = spos[k];
-    int b = me.x >= 0 ? 0 : 0;
-    // find the box of sorted position k
+    // the box of sorted position k
     int lo = 0, hi = g.nb3;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         if (bstart[mid] <= k) lo = mid;
         else hi = mid - 1;
     }
-    b = lo;
+    int b = lo;
     int nb = g.nb, bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
-    unsigned long long np = 0, nl = 0;
     for (int dz = -1; dz <= 1; dz++)
         for (int dy = -1; dy <= 1; dy++)
             for (int dx = -1; dx <= 1; dx++) {
@@ -1501,8 +1503,87 @@ __global__ void k_pair_count(Geo g, const double4 *__restrict__ spos, const int4
                     if (r2 <= g.rc2 && ff->lj_c[(me.y & 255) * 32 + (mj.y & 255)] != 0.0) nl++;
                 }
             }
-    atomicAdd(&cnt[0], np);
-    atomicAdd(&cnt[1], nl);
+}
+
+// pair counts for roofline accounting (unordered pairs, counted from both sides / 2)
+__global__ void k_pair_count(Geo g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
+                             const int *__restrict__ bstart, const int *__restrict__ rkind,
+                             const sesmd_forcefield *__restrict__ ff,
+                             unsigned long long *__restrict__ cnt)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    unsigned long long np, nl;
+    site_pair_count(g, spos, smeta, bstart, rkind, ff, k, np, nl);
+    if (np) atomicAdd(&cnt[0], np);
+    if (nl) atomicAdd(&cnt[1], nl);
+}
+
+// ------------------------------------------------ allocation-free pair counts
+// sesmd_pair_counts_noalloc keeps its partial sums in `scratch`, a file-scope
+// device array that every GPU the library runs on holds until the process
+// ends: two counts per block of NOALLOC_BLOCK sites, for at most
+// NOALLOC_MAX_BLOCKS blocks (1 MiB; 8,388,608 sites).  Every slot a call reads
+// is written by the same call first, so nothing needs clearing.  Host code
+// reaches the array only through the address hipGetSymbolAddress gives on
+// each device (scratch_setup), never through its name.
+constexpr int NOALLOC_BLOCK = 128;
+constexpr int NOALLOC_MAX_BLOCKS = 65536;
+static __device__ __align__(256) unsigned long long scratch[2 * NOALLOC_MAX_BLOCKS];
+
+__global__ __launch_bounds__(NOALLOC_BLOCK) void k_pair_count_part(Geo g, const double4 *__restrict__ spos,
+                                                                   const int4 *__restrict__ smeta,
+                                                                   const int *__restrict__ bstart,
+                                                                   const int *__restrict__ rkind,
+                                                                   const sesmd_forcefield *__restrict__ ff,
+                                                                   unsigned long long *__restrict__ part)
+{
+    __shared__ unsigned long long s[2][NOALLOC_BLOCK];
+    int t = threadIdx.x, k = blockIdx.x * NOALLOC_BLOCK + t;
+    unsigned long long np = 0, nl = 0;
+    if (k < g.n) site_pair_count(g, spos, smeta, bstart, rkind, ff, k, np, nl);
+    s[0][t] = np;
+    s[1][t] = nl;
+    __syncthreads();
+    for (int d = NOALLOC_BLOCK / 2; d > 0; d >>= 1) {
+        if (t < d) {
+            s[0][t] += s[0][t + d];
+            s[1][t] += s[1][t + d];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        part[2 * blockIdx.x] = s[0][0];
+        part[2 * blockIdx.x + 1] = s[1][0];
+    }
+}
+
+// one block: the block counts summed, halved (each pair was counted from both
+// sides) into out[0] (pairs) and out[1] (LJ pairs)
+__global__ __launch_bounds__(256) void k_pair_count_total(const unsigned long long *__restrict__ part, int nblocks,
+                                                          double *__restrict__ out)
+{
+    __shared__ unsigned long long s[2][256];
+    int t = threadIdx.x;
+    unsigned long long a = 0, b = 0;
+    for (int i = t; i < nblocks; i += 256) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    s[0][t] = a;
+    s[1][t] = b;
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if (t < d) {
+            s[0][t] += s[0][t + d];
+            s[1][t] += s[1][t + d];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = 0.5 * (double)s[0][0];
+        out[1] = 0.5 * (double)s[1][0];
+    }
 }
 
 // ------------------------------------------------------------ far field
@@ -3595,6 +3676,33 @@ static hipError_t upload_lj32(sesmd_ctx *c, const sesmd_forcefield *ff)
     return hipMemcpy(c->lj32, t.data(), sizeof(float) * 2048, hipMemcpyHostToDevice);
 }
 
+// Device address of `scratch` on every device a context was created on,
+// looked up once per device with that device current.
+constexpr int SCRATCH_DEVICES = 64;
+static void *g_scratch[SCRATCH_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= SCRATCH_DEVICES) return fail("device %d: the library tracks devices 0..%d", dev,
+                                                       SCRATCH_DEVICES - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[dev]) return 0;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch[dev] = ptr;
+    return 0;
+}
+
+static void *scratch_of(int dev)
+{
+    if (dev < 0 || dev >= SCRATCH_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch[dev];
+}
+
 extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, int p,
                             const sesmd_forcefield *ff, int precision)
 {
@@ -3607,8 +3715,12 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     if (check_topology(*ff)) return 1;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device");
+    int dev = 0;
+    HC(hipGetDevice(&dev));
+    if (scratch_setup(dev)) return 1;
     sesmd_ctx *c = new sesmd_ctx();
     memset((void *)c, 0, sizeof(*c));
+    c->device = dev;
     Geo &g = c->g;
     g.nw = n_water;
     g.ns = n_surf;
@@ -4427,6 +4539,25 @@ extern "C" int sesmd_pair_counts(sesmd_ctx *c, double *pairs, double *lj_pairs)
     HC(hipFree(cnt));
     if (pairs) *pairs = 0.5 * (double)h[0];
     if (lj_pairs) *lj_pairs = 0.5 * (double)h[1];
+    return 0;
+}
+
+extern "C" int sesmd_pair_counts_noalloc(sesmd_ctx *c, double *counts)
+{
+    if (!c || !c->built) return fail("not built");
+    if (!counts) return fail("null output");
+    const Geo &g = c->g;
+    int nb = (g.n + NOALLOC_BLOCK - 1) / NOALLOC_BLOCK;
+    if (nb > NOALLOC_MAX_BLOCKS)
+        return fail("sesmd_pair_counts_noalloc: %d sites exceed its maximum of %d", g.n,
+                    NOALLOC_MAX_BLOCKS * NOALLOC_BLOCK);
+    unsigned long long *part = (unsigned long long *)scratch_of(c->device);
+    if (!part) return fail("sesmd_pair_counts_noalloc: no scratch on device %d", c->device);
+    k_gather_pos<<<nblk(g.n, 256), 256, 0, c->stream>>>(g, c->perm, c->r, c->type, c->dff, c->spos);
+    k_pair_count_part<<<nb, NOALLOC_BLOCK, 0, c->stream>>>(g, c->spos, c->smeta, c->bstart, c->rkind, c->dff,
+                                                          part);
+    k_pair_count_total<<<1, 256, 0, c->stream>>>(part, nb, counts);
+    HC(hipGetLastError());
     return 0;
 }
 
