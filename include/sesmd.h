@@ -80,6 +80,15 @@ const char *sesmd_last_error(void);
 int sesmd_set_state(sesmd_ctx *ctx, const double *r, const double *v, const double *a,
                     double eta);
 int sesmd_get_state(sesmd_ctx *ctx, double *r, double *v, double *a, double *eta);
+/* The Nose-Hoover friction eta and its time integral xi = integral of eta dt
+ * (accumulated by every eta update; it enters only the thermostat energy
+ * SESMD_E_THERMOSTAT).  Together with sesmd_get_state/sesmd_set_state this is
+ * the whole dynamical state: a run restarted from them at a step that
+ * rebuilds (kb % ifar == ifar - 1 before the call) continues bit for bit.
+ * The reference restarts from data_cont2.dat positions and velocities with
+ * eta and xi at zero (src/read_config.f90:19-34, src/mybox.f90:44-51). */
+int sesmd_get_thermostat(sesmd_ctx *ctx, double *eta, double *xi);
+int sesmd_set_thermostat(sesmd_ctx *ctx, double eta, double xi);
 /* Nose-Hoover target tr and the constants pnh1, pnh2 of src/mybox.f90:44-51 */
 int sesmd_set_nh(sesmd_ctx *ctx, double tr, double pnh1, double pnh2);
 /* frozen graphite carbons (reduced units); the per-column selection of
@@ -132,6 +141,14 @@ int sesmd_get_expansions(sesmd_ctx *ctx, double *O4, double *L4, double *O3, dou
                          double *O2, double *L2);
 /* counters: SHAKE/RATTLE molecules that hit the iteration limit, cumulative */
 int sesmd_get_counters(sesmd_ctx *ctx, int *shake_fail);
+/* Error flags counted on the device, cumulative over the context's life:
+ * flags[0] SHAKE/RATTLE molecules that hit the iteration limit (the
+ * reference's "Did not finish" messages, src/move_w1_a.f90:113-121);
+ * flags[1] steps whose kinetic sums were not finite (a NaN or an infinity in
+ * some velocity); flags[2] molecule filings whose filing site lay outside the
+ * (4 nb2)^3-box cube (filed in the nearest edge box instead).  n <= 3 entries
+ * are written (more are zero). */
+int sesmd_get_flags(sesmd_ctx *ctx, int *flags, int n);
 /* device pointers of the resident SoA arrays (x,y,z blocks of N doubles) */
 int sesmd_device_arrays(sesmd_ctx *ctx, void **r, void **v, void **a, void **f);
 /* the HIP stream every kernel of this context is launched on */

@@ -394,8 +394,12 @@ __global__ void k_fill_i(int *p, size_t n, int v)
 
 // Molecule classes and storage boxes: read_config / delete_w1 logic,
 // src/read_config.f90:44-175 (ibox2 + ibox64, int() truncating toward 0).
+// A molecule whose filing site lies outside the cube [-rhs, rhs)^3 is filed
+// in the nearest edge box, as the clamp below does, and counted in
+// counters[2]: the reference's walls keep every site inside, so this flags a
+// blown-up system.
 __global__ void k_classify(Geo g, const double *__restrict__ r, const int *__restrict__ first,
-                           int *__restrict__ cls, int *__restrict__ anc)
+                           int *__restrict__ cls, int *__restrict__ anc, int *__restrict__ counters)
 {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= g.nmol) return;
@@ -419,6 +423,15 @@ __global__ void k_classify(Geo g, const double *__restrict__ r, const int *__res
         return gx + g.nb * (gy + g.nb * gz);
     };
     int f = first[m];
+    {
+        int fs = m < g.nw ? f : f + 13;
+        bool out = false;
+        for (int d = 0; d < 3; d++) {
+            double x = r[(size_t)d * g.n + fs] + g.rhs;
+            out = out || !(x >= 0.0 && x < g.rsd);
+        }
+        if (out) atomicAdd(&counters[2], 1);
+    }
     if (m < g.nw) {
         int ibO = ib2(f), ibG = ib2(f + 1), ibH = ib2(f + 2);
         int lO[3], lG[3], lH[3];
@@ -2785,7 +2798,7 @@ __device__ void block_sum_store(double (&v)[NV], double *out)
 // and after, force zeroing.  One thread per molecule, 256 per block.
 __device__ void nh_update_block(Geo g, const double *__restrict__ partials, int nblocks,
                                 const sesmd_forcefield *__restrict__ ff, double *__restrict__ nh,
-                                int update_eta);
+                                int update_eta, int *__restrict__ counters);
 
 // Sorted-order copies of a site's new position for the next force evaluation
 // (what k_gather_pos / k_gather_pos32 would write), from the integrator.
@@ -3213,10 +3226,12 @@ __device__ __forceinline__ void surf_b(
     }
 }
 
-// kinetic sums -> the eta update of src/move_w2_a.f90:145-157 (single block)
+// kinetic sums -> the eta update of src/move_w2_a.f90:145-157 (single block);
+// a step whose kinetic sums are not finite (a NaN or an infinity reached some
+// velocity) is counted in counters[1]
 __device__ void nh_update_block(Geo g, const double *__restrict__ partials, int nblocks,
                                 const sesmd_forcefield *__restrict__ ff, double *__restrict__ nh,
-                                int update_eta)
+                                int update_eta, int *__restrict__ counters)
 {
     double acc[NH_VALS];
     for (int c = 0; c < NH_VALS; c++) acc[c] = 0.0;
@@ -3237,6 +3252,7 @@ __device__ void nh_update_block(Geo g, const double *__restrict__ partials, int 
         double dt2 = g.dt * g.dt;
         double sum1 = (S[0] + S[2] + S[4]) / dt2;
         double sum2 = (S[1] + S[3] + S[5]) / dt2;
+        if (!isfinite(sum1 + sum2)) counters[1] += 1;  // one thread: a plain store
         if (update_eta) {
             double eta_old = nh[0];
             nh[0] = nh[0] + nh[8] * (sum1 + nh[9] + sum2);
@@ -3249,9 +3265,10 @@ __device__ void nh_update_block(Geo g, const double *__restrict__ partials, int 
 
 __global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restrict__ partials,
                                                    int nblocks, const sesmd_forcefield *__restrict__ ff,
-                                                   double *__restrict__ nh, int update_eta)
+                                                   double *__restrict__ nh, int update_eta,
+                                                   int *__restrict__ counters)
 {
-    nh_update_block(g, partials, nblocks, ff, nh, update_eta);
+    nh_update_block(g, partials, nblocks, ff, nh, update_eta, counters);
 }
 
 // Step pipeline: block 0 updates eta from part A's kinetic sums (k_nh_update);
@@ -3262,10 +3279,11 @@ __global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restri
 __global__ __launch_bounds__(256) void k_nh_intra(Geo g, const double *__restrict__ partials, int nblocks,
                                                   const sesmd_forcefield *__restrict__ ff,
                                                   double *__restrict__ nh, const double *__restrict__ r,
-                                                  const Topo *__restrict__ topo, double *__restrict__ fs)
+                                                  const Topo *__restrict__ topo, double *__restrict__ fs,
+                                                  int *__restrict__ counters)
 {
     if (blockIdx.x == 0) {
-        nh_update_block(g, partials, nblocks, ff, nh, 1);
+        nh_update_block(g, partials, nblocks, ff, nh, 1, counters);
         return;
     }
     __shared__ double tab[29 * INTRA_MAXC * 3];
@@ -4052,7 +4070,7 @@ static int launch_rebuild(sesmd_ctx *c)
     const Geo &g = c->g;
     hipStream_t st = c->stream;
     if (!c->classes_fixed)
-        k_classify<<<nblk(g.nmol, 256), 256, 0, st>>>(g, c->r, c->first, c->cls, c->anc);
+        k_classify<<<nblk(g.nmol, 256), 256, 0, st>>>(g, c->r, c->first, c->cls, c->anc, c->counters);
     // counting sort by key (kcnt: counts, then arrival counters; then bflag), cleared by one fill
     int K = 4 * g.nb3;
     k_fill_i<<<nblk(2 * K + g.nb3, 256), 256, 0, st>>>(c->kcnt, 2 * K + g.nb3, 0);
@@ -4255,7 +4273,8 @@ static int launch_move_a(sesmd_ctx *c, unsigned clsmask = 7u, int update_eta = 1
     else if (c->nblk_s)
         k_fill_d<<<nblk((size_t)c->nblk_s * NH_VALS, 256), 256, 0, st>>>(
             c->partials + (size_t)c->nblk_w * NH_VALS, (size_t)c->nblk_s * NH_VALS, 0.0);
-    k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, update_eta);
+    k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, update_eta,
+                                   c->counters);
     HC(hipGetLastError());
     return 0;
 }
@@ -4336,9 +4355,9 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
                                                               c->nblk_s, so);
         if (g.ns) {  // eta update and the intramolecular terms in one launch
             k_nh_intra<<<1 + g.ns, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, c->r,
-                                                 c->dtopo, c->fs);
+                                                 c->dtopo, c->fs, c->counters);
         } else {
-            k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1);
+            k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1, c->counters);
         }
     } else {
         if (c->nblk_w)
@@ -4347,7 +4366,7 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
         if (c->nblk_s)
             k_move_a_surf<<<c->nblk_s, 64, 0, st>>>(g, c->r, c->v, c->a, c->f, c->dff, c->nh, c->partials,
                                                     c->counters);
-        k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1);
+        k_nh_update<<<1, 256, 0, st>>>(g, c->partials, c->nblk_w + c->nblk_s, c->dff, c->nh, 1, c->counters);
     }
     HC(hipGetLastError());
     if (rebuild) {
@@ -4581,6 +4600,34 @@ extern "C" int sesmd_set_classes(sesmd_ctx *c, const int *cls, const int *anc)
     HC(hipMemcpy(c->anc, anc, sizeof(int) * g.nmol, hipMemcpyHostToDevice));
     if (!c->classes_fixed) drop_graphs(c);  // a rebuild no longer classifies
     c->classes_fixed = true;
+    return 0;
+}
+
+extern "C" int sesmd_get_thermostat(sesmd_ctx *c, double *eta, double *xi)
+{
+    if (!c) return fail("null context");
+    double t[11];
+    HC(hipMemcpy(t, c->nh, sizeof t, hipMemcpyDeviceToHost));
+    if (eta) *eta = t[0];
+    if (xi) *xi = t[10];
+    return 0;
+}
+
+extern "C" int sesmd_set_thermostat(sesmd_ctx *c, double eta, double xi)
+{
+    if (!c) return fail("null context");
+    HC(hipMemcpy(c->nh, &eta, sizeof(double), hipMemcpyHostToDevice));
+    HC(hipMemcpy(c->nh + 10, &xi, sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int sesmd_get_flags(sesmd_ctx *c, int *flags, int n)
+{
+    if (!c) return fail("null context");
+    if (n < 0 || (n > 0 && !flags)) return fail("bad flag buffer");
+    int t[4];
+    HC(hipMemcpy(t, c->counters, sizeof t, hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; k++) flags[k] = k < 3 ? t[k] : 0;
     return 0;
 }
 

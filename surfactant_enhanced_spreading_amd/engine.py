@@ -117,6 +117,9 @@ def lib():
         L.sesmd_fmm_stage.argtypes = [vp, ctypes.c_int]
         L.sesmd_fmm_stage_enqueue.argtypes = [vp, ctypes.c_int]
         L.sesmd_pair_counts.argtypes = [vp, dp, dp]
+        L.sesmd_get_thermostat.argtypes = [vp, dp, dp]
+        L.sesmd_set_thermostat.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.sesmd_get_flags.argtypes = [vp, ip, ctypes.c_int]
         L.sesmd_energies.argtypes = [vp, dp]
         _lib = L
     return _lib
@@ -194,6 +197,42 @@ class Engine:
         eta = ctypes.c_double()
         _ck(self.L.sesmd_get_state(self.h, _p(r), _p(v), _p(a), ctypes.byref(eta)))
         return r, v, a, eta.value
+
+    def thermostat(self):
+        """(eta, xi): the Nose-Hoover friction and its time integral."""
+        eta, xi = ctypes.c_double(), ctypes.c_double()
+        _ck(self.L.sesmd_get_thermostat(self.h, ctypes.byref(eta), ctypes.byref(xi)))
+        return eta.value, xi.value
+
+    def set_thermostat(self, eta, xi):
+        _ck(self.L.sesmd_set_thermostat(self.h, float(eta), float(xi)))
+
+    def checkpoint(self, path, kb):
+        """Write the whole dynamical state (positions, v dt, F dt^2/2m, eta, xi)
+        and the step counter kb to an .npz file."""
+        r, v, a, _ = self.state()
+        eta, xi = self.thermostat()
+        np.savez(path, r=r, v=v, a=a, eta=eta, xi=xi, kb=kb, n_water=self.n_water, n_surf=self.n_surf,
+                 nb2=self.nb2, p=self.p)
+
+    def restore(self, path):
+        """Load a checkpoint written by checkpoint() into this context (same
+        system size and geometry), rebuild, and return its step counter."""
+        d = np.load(path, allow_pickle=False)
+        if (int(d['n_water']), int(d['n_surf']), int(d['nb2']), int(d['p'])) != \
+                (self.n_water, self.n_surf, self.nb2, self.p):
+            raise SesmdError('checkpoint is of another system or geometry')
+        self.set_state(d['r'], d['v'], d['a'], float(d['eta']))
+        self.set_thermostat(float(d['eta']), float(d['xi']))
+        self.rebuild()
+        return int(d['kb'])
+
+    def flags(self):
+        """Device error counters: SHAKE/RATTLE non-convergence, steps with
+        non-finite kinetic sums, molecules filed outside the cube."""
+        f = (ctypes.c_int * 3)()
+        _ck(self.L.sesmd_get_flags(self.h, f, 3))
+        return dict(shake=f[0], nonfinite=f[1], outside=f[2])
 
     # reference routine families ---------------------------------------------
     def rebuild(self):
