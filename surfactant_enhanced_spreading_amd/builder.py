@@ -177,18 +177,23 @@ def _align(v_from, v_to):
     return np.eye(3) + vx + vx @ vx / (1.0 + c)
 
 
-def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6):
+def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6, hemisphere=False):
     """Positions of one droplet: waters on a simple-cubic lattice at liquid
     density with random orientations, cut to the n_water lattice sites closest
     to the centre, and n_surf surfactants spread over the droplet surface with
     the EO chain (site 1 end) pointing inward and the trisiloxane outward.
+
+    hemisphere: only lattice sites at or above the centre's height (a
+    hemispherical droplet whose flat face lies at center_A's z), radius from
+    the hemisphere's volume; surfactants on the curved surface only.
 
     Returns (water_pos (n_water,3,3), surf_pos (n_surf,29,3)) in sigma units.
     """
     rng = np.random.default_rng(seed)
     a = (1.0 / WATER_DENSITY_A3) ** (1.0 / 3.0) / ff.SIGMA
     c = np.asarray(center_A, dtype=float) / ff.SIGMA
-    r_drop = (3.0 * n_water / (4.0 * math.pi * WATER_DENSITY_A3)) ** (1.0 / 3.0) / ff.SIGMA
+    vol = n_water / WATER_DENSITY_A3 * (2.0 if hemisphere else 1.0)
+    r_drop = (3.0 * vol / (4.0 * math.pi)) ** (1.0 / 3.0) / ff.SIGMA
     tmpl = surfactant_template()
     axis = tmpl[17] - tmpl[0]
     length = np.linalg.norm(axis)
@@ -196,6 +201,8 @@ def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6):
     for s in range(n_surf):
         # Fibonacci sphere for evenly spread anchor directions
         zz = 1.0 - 2.0 * (s + 0.5) / max(n_surf, 1)
+        if hemisphere:
+            zz = 1.0 - (s + 0.5) / max(n_surf, 1)  # upper half only
         phi = s * math.pi * (3.0 - math.sqrt(5.0))
         d = np.array([math.sqrt(max(0.0, 1 - zz * zz)) * math.cos(phi),
                       math.sqrt(max(0.0, 1 - zz * zz)) * math.sin(phi), zz])
@@ -214,6 +221,8 @@ def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6):
     X, Y, Z = np.meshgrid(g, g, g, indexing='ij')
     pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
     pts += rng.uniform(-0.02, 0.02, size=3)
+    if hemisphere:
+        pts = pts[pts[:, 2] >= 0.0]
     d2 = np.sum(pts * pts, axis=1)
     pts = pts[np.argsort(d2, kind='stable')] + c
     pts = pts[:int(n_water * 1.5) + 64]
@@ -239,21 +248,21 @@ def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6):
     return waters, surf
 
 
-def graphite_sheet(half_side_A=60.0, z_top=-16.0, layer_gap_A=3.41, ccbond_A=1.421):
-    """Two AB-stacked graphene layers of C atoms (sigma units), the top layer at
-    z = z_top sigma and the second 3.41 A below, like the sheet the reference's
-    own generator writes (``src/make_graphite.f90:14-80``).  Coordinates are
-    rounded to 1e-4 sigma, the precision of the reference's graphite file
-    (``src/read_graphite.f90:216``)."""
+def graphite_sheet(half_side_A=60.0, z_top=-16.0, layer_gap_A=3.41, ccbond_A=1.421, layers=2):
+    """AB-stacked graphene layers of C atoms (sigma units), the top layer at
+    z = z_top sigma and each further layer layer_gap_A below, like the sheet
+    the reference's own generator writes (``src/make_graphite.f90:14-80``,
+    two layers).  Coordinates are rounded to 1e-4 sigma, the precision of the
+    reference's graphite file (``src/read_graphite.f90:216``)."""
     a = ccbond_A / ff.SIGMA
     h = half_side_A / ff.SIGMA
     ax = np.array([1.5 * a, math.sqrt(3) / 2 * a])
     ay = np.array([0.0, math.sqrt(3) * a])
     n = int(h / a) + 3
     out = []
-    for layer in range(2):
+    for layer in range(layers):
         zl = z_top - layer * layer_gap_A / ff.SIGMA
-        shift = np.array([a, 0.0]) * layer
+        shift = np.array([a, 0.0]) * (layer % 2)
         for i in range(-n, n + 1):
             for j in range(-n, n + 1):
                 base = i * ax + j * ay + shift
@@ -300,3 +309,87 @@ def config1_system(n_water=20000, n_surf=200, seed=20261204):
     sheet spanning the domain's 192 x 192 A floor."""
     d = droplet(n_water, n_surf, center_A=(0.0, 0.0, 4.0), seed=seed)
     return assemble([d], graphite=graphite_sheet(half_side_A=96.0), seed=seed)
+
+
+def c0_system(n_water=2000, seed=20261203):
+    """2,000 waters as a hemispherical droplet (radius ~31 A) resting 3.2 A
+    above a three-layer graphite slab, in the 8^3-box domain (nb2 = 2, a 96 A
+    cube): the slab's top layer at z = -41 A, inside the bottom layer of L4
+    boxes, so the droplet's lowest sites feel the graphite."""
+    z_top = -41.0
+    d = droplet(n_water, 0, center_A=(0.0, 0.0, z_top + 3.2), seed=seed, hemisphere=True)
+    g = graphite_sheet(half_side_A=46.0, z_top=z_top / ff.SIGMA, layer_gap_A=3.35, layers=3)
+    return assemble([d], graphite=g, seed=seed)
+
+
+def film(n_water, n_surf, half_side_A, z_base_A, seed=0, min_sep_A=2.6):
+    """A water film on a simple-cubic lattice at liquid density over
+    |x|, |y| <= half_side_A from z_base_A up, with n_surf upright surfactants
+    (site 1 down, the trisiloxane head up, all turned alike): half of them on a
+    square grid whose heads stand above the film's top face, the rest on the
+    same grid shifted by half a cell, within the film at mid-depth.  The grid
+    is wide enough that no two heads meet; waters within min_sep_A of a
+    surfactant site are left out.  Returns (water_pos, surf_pos) in sigma
+    units."""
+    rng = np.random.default_rng(seed)
+    a_A = (1.0 / WATER_DENSITY_A3) ** (1.0 / 3.0)
+    nx = int(2 * half_side_A / a_A)
+    tmpl = surfactant_template()
+    axis = tmpl[17] - tmpl[0]
+    length = np.linalg.norm(axis)
+    R = _align(axis, np.array([0.0, 0.0, 1.0]))
+    body0 = (R @ (tmpl - tmpl[0]).T).T
+    n_top = (n_surf + 1) // 2
+    m = max(1, int(math.ceil(math.sqrt(max(n_top, 1)))))
+    step = 2 * half_side_A / m / ff.SIGMA
+    # film thick enough for the waters plus the volume the surfactants take
+    vol_s = 29 * 30.0  # A^3 per surfactant, roughly
+    thick = (n_water / WATER_DENSITY_A3 + n_surf * vol_s) / (4 * half_side_A ** 2)
+    nz = int(math.ceil(thick / a_A)) + 2
+    gx = (np.arange(nx) - (nx - 1) / 2.0) * a_A
+    gz = z_base_A + (np.arange(nz) + 0.5) * a_A
+    X, Y, Z = np.meshgrid(gx, gx, gz, indexing='ij')
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1) / ff.SIGMA
+    pts += rng.uniform(-0.02, 0.02, size=3)
+    top = (z_base_A + thick) / ff.SIGMA
+    mid = (z_base_A + 0.5 * thick) / ff.SIGMA - 0.5 * length
+    surf = np.zeros((n_surf, 29, 3))
+    for s_ in range(n_surf):
+        lay, k = (0, s_) if s_ < n_top else (1, s_ - n_top)
+        off = 0.5 * lay
+        x0 = -half_side_A / ff.SIGMA + (k % m + 0.5 + off) * step
+        y0 = -half_side_A / ff.SIGMA + (k // m % m + 0.5 + off) * step
+        z0 = top - 0.3 * length if lay == 0 else mid
+        surf[s_] = body0 + np.array([x0, y0, z0])
+    if n_surf:
+        from scipy.spatial import cKDTree
+        near = cKDTree(surf.reshape(-1, 3)).query(pts, k=1)[0]
+        pts = pts[near > min_sep_A / ff.SIGMA]
+    pts = pts[np.argsort(pts[:, 2], kind='stable')]
+    if len(pts) < n_water:
+        raise ValueError("film lattice too small for the requested water count")
+    pts = pts[:n_water]
+    w = water_template()
+    q = rng.normal(size=(n_water, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    qa, qb, qc, qd = q.T
+    Rq = np.stack([
+        np.stack([qa * qa + qb * qb - qc * qc - qd * qd, 2 * (qb * qc - qa * qd),
+                  2 * (qb * qd + qa * qc)], axis=1),
+        np.stack([2 * (qb * qc + qa * qd), qa * qa - qb * qb + qc * qc - qd * qd,
+                  2 * (qc * qd - qa * qb)], axis=1),
+        np.stack([2 * (qb * qd - qa * qc), 2 * (qc * qd + qa * qb),
+                  qa * qa - qb * qb - qc * qc + qd * qd], axis=1)], axis=1)
+    waters = np.einsum('kij,sj->ksi', Rq, w) + pts[:, None, :]
+    return waters, surf
+
+
+def c4_system(n_water=200000, n_surf=4000, seed=20261207):
+    """200,000 waters with 4,000 surfactants as a film 400 x 400 A wide on a
+    two-layer graphite slab, in the 36^3-box domain (nb2 = 9, a 432 A cube:
+    the smallest cube of 12 A boxes that holds the film's width); the slab's
+    top layer at z = -209 A, inside the bottom layer of L4 boxes."""
+    z_top = -209.0
+    d = film(n_water, n_surf, 200.0, z_top + 3.2, seed=seed)
+    g = graphite_sheet(half_side_A=206.0, z_top=z_top / ff.SIGMA, layer_gap_A=3.41)
+    return assemble([d], graphite=g, seed=seed)

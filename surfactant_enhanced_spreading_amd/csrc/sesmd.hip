@@ -64,7 +64,7 @@ constexpr int NEAR_TILE = 64;                 // lanes of a near-field wavefront
 constexpr int NEAR_CHUNK = 128;               // receivers per near-field work item (two per lane)
 constexpr int NH_VALS = 18;                   // kinetic partial sums per block
 
-constexpr int MAXNB = 32;  // L4 boxes per axis (nb2 <= 8)
+constexpr int MAXNB = 48;  // L4 boxes per axis (nb2 <= 12; packed box coordinates hold 0..255)
 
 struct Geo {
     int n, nw, ns, nmol, nb2, nb, nb3, n3, n33, n2, p, lp, lk;
@@ -3727,7 +3727,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     if (!out || !ff) return fail("null argument");
     *out = nullptr;
     if (n_water < 0 || n_surf < 0 || n_water + n_surf == 0) return fail("empty system");
-    if (nb2 < 1 || nb2 > 8) return fail("nb2 must be 1..8");
+    if (nb2 < 1 || 4 * nb2 > MAXNB) return fail("nb2 must be 1..%d", MAXNB / 4);
     if (p != 4 && p != 8) return fail("p must be 4 (the reference) or 8");
     if (precision != SESMD_FP64 && precision != SESMD_MIXED) return fail("bad precision");
     if (check_topology(*ff)) return 1;

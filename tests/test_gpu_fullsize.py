@@ -1,9 +1,11 @@
 """The HIP engine against the oracle on the benchmarked systems at full size.
 
-Two workloads, the ones bench.py times:
+Three workloads:
   c2  -- 20,000 waters + 200 surfactants (65,800 sites) in the 16^3-box domain
-         (nb2 = 4, 4,096 L4 boxes), expansion order p = 8;
-  ref -- the same droplet in the reference's geometry (12^3 boxes, p = 4).
+         (nb2 = 4, 4,096 L4 boxes), expansion order p = 8 (bench.py's headline);
+  ref -- the same droplet in the reference's geometry (12^3 boxes, p = 4);
+  c0  -- 2,000 waters as a hemispherical droplet on a three-layer graphite slab
+         in the 8^3-box domain (nb2 = 2, 512 L4 boxes of 12 A), p = 8.
 
 Bar as in test_gpu_parity.py: classes, storage boxes, central boxes, receiver
 kinds and subtraction masks bit-exact; expansions 1e-12 of their largest
@@ -34,12 +36,12 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
 STEP_TOL = 1e-9
-CASES = {'c2': (4, 8), 'ref': (3, 4)}
+CASES = {'c2': (4, 8), 'ref': (3, 4), 'c0': (2, 8)}
 
 
 def _system(name):
     from surfactant_enhanced_spreading_amd import builder
-    return builder.config1_system() if name == 'c2' else builder.bench_system()
+    return {'c2': builder.config1_system, 'ref': builder.bench_system, 'c0': builder.c0_system}[name]()
 
 
 @pytest.fixture(scope='module', params=sorted(CASES))
@@ -64,8 +66,8 @@ def test_full_rebuild_bit_exact(full):
     a, b = o.classes(), e.classes()
     for k in ('cls', 'anc', 'cbox', 'rkind', 'isub'):
         assert np.array_equal(a[k], b[k]), (name, k)
-    # every molecule filed once; W1 and W2 both occur in a droplet this size
-    assert set(np.unique(a['cls'])) == {0, 1, 2}
+    # W1 and W2 both occur in a droplet this size (and S where it has surfactants)
+    assert {0, 1} <= set(np.unique(a['cls']))
 
 
 def test_full_expansions(full):
@@ -73,7 +75,12 @@ def test_full_expansions(full):
     A, B = o.expansions(), e.expansions()
     for k in ('O4', 'O3', 'O2', 'L2', 'L3', 'L4'):
         scale = np.abs(A[k]).max()
-        assert scale > 0, (name, k)
+        if scale == 0.0:
+            # nb2 = 2: the eight top-level boxes are all neighbours, so no
+            # top-level translation exists and L2 is zero
+            assert k == 'L2' and CASES[name][0] <= 2, (name, k)
+            assert np.abs(B[k]).max() == 0.0, (name, k)
+            continue
         err = np.abs(A[k] - B[k]).max() / scale
         assert err <= 1e-12, (name, k, err)
 

@@ -143,11 +143,13 @@ def test_steps_pipeline(which, ifar, sys_small, sys_medium):
         e.close()
 
 
-@pytest.mark.parametrize('nb2,p', [(3, 8), (4, 8), (4, 4)])
+@pytest.mark.parametrize('nb2,p', [(3, 8), (4, 8), (4, 4), (5, 4)])
 def test_other_orders_and_domains(nb2, p, sys_small):
     """The engine beyond the reference's own p = 4, 27-rank geometry: expansion
-    order 8 and a 16^3-box domain (nb2 = 4), against the oracle run with the
-    same parameters -- classes bit-exact, expansions 1e-12, total force 1e-10."""
+    order 8, a 16^3-box domain (nb2 = 4) and a 20^3-box one (nb2 = 5: more
+    rows than the row list keeps in LDS, and odd half-widths at L3), against
+    the oracle run with the same parameters -- classes bit-exact, expansions
+    1e-12, total force 1e-10."""
     import oracle as orc
     from surfactant_enhanced_spreading_amd import engine
     o = orc.Oracle(sys_small, nb2=nb2, p=p)
