@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 METRIC = "MD steps/s (ns/day) 20k water+surfactant: mixed vs FP64 GPU at RMS F err≤1e-5"
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0  # dense bf16 matrix peak
 HBM_PEAK_GBS = 8000.0
 
 
@@ -163,10 +164,12 @@ def m2l_translations(O4, nb):
     return total
 
 
-def m2l_timing(e, timer, reps=20):
+def m2l_timing(e, timer, reps=20, mixed=False):
     """The L4 M2L (computeLjk4) alone, `reps` launches on the context's stream
-    after a rebuild and far field at the current state; achieved FP64 rate of
-    the algorithmic count 8 ((p+1)(p+2)/2) (p+1)^2 per translation."""
+    after a rebuild and far field at the current state; achieved rate of the
+    algorithmic count 8 ((p+1)(p+2)/2) (p+1)^2 per translation.  FP64 context:
+    against the FP64 matrix peak.  Mixed context (split-BF16x3 products): the
+    six bf16 products per translation against the dense bf16 matrix peak."""
     from surfactant_enhanced_spreading_amd import engine
     e.rebuild()
     e.fmm()
@@ -183,6 +186,12 @@ def m2l_timing(e, timer, reps=20):
     p = e.p
     flop = 8.0 * ((p + 1) * (p + 2) // 2) * (p + 1) ** 2 * T
     ach = flop / (ms * 1e-3) / 1e12
+    if mixed:
+        return dict(kernel='M2L at L4 (k_m2l_count, k_m2l_scan, k_m2l_fill, k_m2l_split, k_m2l_gemm_bf3, '
+                           'k_m2l_accum)', translations=T, ms=round(ms, 4),
+                    achieved_tflops=round(ach, 3),
+                    bf16_products_tflops=round(6 * ach, 3), peak_bf16_tflops=BF16_PEAK_TFLOPS,
+                    frac_bf16_mfma_peak=round(6 * ach / BF16_PEAK_TFLOPS, 4))
     return dict(kernel='M2L at L4 (k_m2l_count, k_m2l_scan, k_m2l_fill, k_m2l_gemm, k_m2l_accum)', translations=T, ms=round(ms, 4),
                 achieved_tflops=round(ach, 3), peak_tflops=FP64_PEAK_TFLOPS,
                 frac_fp64_mfma_peak=round(ach / FP64_PEAK_TFLOPS, 4))
@@ -339,6 +348,7 @@ def main():
     print('bench: fp64 %.1f steps/s' % steps_s, file=sys.stderr, flush=True)
     if not args.no_mixed:
         rm, e = run(sys_, cfg, engine.MIXED, args.steps, args.warmup, timer)
+        m2l_mixed = m2l_timing(e, timer, mixed=True)
         e.close()
         ach = flop / (rm['near_ms'] * 1e-3) / 1e12
         out['mixed'] = dict(steps_per_s=round(rm['steps_s'], 2), ms_per_step=round(rm['ms_step'], 4),
@@ -347,7 +357,8 @@ def main():
                             rms_force_err=rms_force_error(sys_, cfg), dtype='f32 pair / f64',
                             near_kernel_ms=round(rm['near_ms'], 4),
                             near_achieved_tflops=round(ach, 3),
-                            near_frac_fp32_peak=round(ach / FP32_PEAK_TFLOPS, 4))
+                            near_frac_fp32_peak=round(ach / FP32_PEAK_TFLOPS, 4),
+                            m2l=m2l_mixed)
         print('bench: mixed %.1f steps/s' % rm['steps_s'], file=sys.stderr, flush=True)
     if not args.no_secondary and name == 'c2':
         rcfg = CONFIGS['ref']

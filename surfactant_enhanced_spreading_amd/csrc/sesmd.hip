@@ -147,8 +147,11 @@ struct sesmd_ctx {
     int *ne4, *ne3, *ne2;         // nonempty flags per box
     double2 *T4, *T3, *T2;        // M2L operators per offset
     double *AF4, *AF3;            // the same, real-expanded in MFMA operand order (k_m2l_gemm)
-    double *m2l_res;              // M2L results, one row (M2L_RES doubles) per list entry
-    int *m2l_cnt, *m2l_base, *m2l_bbase;  // per (class, candidate) list: entries, first entry, first block
+    unsigned short *AFB4, *AFB3;  // mixed precision: the operators as bf16 parts (k_m2l_gemm_bf3)
+    unsigned short *m2l_ob;       // mixed precision: the source expansions as bf16 parts
+    double *m2l_res;              // M2L results, one row (M2L_RES values) per list entry
+    int *m2l_cnt, *m2l_base;      // per (class, candidate) list: entries, first entry
+    int *m2l_gbase, *m2l_bbase;   // per offset group: first entry, first block
     int2 *m2l_list;               // list entries (target within its class, source box)
     int *m2l_idx;                 // per target box and candidate: its result row, -1 none
     int m2l_max_blocks;           // k_m2l_gemm grid (blocks of the densest possible level)
@@ -964,18 +967,48 @@ __device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict
 }
 
 // Mixed precision (SESMD_MIXED): the same pair set and order, pair arithmetic
-// in FP32 on coordinates relative to the source box's centre, the forces of
-// one (box, class) segment summed in FP32 and then added to the FP64 totals.
+// in FP32 on coordinates relative to the source box's centre, and every pair
+// force component accumulated in 64-bit fixed point with a resolution of 2^-32
+// force units.  The conversion is one FP64 fused multiply-add per component:
+// fma(d, f, 1.5 2^20) rounds the exact product d f (two FP32 factors) once, to
+// the 2^-32 spacing of the doubles in [2^20, 2^21), and while |d f| < 2^19 the
+// bit pattern of the result is bits(1.5 2^20) + round(d f 2^32).  Adding the
+// bit patterns as 64-bit integers therefore adds the fixed-point values; the n
+// copies of bits(1.5 2^20) come off once at the end (n is the same for every
+// lane).  Integer sums do not depend on their order.  Headroom: each lane
+// keeps the largest |f| (pair force over distance) it has seen; an item where
+// that times the longest distance its rows allow reaches 2^19 counts in
+// counters[3].
+constexpr double FIX_MAGIC = 1572864.0;                        // 1.5 2^20
+constexpr unsigned long long FIX_MAGIC_BITS = 0x4138000000000000ull;  // its bit pattern
+constexpr float FIX_LIMIT = 524288.0f;                         // 2^19
+
 struct Recv32 {
     float q, qO, qH, cO, s6O;
 };
+
+struct Fix3 {
+    unsigned long long x, y, z;
+};
+
+// the fixed-point sum of n converted products, as a double (exact: |sum| < 2^21)
+__device__ __forceinline__ double fix_value(unsigned long long acc, int n)
+{
+    long long v = (long long)(acc - (unsigned long long)n * FIX_MAGIC_BITS);
+    return (double)v * 0x1p-32;
+}
+
+__device__ __forceinline__ unsigned long long fix_term(float d, float f)
+{
+    return (unsigned long long)__double_as_longlong(fma((double)d, (double)f, FIX_MAGIC));
+}
 
 template <int CAT, bool CHECK, bool CEN, bool HQ, int NR>
 __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, int bx,
                                             const float (&xr)[NR], const float (&yr)[NR],
                                             const float (&zr)[NR], const Recv (&R)[NR],
                                             const Recv32 (&Q)[NR], const float *__restrict__ lj32,
-                                            float rc2, float (&fx)[NR], float (&fy)[NR], float (&fz)[NR])
+                                            float rc2, Fix3 (&acc)[NR], float &mf, int &nadd)
 {
     bool okc[NR];
     if (CHECK) {
@@ -998,10 +1031,9 @@ __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, in
         r2[r] = dx[r] * dx[r] + dy[r] * dy[r] + dz[r] * dz[r];
         float rinv = __builtin_amdgcn_rsqf(r2[r]);
         rinv2[r] = rinv * rinv;
-        // hydrogen sources of one charge: q_i q_H scales the segment's sum once
-        // (near_segment32), not every pair
-        float qq = CAT == 0 ? Q[r].qO : Q[r].q * pj.w;
-        ft[r] = (CAT == 2 && HQ) ? rinv2[r] * rinv : qq * (rinv2[r] * rinv);
+        // hydrogen sources of one charge (HQ): q_i q_H is the receiver's own
+        float qq = CAT == 0 ? Q[r].qO : (CAT == 2 && HQ) ? Q[r].qH : Q[r].q * pj.w;
+        ft[r] = qq * (rinv2[r] * rinv);
     }
     if (CAT != 2) {
         bool in[NR], any = false;
@@ -1038,10 +1070,12 @@ __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, in
     }
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-        fx[r] = fmaf(dx[r], ft[r], fx[r]);
-        fy[r] = fmaf(dy[r], ft[r], fy[r]);
-        fz[r] = fmaf(dz[r], ft[r], fz[r]);
+        acc[r].x += fix_term(dx[r], ft[r]);
+        acc[r].y += fix_term(dy[r], ft[r]);
+        acc[r].z += fix_term(dz[r], ft[r]);
+        mf = fmaxf(mf, fabsf(ft[r]));
     }
+    nadd++;
 }
 
 template <int CAT, bool CHECK, bool CEN, bool HQ, int NR>
@@ -1051,12 +1085,9 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
                                                const float (&yr)[NR], const float (&zr)[NR],
                                                const Recv (&R)[NR], const Recv32 (&Q)[NR],
                                                const float *__restrict__ lj32, float rc2,
-                                               double (&ax)[NR], double (&ay)[NR], double (&az)[NR])
+                                               Fix3 (&acc)[NR], float &mf, int &nadd)
 {
     if (s0 >= s1) return;
-    float fx[NR], fy[NR], fz[NR];
-#pragma unroll
-    for (int r = 0; r < NR; r++) fx[r] = fy[r] = fz[r] = 0.0f;
     // No branch between the loop body's two sources, so their pair arithmetic
     // interleaves; an odd last source runs after the loop.
     const float4 *ps = spos32 + s0;
@@ -1074,17 +1105,10 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
         pA = ps[0];
         pB = ps[1];
         __asm__ volatile("" ::: "memory");  // keeps these loads ahead of the pair arithmetic
-        near_pair32<CAT, CHECK, CEN, HQ, NR>(pa, ma, bx, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
-        near_pair32<CAT, CHECK, CEN, HQ, NR>(pb, mb, bx, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
+        near_pair32<CAT, CHECK, CEN, HQ, NR>(pa, ma, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
+        near_pair32<CAT, CHECK, CEN, HQ, NR>(pb, mb, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
     }
-    if (t == 1) near_pair32<CAT, CHECK, CEN, HQ, NR>(pA, mA, bx, xr, yr, zr, R, Q, lj32, rc2, fx, fy, fz);
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-        float sc = (CAT == 2 && HQ) ? Q[r].qH : 1.0f;
-        ax[r] += (double)(sc * fx[r]);
-        ay[r] += (double)(sc * fy[r]);
-        az[r] += (double)(sc * fz[r]);
-    }
+    if (t == 1) near_pair32<CAT, CHECK, CEN, HQ, NR>(pA, mA, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
 }
 
 template <bool CHECK, bool HQ, int NR>
@@ -1092,8 +1116,8 @@ __device__ __forceinline__ void near_row32(const Geo &g, const float4 *__restric
                                            const int4 *__restrict__ smeta, const int *__restrict__ bs4,
                                            int r0, int r1, int bc, int bx,
                                            const Recv (&R)[NR], const Recv32 (&Q)[NR],
-                                           const float *__restrict__ lj32, double (&ax)[NR],
-                                           double (&ay)[NR], double (&az)[NR])
+                                           const float *__restrict__ lj32, Fix3 (&acc)[NR], float &mf,
+                                           int &nadd)
 {
     int nb = g.nb;
     float rc2 = (float)g.rc2;
@@ -1113,13 +1137,13 @@ __device__ __forceinline__ void near_row32(const Geo &g, const float4 *__restric
         int k2 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 2]);
         int k3 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 3]);
         if (bb == bc) {
-            near_segment32<0, CHECK, true, HQ, NR>(spos32, smeta, k0, k1, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
-            near_segment32<1, CHECK, true, HQ, NR>(spos32, smeta, k1, k2, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
-            near_segment32<2, CHECK, true, HQ, NR>(spos32, smeta, k2, k3, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<0, CHECK, true, HQ, NR>(spos32, smeta, k0, k1, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
+            near_segment32<1, CHECK, true, HQ, NR>(spos32, smeta, k1, k2, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
+            near_segment32<2, CHECK, true, HQ, NR>(spos32, smeta, k2, k3, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
         } else {
-            near_segment32<0, CHECK, false, HQ, NR>(spos32, smeta, k0, k1, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
-            near_segment32<1, CHECK, false, HQ, NR>(spos32, smeta, k1, k2, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
-            near_segment32<2, CHECK, false, HQ, NR>(spos32, smeta, k2, k3, bx, xr, yr, zr, R, Q, lj32, rc2, ax, ay, az);
+            near_segment32<0, CHECK, false, HQ, NR>(spos32, smeta, k0, k1, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
+            near_segment32<1, CHECK, false, HQ, NR>(spos32, smeta, k1, k2, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
+            near_segment32<2, CHECK, false, HQ, NR>(spos32, smeta, k2, k3, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
         }
     }
 }
@@ -1165,10 +1189,11 @@ __device__ __forceinline__ void near_sources(const Geo &g, const double4 *__rest
                                              unsigned long long mask, const float4 *__restrict__ spos32,
                                              const float *__restrict__ lj32, int r0, int r1, int bc, int bx,
                                              const Recv (&R)[NR], const int (&ti)[NR], double (&ax)[NR],
-                                             double (&ay)[NR], double (&az)[NR])
+                                             double (&ay)[NR], double (&az)[NR], int *__restrict__ counters)
 {
     if (MIX) {
         Recv32 Q[NR];
+        Fix3 acc[NR];
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             Q[r].q = (float)R[r].q;
@@ -1176,11 +1201,25 @@ __device__ __forceinline__ void near_sources(const Geo &g, const double4 *__rest
             Q[r].qH = (float)R[r].qH;
             Q[r].cO = lj32[ti[r] * 32];
             Q[r].s6O = lj32[1024 + ti[r] * 32];
+            acc[r].x = acc[r].y = acc[r].z = 0ull;
         }
+        float mf = 0.0f;
+        int nadd = 0;
         if (check)
-            near_row32<true, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, ax, ay, az);
+            near_row32<true, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, acc, mf, nadd);
         else
-            near_row32<false, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, ax, ay, az);
+            near_row32<false, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, acc, mf, nadd);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            ax[r] += fix_value(acc[r].x, nadd);
+            ay[r] += fix_value(acc[r].y, nadd);
+            az[r] += fix_value(acc[r].z, nadd);
+        }
+        // a receiver in the 3 x 3 boxes around the row's centre box and a
+        // source in the row's three boxes are at most sqrt(17) box sides apart
+        float dmax = 4.1231056f * (float)g.rs;
+        if (__builtin_amdgcn_ballot_w64(mf * dmax >= FIX_LIMIT) && (threadIdx.x & 63) == 0)
+            atomicAdd(&counters[3], 1);
     } else if (MASKED)
         near_row<true, true, HQ, NR>(g, spos, smeta, bs4, r0, r1, bc, bx, R, ff->lj_c, ff->lj_s6, mask, ax, ay,
                                      az);
@@ -1221,7 +1260,7 @@ __device__ __forceinline__ void near_item(
         if (!MASKED)
             for (int bb = r0; bb <= r1; bb++) check = check || bflag[bb];
         near_sources<MASKED, HQ, MIX, NR>(g, spos, smeta, bs4, check, ff, mask, spos32, lj32, r0, r1, b, bx, R,
-                                          ti, ax, ay, az);
+                                          ti, ax, ay, az, nullptr);
     }
 #pragma unroll
     for (int r = 0; r < NR; r++) {
@@ -1383,7 +1422,7 @@ __device__ __forceinline__ void near_rows_item(
     const int *__restrict__ bs4, const int *__restrict__ bflag, const int *__restrict__ rkind,
     const sesmd_forcefield *__restrict__ ff, const float4 *__restrict__ spos32,
     const float *__restrict__ lj32, double *__restrict__ part, const RowRecv &L, int x0, int y, int z,
-    int c0, int cend, int lane)
+    int c0, int cend, int lane, int *__restrict__ counters)
 {
     int nb = g.nb;
     Recv R[NR];
@@ -1422,7 +1461,7 @@ __device__ __forceinline__ void near_rows_item(
     for (int bb = r0; bb <= r1; bb++) check = check || bflag[bb];
     int bc = anycen ? x0 + nb * (y + nb * z) : -1;
     near_sources<false, HQ, MIX, NR>(g, spos, smeta, bs4, check, ff, 0ull, spos32, lj32, r0, r1, bc, x0, R, ti,
-                                     ax, ay, az);
+                                     ax, ay, az, counters);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         if (c0 + NEAR_TILE * r + lane < cend) {
@@ -1440,7 +1479,7 @@ __global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
     const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
     const int *__restrict__ rkind, const int2 *__restrict__ work, const int *__restrict__ nwork,
     const sesmd_forcefield *__restrict__ ff, const float4 *__restrict__ spos32,
-    const float *__restrict__ lj32, double *__restrict__ part)
+    const float *__restrict__ lj32, double *__restrict__ part, int *__restrict__ counters)
 {
     // The full near field, row-centred: one 64-lane wavefront per work item,
     // the item being up to NEAR_CHUNK receivers of the nine boxes that all see
@@ -1468,10 +1507,10 @@ __global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
     int cend = min(L.pre[9], c0 + NEAR_CHUNK);
     if (cend - c0 > NEAR_TILE)
         near_rows_item<HQ, MIX, 2>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, c0,
-                                   cend, lane);
+                                   cend, lane, counters);
     else
         near_rows_item<HQ, MIX, 1>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, c0,
-                                   cend, lane);
+                                   cend, lane, counters);
 }
 
 
@@ -1864,19 +1903,68 @@ __global__ __launch_bounds__(64 * W) void k_m2l(Geo g, int nd, const double2 *__
 // the class would mostly multiply zeros.
 //
 // Per rebuild, from the occupied boxes (ne): k_m2l_count counts each list's
-// targets, k_m2l_scan gives each list its first position and first block,
-// k_m2l_fill writes the lists (targets in box order) and, per target and
-// candidate, the position of its result (-1: none).  k_m2l_gemm takes up to
-// M2L_TPB x 16 consecutive entries of one list per block: the operator slab is
-// the A operand of v_mfma_f64_16x16x4f64 (2 lk rows padded to MT x 16, 2 lp
-// columns padded to KS x 4), the 16 entries of a tile the N = 16 columns, the
-// sources' expansions the B operand.  Each result column goes to its own row
-// of `res`; k_m2l_accum adds a target's results in candidate order, so the
-// sums do not depend on scheduling.
+// targets; k_m2l_scan places the lists grouped by their offset (the up to
+// eight classes that share an operator side by side) and gives each group its
+// first position and first block; k_m2l_fill writes the lists (targets in box
+// order) and, per target and candidate, the position of its result (-1:
+// none).  A GEMM block takes up to M2L_TPB x 16 consecutive entries of one
+// group, so one operator load serves the targets of every class with that
+// offset.  FP64 (k_m2l_gemm): the operator slab is the A operand of
+// v_mfma_f64_16x16x4f64 (2 lk rows padded to MT x 16, 2 lp columns padded to
+// KS x 4), the 16 entries of a tile the N = 16 columns, the sources'
+// expansions the B operand.  Mixed precision (k_m2l_gemm_bf3): the same
+// product on v_mfma_f32_16x16x32_bf16, each FP64 operand rounded to FP32 and
+// split into three bf16 parts (hi + mid + lo) whose six leading cross
+// products are accumulated in FP32.  Each result column goes to its own row
+// of `res`; k_m2l_accum adds a target's results in candidate order, in FP64,
+// so the sums do not depend on scheduling.
 constexpr int M2L_TPB = 8;        // tiles of 16 list entries per block
 constexpr int M2L_LISTS = 8 * 216;
-constexpr int M2L_RES = 96;       // doubles per result row (2 lk <= 90, padded)
+constexpr int M2L_GROUPS = 343;   // operator offsets in [-3, 3]^3
+constexpr int M2L_RES = 96;       // values per result row (2 lk <= 90, padded)
 typedef double m2l_d4 __attribute__((ext_vector_type(4)));
+typedef float m2l_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 m2l_bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned m2l_u4 __attribute__((ext_vector_type(4)));  // eight bf16 bit patterns
+
+// bf16 bit pattern of f, rounded to nearest even (finite f)
+__host__ __device__ __forceinline__ unsigned short bf16_bits(float f)
+{
+    unsigned u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__host__ __device__ __forceinline__ float bf16_value(unsigned short h)
+{
+    unsigned u = (unsigned)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// x rounded to FP32 and split as hi + mid + lo (bf16 each); every remainder
+// is exact in FP32, so the three parts carry the 24 bits of the FP32 value
+__host__ __device__ __forceinline__ void bf16_split3(double x, unsigned short &h, unsigned short &m,
+                                                     unsigned short &l)
+{
+    float f = (float)x;
+    h = bf16_bits(f);
+    float r = f - bf16_value(h);
+    m = bf16_bits(r);
+    l = bf16_bits(r - bf16_value(m));
+}
+
+// operator offset (group) of candidate ci of class par; the class's candidate
+// for group grp, -1 if it has none
+__device__ __forceinline__ int m2l_cand(int par, int grp)
+{
+    int cx = grp % 7 - 3 + 2 + (par & 1), cy = (grp / 7) % 7 - 3 + 2 + ((par >> 1) & 1),
+        cz = grp / 49 - 3 + 2 + (par >> 2);
+    if (cx < 0 || cx > 5 || cy < 0 || cy > 5 || cz < 0 || cz > 5) return -1;
+    return cx + 6 * (cy + 6 * cz);
+}
 
 // candidate ci (0..215, x fastest over the 6^3 children of the parent's
 // neighbours) of parity class par -> offset; false for the 27 near ones
@@ -1919,42 +2007,65 @@ __global__ __launch_bounds__(256) void k_m2l_count(int nd, const int *__restrict
     if (threadIdx.x == 0) cnt[L] = s_c;
 }
 
-// One block: first position (base) and first block (bbase) of every list;
-// bbase[M2L_LISTS] = blocks of the level.
-__global__ __launch_bounds__(1024) void k_m2l_scan(const int *__restrict__ cnt, int *__restrict__ base,
-                                                   int *__restrict__ bbase)
+// One block: the lists grouped by operator offset, classes in order within a
+// group.  base[list] = its first position; gbase[grp] = the group's first
+// position, bbase[grp] its first block (gbase / bbase[M2L_GROUPS] = entries /
+// blocks of the level).
+__global__ __launch_bounds__(512) void k_m2l_scan(const int *__restrict__ cnt, int *__restrict__ base,
+                                                  int *__restrict__ gbase, int *__restrict__ bbase)
 {
-    __shared__ int s_a[1024], s_b[1024];
-    constexpr int PER = (M2L_LISTS + 1023) / 1024;
-    int t = threadIdx.x, a = 0, bsum = 0;
-    for (int e = 0; e < PER; e++) {
-        int L = t * PER + e;
-        if (L < M2L_LISTS) {
-            a += cnt[L];
-            bsum += (cnt[L] + M2L_TPB * 16 - 1) / (M2L_TPB * 16);
+    __shared__ int s_a[512], s_b[512];
+    int t = threadIdx.x, tot = 0;
+    if (t < M2L_GROUPS)
+        for (int par = 0; par < 8; par++) {
+            int ci = m2l_cand(par, t);
+            if (ci >= 0) tot += cnt[par * 216 + ci];
         }
-    }
-    s_a[t] = a;
-    s_b[t] = bsum;
+    int nblk = (tot + M2L_TPB * 16 - 1) / (M2L_TPB * 16);
+    s_a[t] = tot;
+    s_b[t] = nblk;
     __syncthreads();
-    for (int d = 1; d < 1024; d *= 2) {  // inclusive scan of the per-thread sums
+    for (int d = 1; d < 512; d *= 2) {  // inclusive scan
         int va = t >= d ? s_a[t - d] : 0, vb = t >= d ? s_b[t - d] : 0;
         __syncthreads();
         s_a[t] += va;
         s_b[t] += vb;
         __syncthreads();
     }
-    int ra = s_a[t] - a, rb = s_b[t] - bsum;  // exclusive
-    for (int e = 0; e < PER; e++) {
-        int L = t * PER + e;
-        if (L < M2L_LISTS) {
-            base[L] = ra;
-            bbase[L] = rb;
-            ra += cnt[L];
-            rb += (cnt[L] + M2L_TPB * 16 - 1) / (M2L_TPB * 16);
+    if (t < M2L_GROUPS) {
+        int run = s_a[t] - tot;
+        gbase[t] = run;
+        bbase[t] = s_b[t] - nblk;
+        for (int par = 0; par < 8; par++) {
+            int ci = m2l_cand(par, t);
+            if (ci < 0) continue;
+            base[par * 216 + ci] = run;
+            run += cnt[par * 216 + ci];
         }
     }
-    if (t == 1023) bbase[M2L_LISTS] = s_b[1023];
+    if (t == M2L_GROUPS - 1) {
+        gbase[M2L_GROUPS] = s_a[t];
+        bbase[M2L_GROUPS] = s_b[t];
+    }
+}
+
+// the group of GEMM block B (the last group whose first block is <= B), its
+// first entry and entry count; false past the level's last block
+__device__ __forceinline__ bool m2l_block(const int *__restrict__ gbase, const int *__restrict__ bbase, int B,
+                                          int &grp, int &first, int &ne_)
+{
+    if (B >= bbase[M2L_GROUPS]) return false;
+    int lo = 0, hi = M2L_GROUPS - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (bbase[mid] <= B) lo = mid;
+        else hi = mid - 1;
+    }
+    grp = lo;
+    int e0 = (B - bbase[grp]) * M2L_TPB * 16;
+    ne_ = min(M2L_TPB * 16, gbase[grp + 1] - gbase[grp] - e0);
+    first = gbase[grp] + e0;
+    return true;
 }
 
 // One block per list: its entries (target q, source box) in q order, and for
@@ -2004,8 +2115,7 @@ __global__ __launch_bounds__(256) void k_m2l_fill(int nd, const int *__restrict_
 // double-buffered, loaded two chunks ahead: while chunk s's products run, chunk
 // s + 1's data goes into the other buffer and chunk s + 2's loads are in flight.
 template <int MT, int KS, int TPB, int WG>
-__global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__restrict__ cnt,
-                                                           const int *__restrict__ base,
+__global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__restrict__ gbase,
                                                            const int *__restrict__ bbase,
                                                            const int2 *__restrict__ list,
                                                            const double *__restrict__ O,
@@ -2018,24 +2128,11 @@ __global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__r
     constexpr int C = (KS + NCH - 1) / NCH;     // k steps per chunk
     constexpr int BCH = C * TPB * 64;           // doubles of one chunk of expansions
     constexpr int PER = (BCH + NT - 1) / NT;    // chunk loads per thread
-    static_assert(TPB % WG == 0, "tiles per wave");
+    static_assert(TPB % WG == 0 && TPB == M2L_TPB, "tiles per wave");
     __shared__ double s_b[2][BCH];
     __shared__ int s_src[TPB * 16];
-    int B = blockIdx.x;
-    if (B >= bbase[M2L_LISTS]) return;
-    // the list of block B: the last list whose first block is <= B
-    int lo = 0, hi = M2L_LISTS - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (bbase[mid] <= B) lo = mid;
-        else hi = mid - 1;
-    }
-    int L = lo, par = L / 216, ci = L % 216;
-    int e0 = (B - bbase[L]) * TPB * 16, ne_ = min(TPB * 16, cnt[L] - e0);
-    int first = base[L] + e0;
-    int dx, dy, dz;
-    m2l_offset(par, ci, dx, dy, dz);
-    int off = (dx + 3) + 7 * ((dy + 3) + 7 * (dz + 3));  // operator index in [-3, 3]^3
+    int off, first, ne_;  // off: the operator index in [-3, 3]^3
+    if (!m2l_block(gbase, bbase, blockIdx.x, off, first, ne_)) return;
     for (int j = threadIdx.x; j < TPB * 16; j += NT) s_src[j] = j < ne_ ? list[first + j].y : -1;
     __syncthreads();
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -2139,12 +2236,113 @@ __global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__r
     }
 }
 
+// Mixed precision: each box's real-expanded expansion (2 lp values) as the
+// B-operand parts of k_m2l_gemm_bf3: OB[box][ks][part][32], part 0..2 =
+// hi, mid, lo of column 32 ks + c (zero past 2 lp).
+template <int KS>
+__global__ __launch_bounds__(256) void k_m2l_split(int nbox, int lp2, const double *__restrict__ O,
+                                                   unsigned short *__restrict__ OB)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbox * 32 * KS) return;
+    int box = t / (32 * KS), k = t % (32 * KS);
+    double x = k < lp2 ? O[(size_t)box * lp2 + k] : 0.0;
+    unsigned short h, m, l;
+    bf16_split3(x, h, m, l);
+    unsigned short *dst = OB + ((size_t)box * KS + k / 32) * 96 + k % 32;
+    dst[0] = h;
+    dst[32] = m;
+    dst[64] = l;
+}
+
+// Mixed-precision M2L GEMM: one block per M2L_TPB x 16 consecutive entries of
+// one offset group, WM x WT wavefronts; wave (wm, wt) computes the 16-row
+// slabs wm MTW .. + MTW - 1 of the operator against tiles wt TWW .. + TWW - 1,
+// each a register-resident FP32 accumulator.  Per 32-wide k step a wave loads
+// its slabs' three operator parts (AFB: [group][ks][slab][part][lane] x 8
+// bf16, the A-operand layout of v_mfma_f32_16x16x32_bf16: lane l holds row
+// l % 16, columns 8 (l / 16) .. + 7) and its tiles' three expansion parts
+// (lane l: entry l % 16, columns 8 (l / 16) .. + 7), then runs the six
+// products lo x hi, mid x mid, hi x lo, mid x hi, hi x mid, hi x hi (smallest
+// first) per (slab, tile).  Output lane l holds rows 4 (l / 16) .. + 3 of
+// entry l % 16.
+template <int MTW, int WM, int TWW, int WT, int KS>
+__global__ __launch_bounds__(64 * WM * WT) void k_m2l_gemm_bf3(const int *__restrict__ gbase,
+                                                               const int *__restrict__ bbase,
+                                                               const int2 *__restrict__ list,
+                                                               const m2l_u4 *__restrict__ OB,
+                                                               const m2l_u4 *__restrict__ AFB,
+                                                               float *__restrict__ res)
+{
+    constexpr int MT = MTW * WM, TPB = TWW * WT;
+    static_assert(TPB == M2L_TPB, "block tiles");
+    int grp, first, ne_;
+    if (!m2l_block(gbase, bbase, blockIdx.x, grp, first, ne_)) return;
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w % WM, wt = w / WM;
+    int n = lane & 15, kq = lane >> 4;
+    int si[TWW];
+#pragma unroll
+    for (int u = 0; u < TWW; u++) {
+        int j = 16 * (wt * TWW + u) + n;
+        si[u] = j < ne_ ? list[first + j].y : -1;  // -1: no entry (its column stays zero)
+    }
+    m2l_f4 acc[MTW][TWW];
+#pragma unroll
+    for (int i = 0; i < MTW; i++)
+#pragma unroll
+        for (int u = 0; u < TWW; u++) acc[i][u] = (m2l_f4){0.f, 0.f, 0.f, 0.f};
+    const m2l_u4 *ab = AFB + ((size_t)grp * KS * MT + wm * MTW) * 3 * 64 + lane;
+    const m2l_u4 zero = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+        m2l_u4 a[MTW][3], b[TWW][3];
+#pragma unroll
+        for (int i = 0; i < MTW; i++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) a[i][q] = ab[((size_t)(ks * MT + i) * 3 + q) * 64];
+#pragma unroll
+        for (int u = 0; u < TWW; u++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                m2l_u4 v = OB[(((size_t)max(si[u], 0) * KS + ks) * 3 + q) * 4 + kq];
+                b[u][q] = si[u] >= 0 ? v : zero;
+            }
+#pragma unroll
+        for (int i = 0; i < MTW; i++)
+#pragma unroll
+            for (int u = 0; u < TWW; u++) {
+                m2l_bf8 a0 = __builtin_bit_cast(m2l_bf8, a[i][0]), a1 = __builtin_bit_cast(m2l_bf8, a[i][1]),
+                        a2 = __builtin_bit_cast(m2l_bf8, a[i][2]);
+                m2l_bf8 b0 = __builtin_bit_cast(m2l_bf8, b[u][0]), b1 = __builtin_bit_cast(m2l_bf8, b[u][1]),
+                        b2 = __builtin_bit_cast(m2l_bf8, b[u][2]);
+                m2l_f4 c = acc[i][u];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b0, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b2, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, c, 0, 0, 0);
+                acc[i][u] = c;
+            }
+    }
+#pragma unroll
+    for (int u = 0; u < TWW; u++) {
+        int j = 16 * (wt * TWW + u) + n;
+        if (j >= ne_) continue;
+        float *dst = res + (size_t)(first + j) * M2L_RES + 4 * kq;
+#pragma unroll
+        for (int i = 0; i < MTW; i++) *(m2l_f4 *)(dst + 16 * (wm * MTW + i)) = acc[i][u];
+    }
+}
+
 // Adds each target's results in candidate order onto L (the L2L part already
 // there): one wavefront per target box, lane r its k >= 0 row r.  The 216
-// result positions of the target are loaded at once (four per lane) and
-// taken lane by lane; each result row is read by consecutive lanes.
+// result positions of the target are loaded at once (four per lane); the
+// candidates that have a result are taken eight at a time, their rows loaded
+// together and then added in candidate order.
+template <typename T>
 __global__ __launch_bounds__(64) void k_m2l_accum(Geo g, int nd, const int *__restrict__ idx,
-                                                  const double *__restrict__ res, double2 *__restrict__ L)
+                                                  const T *__restrict__ res, double2 *__restrict__ L)
 {
     int b = blockIdx.x, row = threadIdx.x, lk = g.lk;
     const int *ix = idx + (size_t)b * 216;
@@ -2155,12 +2353,31 @@ __global__ __launch_bounds__(64) void k_m2l_accum(Geo g, int nd, const int *__re
     double re = 0.0, im = 0.0;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-        for (int l = 0; l < 64 && 64 * u + l < 216; l++) {
-            int pos = __builtin_amdgcn_readlane(pv[u], l);
-            if (pos < 0) continue;  // wave-uniform
-            const double *r = res + (size_t)pos * M2L_RES + 2 * rr;
-            re += r[0];
-            im += r[1];
+        unsigned long long m = __builtin_amdgcn_ballot_w64(pv[u] >= 0);
+        while (m) {  // wave-uniform
+            int ps[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                ps[q] = -1;
+                if (m) {
+                    int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    ps[q] = __builtin_amdgcn_readlane(pv[u], l);
+                }
+            }
+            T vr[8], vi[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const T *r = res + (size_t)max(ps[q], 0) * M2L_RES + 2 * rr;
+                vr[q] = r[0];
+                vi[q] = r[1];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (ps[q] >= 0) {
+                    re += (double)vr[q];
+                    im += (double)vi[q];
+                }
         }
     }
     if (row >= lk) return;
@@ -3620,10 +3837,40 @@ static int upload_ops(sesmd_ctx *c)
                     af.push_back(v);
                 }
     };
+    // mixed precision: the same operators as bf16 parts in the A-operand
+    // layout of v_mfma_f32_16x16x32_bf16, fragments [ks][slab][part] of 64
+    // lanes x 8 values; lane l of fragment (ks, slab) holds row 16 slab + l % 16,
+    // columns 32 ks + 8 (l / 16) .. + 7
+    const int KS32 = (2 * lp + 31) / 32;
+    std::vector<unsigned short> afb;
+    auto append_bf3 = [&]() {
+        for (int ks = 0; ks < KS32; ks++)
+            for (int mt = 0; mt < MT; mt++) {
+                size_t at = afb.size();
+                afb.resize(at + (size_t)3 * 64 * 8, 0);
+                for (int l = 0; l < 64; l++)
+                    for (int e = 0; e < 8; e++) {
+                        int mm = 16 * mt + l % 16, kk = 32 * ks + 8 * (l / 16) + e;
+                        double v = 0.0;
+                        if (mm < 2 * lk && kk < 2 * lp) {
+                            const hostm::C &z = T[(size_t)(mm / 2) * lp + kk / 2];
+                            int ro = mm & 1, ci = kk & 1;
+                            v = ro == ci ? z.real() : (ro == 0 ? -z.imag() : z.imag());
+                        }
+                        unsigned short h, m, lo;
+                        bf16_split3(v, h, m, lo);
+                        afb[at + (size_t)(0 * 64 + l) * 8 + e] = h;
+                        afb[at + (size_t)(1 * 64 + l) * 8 + e] = m;
+                        afb[at + (size_t)(2 * 64 + l) * 8 + e] = lo;
+                    }
+            }
+    };
+    bool mixed = c->precision == SESMD_MIXED;
     // M2L at L4 and L3: offsets in [-3, 3]^3 (spacing rs and 2 rs)
     for (int lvl = 0; lvl < 2; lvl++) {
         buf.clear();
         af.clear();
+        afb.clear();
         double sp = lvl == 0 ? g.rs : 2.0 * g.rs;
         for (int dz = -3; dz <= 3; dz++)
             for (int dy = -3; dy <= 3; dy++)
@@ -3635,8 +3882,12 @@ static int upload_ops(sesmd_ctx *c)
                     }
                     append();
                     append_frag();
+                    if (mixed) append_bf3();
                 }
         CK(put(lvl == 0 ? c->T4 : c->T3, buf.size()));
+        if (mixed)
+            HC(hipMemcpy(lvl == 0 ? c->AFB4 : c->AFB3, afb.data(), sizeof(unsigned short) * afb.size(),
+                         hipMemcpyHostToDevice));
         af.resize(af.size() + (size_t)MT * KS * 64, 0.0);  // the spare offset
         HC(hipMemcpy(lvl == 0 ? c->AF4 : c->AF3, af.data(), sizeof(double) * af.size(),
                      hipMemcpyHostToDevice));
@@ -3848,8 +4099,15 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
         A(dalloc(&c->m2l_idx, pairs));
         A(dalloc(&c->m2l_cnt, M2L_LISTS));
         A(dalloc(&c->m2l_base, M2L_LISTS));
-        A(dalloc(&c->m2l_bbase, M2L_LISTS + 1));
-        c->m2l_max_blocks = (int)(pairs / (M2L_TPB * 16)) + M2L_LISTS;
+        A(dalloc(&c->m2l_gbase, M2L_GROUPS + 1));
+        A(dalloc(&c->m2l_bbase, M2L_GROUPS + 1));
+        c->m2l_max_blocks = (int)(pairs / (M2L_TPB * 16)) + M2L_GROUPS;
+        if (precision == SESMD_MIXED) {
+            size_t ks32 = (2 * g.lp + 31) / 32;
+            A(dalloc(&c->AFB4, (size_t)M2L_GROUPS * ks32 * mt * 3 * 64 * 8));
+            A(dalloc(&c->AFB3, (size_t)M2L_GROUPS * ks32 * mt * 3 * 64 * 8));
+            A(dalloc(&c->m2l_ob, (size_t)g.nb3 * ks32 * 96));
+        }
     }
     A(dalloc(&c->T3, (size_t)343 * g.lk * g.lp));
     int w2 = 2 * nb2 - 1;
@@ -3956,7 +4214,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
                     c->AF4, c->AF3, c->m2l_res, c->m2l_list, c->m2l_idx, c->m2l_cnt, c->m2l_base,
-                    c->m2l_bbase,
+                    c->m2l_gbase, c->m2l_bbase, c->AFB4, c->AFB3, c->m2l_ob,
                     c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -4105,18 +4363,35 @@ static void launch_m2l(sesmd_ctx *c, int nd, const double2 *O, const int *ne, co
     int h = nd / 2;
     if (g.p == 4 || g.p == 8) {
         hipStream_t st = c->stream;
-        const double *AF = nd == g.nb ? c->AF4 : c->AF3;
         const double *Od = (const double *)O;
         k_m2l_count<<<M2L_LISTS, 256, 0, st>>>(nd, ne, c->m2l_cnt);
-        k_m2l_scan<<<1, 1024, 0, st>>>(c->m2l_cnt, c->m2l_base, c->m2l_bbase);
+        k_m2l_scan<<<1, 512, 0, st>>>(c->m2l_cnt, c->m2l_base, c->m2l_gbase, c->m2l_bbase);
         k_m2l_fill<<<M2L_LISTS, 256, 0, st>>>(nd, ne, c->m2l_base, c->m2l_list, c->m2l_idx);
+        if (c->precision == SESMD_MIXED) {
+            const m2l_u4 *AFB = (const m2l_u4 *)(nd == g.nb ? c->AFB4 : c->AFB3);
+            const m2l_u4 *OB = (const m2l_u4 *)c->m2l_ob;
+            float *res = (float *)c->m2l_res;
+            int nbox = nd * nd * nd;
+            if (g.p == 4) {  // two slabs, two 32-wide k steps: four waves of two tiles
+                k_m2l_split<2><<<nblk(nbox * 64, 256), 256, 0, st>>>(nbox, 2 * g.lp, Od, c->m2l_ob);
+                k_m2l_gemm_bf3<2, 1, 2, 4, 2><<<c->m2l_max_blocks, 256, 0, st>>>(c->m2l_gbase, c->m2l_bbase,
+                                                                                  c->m2l_list, OB, AFB, res);
+            } else {  // six slabs, six k steps: four waves of 3 slabs x 4 tiles
+                k_m2l_split<6><<<nblk(nbox * 192, 256), 256, 0, st>>>(nbox, 2 * g.lp, Od, c->m2l_ob);
+                k_m2l_gemm_bf3<3, 2, 4, 2, 6><<<c->m2l_max_blocks, 256, 0, st>>>(c->m2l_gbase, c->m2l_bbase,
+                                                                                  c->m2l_list, OB, AFB, res);
+            }
+            k_m2l_accum<float><<<nbox, 64, 0, st>>>(g, nd, c->m2l_idx, res, L);
+            return;
+        }
+        const double *AF = nd == g.nb ? c->AF4 : c->AF3;
         if (g.p == 4)  // two slabs of rows: two tile halves per block, four waves
             k_m2l_gemm<2, 13, M2L_TPB, 2><<<c->m2l_max_blocks, 256, 0, st>>>(
-                g, c->m2l_cnt, c->m2l_base, c->m2l_bbase, c->m2l_list, Od, AF, c->m2l_res);
+                g, c->m2l_gbase, c->m2l_bbase, c->m2l_list, Od, AF, c->m2l_res);
         else  // six slabs: six waves
             k_m2l_gemm<6, 41, M2L_TPB, 1><<<c->m2l_max_blocks, 384, 0, st>>>(
-                g, c->m2l_cnt, c->m2l_base, c->m2l_bbase, c->m2l_list, Od, AF, c->m2l_res);
-        k_m2l_accum<<<nd * nd * nd, 64, 0, st>>>(g, nd, c->m2l_idx, c->m2l_res, L);
+                g, c->m2l_gbase, c->m2l_bbase, c->m2l_list, Od, AF, c->m2l_res);
+        k_m2l_accum<double><<<nd * nd * nd, 64, 0, st>>>(g, nd, c->m2l_idx, c->m2l_res, L);
         return;
     }
     if (g.lk <= 16) {
@@ -4182,7 +4457,7 @@ static void launch_near(sesmd_ctx *c, uint64_t mask)
         dim3 grid(c->max_rwork), block(NEAR_TILE);
 #define ROWS_ARGS \
     g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind, c->rwork, c->nrwork, c->dff, c->spos32, \
-        c->lj32, c->npart
+        c->lj32, c->npart, c->counters
         if (mix && hq) k_near_rows<true, true><<<grid, block, 0, c->stream>>>(ROWS_ARGS);
         else if (mix) k_near_rows<false, true><<<grid, block, 0, c->stream>>>(ROWS_ARGS);
         else if (hq) k_near_rows<true, false><<<grid, block, 0, c->stream>>>(ROWS_ARGS);
@@ -4627,7 +4902,7 @@ extern "C" int sesmd_get_flags(sesmd_ctx *c, int *flags, int n)
     if (n < 0 || (n > 0 && !flags)) return fail("bad flag buffer");
     int t[4];
     HC(hipMemcpy(t, c->counters, sizeof t, hipMemcpyDeviceToHost));
-    for (int k = 0; k < n; k++) flags[k] = k < 3 ? t[k] : 0;
+    for (int k = 0; k < n; k++) flags[k] = k < 4 ? t[k] : 0;
     return 0;
 }
 

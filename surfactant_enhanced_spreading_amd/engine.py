@@ -229,10 +229,11 @@ class Engine:
 
     def flags(self):
         """Device error counters: SHAKE/RATTLE non-convergence, steps with
-        non-finite kinetic sums, molecules filed outside the cube."""
-        f = (ctypes.c_int * 3)()
-        _ck(self.L.sesmd_get_flags(self.h, f, 3))
-        return dict(shake=f[0], nonfinite=f[1], outside=f[2])
+        non-finite kinetic sums, molecules filed outside the cube, mixed
+        near-field items whose fixed-point accumulation could leave its range."""
+        f = (ctypes.c_int * 4)()
+        _ck(self.L.sesmd_get_flags(self.h, f, 4))
+        return dict(shake=f[0], nonfinite=f[1], outside=f[2], fixed_range=f[3])
 
     # reference routine families ---------------------------------------------
     def rebuild(self):

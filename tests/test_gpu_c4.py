@@ -79,7 +79,7 @@ def test_c4_mixed_tracks_fp64(c4):
         flags = e32.flags()
         print('c4 mixed rms at start', rms0, 'after 100 steps', rms100, 'flags', flags, flush=True)
         assert rms0 <= 1e-5 and rms100 <= 1e-5
-        assert flags['nonfinite'] == 0 and flags['outside'] == 0
+        assert flags['nonfinite'] == 0 and flags['outside'] == 0 and flags['fixed_range'] == 0
     finally:
         e32.close()
         e64.close()

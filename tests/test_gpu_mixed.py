@@ -1,4 +1,5 @@
-"""Mixed precision (SESMD_MIXED: FP32 pair arithmetic, FP64 accumulation and
+"""Mixed precision (SESMD_MIXED: FP32 pair arithmetic with 64-bit fixed-point
+accumulation, M2L as split-BF16x3 MFMA products accumulated in FP32, and
 everything else FP64) against the FP64 engine, which is itself checked
 against the oracle to 1e-10.
 
@@ -6,7 +7,10 @@ Bar (the accuracy target of the mixed mode): RMS relative force error
 ||F_mixed - F_fp64||_2 / ||F_fp64||_2 <= 1e-5 over all sites, on the test
 systems and on the 65,800-site droplet the benchmark runs; no site worse than
 1e-3 of its own force scale (per-site error floored at 1e-3 of the RMS force).
-Integer work (classes, boxes, sort) is shared and must be identical.
+Integer work (classes, boxes, sort) is shared and must be identical.  The
+expansions: multipoles and the top level identical (FP64 in both modes); the
+M2L levels within 1e-6 of their largest coefficient (three bf16 parts carry
+the 24 bits of an FP32 value; six of the nine cross products are kept).
 """
 import numpy as np
 import pytest
@@ -77,3 +81,24 @@ def test_mixed_steps_track_fp64(sys_medium):
     finally:
         e64.close()
         e32.close()
+
+
+@pytest.mark.parametrize('nb2,p', [(3, 4), (4, 8)])
+def test_mixed_m2l_bf3(sys_medium, nb2, p):
+    from surfactant_enhanced_spreading_amd import engine
+    es = [engine.Engine(sys_medium, nb2=nb2, p=p, precision=prec) for prec in (engine.FP64, engine.MIXED)]
+    try:
+        for e in es:
+            e.rebuild()
+            e.fmm()
+        A, B = es[0].expansions(), es[1].expansions()
+        for k in ('O4', 'O3', 'O2', 'L2'):
+            assert np.array_equal(A[k], B[k]), k
+        for k in ('L3', 'L4'):
+            scale = np.abs(A[k]).max()
+            err = np.abs(A[k] - B[k]).max() / scale
+            print('p', p, k, 'bf16x3 error / largest coefficient', err, flush=True)
+            assert 0.0 < err <= 1e-6, (k, err)
+    finally:
+        for e in es:
+            e.close()

@@ -55,7 +55,7 @@ def test_error_flags(sys_small):
         e.bootstrap()
         e.steps(12, 10, 0)
         f = e.flags()
-        assert f == dict(shake=e.shake_failures(), nonfinite=0, outside=0)
+        assert f == dict(shake=e.shake_failures(), nonfinite=0, outside=0, fixed_range=0)
         r, v, a, eta = e.state()
         v[0] = 1e160  # one water oxygen: |v|^2 overflows to infinity
         e.set_state(r, v, a, eta)
