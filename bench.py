@@ -250,6 +250,75 @@ def rms_force_error(sys_, cfg, state=None):
     return float(np.sqrt(((f[1] - f[0]) ** 2).sum() / (f[0] ** 2).sum()))
 
 
+def ensemble_leg(timer, steps, warmup, n=64):
+    """The batched-ensemble workload: n droplets (builder.ensemble_systems),
+    mixed precision, p = 8, one context each, advanced by one ensemble (one
+    graph per step, the near field of all droplets in one persistent launch
+    whose shape the autotuner picks).  Times `steps` ensemble steps after
+    `warmup` (whole ten-step cycles; graphs captured before the timed region)
+    and the ensemble near-field launch alone; the FP32 rate counts 21 flops
+    per pair and 9 more per LJ pair, as the headline roofline does."""
+    from surfactant_enhanced_spreading_amd import builder, engine
+    t0 = time.perf_counter()
+    systems = builder.ensemble_systems(n=n)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    members = []
+    for s, nb2 in systems:
+        e = engine.Engine(s, nb2=nb2, p=8, precision=engine.MIXED)
+        e.bootstrap()
+        members.append(e)
+    t_create = time.perf_counter() - t0
+    print('bench: ensemble of %d built (%.1f s) and created (%.1f s)' % (n, t_build, t_create),
+          file=sys.stderr, flush=True)
+    pairs = lj = 0.0
+    for e in members:
+        a, b = e.pair_counts()
+        pairs += a
+        lj += b
+    ens = engine.Ensemble(members)
+    shape, tune_ms = ens.tune(reps=3)
+    stream = ens.stream()
+    kb = ens.steps(warmup, 10, 9)  # kb = 9: the first step is a rebuild, every run whole cycles
+    timer.sync()
+    e0, e1 = timer.event(), timer.event()
+    timer.record(e0, stream)
+    kb = ens.steps(steps, 10, kb)
+    timer.record(e1, stream)
+    ms = timer.elapsed_ms(e0, e1) / steps
+    timer.sync()
+    reps = 10
+    timer.record(e0, stream)
+    for _ in range(reps):
+        ens.near_only()
+    timer.record(e1, stream)
+    near_ms = timer.elapsed_ms(e0, e1) / reps
+    timer.sync()
+    flags = [e.flags() for e in members]
+    ens.close()
+    sites = sum(s.n_sites for s, _ in systems)
+    for e in members:
+        e.close()
+    ach = (21.0 * pairs + 9.0 * lj) / (near_ms * 1e-3) / 1e12
+    return dict(workload='%d droplets, waters log-uniform 500-30,000, 0-5 wt%% surfactant, graphite slab '
+                         '3 diameters wide, domains of (4 nb2)^3 boxes of 12 A by size, p=8, mixed '
+                         'precision, ifar=10, 2 fs' % n,
+                droplets=n, sites=sites, carbons=int(sum(len(s.graphite) for s, _ in systems)),
+                nb2_counts={str(k): sum(1 for _, b in systems if b == k) for k in sorted({b for _, b in systems})},
+                steps=steps, warmup=warmup, ms_per_step=round(ms, 4),
+                steps_per_s=round(1000.0 / ms, 2),
+                droplet_ns_per_day=round(1000.0 / ms * n * 2.0 * 0.0864, 2),
+                near_kernel='k_near_rows_ens (persistent, %d waves/workgroup, %d receivers/wave, %d workgroups)'
+                            % shape,
+                near_kernel_ms=round(near_ms, 4), tune_ms=round(tune_ms, 4),
+                pairs=pairs, lj_pairs=lj, near_achieved_tflops=round(ach, 3),
+                near_frac_fp32_peak=round(ach / FP32_PEAK_TFLOPS, 4),
+                near_frac_fp32_unpacked_peak=round(ach / FP64_PEAK_TFLOPS, 4),
+                shake_failures=sum(f['shake'] for f in flags),
+                flags_nonzero=sum(1 for f in flags if any(f.values())),
+                build_s=round(t_build, 1), create_s=round(t_create, 1))
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_near_rows_counters_c2_fp64.md')
 
 
@@ -303,6 +372,9 @@ def main():
                     help='steps of the energy-drift runs (FP64 and mixed); 0 skips them')
     ap.add_argument('--ifar-steps', type=int, default=2000,
                     help='steps of each run of the drift-vs-ifar series (ifar 1, 2, 5, 10); 0 skips it')
+    ap.add_argument('--no-ensemble', action='store_true', help='skip the batched-ensemble leg')
+    ap.add_argument('--ensemble-only', action='store_true', help='run only the batched-ensemble leg')
+    ap.add_argument('--ensemble-steps', type=int, default=50)
     ap.add_argument('--copy-calibrate', type=int, default=0, metavar='BYTES',
                     help='only run a device copy of BYTES (counter calibration) and exit')
     args = ap.parse_args()
@@ -317,8 +389,11 @@ def main():
     from surfactant_enhanced_spreading_amd import engine
     name = args.config
     cfg = CONFIGS[name]
-    sys_ = make_system(name)
     timer = HipTimer()
+    if args.ensemble_only:
+        print(json.dumps(ensemble_leg(timer, args.ensemble_steps, 10)), flush=True)
+        return
+    sys_ = make_system(name)
     print('bench: %s, %d sites, nb2=%d, p=%d' % (name, sys_.n_sites, cfg['nb2'], cfg['p']),
           file=sys.stderr, flush=True)
     r64, e = run(sys_, cfg, engine.FP64, args.steps, args.warmup, timer)
@@ -387,6 +462,9 @@ def main():
         out['drift_vs_ifar'] = [energy_drift(sys_, cfg, engine.FP64, args.ifar_steps, ifar=k)
                                 for k in (1, 2, 5, 10)]
         print('bench: ifar series done', file=sys.stderr, flush=True)
+    if not args.no_ensemble and name == 'c2':
+        out['ensemble'] = ensemble_leg(timer, args.ensemble_steps, 10)
+        print('bench: ensemble done', file=sys.stderr, flush=True)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(sys_, cfg['nb2'], cfg['p'], args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -245,6 +245,33 @@ enum {
  * steps run since the context was created. */
 int sesmd_energies(sesmd_ctx *ctx, double *e);
 
+/* Ensembles: n independent systems (one built context each, same precision
+ * and device, equal hydrogen charges) advanced together -- the reference's
+ * step loop (src/mybox.f90:130-241) for every member in one captured graph,
+ * with the near field of all members in one persistent launch over their
+ * flattened row-item lists.  Each member's state after sesmd_ensemble_steps
+ * is bit-for-bit what sesmd_steps on that context alone gives.  Members stay
+ * owned by the caller and must outlive the ensemble; while an ensemble steps
+ * them, their own calls must not run.  No counterpart in the reference (it
+ * runs one system per MPI job). */
+typedef struct sesmd_ensemble sesmd_ensemble;
+int sesmd_ensemble_create(sesmd_ensemble **out, sesmd_ctx *const *members, int n);
+int sesmd_ensemble_destroy(sesmd_ensemble *ens);
+/* launch shape of the ensemble's near field: wavefronts per workgroup (1, 2,
+ * 4), receivers per wavefront (64, 128), workgroups (the persistent grid) */
+int sesmd_ensemble_set_shape(sesmd_ensemble *ens, int waves, int chunk, int grid);
+int sesmd_ensemble_get_shape(sesmd_ensemble *ens, int *shape3);
+/* the autotuner: times every shape (reps launches each) at the members'
+ * current state, keeps and returns the fastest and its time per launch */
+int sesmd_ensemble_tune(sesmd_ensemble *ens, int reps, int *shape3, float *best_ms);
+/* the near field of every member once, enqueued on the ensemble's stream
+ * (timing) */
+int sesmd_ensemble_near_only(sesmd_ensemble *ens);
+/* the ensemble's stream (a hipStream_t) */
+void *sesmd_ensemble_stream(sesmd_ensemble *ens);
+/* nsteps steps of every member; kb is the shared step counter (sesmd_steps) */
+int sesmd_ensemble_steps(sesmd_ensemble *ens, int nsteps, int ifar, int *kb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,7 +110,18 @@ def surfactant_template():
     """One extended M(D'E4OH)M conformer whose 28 bond lengths equal the
     constraint lengths exactly and whose angles sit at their equilibria; the
     siloxane-head dihedrals are picked on a small grid to maximise the closest
-    approach of sites three or more bonds apart."""
+    approach of sites three or more bonds apart.  (Computed once; a copy is
+    returned.)"""
+    global _TEMPLATE
+    if _TEMPLATE is None:
+        _TEMPLATE = _surfactant_template()
+    return _TEMPLATE.copy()
+
+
+_TEMPLATE = None
+
+
+def _surfactant_template():
     blen = {}
     for b in range(28):
         i, j = ff.IBOND[b], ff.JBOND[b]
@@ -259,18 +270,17 @@ def graphite_sheet(half_side_A=60.0, z_top=-16.0, layer_gap_A=3.41, ccbond_A=1.4
     ax = np.array([1.5 * a, math.sqrt(3) / 2 * a])
     ay = np.array([0.0, math.sqrt(3) * a])
     n = int(h / a) + 3
+    k = np.arange(-n, n + 1, dtype=float)
     out = []
     for layer in range(layers):
         zl = z_top - layer * layer_gap_A / ff.SIGMA
         shift = np.array([a, 0.0]) * (layer % 2)
-        for i in range(-n, n + 1):
-            for j in range(-n, n + 1):
-                base = i * ax + j * ay + shift
-                for b in (np.zeros(2), np.array([a, 0.0])):
-                    p = base + b
-                    if abs(p[0]) <= h and abs(p[1]) <= h:
-                        out.append((p[0], p[1], zl))
-    g = np.round(np.array(out), 4)
+        # lattice points in (i, j, basis atom) order
+        base = k[:, None, None] * ax + k[None, :, None] * ay + shift      # (i, j, 2)
+        pts = (base[:, :, None, :] + np.array([[0.0, 0.0], [a, 0.0]])[None, None]).reshape(-1, 2)
+        pts = pts[(np.abs(pts[:, 0]) <= h) & (np.abs(pts[:, 1]) <= h)]
+        out.append(np.column_stack([pts, np.full(len(pts), zl)]))
+    g = np.round(np.concatenate(out, axis=0), 4)
     return g
 
 
@@ -393,3 +403,39 @@ def c4_system(n_water=200000, n_surf=4000, seed=20261207):
     d = film(n_water, n_surf, 200.0, z_top + 3.2, seed=seed)
     g = graphite_sheet(half_side_A=206.0, z_top=z_top / ff.SIGMA, layer_gap_A=3.41)
     return assemble([d], graphite=g, seed=seed)
+
+
+def ensemble_systems(n=64, seed=20261208, n_min=500, n_max=30000, surf_wt_max=0.05, layers=2):
+    """The batched-ensemble workload: n independent droplets, each a system of
+    its own.  Water counts log-uniform in [n_min, n_max]; surfactant loading
+    uniform in [0, surf_wt_max] by mass; each droplet (with the surfactant
+    chains standing out of its surface) rests 3.5 A above a graphite slab whose
+    side is three droplet diameters, in the smallest domain of (4 nb2)^3
+    boxes of 12 A (nb2 >= 2) that holds the slab and the droplet.
+
+    Returns a list of (System, nb2)."""
+    rng = np.random.default_rng(seed)
+    m_w = float(sum(ff.EMW))
+    m_s = float(sum(ff.EMS))
+    rs_A = ff.RS * ff.SIGMA
+    tmpl = surfactant_template()
+    l_A = float(np.linalg.norm(tmpl[17] - tmpl[0])) * ff.SIGMA  # surfactant length
+    out = []
+    for k in range(n):
+        n_w = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        wt = rng.uniform(0.0, surf_wt_max)
+        n_s = int(wt * n_w * m_w / ((1.0 - wt) * m_s))
+        r_A = (3.0 * n_w / WATER_DENSITY_A3 / (4.0 * math.pi)) ** (1.0 / 3.0)
+        # outer radius: droplet() anchors site 1 at max(r - 0.65 l, 0.75 r)
+        # from the centre and points the chain outward
+        r_out = r_A if n_s == 0 else max(r_A + 0.35 * l_A, 0.75 * r_A + l_A) + 1.0
+        half = 3.0 * r_A  # slab side: three diameters
+        need = max(2.0 * half, 2.0 * r_out + 7.0 + 3.5 + 8.0)  # slab; droplet between slab and top face
+        nb2 = max(2, int(math.ceil(need / (4.0 * rs_A))))
+        dom = 4 * nb2 * rs_A
+        half = min(half, dom / 2.0 - 0.5)
+        z_top_A = -dom / 2.0 + 7.0
+        d = droplet(n_w, n_s, center_A=(0.0, 0.0, z_top_A + r_out + 3.5), seed=seed + 101 * k)
+        g = graphite_sheet(half_side_A=half, z_top=z_top_A / ff.SIGMA, layers=layers)
+        out.append((assemble([d], graphite=g, seed=seed + 101 * k), nb2))
+    return out

@@ -4094,7 +4094,8 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
         // M2L lists: every (target, candidate) pair of the L4 level at most
         // (8 classes x 216 candidates x (nb/2)^3 targets), the L3 level less
         size_t pairs = (size_t)216 * g.nb3;
-        A(dalloc(&c->m2l_res, pairs * M2L_RES));
+        // result rows: M2L_RES doubles, or floats for the bf16x3 products
+        A(dalloc(&c->m2l_res, pairs * M2L_RES / (precision == SESMD_MIXED ? 2 : 1)));
         A(dalloc(&c->m2l_list, pairs));
         A(dalloc(&c->m2l_idx, pairs));
         A(dalloc(&c->m2l_cnt, M2L_LISTS));
@@ -4478,7 +4479,10 @@ static void launch_near(sesmd_ctx *c, uint64_t mask)
 // gathered: the sorted copies of the positions are already current (written
 // by the fused part A of a regular step)
 // fold: graphite evaluated inside k_site_terms (the step pipeline's form)
-static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask, bool gathered = false, bool fold = false)
+// The forces around the near field: first the sorted copies of the
+// positions that the near field and the site terms read (unless part A wrote
+// them), then, after the near field, the per-site terms.
+static void forces_gather(sesmd_ctx *c, unsigned what, uint64_t mask, bool gathered)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
@@ -4487,7 +4491,21 @@ static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask, bool gather
     if ((what & SESMD_NEAR) && c->precision == SESMD_MIXED && mask == ~0ull && !gathered)
         k_gather_pos32<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->key_out, c->r, c->type, c->dff,
                                                        c->spos32);
+}
+
+static int forces_sites(sesmd_ctx *c, unsigned what, bool fold);
+
+static int launch_forces(sesmd_ctx *c, unsigned what, uint64_t mask, bool gathered = false, bool fold = false)
+{
+    forces_gather(c, what, mask, gathered);
     if (what & SESMD_NEAR) launch_near(c, mask);
+    return forces_sites(c, what, fold);
+}
+
+static int forces_sites(sesmd_ctx *c, unsigned what, bool fold)
+{
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
     const unsigned intra = SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE;
     (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<nblk(g.n, 128), 128, 0, st>>>(
         g, what, c->r, c->key_out, c->L4, c->lfac, c->dff, c->npart, c->spos, c->smeta, c->f, c->gr,
@@ -4611,11 +4629,34 @@ extern "C" int sesmd_init_accel(sesmd_ctx *c)
 //   A(surf+water) - eta + intra -> fs - [rebuild, far field] - near - site terms - B(surf+water)
 // (A second stream for the eta/intra launch, beside the forces, measured
 // slower: the join before part B costs more than the launch it hides.)
+static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered);
+static int step_part_b(sesmd_ctx *c);
+static const unsigned STEP_INTRA =
+    SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE;
+
+// the force components a step evaluates in launch_forces (with the reference
+// topology the intramolecular terms run beside the eta update)
+static unsigned step_what(const sesmd_ctx *c) { return c->surf_fast ? SESMD_ALL & ~STEP_INTRA : SESMD_ALL; }
+
 static int launch_step(sesmd_ctx *c, bool rebuild)
+{
+    bool gathered = false;
+    CK(step_part_a(c, rebuild, gathered));
+    // one stream, no cross-queue fork and join: with the reference topology
+    // the intramolecular terms were evaluated with the eta update and part B
+    // adds them per site (with the subtraction); graphite runs inside
+    // k_site_terms
+    CK(launch_forces(c, step_what(c), ~0ull, gathered, true));
+    return step_part_b(c);
+}
+
+// part A of a step (with the rebuild and far field when `rebuild`); gathered:
+// part A wrote the sorted position copies
+static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
-    bool gathered = false;
+    gathered = false;
     if (c->surf_fast) {
         SortedOut so;
         so.iperm = c->iperm;
@@ -4648,12 +4689,13 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
         CK(launch_rebuild(c));
         CK(launch_fmm(c));
     }
-    // one stream, no cross-queue fork and join: with the reference topology
-    // the intramolecular terms were evaluated with the eta update and part B
-    // adds them per site (with the subtraction); graphite runs inside
-    // k_site_terms
-    const unsigned intra = SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE;
-    CK(launch_forces(c, c->surf_fast ? SESMD_ALL & ~intra : SESMD_ALL, ~0ull, gathered, true));
+    return 0;
+}
+
+static int step_part_b(sesmd_ctx *c)
+{
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
     if (c->surf_fast) {
         IntraIn in;
         if (g.ns) {
@@ -4764,6 +4806,379 @@ extern "C" int sesmd_steps(sesmd_ctx *c, int nsteps, int ifar, int *kb)
         s++;
     }
     HC(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent systems, one context each, advanced together.  A
+// step of the ensemble is every member's part A (with its rebuild and far
+// field in the first step of a cycle) in member order, then ONE near-field
+// launch over the flattened row-item lists of all members, then every
+// member's site terms and part B -- the whole of it one captured graph on the
+// ensemble's stream.  The near-field launch is persistent: a fixed grid whose
+// wavefronts stride over the flattened items.  An item (member m, row q,
+// receivers c0 ..) runs the arithmetic of k_near_rows on member m's arrays
+// for each receiver -- the same sources in the same order -- so a member's
+// forces are bit-for-bit those of its context stepped alone, whatever the
+// launch shape.  The launch shape (wavefronts per workgroup, receivers per
+// wavefront, workgroups) is chosen by sesmd_ensemble_tune from timed runs.
+struct EnsMember {
+    Geo g;
+    const double4 *spos;
+    const int4 *smeta;
+    const int *bstart, *bs4, *bflag, *rkind;
+    const int2 *rwork;
+    const int *nrwork;
+    const sesmd_forcefield *ff;
+    const float4 *spos32;
+    const float *lj32;
+    double *part;
+    int *counters;
+};
+
+struct sesmd_ensemble {
+    int n;
+    std::vector<sesmd_ctx *> m;
+    std::vector<hipStream_t> own;  // the members' own streams (restored after each use)
+    hipStream_t stream;
+    EnsMember *dm;                 // device copy of the member table
+    int *prefix;                   // first flattened item of each member; [n] = total
+    bool mix;
+    int waves, chunk, grid;        // launch shape: wavefronts per workgroup, receivers per wavefront, workgroups
+    hipGraphExec_t g_cycle, g_step;
+    int cycle_len;
+    bool step_ok;
+};
+
+// flattened item counts: member i contributes its row items x (NEAR_CHUNK / chunk)
+__global__ void k_ens_prefix(const EnsMember *__restrict__ dm, int n, int per, int *__restrict__ prefix)
+{
+    if (threadIdx.x != 0) return;
+    int run = 0;
+    for (int i = 0; i < n; i++) {
+        prefix[i] = run;
+        run += *dm[i].nrwork * per;
+    }
+    prefix[n] = run;
+}
+
+template <bool MIX>
+__global__ __launch_bounds__(256) void k_near_rows_ens(const EnsMember *__restrict__ dm,
+                                                       const int *__restrict__ prefix, int n, int chunk)
+{
+    int lane = threadIdx.x & 63;
+    int W = blockDim.x >> 6;
+    int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int per = NEAR_CHUNK / chunk;
+    int total = prefix[n];
+    int stride = gridDim.x * W;
+    for (int i = blockIdx.x * W + wv; i < total; i += stride) {  // wave-uniform
+        int lo = 0, hi = n - 1;  // the member: the last whose first item is <= i
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const EnsMember &M = dm[lo];
+        int it = i - prefix[lo];
+        int item = it / per, sub = it % per;
+        const Geo &g = M.g;
+        int nb = g.nb;
+        int2 w = M.rwork[item];
+        int q = w.x;
+        int x0 = q % nb, y = (q / nb) % nb, z = q / (nb * nb);
+        // the row's receiver list in the wave's LDS slot, as in k_near_rows;
+        // every lane stores the same values, so each lane reads back the
+        // list of this item whichever store lands last
+        __shared__ RowRecv s_L[4];
+        RowRecv &L = s_L[wv];
+        {
+            RowRecv Lr;
+            row_receivers(g, M.bstart, x0, y, z, Lr);
+            L = Lr;
+        }
+        int cend = min(L.pre[9], w.y + NEAR_CHUNK);
+        int s0 = w.y + sub * chunk, s1 = min(cend, s0 + chunk);
+        if (s0 >= s1) continue;
+        if (s1 - s0 > NEAR_TILE)
+            near_rows_item<true, MIX, 2>(g, M.spos, M.smeta, M.bs4, M.bflag, M.rkind, M.ff, M.spos32, M.lj32,
+                                         M.part, L, x0, y, z, s0, s1, lane, M.counters);
+        else
+            near_rows_item<true, MIX, 1>(g, M.spos, M.smeta, M.bs4, M.bflag, M.rkind, M.ff, M.spos32, M.lj32,
+                                         M.part, L, x0, y, z, s0, s1, lane, M.counters);
+    }
+}
+
+// every member launches on the ensemble's stream while one of these lives
+struct EnsStreams {
+    sesmd_ensemble *e;
+    explicit EnsStreams(sesmd_ensemble *e_) : e(e_)
+    {
+        for (int i = 0; i < e->n; i++) e->m[i]->stream = e->stream;
+    }
+    ~EnsStreams()
+    {
+        for (int i = 0; i < e->n; i++) e->m[i]->stream = e->own[i];
+    }
+};
+
+static void ens_near(sesmd_ensemble *e)
+{
+    k_ens_prefix<<<1, 64, 0, e->stream>>>(e->dm, e->n, NEAR_CHUNK / e->chunk, e->prefix);
+    if (e->mix)
+        k_near_rows_ens<true><<<e->grid, 64 * e->waves, 0, e->stream>>>(e->dm, e->prefix, e->n, e->chunk);
+    else
+        k_near_rows_ens<false><<<e->grid, 64 * e->waves, 0, e->stream>>>(e->dm, e->prefix, e->n, e->chunk);
+}
+
+static int ens_step(sesmd_ensemble *e, bool rebuild)
+{
+    EnsStreams on(e);
+    for (int i = 0; i < e->n; i++) {
+        sesmd_ctx *c = e->m[i];
+        bool gathered = false;
+        CK(step_part_a(c, rebuild, gathered));
+        forces_gather(c, step_what(c), ~0ull, gathered);
+    }
+    ens_near(e);
+    for (int i = 0; i < e->n; i++) {
+        sesmd_ctx *c = e->m[i];
+        CK(forces_sites(c, step_what(c), true));
+        CK(step_part_b(c));
+    }
+    HC(hipGetLastError());
+    return 0;
+}
+
+static int ens_capture(sesmd_ensemble *e, int nsteps, bool rebuild, hipGraphExec_t *out)
+{
+    hipGraph_t gr;
+    HC(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int r = 0;
+    for (int k = 0; k < nsteps && !r; k++) r = ens_step(e, rebuild && k == 0);
+    hipError_t er = hipStreamEndCapture(e->stream, &gr);
+    if (r) return 1;
+    HC(er);
+    hipError_t ei = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    HC(ei);
+    return 0;
+}
+
+static void ens_drop_graphs(sesmd_ensemble *e)
+{
+    if (e->cycle_len) hipGraphExecDestroy(e->g_cycle);
+    if (e->step_ok) hipGraphExecDestroy(e->g_step);
+    e->cycle_len = 0;
+    e->step_ok = false;
+}
+
+extern "C" int sesmd_ensemble_create(sesmd_ensemble **out, sesmd_ctx *const *members, int n)
+{
+    if (!out || !members) return fail("null argument");
+    *out = nullptr;
+    if (n < 1) return fail("an ensemble needs at least one member");
+    for (int i = 0; i < n; i++) {
+        const sesmd_ctx *c = members[i];
+        if (!c) return fail("null member %d", i);
+        if (!c->built) return fail("member %d not built (sesmd_rebuild first)", i);
+        if (c->precision != members[0]->precision) return fail("members differ in precision");
+        if (c->device != members[0]->device) return fail("members live on different devices");
+        if (c->ff.q[1] != c->ff.q[2]) return fail("member %d: the ensemble needs equal hydrogen charges", i);
+        for (int j = 0; j < i; j++)
+            if (members[j] == c) return fail("member %d repeated", i);
+    }
+    sesmd_ensemble *e = new sesmd_ensemble();
+    e->n = n;
+    e->m.assign(members, members + n);
+    for (int i = 0; i < n; i++) e->own.push_back(members[i]->stream);
+    e->mix = members[0]->precision == SESMD_MIXED;
+    e->waves = 1;
+    e->chunk = NEAR_CHUNK;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, members[0]->device) != hipSuccess ||
+        cus < 1)
+        cus = 256;
+    e->grid = 8 * 4 * cus;  // eight wavefronts per SIMD
+    std::vector<EnsMember> hm(n);
+    for (int i = 0; i < n; i++) {
+        sesmd_ctx *c = members[i];
+        EnsMember &M = hm[i];
+        M.g = c->g;
+        M.spos = c->spos;
+        M.smeta = c->smeta;
+        M.bstart = c->bstart;
+        M.bs4 = c->bs4;
+        M.bflag = c->bflag;
+        M.rkind = c->rkind;
+        M.rwork = c->rwork;
+        M.nrwork = c->nrwork;
+        M.ff = c->dff;
+        M.spos32 = c->spos32;
+        M.lj32 = c->lj32;
+        M.part = c->npart;
+        M.counters = c->counters;
+    }
+    int rc = 0;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) rc = fail("stream create failed");
+    if (!rc && dalloc(&e->dm, (size_t)n)) rc = 1;
+    if (!rc && dalloc(&e->prefix, (size_t)n + 1)) rc = 1;
+    if (!rc && hipMemcpy(e->dm, hm.data(), sizeof(EnsMember) * n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail("member table upload failed");
+    for (int i = 0; i < n && !rc; i++)  // the members' queued work before the ensemble's
+        if (hipStreamSynchronize(e->own[i]) != hipSuccess) rc = fail("member %d stream failed", i);
+    if (rc) {
+        if (e->dm) hipFree(e->dm);
+        if (e->prefix) hipFree(e->prefix);
+        if (e->stream) hipStreamDestroy(e->stream);
+        delete e;
+        return 1;
+    }
+    *out = e;
+    return 0;
+}
+
+extern "C" int sesmd_ensemble_destroy(sesmd_ensemble *e)
+{
+    if (!e) return 0;
+    hipStreamSynchronize(e->stream);
+    ens_drop_graphs(e);
+    hipFree(e->dm);
+    hipFree(e->prefix);
+    hipStreamDestroy(e->stream);
+    delete e;
+    return 0;
+}
+
+extern "C" int sesmd_ensemble_set_shape(sesmd_ensemble *e, int waves, int chunk, int grid)
+{
+    if (!e) return fail("null ensemble");
+    if (waves != 1 && waves != 2 && waves != 4) return fail("waves per workgroup must be 1, 2 or 4");
+    if (!e->mix && waves != 1)  // the FP64 pair loop stages per-lane constants in LDS by lane
+        return fail("an FP64 ensemble runs one wavefront per workgroup");
+    if (chunk != 64 && chunk != 128) return fail("receivers per wavefront must be 64 or 128");
+    if (grid < 1) return fail("grid must be positive");
+    if (waves != e->waves || chunk != e->chunk || grid != e->grid) ens_drop_graphs(e);
+    e->waves = waves;
+    e->chunk = chunk;
+    e->grid = grid;
+    return 0;
+}
+
+extern "C" int sesmd_ensemble_get_shape(sesmd_ensemble *e, int *shape3)
+{
+    if (!e || !shape3) return fail("null argument");
+    shape3[0] = e->waves;
+    shape3[1] = e->chunk;
+    shape3[2] = e->grid;
+    return 0;
+}
+
+extern "C" void *sesmd_ensemble_stream(sesmd_ensemble *e) { return e ? (void *)e->stream : nullptr; }
+
+// the ensemble's near field once, at the members' current sorted positions
+extern "C" int sesmd_ensemble_near_only(sesmd_ensemble *e)
+{
+    if (!e) return fail("null ensemble");
+    ens_near(e);
+    HC(hipGetLastError());
+    return 0;
+}
+
+// Times every launch shape (wavefronts per workgroup 1, 2, 4 -- 1 for FP64;
+// receivers per wavefront 64, 128; 2, 4, 8 or 16 wavefronts per SIMD) on the members'
+// current state -- `reps` launches each after one untimed -- and keeps the
+// fastest.  The shapes give identical forces, so the choice changes speed only.
+extern "C" int sesmd_ensemble_tune(sesmd_ensemble *e, int reps, int *shape3, float *best_ms)
+{
+    if (!e) return fail("null ensemble");
+    if (reps < 1) return fail("reps must be >= 1");
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->m[0]->device) != hipSuccess ||
+        cus < 1)
+        cus = 256;
+    {
+        EnsStreams on(e);
+        for (int i = 0; i < e->n; i++) forces_gather(e->m[i], step_what(e->m[i]), ~0ull, false);
+    }
+    hipEvent_t t0, t1;
+    HC(hipEventCreate(&t0));
+    HC(hipEventCreate(&t1));
+    int bw = e->waves, bc = e->chunk, bg = e->grid;
+    float best = -1.f;
+    for (int waves : {1, 2, 4})
+        for (int chunk : {64, 128})
+            for (int per_simd : {2, 4, 8, 16}) {
+                if (!e->mix && waves != 1) continue;
+                e->waves = waves;
+                e->chunk = chunk;
+                e->grid = per_simd * 4 * cus / waves;
+                ens_near(e);
+                HC(hipEventRecord(t0, e->stream));
+                for (int r = 0; r < reps; r++) ens_near(e);
+                HC(hipEventRecord(t1, e->stream));
+                HC(hipEventSynchronize(t1));
+                float ms = 0.f;
+                HC(hipEventElapsedTime(&ms, t0, t1));
+                ms /= reps;
+                if (best < 0.f || ms < best) {
+                    best = ms;
+                    bw = waves;
+                    bc = chunk;
+                    bg = e->grid;
+                }
+            }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    e->waves = e->chunk = e->grid = 0;  // force the graphs to be captured again
+    CK(sesmd_ensemble_set_shape(e, bw, bc, bg));
+    if (shape3) {
+        shape3[0] = bw;
+        shape3[1] = bc;
+        shape3[2] = bg;
+    }
+    if (best_ms) *best_ms = best;
+    return 0;
+}
+
+// nsteps steps of every member (kb: the shared step counter, as in
+// sesmd_steps): whole cycles replay one graph, other steps one step graph or,
+// for a rebuild step outside a whole cycle, eager launches.
+extern "C" int sesmd_ensemble_steps(sesmd_ensemble *e, int nsteps, int ifar, int *kb)
+{
+    if (!e || !kb) return fail("null argument");
+    if (ifar < 1) return fail("ifar must be >= 1");
+    if (nsteps < 0) return fail("nsteps must be >= 0");
+    if (nsteps > 0) {
+        if (e->cycle_len != ifar) {
+            if (e->cycle_len) HC(hipGraphExecDestroy(e->g_cycle));
+            e->cycle_len = 0;
+            CK(ens_capture(e, ifar, true, &e->g_cycle));
+            e->cycle_len = ifar;
+        }
+        if (!e->step_ok) {
+            CK(ens_capture(e, 1, false, &e->g_step));
+            e->step_ok = true;
+        }
+    }
+    for (int s = 0; s < nsteps;) {
+        if ((*kb + 1) % ifar == 0 && nsteps - s >= ifar) {
+            HC(hipGraphLaunch(e->g_cycle, e->stream));
+            for (int i = 0; i < e->n; i++) e->m[i]->boxlist = false;  // the replayed rebuild re-sorted
+            *kb += ifar;
+            s += ifar;
+        } else if ((*kb + 1) % ifar == 0) {
+            CK(ens_step(e, true));
+            for (int i = 0; i < e->n; i++) e->m[i]->boxlist = false;
+            (*kb)++;
+            s++;
+        } else {
+            HC(hipGraphLaunch(e->g_step, e->stream));
+            (*kb)++;
+            s++;
+        }
+    }
+    HC(hipStreamSynchronize(e->stream));
     return 0;
 }
 

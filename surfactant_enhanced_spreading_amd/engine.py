@@ -121,6 +121,15 @@ def lib():
         L.sesmd_set_thermostat.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         L.sesmd_get_flags.argtypes = [vp, ip, ctypes.c_int]
         L.sesmd_energies.argtypes = [vp, dp]
+        L.sesmd_ensemble_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int]
+        L.sesmd_ensemble_destroy.argtypes = [vp]
+        L.sesmd_ensemble_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.sesmd_ensemble_get_shape.argtypes = [vp, ip]
+        L.sesmd_ensemble_tune.argtypes = [vp, ctypes.c_int, ip, ctypes.POINTER(ctypes.c_float)]
+        L.sesmd_ensemble_near_only.argtypes = [vp]
+        L.sesmd_ensemble_stream.argtypes = [vp]
+        L.sesmd_ensemble_stream.restype = vp
+        L.sesmd_ensemble_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip]
         _lib = L
     return _lib
 
@@ -333,3 +342,57 @@ class Engine:
         r, v, a, eta = self.state()
         m = np.concatenate([np.tile(ff.EMW, self.n_water), np.tile(ff.EMS, self.n_surf)])
         return float((m[:, None] * v * v).sum() / ff.DT ** 2 / self.df)
+
+
+class Ensemble:
+    """Independent systems advanced together (sesmd_ensemble_*): every
+    member's step in one captured graph, the near field of all members in one
+    persistent launch.  Members are Engine objects of the same precision,
+    built (bootstrap() done); they must stay open while the ensemble lives,
+    and each keeps its own state (state(), forces() ... read it as usual)."""
+
+    def __init__(self, members):
+        self.L = lib()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        _ck(self.L.sesmd_ensemble_create(ctypes.byref(h), arr, len(self.members)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.L.sesmd_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def steps(self, nsteps, ifar=10, kb=0):
+        k = ctypes.c_int(kb)
+        _ck(self.L.sesmd_ensemble_steps(self.h, nsteps, ifar, ctypes.byref(k)))
+        return k.value
+
+    def tune(self, reps=3):
+        """The autotuner: returns ((waves, chunk, grid), ms per launch) of the
+        fastest near-field launch shape, which the ensemble then keeps."""
+        sh = (ctypes.c_int * 3)()
+        ms = ctypes.c_float()
+        _ck(self.L.sesmd_ensemble_tune(self.h, reps, sh, ctypes.byref(ms)))
+        return tuple(sh), ms.value
+
+    def set_shape(self, waves, chunk, grid):
+        _ck(self.L.sesmd_ensemble_set_shape(self.h, waves, chunk, grid))
+
+    def shape(self):
+        sh = (ctypes.c_int * 3)()
+        _ck(self.L.sesmd_ensemble_get_shape(self.h, sh))
+        return tuple(sh)
+
+    def near_only(self):
+        _ck(self.L.sesmd_ensemble_near_only(self.h))
+
+    def stream(self):
+        return self.L.sesmd_ensemble_stream(self.h)
