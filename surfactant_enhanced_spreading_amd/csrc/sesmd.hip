@@ -2619,17 +2619,33 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
             int t = me.y & 255;
             double c = ff->gr_c[t], s6 = ff->gr_s6[t];
             if (c != 0.0) {
+                // the column's carbons eight at a time: their loads are issued
+                // together (a column holds ~800; one at a time, each a pair of
+                // dependent loads, the walk is latency-bound), the terms are
+                // added in list order
                 double gx = 0.0, gy = 0.0, gz = 0.0;
-                for (int q = gstart[b]; q < gstart[b + 1]; q++) {
-                    int a = gidx[q];
-                    double xij = x - gr[3 * a], yij = y - gr[3 * a + 1], zij = z - gr[3 * a + 2];
-                    double rijsq = xij * xij + yij * yij + zij * zij;
-                    if (sqrt(rijsq) <= g.rc) {
-                        double rsi = 1.0 / rijsq, r6 = rsi * rsi * rsi;
-                        double flj = c * r6 * (s6 * r6 - 0.5) * rsi;
-                        gx += xij * flj;
-                        gy += yij * flj;
-                        gz += zij * flj;
+                constexpr int U = 8;
+                int q0 = gstart[b], q1 = gstart[b + 1];
+                for (int q = q0; q < q1; q += U) {
+                    int a[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) a[u] = gidx[min(q + u, q1 - 1)];
+                    double cxyz[U][3];
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+#pragma unroll
+                        for (int d = 0; d < 3; d++) cxyz[u][d] = gr[3 * a[u] + d];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        double xij = x - cxyz[u][0], yij = y - cxyz[u][1], zij = z - cxyz[u][2];
+                        double rijsq = xij * xij + yij * yij + zij * zij;
+                        if (q + u < q1 && sqrt(rijsq) <= g.rc) {
+                            double rsi = 1.0 / rijsq, r6 = rsi * rsi * rsi;
+                            double flj = c * r6 * (s6 * r6 - 0.5) * rsi;
+                            gx += xij * flj;
+                            gy += yij * flj;
+                            gz += zij * flj;
+                        }
                     }
                 }
                 fx += gx;
@@ -4836,11 +4852,15 @@ struct EnsMember {
     int *counters;
 };
 
+constexpr int ENS_BRANCHES = 4;  // streams the members' launches spread over (graph branches)
+
 struct sesmd_ensemble {
     int n;
     std::vector<sesmd_ctx *> m;
     std::vector<hipStream_t> own;  // the members' own streams (restored after each use)
     hipStream_t stream;
+    hipStream_t branch[ENS_BRANCHES];
+    hipEvent_t ev_fork, ev_join[ENS_BRANCHES];
     EnsMember *dm;                 // device copy of the member table
     int *prefix;                   // first flattened item of each member; [n] = total
     bool mix;
@@ -4850,16 +4870,26 @@ struct sesmd_ensemble {
     bool step_ok;
 };
 
-// flattened item counts: member i contributes its row items x (NEAR_CHUNK / chunk)
-__global__ void k_ens_prefix(const EnsMember *__restrict__ dm, int n, int per, int *__restrict__ prefix)
+// flattened item counts: member i contributes its row items x (NEAR_CHUNK /
+// chunk); one block of 1024 threads, each member's count loaded by its own
+// thread, then a scan in LDS
+constexpr int ENS_MAX = 1024;
+__global__ __launch_bounds__(ENS_MAX) void k_ens_prefix(const EnsMember *__restrict__ dm, int n, int per,
+                                                        int *__restrict__ prefix)
 {
-    if (threadIdx.x != 0) return;
-    int run = 0;
-    for (int i = 0; i < n; i++) {
-        prefix[i] = run;
-        run += *dm[i].nrwork * per;
+    __shared__ int s_c[ENS_MAX];
+    int t = threadIdx.x;
+    int c = t < n ? *dm[t].nrwork * per : 0;
+    s_c[t] = c;
+    __syncthreads();
+    for (int d = 1; d < ENS_MAX; d *= 2) {  // inclusive scan
+        int v = t >= d ? s_c[t - d] : 0;
+        __syncthreads();
+        s_c[t] += v;
+        __syncthreads();
     }
-    prefix[n] = run;
+    if (t < n) prefix[t] = s_c[t] - c;
+    if (t == n - 1) prefix[n] = s_c[t];
 }
 
 template <bool MIX>
@@ -4909,12 +4939,13 @@ __global__ __launch_bounds__(256) void k_near_rows_ens(const EnsMember *__restri
     }
 }
 
-// every member launches on the ensemble's stream while one of these lives
+// while one of these lives, member i launches on the ensemble's branch
+// stream i % ENS_BRANCHES (or on the ensemble's stream: one = true)
 struct EnsStreams {
     sesmd_ensemble *e;
-    explicit EnsStreams(sesmd_ensemble *e_) : e(e_)
+    EnsStreams(sesmd_ensemble *e_, bool one = false) : e(e_)
     {
-        for (int i = 0; i < e->n; i++) e->m[i]->stream = e->stream;
+        for (int i = 0; i < e->n; i++) e->m[i]->stream = one ? e->stream : e->branch[i % ENS_BRANCHES];
     }
     ~EnsStreams()
     {
@@ -4922,30 +4953,56 @@ struct EnsStreams {
     }
 };
 
+// the branch streams start after the work queued on the ensemble's stream ...
+static int ens_fork(sesmd_ensemble *e)
+{
+    HC(hipEventRecord(e->ev_fork, e->stream));
+    for (int b = 0; b < ENS_BRANCHES; b++) HC(hipStreamWaitEvent(e->branch[b], e->ev_fork, 0));
+    return 0;
+}
+
+// ... and the ensemble's stream continues after all of theirs
+static int ens_join(sesmd_ensemble *e)
+{
+    for (int b = 0; b < ENS_BRANCHES; b++) {
+        HC(hipEventRecord(e->ev_join[b], e->branch[b]));
+        HC(hipStreamWaitEvent(e->stream, e->ev_join[b], 0));
+    }
+    return 0;
+}
+
 static void ens_near(sesmd_ensemble *e)
 {
-    k_ens_prefix<<<1, 64, 0, e->stream>>>(e->dm, e->n, NEAR_CHUNK / e->chunk, e->prefix);
+    k_ens_prefix<<<1, ENS_MAX, 0, e->stream>>>(e->dm, e->n, NEAR_CHUNK / e->chunk, e->prefix);
     if (e->mix)
         k_near_rows_ens<true><<<e->grid, 64 * e->waves, 0, e->stream>>>(e->dm, e->prefix, e->n, e->chunk);
     else
         k_near_rows_ens<false><<<e->grid, 64 * e->waves, 0, e->stream>>>(e->dm, e->prefix, e->n, e->chunk);
 }
 
+// One step: the members' parts A spread over the branch streams, joined;
+// the near field of all members on the ensemble's stream; the members' site
+// terms and parts B on the branches, joined again.  Captured, the branches
+// are independent paths of the graph.
 static int ens_step(sesmd_ensemble *e, bool rebuild)
 {
     EnsStreams on(e);
+    CK(ens_fork(e));
     for (int i = 0; i < e->n; i++) {
         sesmd_ctx *c = e->m[i];
         bool gathered = false;
         CK(step_part_a(c, rebuild, gathered));
         forces_gather(c, step_what(c), ~0ull, gathered);
     }
+    CK(ens_join(e));
     ens_near(e);
+    CK(ens_fork(e));
     for (int i = 0; i < e->n; i++) {
         sesmd_ctx *c = e->m[i];
         CK(forces_sites(c, step_what(c), true));
         CK(step_part_b(c));
     }
+    CK(ens_join(e));
     HC(hipGetLastError());
     return 0;
 }
@@ -4977,7 +5034,7 @@ extern "C" int sesmd_ensemble_create(sesmd_ensemble **out, sesmd_ctx *const *mem
 {
     if (!out || !members) return fail("null argument");
     *out = nullptr;
-    if (n < 1) return fail("an ensemble needs at least one member");
+    if (n < 1 || n > ENS_MAX) return fail("an ensemble has 1..%d members", ENS_MAX);
     for (int i = 0; i < n; i++) {
         const sesmd_ctx *c = members[i];
         if (!c) return fail("null member %d", i);
@@ -5021,6 +5078,12 @@ extern "C" int sesmd_ensemble_create(sesmd_ensemble **out, sesmd_ctx *const *mem
     }
     int rc = 0;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) rc = fail("stream create failed");
+    for (int b = 0; b < ENS_BRANCHES && !rc; b++)
+        if (hipStreamCreateWithFlags(&e->branch[b], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e->ev_join[b], hipEventDisableTiming) != hipSuccess)
+            rc = fail("branch stream create failed");
+    if (!rc && hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess)
+        rc = fail("event create failed");
     if (!rc && dalloc(&e->dm, (size_t)n)) rc = 1;
     if (!rc && dalloc(&e->prefix, (size_t)n + 1)) rc = 1;
     if (!rc && hipMemcpy(e->dm, hm.data(), sizeof(EnsMember) * n, hipMemcpyHostToDevice) != hipSuccess)
@@ -5031,6 +5094,11 @@ extern "C" int sesmd_ensemble_create(sesmd_ensemble **out, sesmd_ctx *const *mem
         if (e->dm) hipFree(e->dm);
         if (e->prefix) hipFree(e->prefix);
         if (e->stream) hipStreamDestroy(e->stream);
+        for (int b = 0; b < ENS_BRANCHES; b++) {
+            if (e->branch[b]) hipStreamDestroy(e->branch[b]);
+            if (e->ev_join[b]) hipEventDestroy(e->ev_join[b]);
+        }
+        if (e->ev_fork) hipEventDestroy(e->ev_fork);
         delete e;
         return 1;
     }
@@ -5046,6 +5114,11 @@ extern "C" int sesmd_ensemble_destroy(sesmd_ensemble *e)
     hipFree(e->dm);
     hipFree(e->prefix);
     hipStreamDestroy(e->stream);
+    for (int b = 0; b < ENS_BRANCHES; b++) {
+        hipStreamDestroy(e->branch[b]);
+        hipEventDestroy(e->ev_join[b]);
+    }
+    hipEventDestroy(e->ev_fork);
     delete e;
     return 0;
 }
@@ -5098,7 +5171,7 @@ extern "C" int sesmd_ensemble_tune(sesmd_ensemble *e, int reps, int *shape3, flo
         cus < 1)
         cus = 256;
     {
-        EnsStreams on(e);
+        EnsStreams on(e, true);
         for (int i = 0; i < e->n; i++) forces_gather(e->m[i], step_what(e->m[i]), ~0ull, false);
     }
     hipEvent_t t0, t1;
