@@ -319,6 +319,53 @@ def ensemble_leg(timer, steps, warmup, n=64):
                 build_s=round(t_build, 1), create_s=round(t_create, 1))
 
 
+def reference_leg(timer, seconds, steps, warmup):
+    """The reference's workload of record (9,997 waters + 475 surfactants, its
+    PARAMETER sizes, src/mybox.cmns:59-60) as the data_cont2.dat its program
+    reads: the GPU engine on it in the reference geometry (12^3 boxes, p = 4),
+    FP64 and mixed, and beside it the reference program itself
+    (oracle/_ref/mybox_ref, 27 MPI ranks on this host's cores, about
+    `seconds` of stepping; median step time from step 21 on, regular and
+    rebuild + far-field steps apart)."""
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import run_reference as rr
+    from surfactant_enhanced_spreading_amd import builder, engine, formats
+    s0 = builder.r2_system()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, 'data_cont2.dat')
+        formats.write_config(path, s0)
+        s = formats.read_config(path, s0.n_water, s0.n_surf)
+    s.graphite = s0.graphite
+    cfg = dict(nb2=3, p=4)
+    out = dict(workload='9,997 SPC/E waters + 475 M(D\'E4OH)M surfactants (the reference\'s PARAMETER '
+                        'sizes) as one droplet over the reference\'s own graphite sheet, 12^3 L4 boxes '
+                        'of 12 A, p=4, ifar=10, 2 fs', sites=s.n_sites)
+    for prec, key in ((engine.FP64, 'gpu_fp64'), (engine.MIXED, 'gpu_mixed')):
+        r, e = run(s, cfg, prec, steps, warmup, timer)
+        out[key] = dict(steps_per_s=round(r['steps_s'], 2), ms_per_step=round(r['ms_step'], 4),
+                        shake_failures=e.shake_failures())
+        e.close()
+    if rr.available():
+        res = rr.run(s, seconds)
+        sm = rr.summary(res, skip=20)
+        ref = dict(kind='reference', program='oracle/_ref/mybox_ref (src/mybox.f90 and its routines, '
+                                             'amdflang -O2, MPICH 3.3.2)',
+                   ranks=sm['ranks'], cores=sm['cores'], steps_timed=sm['steps'],
+                   startup_s=round(res['startup_s'], 2) if res['startup_s'] else None,
+                   shake_errors=res['shake_errors'])
+        for k in ('regular_median_s', 'far_field_median_s', 'cycle_mean_s_per_step'):
+            if k in sm:
+                ref[k] = round(sm[k], 4)
+        if 'cycle_mean_s_per_step' in sm:
+            ref['steps_per_s'] = round(1.0 / sm['cycle_mean_s_per_step'], 4)
+            out['gpu_fp64_over_reference'] = round(out['gpu_fp64']['steps_per_s'] * sm['cycle_mean_s_per_step'], 1)
+        out['reference_cpu'] = ref
+    else:
+        out['reference_cpu'] = None
+    return out
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_near_rows_counters_c2_fp64.md')
 
 
@@ -373,6 +420,10 @@ def main():
     ap.add_argument('--ifar-steps', type=int, default=2000,
                     help='steps of each run of the drift-vs-ifar series (ifar 1, 2, 5, 10); 0 skips it')
     ap.add_argument('--no-ensemble', action='store_true', help='skip the batched-ensemble leg')
+    ap.add_argument('--no-reference', action='store_true',
+                    help='skip the reference-program leg (its workload of record, timed beside the GPU)')
+    ap.add_argument('--reference-seconds', type=float, default=40.0)
+    ap.add_argument('--reference-only', action='store_true', help='run only the reference-program leg')
     ap.add_argument('--ensemble-only', action='store_true', help='run only the batched-ensemble leg')
     ap.add_argument('--ensemble-steps', type=int, default=50)
     ap.add_argument('--copy-calibrate', type=int, default=0, metavar='BYTES',
@@ -392,6 +443,9 @@ def main():
     timer = HipTimer()
     if args.ensemble_only:
         print(json.dumps(ensemble_leg(timer, args.ensemble_steps, 10)), flush=True)
+        return
+    if args.reference_only:
+        print(json.dumps(reference_leg(timer, args.reference_seconds, 200, 20)), flush=True)
         return
     sys_ = make_system(name)
     print('bench: %s, %d sites, nb2=%d, p=%d' % (name, sys_.n_sites, cfg['nb2'], cfg['p']),
@@ -465,6 +519,9 @@ def main():
     if not args.no_ensemble and name == 'c2':
         out['ensemble'] = ensemble_leg(timer, args.ensemble_steps, 10)
         print('bench: ensemble done', file=sys.stderr, flush=True)
+    if not args.no_reference and name == 'c2':
+        out['reference_workload'] = reference_leg(timer, args.reference_seconds, 200, 20)
+        print('bench: reference workload done', file=sys.stderr, flush=True)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(sys_, cfg['nb2'], cfg['p'], args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -188,7 +188,53 @@ def _align(v_from, v_to):
     return np.eye(3) + vx + vx @ vx / (1.0 + c)
 
 
-def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6, hemisphere=False):
+def _surfactants_rsa(n_surf, r_drop, c, body0, axis, length, rng, min_ss, hemisphere=False):
+    """n_surf surfactants on the droplet's surface, placed one by one: evenly
+    spread anchor directions (a Fibonacci sphere, in random order), and per
+    chain 16 spins about its axis x 8 anchor radii (from the radius droplet()
+    uses, 0.25 sigma apart, the chain pointing outward), the first of them in
+    that order whose sites all lie at least min_ss from every site placed
+    before (the last radius if none)."""
+    from scipy.spatial import cKDTree
+    out = np.zeros((n_surf, 29, 3))
+    r0 = max(r_drop + 0.35 * length - length, 0.75 * r_drop)
+    order = rng.permutation(n_surf)
+    spins = rng.uniform(0, 2 * math.pi, size=(n_surf, 16))
+    tree, ntree = None, 0  # tree over the first ntree chains; later ones checked directly
+    for k, s in enumerate(order):
+        zz = 1.0 - 2.0 * (s + 0.5) / n_surf
+        if hemisphere:
+            zz = 1.0 - (s + 0.5) / n_surf
+        phi = s * math.pi * (3.0 - math.sqrt(5.0))
+        d = np.array([math.sqrt(max(0.0, 1 - zz * zz)) * math.cos(phi),
+                      math.sqrt(max(0.0, 1 - zz * zz)) * math.sin(phi), zz])
+        R = _align(axis, d)
+        K = np.array([[0, -d[2], d[1]], [d[2], 0, -d[0]], [-d[1], d[0], 0]])
+        base = (R @ body0.T).T
+        cands = []
+        for step in range(8):
+            for ang in spins[s]:
+                spin = np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * K @ K
+                cands.append((spin @ base.T).T + c + d * (r0 + 0.25 * step))
+        cands = np.array(cands)  # (128, 29, 3)
+        dmin = np.full(len(cands), np.inf)
+        if tree is not None:
+            dd, _ = tree.query(cands.reshape(-1, 3), k=1)
+            dmin = dd.reshape(len(cands), 29).min(axis=1)
+        if k > ntree:
+            recent = out[order[ntree:k]].reshape(-1, 3)
+            dr = np.sqrt(((cands.reshape(-1, 1, 3) - recent[None]) ** 2).sum(axis=2)).min(axis=1)
+            dmin = np.minimum(dmin, dr.reshape(len(cands), 29).min(axis=1))
+        ok = np.nonzero(dmin >= min_ss)[0]
+        out[s] = cands[ok[0]] if len(ok) else cands[-1]
+        if k + 1 - ntree >= 8:
+            ntree = k + 1
+            tree = cKDTree(out[order[:ntree]].reshape(-1, 3))
+    return out
+
+
+def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6, hemisphere=False,
+            spread='fibonacci', min_ss_A=3.0):
     """Positions of one droplet: waters on a simple-cubic lattice at liquid
     density with random orientations, cut to the n_water lattice sites closest
     to the centre, and n_surf surfactants spread over the droplet surface with
@@ -197,6 +243,11 @@ def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6, he
     hemisphere: only lattice sites at or above the centre's height (a
     hemispherical droplet whose flat face lies at center_A's z), radius from
     the hemisphere's volume; surfactants on the curved surface only.
+
+    spread: 'fibonacci' anchors the surfactants at evenly spread directions
+    with one random spin each (fine while they are few); 'rsa' places them
+    one by one, choosing spin and anchor radius so that each lies at least
+    min_ss_A from the ones before (dense coverage).
 
     Returns (water_pos (n_water,3,3), surf_pos (n_surf,29,3)) in sigma units.
     """
@@ -209,7 +260,10 @@ def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6, he
     axis = tmpl[17] - tmpl[0]
     length = np.linalg.norm(axis)
     surf = np.zeros((n_surf, 29, 3))
-    for s in range(n_surf):
+    if spread == 'rsa':
+        surf = _surfactants_rsa(n_surf, r_drop, c, tmpl - tmpl[0], axis, length, rng,
+                                min_ss_A / ff.SIGMA, hemisphere)
+    for s in range(n_surf if spread != 'rsa' else 0):
         # Fibonacci sphere for evenly spread anchor directions
         zz = 1.0 - 2.0 * (s + 0.5) / max(n_surf, 1)
         if hemisphere:
@@ -236,7 +290,10 @@ def droplet(n_water, n_surf, center_A=(0.0, 0.0, 0.0), seed=0, min_sep_A=2.6, he
         pts = pts[pts[:, 2] >= 0.0]
     d2 = np.sum(pts * pts, axis=1)
     pts = pts[np.argsort(d2, kind='stable')] + c
-    pts = pts[:int(n_water * 1.5) + 64]
+    # candidates: enough for the waters plus the lattice points the
+    # surfactant sites exclude (the first n_water survivors are the same for
+    # any larger pool)
+    pts = pts[:int(n_water * 1.5) + 64 + 60 * n_surf]
     if n_surf:
         from scipy.spatial import cKDTree
         near = cKDTree(surf.reshape(-1, 3)).query(pts, k=1)[0]
@@ -439,3 +496,21 @@ def ensemble_systems(n=64, seed=20261208, n_min=500, n_max=30000, surf_wt_max=0.
         g = graphite_sheet(half_side_A=half, z_top=z_top_A / ff.SIGMA, layers=layers)
         out.append((assemble([d], graphite=g, seed=seed + 101 * k), nb2))
     return out
+
+
+def r2_system(n_water=9997, n_surf=475, seed=20261209, jitter=0.012):
+    """The reference's workload of record (its PARAMETER sizes: 9,997 waters +
+    475 surfactants, src/mybox.cmns:59-60) as one droplet centred 12 A above
+    the domain centre (surfactants by random sequential placement: 475 are too
+    many for evenly spread anchors), over the reference's own two-layer graphite sheet
+    (formats.make_graphite, top layer at z = -16 sigma), in the reference
+    geometry (12^3 L4 boxes of 12 A, nb2 = 3).  Every site is then displaced
+    by a normal deviate of `jitter` sigma per axis (~0.04 A), as in a thermal
+    configuration: the template surfactant is all-trans at its angle
+    equilibria, where the angle and dihedral forces are differences of nearly
+    equal numbers."""
+    from . import formats
+    d = droplet(n_water, n_surf, center_A=(0.0, 0.0, 12.0), seed=seed, spread='rsa')
+    s = assemble([d], graphite=formats.make_graphite(), seed=seed)
+    s.pos = s.pos + np.random.default_rng(seed + 1000).normal(0.0, jitter, size=s.pos.shape)
+    return s

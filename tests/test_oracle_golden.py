@@ -114,6 +114,39 @@ def test_medium_total_force():
     assert rel_err(o.forces(), z['F']) <= TOL
 
 
+def r2_system_from_golden(z):
+    """The R2 system as read_config reads its data_cont2.dat (the golden keeps
+    the file's fields as integers in units of 1e-3)."""
+    from surfactant_enhanced_spreading_amd import formats
+    from surfactant_enhanced_spreading_amd.builder import System
+    pos = z['pos_milli'] / 1000.0 + formats.CONFIG_SHIFT
+    vel = (z['vel_milli'] / 1000.0 - 0.5) * 0.01
+    return System(int(z['n_water']), int(z['n_surf']), pos, vel, graphite=formats.make_graphite())
+
+
+def test_r2_workload_of_record():
+    """The reference's own PARAMETER-size system (9,997 waters + 475
+    surfactants, src/mybox.cmns:59-60) through the compiled reference routines
+    (tests/golden/make_goldens.py r2): bootstrap total force and one full step
+    (part A, rebuild, forces, part B) at every 7th site."""
+    import oracle as orc
+    z = load('r2_system.npz')
+    s = r2_system_from_golden(z)
+    k = z['sites']
+    o = orc.Oracle(s, nb2=3, p=4)
+    o.rebuild()
+    o.fmm()
+    f = o.forces()
+    assert rel_err(f[k], z['F']) <= TOL
+    o.init_accel(f)
+    assert o.steps(1, 10, 9) == 10
+    r, v, a, eta = o.state()
+    assert np.abs(r[k] - z['A_pos']).max() <= 1e-12 * np.abs(z['A_pos']).max()
+    assert abs(eta - float(z['A_eta'])) <= 1e-12 * abs(float(z['A_eta']))
+    assert rel_err(v[k], z['B_vel']) <= TOL
+    assert rel_err(a[k], z['B_acc']) <= TOL
+
+
 def test_ylm_and_legendre_primitives():
     """Ylm with the single-precision phase of src/ifma.f90:24 and rLegendreP,
     value for value."""
