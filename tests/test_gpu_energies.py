@@ -48,7 +48,9 @@ def test_energies_match_oracle(which, sys_small, sys_medium):
 
 def test_mixed_context_energies_are_fp64(sys_small):
     """A mixed-precision context evaluates energies in FP64: at the same state
-    they equal the FP64 context's."""
+    they equal the FP64 context's, except the far-field term, which reads the
+    local expansions that the mixed context's split-BF16x3 M2L produced (FP32
+    accumulation: relative error of order 1e-7, bound 1e-5 here)."""
     from surfactant_enhanced_spreading_amd import engine
     e64 = engine.Engine(sys_small)
     em = engine.Engine(sys_small, precision=engine.MIXED)
@@ -57,8 +59,13 @@ def test_mixed_context_energies_are_fp64(sys_small):
             e.rebuild()
             e.fmm()
         a, b = e64.energies(), em.energies()
+        d = a['far'] - b['far']
+        assert a['far'] != 0.0 and abs(d) <= 1e-5 * abs(a['far']), (a['far'], b['far'])
         for k in COMPS:
-            assert a[k] == b[k], k
+            if k in ('potential', 'total'):  # sums that hold the far term
+                assert abs((a[k] - b[k]) - d) <= 1e-12 * abs(a[k]), k
+            elif k != 'far':
+                assert a[k] == b[k], k
     finally:
         e64.close()
         em.close()
