@@ -1655,13 +1655,26 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
     constexpr int p = PP, lk = (PP + 1) * (PP + 2) / 2, lp = (PP + 1) * (PP + 1);
     // sized by the order: 15 KB at p = 4, so every box's block is resident at once
     __shared__ double s_re[64][lk], s_im[64][lk];
+    __shared__ double s_fac[lk];  // sqrt((n - m)! / (n + m)!) per row n (n + 1) / 2 + m
     int b = blockIdx.x;
     int nb = g.nb;
+    int s0 = bstart[b], s1 = bstart[b + 1];
+    if (s0 == s1) {  // an empty box: zero expansion (block-uniform exit)
+        for (int c = threadIdx.x; c < lp; c += 64) O4[(size_t)b * lp + c] = make_double2(0.0, 0.0);
+        if (threadIdx.x == 0) ne4[b] = 0;
+        return;
+    }
+    for (int c = threadIdx.x; c < lk; c += 64) {
+        int n = 0;
+        while ((n + 1) * (n + 2) / 2 <= c) n++;
+        int m = c - n * (n + 1) / 2;
+        s_fac[c] = sqrt(ff->rifact[n - m] / ff->rifact[n + m]);
+    }
+    __syncthreads();
     double cx = d_cen4(g, b % nb), cy = d_cen4(g, (b / nb) % nb), cz = d_cen4(g, b / (nb * nb));
     double are[MAXLK], aim[MAXLK];
 #pragma unroll
     for (int c = 0; c < lk; c++) are[c] = aim[c] = 0.0;
-    int s0 = bstart[b], s1 = bstart[b + 1];
     int any = 0;
     for (int s = s0 + threadIdx.x; s < s1; s += 64) {
         int i = perm[s];
@@ -1698,7 +1711,8 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
                 double pm2 = pmm;
 #pragma unroll
                 for (int ll = m + 2; ll <= p; ll++) {
-                    double pll = (ca * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pm2) / (ll - m);
+                    // 1 / (ll - m) as a constant: a product where rLegendreP divides
+                    double pll = (ca * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pm2) * (1.0 / (ll - m));
                     pm2 = pmmp1;
                     pmmp1 = pll;
                     P[ll][m] = pll;
@@ -1712,7 +1726,7 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
             for (int m = 0; m <= n; m++) {
                 // Y_n^{-m}: phase exp(-i m beta) rounded to single precision
                 float cr = (float)em[m].x, ci = (float)em[m].y;
-                double sc = sqrt(ff->rifact[n - m] / ff->rifact[n + m]) * P[n][m];
+                double sc = s_fac[n * (n + 1) / 2 + m] * P[n][m];
                 double w = q * rhon;
                 int c = n * (n + 1) / 2 + m;
                 are[c] += w * (sc * (double)cr);

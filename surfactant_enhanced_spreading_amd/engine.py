@@ -108,6 +108,7 @@ def lib():
         L.sesmd_init_accel.argtypes = [vp]
         L.sesmd_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip]
         L.sesmd_get_classes.argtypes = [vp, ip, ip, ip, ip, ctypes.POINTER(ctypes.c_uint)]
+        L.sesmd_set_classes.argtypes = [vp, ip, ip]
         L.sesmd_get_expansions.argtypes = [vp] + [dp] * 6
         L.sesmd_get_counters.argtypes = [vp, ip]
         L.sesmd_device_arrays.argtypes = [vp] + [ctypes.POINTER(vp)] * 4
@@ -216,13 +217,27 @@ class Engine:
     def set_thermostat(self, eta, xi):
         _ck(self.L.sesmd_set_thermostat(self.h, float(eta), float(xi)))
 
+    def set_classes(self, cls, anc):
+        """Molecule classes (0 W1, 1 W2, 2 surfactant) and storage boxes that
+        every rebuild keeps instead of classifying (sesmd_set_classes); None
+        lets the rebuilds classify again, from the classes held now."""
+        if cls is None:
+            _ck(self.L.sesmd_set_classes(self.h, None, None))
+            return
+        cls = np.ascontiguousarray(cls, dtype=np.int32)
+        anc = np.ascontiguousarray(anc, dtype=np.int32)
+        _ck(self.L.sesmd_set_classes(self.h, _p(cls, ctypes.c_int), _p(anc, ctypes.c_int)))
+
     def checkpoint(self, path, kb):
-        """Write the whole dynamical state (positions, v dt, F dt^2/2m, eta, xi)
-        and the step counter kb to an .npz file."""
+        """Write the whole dynamical state (positions, v dt, F dt^2/2m, eta, xi),
+        the molecule classes and storage boxes of the last rebuild (part A of
+        the next step groups its kinetic sums by class) and the step counter
+        kb to an .npz file."""
         r, v, a, _ = self.state()
         eta, xi = self.thermostat()
+        c = self.classes()
         np.savez(path, r=r, v=v, a=a, eta=eta, xi=xi, kb=kb, n_water=self.n_water, n_surf=self.n_surf,
-                 nb2=self.nb2, p=self.p)
+                 nb2=self.nb2, p=self.p, cls=c['cls'], anc=c['anc'])
 
     def restore(self, path):
         """Load a checkpoint written by checkpoint() into this context (same
@@ -233,7 +248,15 @@ class Engine:
             raise SesmdError('checkpoint is of another system or geometry')
         self.set_state(d['r'], d['v'], d['a'], float(d['eta']))
         self.set_thermostat(float(d['eta']), float(d['xi']))
-        self.rebuild()
+        if 'cls' in d:
+            # lists from the restored positions with the checkpointed classes,
+            # which part A of the next step reads; the next rebuild classifies
+            # again
+            self.set_classes(d['cls'], d['anc'])
+            self.rebuild()
+            self.set_classes(None, None)
+        else:
+            self.rebuild()
         return int(d['kb'])
 
     def flags(self):

@@ -2,7 +2,9 @@
 
 Resume: the whole dynamical state is positions, v dt, F dt^2/2m, the
 Nose-Hoover friction eta and its integral xi (sesmd_get_state /
-sesmd_get_thermostat).  A run stopped after a step whose successor rebuilds,
+sesmd_get_thermostat), plus the molecule classes of the last rebuild, by
+which part A of the next step groups the kinetic sums that update eta (the
+grouping changes eta's rounding).  A run stopped after a step whose successor rebuilds,
 written to a checkpoint file, and resumed in a new context must continue bit
 for bit like the uninterrupted run -- including the thermostat energy, which
 needs xi.
