@@ -113,6 +113,10 @@ struct sesmd_ctx {
     int precision;
     sesmd_forcefield ff;
     hipStream_t stream;    // every launch of the context, in order
+    // a rebuild step's far field runs on far_stream beside the near field
+    // (forked after the sort, joined before the site terms; default priority)
+    hipStream_t far_stream;
+    hipEvent_t ev_far_fork, ev_far_join;
     double *fs;            // intramolecular forces of a step, 29 ns sites (x, y, z blocks)
     // device state
     double *r, *v, *a, *f;  // each 3*N: x block, y block, z block
@@ -4050,6 +4054,11 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     };
     A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ? fail("stream create failed")
                                                                                  : 0);
+    A(hipStreamCreateWithFlags(&c->far_stream, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&c->ev_far_fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&c->ev_far_join, hipEventDisableTiming) != hipSuccess
+          ? fail("far-field stream create failed")
+          : 0);
     size_t n = g.n;
     {  // box centres, computed on the device with the classification's own formula
         double *tab = nullptr;
@@ -4249,6 +4258,12 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
                     c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    if (c->far_stream) {
+        hipStreamSynchronize(c->far_stream);
+        hipStreamDestroy(c->far_stream);
+    }
+    if (c->ev_far_fork) hipEventDestroy(c->ev_far_fork);
+    if (c->ev_far_join) hipEventDestroy(c->ev_far_join);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -4660,7 +4675,7 @@ extern "C" int sesmd_init_accel(sesmd_ctx *c)
 //   A(surf+water) - eta + intra -> fs - [rebuild, far field] - near - site terms - B(surf+water)
 // (A second stream for the eta/intra launch, beside the forces, measured
 // slower: the join before part B costs more than the launch it hides.)
-static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered);
+static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered, bool far_aside = false);
 static int step_part_b(sesmd_ctx *c);
 static const unsigned STEP_INTRA =
     SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE;
@@ -4672,18 +4687,30 @@ static unsigned step_what(const sesmd_ctx *c) { return c->surf_fast ? SESMD_ALL 
 static int launch_step(sesmd_ctx *c, bool rebuild)
 {
     bool gathered = false;
-    CK(step_part_a(c, rebuild, gathered));
-    // one stream, no cross-queue fork and join: with the reference topology
-    // the intramolecular terms were evaluated with the eta update and part B
-    // adds them per site (with the subtraction); graphite runs inside
-    // k_site_terms
-    CK(launch_forces(c, step_what(c), ~0ull, gathered, true));
+    // a rebuild step's far field goes to far_stream, forked after the sort,
+    // and runs beside the near field: the two touch disjoint buffers (P2M
+    // reads the positions through the permutation, the near field the sorted
+    // copies, and writes only its partials); the site terms need both, so the
+    // join comes before them.  Regular steps stay on one stream: a fork and
+    // join between queues costs more than the ~10 us they could hide.
+    CK(step_part_a(c, rebuild, gathered, rebuild));
+    // with the reference topology the intramolecular terms were evaluated
+    // with the eta update and part B adds them per site (with the
+    // subtraction); graphite runs inside k_site_terms
+    unsigned what = step_what(c);
+    forces_gather(c, what, ~0ull, gathered);
+    if (what & SESMD_NEAR) launch_near(c, ~0ull);
+    if (rebuild) {
+        HC(hipEventRecord(c->ev_far_join, c->far_stream));
+        HC(hipStreamWaitEvent(c->stream, c->ev_far_join, 0));
+    }
+    CK(forces_sites(c, what, true));
     return step_part_b(c);
 }
 
 // part A of a step (with the rebuild and far field when `rebuild`); gathered:
 // part A wrote the sorted position copies
-static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered)
+static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered, bool far_aside)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
@@ -4718,7 +4745,17 @@ static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered)
     HC(hipGetLastError());
     if (rebuild) {
         CK(launch_rebuild(c));
-        CK(launch_fmm(c));
+        if (far_aside) {  // on far_stream; the caller joins it
+            HC(hipEventRecord(c->ev_far_fork, c->stream));
+            HC(hipStreamWaitEvent(c->far_stream, c->ev_far_fork, 0));
+            hipStream_t own = c->stream;
+            c->stream = c->far_stream;
+            int r = launch_fmm(c);
+            c->stream = own;
+            CK(r);
+        } else {
+            CK(launch_fmm(c));
+        }
     }
     return 0;
 }
