@@ -152,13 +152,20 @@ struct sesmd_ctx {
     double2 *T4, *T3, *T2;        // M2L operators per offset
     double *AF4, *AF3;            // the same, real-expanded in MFMA operand order (k_m2l_gemm)
     unsigned short *AFB4, *AFB3;  // mixed precision: the operators as bf16 parts (k_m2l_gemm_bf3)
-    unsigned short *m2l_ob;       // mixed precision: the source expansions as bf16 parts
-    double *m2l_res;              // M2L results, one row (M2L_RES values) per list entry
-    int *m2l_cnt, *m2l_base;      // per (class, candidate) list: entries, first entry
-    int *m2l_gbase, *m2l_bbase;   // per offset group: first entry, first block
-    int2 *m2l_list;               // list entries (target within its class, source box)
-    int *m2l_idx;                 // per target box and candidate: its result row, -1 none
-    int m2l_max_blocks;           // k_m2l_gemm grid (blocks of the densest possible level)
+    // M2L lists and results, one set per level (L4: m2l[0], L3: m2l[1]), so
+    // the two levels' products can run at the same time
+    struct M2LBuf {
+        unsigned short *ob;   // mixed precision: the source expansions as bf16 parts
+        double *res;          // results, one row (M2L_RES values) per list entry
+        int *cnt, *base;      // per (class, candidate) list: entries, first entry
+        int *gbase, *bbase;   // per offset group: first entry, first block
+        int2 *list;           // list entries (target within its class, source box)
+        int *idx;             // per target box and candidate: its result row, -1 none
+        int max_blocks;       // k_m2l_gemm grid (blocks of the densest possible level)
+    } m2l[2];
+    // the L4 products run on m2l_stream beside the L3 chain of the far field
+    hipStream_t m2l_stream;
+    hipEvent_t ev_m2l_fork, ev_m2l_join;
     double2 *top_part;            // top-level M2L partials per (target, source)
     double2 *M43, *M32, *S23, *S34; // M2M / L2L operators per child
     double *c4;                   // L4 centres (3 per box)
@@ -4054,6 +4061,11 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     };
     A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ? fail("stream create failed")
                                                                                  : 0);
+    A(hipStreamCreateWithFlags(&c->m2l_stream, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&c->ev_m2l_fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&c->ev_m2l_join, hipEventDisableTiming) != hipSuccess
+          ? fail("M2L stream create failed")
+          : 0);
     A(hipStreamCreateWithFlags(&c->far_stream, hipStreamNonBlocking) != hipSuccess ||
               hipEventCreateWithFlags(&c->ev_far_fork, hipEventDisableTiming) != hipSuccess ||
               hipEventCreateWithFlags(&c->ev_far_join, hipEventDisableTiming) != hipSuccess
@@ -4130,23 +4142,27 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
         // run unconditionally up to a chunk past an offset's last k step
         A(dalloc(&c->AF4, (size_t)344 * mt * ks * 64));
         A(dalloc(&c->AF3, (size_t)344 * mt * ks * 64));
-        // M2L lists: every (target, candidate) pair of the L4 level at most
-        // (8 classes x 216 candidates x (nb/2)^3 targets), the L3 level less
-        size_t pairs = (size_t)216 * g.nb3;
-        // result rows: M2L_RES doubles, or floats for the bf16x3 products
-        A(dalloc(&c->m2l_res, pairs * M2L_RES / (precision == SESMD_MIXED ? 2 : 1)));
-        A(dalloc(&c->m2l_list, pairs));
-        A(dalloc(&c->m2l_idx, pairs));
-        A(dalloc(&c->m2l_cnt, M2L_LISTS));
-        A(dalloc(&c->m2l_base, M2L_LISTS));
-        A(dalloc(&c->m2l_gbase, M2L_GROUPS + 1));
-        A(dalloc(&c->m2l_bbase, M2L_GROUPS + 1));
-        c->m2l_max_blocks = (int)(pairs / (M2L_TPB * 16)) + M2L_GROUPS;
+        // M2L lists per level: every (target, candidate) pair at most (8
+        // classes x 216 candidates x (nd/2)^3 targets)
+        for (int lv = 0; lv < 2; lv++) {
+            size_t nbox = lv == 0 ? (size_t)g.nb3 : (size_t)g.n33;
+            size_t pairs = (size_t)216 * nbox;
+            sesmd_ctx::M2LBuf &B = c->m2l[lv];
+            // result rows: M2L_RES doubles, or floats for the bf16x3 products
+            A(dalloc(&B.res, pairs * M2L_RES / (precision == SESMD_MIXED ? 2 : 1)));
+            A(dalloc(&B.list, pairs));
+            A(dalloc(&B.idx, pairs));
+            A(dalloc(&B.cnt, M2L_LISTS));
+            A(dalloc(&B.base, M2L_LISTS));
+            A(dalloc(&B.gbase, M2L_GROUPS + 1));
+            A(dalloc(&B.bbase, M2L_GROUPS + 1));
+            B.max_blocks = (int)(pairs / (M2L_TPB * 16)) + M2L_GROUPS;
+            if (precision == SESMD_MIXED) A(dalloc(&B.ob, nbox * ((2 * g.lp + 31) / 32) * 96));
+        }
         if (precision == SESMD_MIXED) {
             size_t ks32 = (2 * g.lp + 31) / 32;
             A(dalloc(&c->AFB4, (size_t)M2L_GROUPS * ks32 * mt * 3 * 64 * 8));
             A(dalloc(&c->AFB3, (size_t)M2L_GROUPS * ks32 * mt * 3 * 64 * 8));
-            A(dalloc(&c->m2l_ob, (size_t)g.nb3 * ks32 * 96));
         }
     }
     A(dalloc(&c->T3, (size_t)343 * g.lk * g.lp));
@@ -4253,8 +4269,10 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
                     c->wkey_out, c->widx_in, c->widx_out, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
-                    c->AF4, c->AF3, c->m2l_res, c->m2l_list, c->m2l_idx, c->m2l_cnt, c->m2l_base,
-                    c->m2l_gbase, c->m2l_bbase, c->AFB4, c->AFB3, c->m2l_ob,
+                    c->AF4, c->AF3, c->m2l[0].res, c->m2l[0].list, c->m2l[0].idx, c->m2l[0].cnt, c->m2l[0].base,
+                    c->m2l[0].gbase, c->m2l[0].bbase, c->m2l[0].ob, c->m2l[1].res, c->m2l[1].list,
+                    c->m2l[1].idx, c->m2l[1].cnt, c->m2l[1].base, c->m2l[1].gbase, c->m2l[1].bbase,
+                    c->m2l[1].ob, c->AFB4, c->AFB3,
                     c->dtopo, c->partials, c->nh, c->esum, c->counters, c->spos32, c->lj32, c->fs};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -4262,6 +4280,12 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
         hipStreamSynchronize(c->far_stream);
         hipStreamDestroy(c->far_stream);
     }
+    if (c->m2l_stream) {
+        hipStreamSynchronize(c->m2l_stream);
+        hipStreamDestroy(c->m2l_stream);
+    }
+    if (c->ev_m2l_fork) hipEventDestroy(c->ev_m2l_fork);
+    if (c->ev_m2l_join) hipEventDestroy(c->ev_m2l_join);
     if (c->ev_far_fork) hipEventDestroy(c->ev_far_fork);
     if (c->ev_far_join) hipEventDestroy(c->ev_far_join);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -4403,50 +4427,69 @@ extern "C" int sesmd_rebuild(sesmd_ctx *c)
 // M2L over the interaction lists of one level (computeLjk3 / computeLjk4);
 // nd is even at every level (each level halves the one below it).  Matrix
 // cores for p = 4 and p = 8; the vector kernel k_m2l stays for other orders.
-static void launch_m2l(sesmd_ctx *c, int nd, const double2 *O, const int *ne, const double2 *T, double2 *L)
+// The products of one level (lists, then the GEMMs into the level's result
+// rows): they read only the level's multipoles, so the L4 products can run
+// before the L4 locals exist.  M2L_accum adds them in; the vector kernel k_m2l
+// (orders other than 4 and 8) does both at once.
+static void launch_m2l_products(sesmd_ctx *c, int nd, const double2 *O, const int *ne)
 {
     const Geo &g = c->g;
-    int h = nd / 2;
-    if (g.p == 4 || g.p == 8) {
-        hipStream_t st = c->stream;
-        const double *Od = (const double *)O;
-        k_m2l_count<<<M2L_LISTS, 256, 0, st>>>(nd, ne, c->m2l_cnt);
-        k_m2l_scan<<<1, 512, 0, st>>>(c->m2l_cnt, c->m2l_base, c->m2l_gbase, c->m2l_bbase);
-        k_m2l_fill<<<M2L_LISTS, 256, 0, st>>>(nd, ne, c->m2l_base, c->m2l_list, c->m2l_idx);
-        if (c->precision == SESMD_MIXED) {
-            const m2l_u4 *AFB = (const m2l_u4 *)(nd == g.nb ? c->AFB4 : c->AFB3);
-            const m2l_u4 *OB = (const m2l_u4 *)c->m2l_ob;
-            float *res = (float *)c->m2l_res;
-            int nbox = nd * nd * nd;
-            if (g.p == 4) {  // two slabs, two 32-wide k steps: four waves of two tiles
-                k_m2l_split<2><<<nblk(nbox * 64, 256), 256, 0, st>>>(nbox, 2 * g.lp, Od, c->m2l_ob);
-                k_m2l_gemm_bf3<2, 1, 2, 4, 2><<<c->m2l_max_blocks, 256, 0, st>>>(c->m2l_gbase, c->m2l_bbase,
-                                                                                  c->m2l_list, OB, AFB, res);
-            } else {  // six slabs, six k steps: four waves of 3 slabs x 4 tiles
-                k_m2l_split<6><<<nblk(nbox * 192, 256), 256, 0, st>>>(nbox, 2 * g.lp, Od, c->m2l_ob);
-                k_m2l_gemm_bf3<3, 2, 4, 2, 6><<<c->m2l_max_blocks, 256, 0, st>>>(c->m2l_gbase, c->m2l_bbase,
-                                                                                  c->m2l_list, OB, AFB, res);
-            }
-            k_m2l_accum<float><<<nbox, 64, 0, st>>>(g, nd, c->m2l_idx, res, L);
-            return;
+    if (g.p != 4 && g.p != 8) return;
+    const sesmd_ctx::M2LBuf &B = c->m2l[nd == g.nb ? 0 : 1];
+    hipStream_t st = c->stream;
+    const double *Od = (const double *)O;
+    k_m2l_count<<<M2L_LISTS, 256, 0, st>>>(nd, ne, B.cnt);
+    k_m2l_scan<<<1, 512, 0, st>>>(B.cnt, B.base, B.gbase, B.bbase);
+    k_m2l_fill<<<M2L_LISTS, 256, 0, st>>>(nd, ne, B.base, B.list, B.idx);
+    if (c->precision == SESMD_MIXED) {
+        const m2l_u4 *AFB = (const m2l_u4 *)(nd == g.nb ? c->AFB4 : c->AFB3);
+        const m2l_u4 *OB = (const m2l_u4 *)B.ob;
+        float *res = (float *)B.res;
+        int nbox = nd * nd * nd;
+        if (g.p == 4) {  // two slabs, two 32-wide k steps: four waves of two tiles
+            k_m2l_split<2><<<nblk(nbox * 64, 256), 256, 0, st>>>(nbox, 2 * g.lp, Od, B.ob);
+            k_m2l_gemm_bf3<2, 1, 2, 4, 2><<<B.max_blocks, 256, 0, st>>>(B.gbase, B.bbase, B.list, OB, AFB, res);
+        } else {  // six slabs, six k steps: four waves of 3 slabs x 4 tiles
+            k_m2l_split<6><<<nblk(nbox * 192, 256), 256, 0, st>>>(nbox, 2 * g.lp, Od, B.ob);
+            k_m2l_gemm_bf3<3, 2, 4, 2, 6><<<B.max_blocks, 256, 0, st>>>(B.gbase, B.bbase, B.list, OB, AFB, res);
         }
-        const double *AF = nd == g.nb ? c->AF4 : c->AF3;
-        if (g.p == 4)  // two slabs of rows: two tile halves per block, four waves
-            k_m2l_gemm<2, 13, M2L_TPB, 2><<<c->m2l_max_blocks, 256, 0, st>>>(
-                g, c->m2l_gbase, c->m2l_bbase, c->m2l_list, Od, AF, c->m2l_res);
-        else  // six slabs x two tile halves: twelve waves (139 VGPRs: 12 waves per CU, against
-              // 6 for the six-wave form at 214; the L4 stage 0.230 -> 0.210 ms, same sums)
-            k_m2l_gemm<6, 41, M2L_TPB, 2><<<c->m2l_max_blocks, 768, 0, st>>>(
-                g, c->m2l_gbase, c->m2l_bbase, c->m2l_list, Od, AF, c->m2l_res);
-        k_m2l_accum<double><<<nd * nd * nd, 64, 0, st>>>(g, nd, c->m2l_idx, c->m2l_res, L);
         return;
     }
+    const double *AF = nd == g.nb ? c->AF4 : c->AF3;
+    if (g.p == 4)  // two slabs of rows: two tile halves per block, four waves
+        k_m2l_gemm<2, 13, M2L_TPB, 2><<<B.max_blocks, 256, 0, st>>>(g, B.gbase, B.bbase, B.list, Od, AF, B.res);
+    else  // six slabs x two tile halves: twelve waves (139 VGPRs: 12 waves per CU, against
+          // 6 for the six-wave form at 214; the L4 stage 0.230 -> 0.210 ms, same sums)
+        k_m2l_gemm<6, 41, M2L_TPB, 2><<<B.max_blocks, 768, 0, st>>>(g, B.gbase, B.bbase, B.list, Od, AF, B.res);
+}
+
+// The level's results added into its locals L (after its L2L wrote them).
+static void launch_m2l_accum(sesmd_ctx *c, int nd, const double2 *O, const int *ne, const double2 *T, double2 *L)
+{
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
+    if (g.p == 4 || g.p == 8) {
+        const sesmd_ctx::M2LBuf &B = c->m2l[nd == g.nb ? 0 : 1];
+        if (c->precision == SESMD_MIXED)
+            k_m2l_accum<float><<<nd * nd * nd, 64, 0, st>>>(g, nd, B.idx, (const float *)B.res, L);
+        else
+            k_m2l_accum<double><<<nd * nd * nd, 64, 0, st>>>(g, nd, B.idx, B.res, L);
+        return;
+    }
+    int h = nd / 2;
     if (g.lk <= 16) {
         int nblk_ = 8 * ((h + 1) / 2) * ((h + 1) / 2) * h;
-        k_m2l<16, 16><<<nblk_, 64 * 16, 0, c->stream>>>(g, nd, O, ne, T, L);
+        k_m2l<16, 16><<<nblk_, 64 * 16, 0, st>>>(g, nd, O, ne, T, L);
     } else {
-        k_m2l<64, 8><<<8 * h * h * h, 64 * 8, 0, c->stream>>>(g, nd, O, ne, T, L);
+        k_m2l<64, 8><<<8 * h * h * h, 64 * 8, 0, st>>>(g, nd, O, ne, T, L);
     }
+}
+
+// M2L over the interaction lists of one level (computeLjk3 / computeLjk4).
+static void launch_m2l(sesmd_ctx *c, int nd, const double2 *O, const int *ne, const double2 *T, double2 *L)
+{
+    launch_m2l_products(c, nd, O, ne);
+    launch_m2l_accum(c, nd, O, ne, T, L);
 }
 
 static int launch_fmm(sesmd_ctx *c)
@@ -4457,6 +4500,15 @@ static int launch_fmm(sesmd_ctx *c)
         k_p2m<4><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
     else
         k_p2m<8><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
+    // the L4 products need only O4: on m2l_stream, beside the L3 chain
+    // (M2M, top level, L2L, L3 M2L, L2L), joined before their sums go into
+    // L4.  The two levels' lists and results live in separate buffers.
+    HC(hipEventRecord(c->ev_m2l_fork, st));
+    HC(hipStreamWaitEvent(c->m2l_stream, c->ev_m2l_fork, 0));
+    c->stream = c->m2l_stream;
+    launch_m2l_products(c, g.nb, c->O4, c->ne4);
+    c->stream = st;
+    HC(hipEventRecord(c->ev_m2l_join, c->m2l_stream));
     k_m2m<<<g.n33, 512, 0, st>>>(g, g.n3, c->O4, c->ne4, c->M43, c->O3, c->ne3);
     k_m2m<<<g.n2, 512, 0, st>>>(g, g.nb2, c->O3, c->ne3, c->M32, c->O2, c->ne2);
     k_m2l_top_pair<<<g.n2 * g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->top_part);
@@ -4464,7 +4516,8 @@ static int launch_fmm(sesmd_ctx *c)
     k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3);
     launch_m2l(c, g.n3, c->O3, c->ne3, c->T3, c->L3);
     k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4);
-    launch_m2l(c, g.nb, c->O4, c->ne4, c->T4, c->L4);
+    HC(hipStreamWaitEvent(st, c->ev_m2l_join, 0));
+    launch_m2l_accum(c, g.nb, c->O4, c->ne4, c->T4, c->L4);
     HC(hipGetLastError());
     return 0;
 }
@@ -4693,7 +4746,13 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
     // copies, and writes only its partials); the site terms need both, so the
     // join comes before them.  Regular steps stay on one stream: a fork and
     // join between queues costs more than the ~10 us they could hide.
+    // The near-field launches are issued before the far field's, so a
+    // captured graph lists them first.
     CK(step_part_a(c, rebuild, gathered, rebuild));
+    if (rebuild) {
+        HC(hipEventRecord(c->ev_far_fork, c->stream));
+        HC(hipStreamWaitEvent(c->far_stream, c->ev_far_fork, 0));
+    }
     // with the reference topology the intramolecular terms were evaluated
     // with the eta update and part B adds them per site (with the
     // subtraction); graphite runs inside k_site_terms
@@ -4701,6 +4760,11 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
     forces_gather(c, what, ~0ull, gathered);
     if (what & SESMD_NEAR) launch_near(c, ~0ull);
     if (rebuild) {
+        hipStream_t own = c->stream;
+        c->stream = c->far_stream;
+        int r = launch_fmm(c);
+        c->stream = own;
+        CK(r);
         HC(hipEventRecord(c->ev_far_join, c->far_stream));
         HC(hipStreamWaitEvent(c->stream, c->ev_far_join, 0));
     }
@@ -4745,17 +4809,7 @@ static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered, bool far_asid
     HC(hipGetLastError());
     if (rebuild) {
         CK(launch_rebuild(c));
-        if (far_aside) {  // on far_stream; the caller joins it
-            HC(hipEventRecord(c->ev_far_fork, c->stream));
-            HC(hipStreamWaitEvent(c->far_stream, c->ev_far_fork, 0));
-            hipStream_t own = c->stream;
-            c->stream = c->far_stream;
-            int r = launch_fmm(c);
-            c->stream = own;
-            CK(r);
-        } else {
-            CK(launch_fmm(c));
-        }
+        if (!far_aside) CK(launch_fmm(c));  // else the caller runs it, on far_stream
     }
     return 0;
 }
