@@ -2515,7 +2515,9 @@ __device__ __forceinline__ void legendre_table(double rp1, double (&P)[MAXP + 2]
             double pm2 = pmm;
 #pragma unroll
             for (int ll = m + 2; ll <= p; ll++) {
-                double pll = (rp1 * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pm2) / (ll - m);
+                // 1 / (ll - m) as a constant (the loops are unrolled): a product
+                // where rLegendreP divides, within an ulp of the quotient
+                double pll = (rp1 * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pm2) * (1.0 / (ll - m));
                 pm2 = pmmp1;
                 pmmp1 = pll;
                 P[ll + 1][m] = pll;
@@ -2524,13 +2526,22 @@ __device__ __forceinline__ void legendre_table(double rp1, double (&P)[MAXP + 2]
     }
 }
 
-// the 1/x^13 walls (src/eval_wall.f90:149-188), one thread per site.
-template <int PP>
 // Per-site terms, and the first write of f: f = (near field: the nine row
-// partials of k_near, in row order) + L2P + wall.  The pair kernels before it
+// partials of k_near_rows, in row order) + L2P (src/evalLjk4.f90) + the 1/x^13
+// walls (src/eval_wall.f90:149-188) + graphite.  The pair kernels before it
 // leave their results in `part`; the kernels after it add to f.
-__global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const double *__restrict__ r,
-                             const int *__restrict__ key, const double2 *__restrict__ L4,
+//
+// One block per L4 box, one thread per site of the box (sorted order, so the
+// reads of the sorted copies are coalesced).  The box's local expansion is
+// staged in LDS by one round of loads, so every site's L2P reads its (p + 1)^2
+// coefficients at LDS latency; a site-indexed thread layout, whose lanes may
+// belong to different boxes, has to fetch them from memory in the middle of the
+// arithmetic (21 dependent rounds at p = 8, which dominated it).  The rfctr
+// factors go to LDS beside it.
+constexpr int SITE_TPB = 192;
+template <int PP>
+__global__ __launch_bounds__(SITE_TPB) void k_site_terms(Geo g, unsigned what, const int *__restrict__ bstart,
+                             const double2 *__restrict__ L4,
                              const double *__restrict__ lfac, const sesmd_forcefield *__restrict__ ff,
                              const double *__restrict__ part, const double4 *__restrict__ spos,
                              const int4 *__restrict__ smeta, double *__restrict__ f,
@@ -2538,151 +2549,186 @@ __global__ __launch_bounds__(128) void k_site_terms(Geo g, unsigned what, const 
                              const int *__restrict__ gidx, int ngr)
 {
     // ngr > 0: the graphite term too (the step pipeline's form)
-    // one thread per sorted position k (coalesced reads of the sorted copies;
-    // the site's box, receiver kind and type come from them, not from per-site
-    // arrays indexed by the site, so no load waits for another)
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= g.n) return;
-    int4 me = smeta[k];
-    int i = me.x, rk = ((me.y >> 11) & 7) - 1, bk = key[k] >> 2;
-    double4 pk = spos[k];
-    double x = pk.x, y = pk.y, z = pk.z;
-    double nx = 0.0, ny = 0.0, nz = 0.0;
-    if (what & SESMD_NEAR) {
-        // the nine row partials are loaded with the sorted records, not after
-        // them (one wavefront per SIMD here: nothing hides a second round of
-        // loads); the slots of rows outside the grid hold stale values, and
-        // only the rows (dz, dy) of the receiver's box inside the grid are added
-        double pr[NEAR_ROWS][3];
+    constexpr int p = PP, lp = (PP + 1) * (PP + 1);
+    __shared__ double2 sL[lp];
+    __shared__ double sF[lp];
+    const int b = blockIdx.x, nb = g.nb;
+    const int k0 = bstart[b], k1 = bstart[b + 1];
+    if (k0 >= k1) return;  // empty box (uniform over the block)
+    const bool far = (what & SESMD_FAR) != 0;
+    if (far) {
+        for (int t = threadIdx.x; t < lp; t += SITE_TPB) {
+            sL[t] = L4[(size_t)b * lp + t];
+            sF[t] = lfac[t];
+        }
+        __syncthreads();
+    }
+    const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+    const double cx = d_cen4(g, bx), cy = d_cen4(g, by), cz = d_cen4(g, bz);
+    for (int k = k0 + threadIdx.x; k < k1; k += SITE_TPB) {
+        // the LDS coefficients are read inside each site's sum, not hoisted out of
+        // this loop into hundreds of live registers per lane
+        asm volatile("" ::: "memory");
+        int4 me = smeta[k];
+        int i = me.x, rk = ((me.y >> 11) & 7) - 1;
+        double4 pk = spos[k];
+        double x = pk.x, y = pk.y, z = pk.z;
+        double nx = 0.0, ny = 0.0, nz = 0.0;
+        if (what & SESMD_NEAR) {
+            // the slots of rows outside the grid hold stale values, and only the
+            // rows (dz, dy) of the receiver's box inside the grid are added
+            double pr[NEAR_ROWS][3];
 #pragma unroll
-        for (int q = 0; q < NEAR_ROWS; q++)
+            for (int q = 0; q < NEAR_ROWS; q++)
 #pragma unroll
-            for (int d = 0; d < 3; d++) pr[q][d] = part[(size_t)(3 * q + d) * g.n + k];
-        if (rk >= 0) {
-            int b = bk, nb = g.nb, by = (b / nb) % nb, bz = b / (nb * nb);
+                for (int d = 0; d < 3; d++) pr[q][d] = part[(size_t)(3 * q + d) * g.n + k];
+            if (rk >= 0) {
 #pragma unroll
-            for (int q = 0; q < NEAR_ROWS; q++) {
-                int y = by + q % 3 - 1, z = bz + q / 3 - 1;
-                if (y < 0 || y >= nb || z < 0 || z >= nb) continue;
-                nx += pr[q][0];
-                ny += pr[q][1];
-                nz += pr[q][2];
+                for (int q = 0; q < NEAR_ROWS; q++) {
+                    int yy = by + q % 3 - 1, zz = bz + q / 3 - 1;
+                    if (yy < 0 || yy >= nb || zz < 0 || zz >= nb) continue;
+                    nx += pr[q][0];
+                    ny += pr[q][1];
+                    nz += pr[q][2];
+                }
             }
         }
-    }
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    if ((what & SESMD_FAR) && rk >= 0) {
-        int b = bk, nb = g.nb;
-        constexpr int p = PP;
-        double rxpb = x - d_cen4(g, b % nb), rypb = y - d_cen4(g, (b / nb) % nb),
-               rzpb = z - d_cen4(g, b / (nb * nb));
-        double rqq = pk.w * g.xpi;
-        double rho2 = rxpb * rxpb + rypb * rypb;
-        double rr = sqrt(rho2 + rzpb * rzpb);
-        double rp1 = rzpb / rr;
-        double sr = sqrt(rho2), rsr = 1.0 / sr;
-        double2 rp2 = make_double2(rxpb * rsr, rypb * rsr);
-        double rp3 = 1.0 / rho2;
-        double P[MAXP + 2][MAXP + 1];
-        legendre_table<p>(rp1, P);
-        // e^{i k phi} and e^{-i k phi} for k = 0..p by repeated multiplication
-        // (the inverse of e^{i phi}: its conjugate over |e|^2)
-        double2 pw[MAXP + 1], pwi[MAXP + 1];
-        double rd = 1.0 / (rp2.x * rp2.x + rp2.y * rp2.y);
-        double2 rp2i = make_double2(rp2.x * rd, -rp2.y * rd);
-        pw[0] = pwi[0] = make_double2(1.0, 0.0);
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        if (far && rk >= 0) {
+            double rxpb = x - cx, rypb = y - cy, rzpb = z - cz;
+            double rqq = pk.w * g.xpi;
+            double rho2 = rxpb * rxpb + rypb * rypb;
+            double rr = sqrt(rho2 + rzpb * rzpb);
+            double rp1 = rzpb / rr;
+            double sr = sqrt(rho2), rsr = 1.0 / sr;
+            double2 rp2 = make_double2(rxpb * rsr, rypb * rsr);
+            double rp3 = 1.0 / rho2;
+            // the inverse of e^{i phi}: its conjugate over |e|^2
+            double rd = 1.0 / (rp2.x * rp2.x + rp2.y * rp2.y);
+            double2 rp2i = make_double2(rp2.x * rd, -rp2.y * rd);
+            double rrt[MAXP + 1];  // rr^(j-1)
+            rrt[0] = 1.0 / rr;
 #pragma unroll
-        for (int k2 = 1; k2 <= p; k2++) {
-            pw[k2] = cmul(pw[k2 - 1], rp2);
-            pwi[k2] = cmul(pwi[k2 - 1], rp2i);
-        }
-        const double2 *L = L4 + (size_t)b * g.lp;
-        double rrj = 1.0 / rr;  // rr^(j-1)
-        double ax = 0.0, ay = 0.0, az = 0.0;
+            for (int j = 1; j <= p; j++) rrt[j] = rrt[j - 1] * rr;
+            // The sum runs over m = |k| outermost: column m of the associated
+            // Legendre functions comes from the recurrence of rLegendreP
+            // (src/rLegendreP.f90:3-48) as j rises, in the same operations as
+            // legendre_table, and e^{+-i m phi} by one more product per column,
+            // so no table of either is held.
+            //
+            // Each term of src/evalLjk4.f90 adds rqq rp3 Re(rq0 (rq3 + rq4)) to
+            // fx, with rq0 = sc L e^{ik phi}, sc = rfctr(j, |k|) rr^(j-1),
+            // rq3 = x z rq1, rq4 = (-j x rr rq2, k y rr rq2); expanded, that is
+            // x U - y V with U = Re(rq0) (z rq1 - j rr rq2), V = k rr rq2 Im(rq0);
+            // fy gets y U + x V and fz -rqq Re(rq0) rq1.  So three sums carry the
+            // whole expansion, and the terms k = +m and -m, which share rq1, rq2
+            // and sc, enter them together: U and the fz sum through
+            // Re(L+ e+) + Re(L- e-), V through m (Im(L+ e+) - Im(L- e-)).
+            double2 ep = make_double2(1.0, 0.0), em = ep;  // e^{i m phi}, e^{-i m phi}
+            double somx2 = sqrt((1.0 - rp1) * (1.0 + rp1));
+            double pmm = 1.0;
+            double su = 0.0, sv = 0.0, sw = 0.0;
 #pragma unroll
-        for (int j = 0; j <= p; j++) {
+            for (int m = 0; m <= p; m++) {
+                if (m > 0) {
+                    pmm = -pmm * (double)(2 * m - 1) * somx2;
+                    ep = cmul(ep, rp2);
+                    em = cmul(em, rp2i);
+                }
+                const double mrr = m * rr;
+                // P(m-1, m) = 0; at m = 0 the l = -1 entry, 1, which meets j + m = 0
+                double plm1 = m == 0 ? 1.0 : 0.0, pl = pmm;
 #pragma unroll
-            for (int k = -j; k <= j; k++) {
-                int ak = k < 0 ? -k : k;
-                double2 e = k < 0 ? pwi[ak] : pw[ak];  // e^{i k phi}
-                double2 l = L[j * (j + 1) + k];
-                double sc = lfac[j * (j + 1) + k] * rrj;
-                double2 rq0 = cmul(make_double2(l.x * sc, l.y * sc), e);
-                double rq1 = (j + ak) * P[j][ak];      // P(j-1, |k|)
-                double rq2 = P[j + 1][ak];             // P(j, |k|)
-                double rq3 = rxpb * rzpb * rq1;
-                double2 rq4 = make_double2(-j * rxpb * rr * rq2, k * rypb * rr * rq2);
-                double rq5 = rypb * rzpb * rq1;
-                double2 rq6 = make_double2(-j * rypb * rr * rq2, -k * rxpb * rr * rq2);
-                double2 t1 = cmul(rq0, make_double2(rq3 + rq4.x, rq4.y));
-                double2 t2 = cmul(rq0, make_double2(rq5 + rq6.x, rq6.y));
-                ax += rqq * rp3 * t1.x;
-                ay += rqq * rp3 * t2.x;
-                az -= rqq * rq0.x * rq1;
-            }
-            rrj *= rr;
-        }
-        fx += ax;
-        fy += ay;
-        fz += az;
-    }
-    if (what & SESMD_WALL) {
-        double h = g.rhs;
-        auto p13 = [](double u) {
-            double u2 = u * u, u4 = u2 * u2, u8 = u4 * u4;
-            return u8 * u4 * u;
-        };
-        fx += 48.0 / p13(x - h) + 48.0 / p13(h + x);
-        fy += 48.0 / p13(y - h) + 48.0 / p13(h + y);
-        fz += 48.0 / p13(z - h) + 48.0 / p13(h + z);
-    }
-    // graphite (src/eval_graphite.f90:13-132): receivers filed in their own box
-    // of the bottom layer, against that column's carbons
-    if (ngr > 0 && (what & SESMD_GRAPHITE)) {
-        int b = bk;
-        if (b < g.nb * g.nb && (rk == 0 || rk == 1 || rk == 3)) {
-            int t = me.y & 255;
-            double c = ff->gr_c[t], s6 = ff->gr_s6[t];
-            if (c != 0.0) {
-                // the column's carbons eight at a time: their loads are issued
-                // together (a column holds ~800; one at a time, each a pair of
-                // dependent loads, the walk is latency-bound), the terms are
-                // added in list order
-                double gx = 0.0, gy = 0.0, gz = 0.0;
-                constexpr int U = 8;
-                int q0 = gstart[b], q1 = gstart[b + 1];
-                for (int q = q0; q < q1; q += U) {
-                    int a[U];
-#pragma unroll
-                    for (int u = 0; u < U; u++) a[u] = gidx[min(q + u, q1 - 1)];
-                    double cxyz[U][3];
-#pragma unroll
-                    for (int u = 0; u < U; u++)
-#pragma unroll
-                        for (int d = 0; d < 3; d++) cxyz[u][d] = gr[3 * a[u] + d];
-#pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        double xij = x - cxyz[u][0], yij = y - cxyz[u][1], zij = z - cxyz[u][2];
-                        double rijsq = xij * xij + yij * yij + zij * zij;
-                        if (q + u < q1 && sqrt(rijsq) <= g.rc) {
-                            double rsi = 1.0 / rijsq, r6 = rsi * rsi * rsi;
-                            double flj = c * r6 * (s6 * r6 - 0.5) * rsi;
-                            gx += xij * flj;
-                            gy += yij * flj;
-                            gz += zij * flj;
-                        }
+                for (int j = m; j <= p; j++) {
+                    double rq1 = (j + m) * plm1;  // (j + |k|) P(j-1, |k|)
+                    double rq2 = pl;              // P(j, |k|)
+                    double sc = sF[j * (j + 1) + m] * rrt[j];
+                    double t = rzpb * rq1 - (j * rr) * rq2;
+                    double a;
+                    if (m == 0) {
+                        a = sc * sL[j * (j + 1)].x;
+                    } else {
+                        double2 lp = sL[j * (j + 1) + m], lm = sL[j * (j + 1) - m];
+                        double re = lp.x * ep.x - lp.y * ep.y + lm.x * em.x - lm.y * em.y;
+                        double im = lp.x * ep.y + lp.y * ep.x - lm.x * em.y - lm.y * em.x;
+                        a = sc * re;
+                        sv = fma(sc * im, mrr * rq2, sv);
+                    }
+                    su = fma(a, t, su);
+                    sw = fma(a, rq1, sw);
+                    if (j < p) {  // P(j + 1, m)
+                        double pn = j == m ? rp1 * (2 * m + 1) * pmm
+                                           : (rp1 * (2 * (j + 1) - 1) * pl - (j + m) * plm1) *
+                                                 (1.0 / (j + 1 - m));
+                        plm1 = pl;
+                        pl = pn;
                     }
                 }
-                fx += gx;
-                fy += gy;
-                fz += gz;
+            }
+            double ax = rqq * rp3 * (rxpb * su - rypb * sv);
+            double ay = rqq * rp3 * (rypb * su + rxpb * sv);
+            double az = -rqq * sw;
+            fx += ax;
+            fy += ay;
+            fz += az;
+        }
+        if (what & SESMD_WALL) {
+            double h = g.rhs;
+            auto p13 = [](double u) {
+                double u2 = u * u, u4 = u2 * u2, u8 = u4 * u4;
+                return u8 * u4 * u;
+            };
+            fx += 48.0 / p13(x - h) + 48.0 / p13(h + x);
+            fy += 48.0 / p13(y - h) + 48.0 / p13(h + y);
+            fz += 48.0 / p13(z - h) + 48.0 / p13(h + z);
+        }
+        // graphite (src/eval_graphite.f90:13-132): receivers filed in their own box
+        // of the bottom layer, against that column's carbons
+        if (ngr > 0 && (what & SESMD_GRAPHITE)) {
+            if (b < g.nb * g.nb && (rk == 0 || rk == 1 || rk == 3)) {
+                int t = me.y & 255;
+                double c = ff->gr_c[t], s6 = ff->gr_s6[t];
+                if (c != 0.0) {
+                    // the column's carbons eight at a time: their loads are issued
+                    // together (a column holds ~800; one at a time, each a pair of
+                    // dependent loads, the walk is latency-bound), the terms are
+                    // added in list order
+                    double gx = 0.0, gy = 0.0, gz = 0.0;
+                    constexpr int U = 8;
+                    int q0 = gstart[b], q1 = gstart[b + 1];
+                    for (int q = q0; q < q1; q += U) {
+                        int a[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) a[u] = gidx[min(q + u, q1 - 1)];
+                        double cxyz[U][3];
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+#pragma unroll
+                            for (int d = 0; d < 3; d++) cxyz[u][d] = gr[3 * a[u] + d];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            double xij = x - cxyz[u][0], yij = y - cxyz[u][1], zij = z - cxyz[u][2];
+                            double rijsq = xij * xij + yij * yij + zij * zij;
+                            if (q + u < q1 && sqrt(rijsq) <= g.rc) {
+                                double rsi = 1.0 / rijsq, r6 = rsi * rsi * rsi;
+                                double flj = c * r6 * (s6 * r6 - 0.5) * rsi;
+                                gx += xij * flj;
+                                gy += yij * flj;
+                                gz += zij * flj;
+                            }
+                        }
+                    }
+                    fx += gx;
+                    fy += gy;
+                    fz += gz;
+                }
             }
         }
-    }
 
-    f[i] = nx + fx;
-    f[g.n + i] = ny + fy;
-    f[2 * g.n + i] = nz + fz;
+        f[i] = nx + fx;
+        f[g.n + i] = ny + fy;
+        f[2 * g.n + i] = nz + fz;
+    }
 }
 
 // ------------------------------------------------------------ intramolecular
@@ -4606,8 +4652,8 @@ static int forces_sites(sesmd_ctx *c, unsigned what, bool fold)
     const Geo &g = c->g;
     hipStream_t st = c->stream;
     const unsigned intra = SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE;
-    (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<nblk(g.n, 128), 128, 0, st>>>(
-        g, what, c->r, c->key_out, c->L4, c->lfac, c->dff, c->npart, c->spos, c->smeta, c->f, c->gr,
+    (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<g.nb3, SITE_TPB, 0, st>>>(
+        g, what, c->bstart, c->L4, c->lfac, c->dff, c->npart, c->spos, c->smeta, c->f, c->gr,
         c->gstart, c->gidx, fold ? c->ngr : 0);
     if (g.ns && (what & intra))
         k_intra<<<nblk(29 * g.ns, 64), 64, 0, st>>>(g, what, c->r, c->isub, c->dff, c->dtopo, c->f,
