@@ -1935,9 +1935,9 @@ __global__ __launch_bounds__(64 * W) void k_m2l(Geo g, int nd, const double2 *__
 // none).  A GEMM block takes up to M2L_TPB x 16 consecutive entries of one
 // group, so one operator load serves the targets of every class with that
 // offset.  FP64 (k_m2l_gemm): the operator slab is the A operand of
-// v_mfma_f64_16x16x4f64 (2 lk rows padded to MT x 16, 2 lp columns padded to
-// KS x 4), the 16 entries of a tile the N = 16 columns, the sources'
-// expansions the B operand.  Mixed precision (k_m2l_gemm_bf3): the same
+// v_mfma_f64_16x16x4f64 (2 lk rows padded to MT x 16, 2 lk columns -- the
+// half-space inputs of m2l_half_off -- padded to KS x 4), the 16 entries of a
+// tile the N = 16 columns, the sources' expansions the B operand.  Mixed precision (k_m2l_gemm_bf3): the same
 // product on v_mfma_f32_16x16x32_bf16, each FP64 operand rounded to FP32 and
 // split into three bf16 parts (hi + mid + lo) whose six leading cross
 // products are accumulated in FP32.  Each result column goes to its own row
@@ -1951,6 +1951,20 @@ typedef double m2l_d4 __attribute__((ext_vector_type(4)));
 typedef float m2l_f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 m2l_bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned m2l_u4 __attribute__((ext_vector_type(4)));  // eight bf16 bit patterns
+
+// The products run over half of each source expansion.  Every expansion here
+// holds O(n, -m) = conj(O(n, m)) (P2M and M2M write the m < 0 entries so), so
+// L(j, k) = sum_nm T(jk, nm) O(n, m) is a real-linear map of the 2 lk real
+// values Re, Im O(n, m), m >= 0: real input 2 c + {0, 1} is Re / Im of O(n, m)
+// with c = n (n + 1) / 2 + m, at double offset 2 (n (n + 1) + m) + {0, 1} of
+// the box's expansion.
+__host__ __device__ __forceinline__ int m2l_half_off(int kk)
+{
+    int c = kk >> 1;
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= c) n++;
+    return 2 * (n * (n + 1) + c - n * (n + 1) / 2) + (kk & 1);
+}
 
 // bf16 bit pattern of f, rounded to nearest even (finite f)
 __host__ __device__ __forceinline__ unsigned short bf16_bits(float f)
@@ -2002,21 +2016,25 @@ __device__ __forceinline__ bool m2l_offset(int par, int ci, int &dx, int &dy, in
 }
 
 // target q (0..h^3 - 1) of class par -> box coordinates; source box for the
-// offset, or -1 (outside the grid / without charge)
-__device__ __forceinline__ int m2l_src(int nd, const int *__restrict__ ne, int par, int q, int dx, int dy, int dz,
-                                       int *tb)
+// offset, or -1 (outside the grid / without charge; or, with `tne`, a target
+// whose flag there is 0)
+__device__ __forceinline__ int m2l_src(int nd, const int *__restrict__ ne, const int *__restrict__ tne, int par,
+                                       int q, int dx, int dy, int dz, int *tb)
 {
     int h = nd / 2;
     int tx = 2 * (q % h) + (par & 1), ty = 2 * ((q / h) % h) + ((par >> 1) & 1), tz = 2 * (q / (h * h)) + (par >> 2);
     *tb = tx + nd * (ty + nd * tz);
     int sx = tx + dx, sy = ty + dy, sz = tz + dz;
     if (sx < 0 || sx >= nd || sy < 0 || sy >= nd || sz < 0 || sz >= nd) return -1;
+    if (tne && !tne[*tb]) return -1;
     int si = sx + nd * (sy + nd * sz);
     return ne[si] ? si : -1;
 }
 
-// One block per list (class, candidate): its target count.
-__global__ __launch_bounds__(256) void k_m2l_count(int nd, const int *__restrict__ ne, int *__restrict__ cnt)
+// One block per list (class, candidate): its target count.  tne (or null):
+// translate only into targets flagged there.
+__global__ __launch_bounds__(256) void k_m2l_count(int nd, const int *__restrict__ ne, const int *__restrict__ tne,
+                                                   int *__restrict__ cnt)
 {
     int L = blockIdx.x, par = L / 216, ci = L % 216, h = nd / 2, h3 = h * h * h;
     int dx, dy, dz, tb;
@@ -2026,7 +2044,7 @@ __global__ __launch_bounds__(256) void k_m2l_count(int nd, const int *__restrict
     __syncthreads();
     int c = 0;
     if (far)
-        for (int q = threadIdx.x; q < h3; q += blockDim.x) c += m2l_src(nd, ne, par, q, dx, dy, dz, &tb) >= 0;
+        for (int q = threadIdx.x; q < h3; q += blockDim.x) c += m2l_src(nd, ne, tne, par, q, dx, dy, dz, &tb) >= 0;
     if (c) atomicAdd(&s_c, c);  // integer: the total does not depend on order
     __syncthreads();
     if (threadIdx.x == 0) cnt[L] = s_c;
@@ -2096,8 +2114,9 @@ __device__ __forceinline__ bool m2l_block(const int *__restrict__ gbase, const i
 // One block per list: its entries (target q, source box) in q order, and for
 // every target of the class the position of its result for this candidate
 // (idx[target box][ci], -1 none).
-__global__ __launch_bounds__(256) void k_m2l_fill(int nd, const int *__restrict__ ne, const int *__restrict__ base,
-                                                  int2 *__restrict__ list, int *__restrict__ idx)
+__global__ __launch_bounds__(256) void k_m2l_fill(int nd, const int *__restrict__ ne, const int *__restrict__ tne,
+                                                  const int *__restrict__ base, int2 *__restrict__ list,
+                                                  int *__restrict__ idx)
 {
     int L = blockIdx.x, par = L / 216, ci = L % 216, h = nd / 2, h3 = h * h * h;
     int dx, dy, dz;
@@ -2108,7 +2127,7 @@ __global__ __launch_bounds__(256) void k_m2l_fill(int nd, const int *__restrict_
     for (int q0 = 0; q0 < h3; q0 += 256) {
         int q = q0 + (int)threadIdx.x, tb = -1, si = -1;
         if (q < h3) {
-            si = m2l_src(nd, ne, par, q, dx, dy, dz, &tb);
+            si = m2l_src(nd, ne, tne, par, q, dx, dy, dz, &tb);
             if (!far) si = -1;
         }
         unsigned long long m = __builtin_amdgcn_ballot_w64(si >= 0);
@@ -2162,7 +2181,7 @@ __global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__r
     __syncthreads();
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int mt = w % MT, tw0 = (w / MT) * TW;
-    int lp2 = 2 * g.lp;
+    int lp2 = 2 * g.lp, kh = 2 * g.lk;
     int ntile = (ne_ + 15) / 16;  // tiles with entries (block-uniform)
     m2l_d4 acc[TW];
 #pragma unroll
@@ -2175,9 +2194,10 @@ __global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__r
     double acur[C];
     const double *abase = AF + (size_t)off * KS * MT * 64 + mt * 64 + lane;
     // element idx = thread + NT e of a chunk: k step idx / (64 TPB), tile
-    // (idx / 64) % TPB, lane idx % 64 -- expansion entry 4 k + lane / 16 of the
-    // source of entry 16 tile + lane % 16.  Every load is in bounds (missing
-    // entries read box 0 and are replaced by 0), so none waits on a branch.
+    // (idx / 64) % TPB, lane idx % 64 -- half-space input 4 k + lane / 16 of
+    // the source of entry 16 tile + lane % 16.  Every load is in bounds
+    // (missing entries read box 0 and are replaced by 0), so none waits on a
+    // branch.
     auto load = [&](int ch, Stage &P) {
         int cs = min(C, KS - ch * C);
         const double *src = abase + (size_t)ch * C * MT * 64;
@@ -2196,8 +2216,8 @@ __global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__r
         for (int e = 0; e < PER; e++) {
             int idx = (int)threadIdx.x + NT * e;
             int kk = 4 * (ch * C + idx / (TPB * 64)) + (idx % 64) / 16;
-            bool ok = si[e] >= 0 && kk < lp2;
-            double v = O[(size_t)max(si[e], 0) * lp2 + min(kk, lp2 - 1)];
+            bool ok = si[e] >= 0 && kk < kh;
+            double v = O[(size_t)max(si[e], 0) * lp2 + m2l_half_off(min(kk, kh - 1))];
             P.b[e] = ok ? v : 0.0;
         }
     };
@@ -2261,17 +2281,17 @@ __global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__r
     }
 }
 
-// Mixed precision: each box's real-expanded expansion (2 lp values) as the
-// B-operand parts of k_m2l_gemm_bf3: OB[box][ks][part][32], part 0..2 =
-// hi, mid, lo of column 32 ks + c (zero past 2 lp).
+// Mixed precision: each box's half-space inputs (2 lk values, m2l_half_off)
+// as the B-operand parts of k_m2l_gemm_bf3: OB[box][ks][part][32], part 0..2 =
+// hi, mid, lo of column 32 ks + c (zero past 2 lk).
 template <int KS>
-__global__ __launch_bounds__(256) void k_m2l_split(int nbox, int lp2, const double *__restrict__ O,
+__global__ __launch_bounds__(256) void k_m2l_split(int nbox, int lp2, int kh, const double *__restrict__ O,
                                                    unsigned short *__restrict__ OB)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nbox * 32 * KS) return;
     int box = t / (32 * KS), k = t % (32 * KS);
-    double x = k < lp2 ? O[(size_t)box * lp2 + k] : 0.0;
+    double x = k < kh ? O[(size_t)box * lp2 + m2l_half_off(k)] : 0.0;
     unsigned short h, m, l;
     bf16_split3(x, h, m, l);
     unsigned short *dst = OB + ((size_t)box * KS + k / 32) * 96 + k % 32;
@@ -2366,10 +2386,12 @@ __global__ __launch_bounds__(64 * WM * WT) void k_m2l_gemm_bf3(const int *__rest
 // candidates that have a result are taken eight at a time, their rows loaded
 // together and then added in candidate order.
 template <typename T>
-__global__ __launch_bounds__(64) void k_m2l_accum(Geo g, int nd, const int *__restrict__ idx,
-                                                  const T *__restrict__ res, double2 *__restrict__ L)
+__global__ __launch_bounds__(64) void k_m2l_accum(Geo g, int nd, const int *__restrict__ tne,
+                                                  const int *__restrict__ idx, const T *__restrict__ res,
+                                                  double2 *__restrict__ L)
 {
     int b = blockIdx.x, row = threadIdx.x, lk = g.lk;
+    if (tne && !tne[b]) return;  // no translation was listed into it
     const int *ix = idx + (size_t)b * 216;
     int pv[4];
 #pragma unroll
@@ -3902,33 +3924,41 @@ static int upload_ops(sesmd_ctx *c)
                 buf.push_back(make_double2(z.real(), z.imag()));
             }
     };
-    // MFMA operand order of a real-expanded operator: rows 2r + {re, im} of
-    // the output, columns 2c + {re, im} of the input, padded to MT x 16 rows
-    // and KS x 4 columns; fragments ordered (ks, mt), so the fragments of
+    // Real operator of the half-space products (m2l_half_off): output row
+    // 2r + {re, im} of target row r, input 2c + {re, im} of O(n, m), m >= 0.
+    // With A = T(r, (n, m)), B = T(r, (n, -m)) (zero for m = 0) and
+    // O(n, -m) = conj(O(n, m)) = a - ib, the output gains A (a + ib) + B (a - ib):
+    // Re: (A.re + B.re) a + (B.im - A.im) b, Im: (A.im + B.im) a + (A.re - B.re) b.
+    auto half_op = [&](int mm, int kk) -> double {
+        int r = mm >> 1, ro = mm & 1, ci = kk & 1, cc = kk >> 1;
+        int n = 0;
+        while ((n + 1) * (n + 2) / 2 <= cc) n++;
+        int m = cc - n * (n + 1) / 2;
+        const hostm::C &A = T[(size_t)r * lp + n * (n + 1) + m];
+        hostm::C B = m > 0 ? T[(size_t)r * lp + n * (n + 1) - m] : hostm::C(0.0);
+        if (ro == 0) return ci == 0 ? A.real() + B.real() : B.imag() - A.imag();
+        return ci == 0 ? A.imag() + B.imag() : A.real() - B.real();
+    };
+    // MFMA operand order of the real operator: padded to MT x 16 rows and
+    // KS x 4 columns; fragments ordered (ks, mt), so the fragments of
     // consecutive k steps are contiguous; lane l of fragment (ks, mt) holds
     // element (16 mt + l % 16, 4 ks + l / 16) -- the A operand of
     // v_mfma_f64_16x16x4f64.
-    const int MT = (2 * lk + 15) / 16, KS = (2 * lp + 3) / 4;
+    const int MT = (2 * lk + 15) / 16, KS = (2 * lk + 3) / 4;
     std::vector<double> af;
     auto append_frag = [&]() {
         for (int ks = 0; ks < KS; ks++)
             for (int mt = 0; mt < MT; mt++)
                 for (int l = 0; l < 64; l++) {
                     int mm = 16 * mt + l % 16, kk = 4 * ks + l / 16;
-                    double v = 0.0;
-                    if (mm < 2 * lk && kk < 2 * lp) {
-                        const hostm::C &z = T[(size_t)(mm / 2) * lp + kk / 2];
-                        int ro = mm & 1, ci = kk & 1;
-                        v = ro == ci ? z.real() : (ro == 0 ? -z.imag() : z.imag());
-                    }
-                    af.push_back(v);
+                    af.push_back(mm < 2 * lk && kk < 2 * lk ? half_op(mm, kk) : 0.0);
                 }
     };
     // mixed precision: the same operators as bf16 parts in the A-operand
     // layout of v_mfma_f32_16x16x32_bf16, fragments [ks][slab][part] of 64
     // lanes x 8 values; lane l of fragment (ks, slab) holds row 16 slab + l % 16,
     // columns 32 ks + 8 (l / 16) .. + 7
-    const int KS32 = (2 * lp + 31) / 32;
+    const int KS32 = (2 * lk + 31) / 32;
     std::vector<unsigned short> afb;
     auto append_bf3 = [&]() {
         for (int ks = 0; ks < KS32; ks++)
@@ -3938,12 +3968,7 @@ static int upload_ops(sesmd_ctx *c)
                 for (int l = 0; l < 64; l++)
                     for (int e = 0; e < 8; e++) {
                         int mm = 16 * mt + l % 16, kk = 32 * ks + 8 * (l / 16) + e;
-                        double v = 0.0;
-                        if (mm < 2 * lk && kk < 2 * lp) {
-                            const hostm::C &z = T[(size_t)(mm / 2) * lp + kk / 2];
-                            int ro = mm & 1, ci = kk & 1;
-                            v = ro == ci ? z.real() : (ro == 0 ? -z.imag() : z.imag());
-                        }
+                        double v = mm < 2 * lk && kk < 2 * lk ? half_op(mm, kk) : 0.0;
                         unsigned short h, m, lo;
                         bf16_split3(v, h, m, lo);
                         afb[at + (size_t)(0 * 64 + l) * 8 + e] = h;
@@ -4183,7 +4208,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->ne2, g.n2));
     A(dalloc(&c->T4, (size_t)343 * g.lk * g.lp));
     {
-        size_t mt = (2 * g.lk + 15) / 16, ks = (2 * g.lp + 3) / 4;
+        size_t mt = (2 * g.lk + 15) / 16, ks = (2 * g.lk + 3) / 4;
         // one spare offset of zeros at the end: k_m2l_gemm's operator loads
         // run unconditionally up to a chunk past an offset's last k step
         A(dalloc(&c->AF4, (size_t)344 * mt * ks * 64));
@@ -4203,10 +4228,10 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
             A(dalloc(&B.gbase, M2L_GROUPS + 1));
             A(dalloc(&B.bbase, M2L_GROUPS + 1));
             B.max_blocks = (int)(pairs / (M2L_TPB * 16)) + M2L_GROUPS;
-            if (precision == SESMD_MIXED) A(dalloc(&B.ob, nbox * ((2 * g.lp + 31) / 32) * 96));
+            if (precision == SESMD_MIXED) A(dalloc(&B.ob, nbox * ((2 * g.lk + 31) / 32) * 96));
         }
         if (precision == SESMD_MIXED) {
-            size_t ks32 = (2 * g.lp + 31) / 32;
+            size_t ks32 = (2 * g.lk + 31) / 32;
             A(dalloc(&c->AFB4, (size_t)M2L_GROUPS * ks32 * mt * 3 * 64 * 8));
             A(dalloc(&c->AFB3, (size_t)M2L_GROUPS * ks32 * mt * 3 * 64 * 8));
         }
@@ -4477,49 +4502,57 @@ extern "C" int sesmd_rebuild(sesmd_ctx *c)
 // rows): they read only the level's multipoles, so the L4 products can run
 // before the L4 locals exist.  M2L_accum adds them in; the vector kernel k_m2l
 // (orders other than 4 and 8) does both at once.
-static void launch_m2l_products(sesmd_ctx *c, int nd, const double2 *O, const int *ne)
+// occ: translate only into the boxes flagged in ne (those holding a site of a
+// receiver kind): the step pipeline's form, whose only reader of a local
+// expansion is the L2P of the box's own sites (at L3, the L2L into such boxes).
+// Without it every box gets its full local expansion, as computeLjk3 /
+// computeLjk4 write it (the sesmd_fmm and sesmd_fmm_stage form).
+static void launch_m2l_products(sesmd_ctx *c, int nd, const double2 *O, const int *ne, bool occ)
 {
     const Geo &g = c->g;
     if (g.p != 4 && g.p != 8) return;
     const sesmd_ctx::M2LBuf &B = c->m2l[nd == g.nb ? 0 : 1];
     hipStream_t st = c->stream;
     const double *Od = (const double *)O;
-    k_m2l_count<<<M2L_LISTS, 256, 0, st>>>(nd, ne, B.cnt);
+    const int *tne = occ ? ne : nullptr;
+    k_m2l_count<<<M2L_LISTS, 256, 0, st>>>(nd, ne, tne, B.cnt);
     k_m2l_scan<<<1, 512, 0, st>>>(B.cnt, B.base, B.gbase, B.bbase);
-    k_m2l_fill<<<M2L_LISTS, 256, 0, st>>>(nd, ne, B.base, B.list, B.idx);
+    k_m2l_fill<<<M2L_LISTS, 256, 0, st>>>(nd, ne, tne, B.base, B.list, B.idx);
     if (c->precision == SESMD_MIXED) {
         const m2l_u4 *AFB = (const m2l_u4 *)(nd == g.nb ? c->AFB4 : c->AFB3);
         const m2l_u4 *OB = (const m2l_u4 *)B.ob;
         float *res = (float *)B.res;
         int nbox = nd * nd * nd;
-        if (g.p == 4) {  // two slabs, two 32-wide k steps: four waves of two tiles
-            k_m2l_split<2><<<nblk(nbox * 64, 256), 256, 0, st>>>(nbox, 2 * g.lp, Od, B.ob);
-            k_m2l_gemm_bf3<2, 1, 2, 4, 2><<<B.max_blocks, 256, 0, st>>>(B.gbase, B.bbase, B.list, OB, AFB, res);
-        } else {  // six slabs, six k steps: four waves of 3 slabs x 4 tiles
-            k_m2l_split<6><<<nblk(nbox * 192, 256), 256, 0, st>>>(nbox, 2 * g.lp, Od, B.ob);
-            k_m2l_gemm_bf3<3, 2, 4, 2, 6><<<B.max_blocks, 256, 0, st>>>(B.gbase, B.bbase, B.list, OB, AFB, res);
+        if (g.p == 4) {  // two slabs, one 32-wide k step: four waves of two tiles
+            k_m2l_split<1><<<nblk(nbox * 32, 256), 256, 0, st>>>(nbox, 2 * g.lp, 2 * g.lk, Od, B.ob);
+            k_m2l_gemm_bf3<2, 1, 2, 4, 1><<<B.max_blocks, 256, 0, st>>>(B.gbase, B.bbase, B.list, OB, AFB, res);
+        } else {  // six slabs, three k steps: four waves of 3 slabs x 4 tiles
+            k_m2l_split<3><<<nblk(nbox * 96, 256), 256, 0, st>>>(nbox, 2 * g.lp, 2 * g.lk, Od, B.ob);
+            k_m2l_gemm_bf3<3, 2, 4, 2, 3><<<B.max_blocks, 256, 0, st>>>(B.gbase, B.bbase, B.list, OB, AFB, res);
         }
         return;
     }
     const double *AF = nd == g.nb ? c->AF4 : c->AF3;
     if (g.p == 4)  // two slabs of rows: two tile halves per block, four waves
-        k_m2l_gemm<2, 13, M2L_TPB, 2><<<B.max_blocks, 256, 0, st>>>(g, B.gbase, B.bbase, B.list, Od, AF, B.res);
-    else  // six slabs x two tile halves: twelve waves (139 VGPRs: 12 waves per CU, against
-          // 6 for the six-wave form at 214; the L4 stage 0.230 -> 0.210 ms, same sums)
-        k_m2l_gemm<6, 41, M2L_TPB, 2><<<B.max_blocks, 768, 0, st>>>(g, B.gbase, B.bbase, B.list, Od, AF, B.res);
+        k_m2l_gemm<2, 8, M2L_TPB, 2><<<B.max_blocks, 256, 0, st>>>(g, B.gbase, B.bbase, B.list, Od, AF, B.res);
+    else  // six slabs x two tile halves: twelve waves (12 waves per CU, against 6
+          // for a six-wave form at 214 VGPRs)
+        k_m2l_gemm<6, 23, M2L_TPB, 2><<<B.max_blocks, 768, 0, st>>>(g, B.gbase, B.bbase, B.list, Od, AF, B.res);
 }
 
 // The level's results added into its locals L (after its L2L wrote them).
-static void launch_m2l_accum(sesmd_ctx *c, int nd, const double2 *O, const int *ne, const double2 *T, double2 *L)
+static void launch_m2l_accum(sesmd_ctx *c, int nd, const double2 *O, const int *ne, const double2 *T, double2 *L,
+                             bool occ)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
     if (g.p == 4 || g.p == 8) {
         const sesmd_ctx::M2LBuf &B = c->m2l[nd == g.nb ? 0 : 1];
         if (c->precision == SESMD_MIXED)
-            k_m2l_accum<float><<<nd * nd * nd, 64, 0, st>>>(g, nd, B.idx, (const float *)B.res, L);
+            k_m2l_accum<float><<<nd * nd * nd, 64, 0, st>>>(g, nd, occ ? ne : nullptr, B.idx,
+                                                             (const float *)B.res, L);
         else
-            k_m2l_accum<double><<<nd * nd * nd, 64, 0, st>>>(g, nd, B.idx, B.res, L);
+            k_m2l_accum<double><<<nd * nd * nd, 64, 0, st>>>(g, nd, occ ? ne : nullptr, B.idx, B.res, L);
         return;
     }
     int h = nd / 2;
@@ -4532,13 +4565,14 @@ static void launch_m2l_accum(sesmd_ctx *c, int nd, const double2 *O, const int *
 }
 
 // M2L over the interaction lists of one level (computeLjk3 / computeLjk4).
-static void launch_m2l(sesmd_ctx *c, int nd, const double2 *O, const int *ne, const double2 *T, double2 *L)
+static void launch_m2l(sesmd_ctx *c, int nd, const double2 *O, const int *ne, const double2 *T, double2 *L,
+                       bool occ)
 {
-    launch_m2l_products(c, nd, O, ne);
-    launch_m2l_accum(c, nd, O, ne, T, L);
+    launch_m2l_products(c, nd, O, ne, occ);
+    launch_m2l_accum(c, nd, O, ne, T, L, occ);
 }
 
-static int launch_fmm(sesmd_ctx *c)
+static int launch_fmm(sesmd_ctx *c, bool occ)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
@@ -4552,7 +4586,7 @@ static int launch_fmm(sesmd_ctx *c)
     HC(hipEventRecord(c->ev_m2l_fork, st));
     HC(hipStreamWaitEvent(c->m2l_stream, c->ev_m2l_fork, 0));
     c->stream = c->m2l_stream;
-    launch_m2l_products(c, g.nb, c->O4, c->ne4);
+    launch_m2l_products(c, g.nb, c->O4, c->ne4, occ);
     c->stream = st;
     HC(hipEventRecord(c->ev_m2l_join, c->m2l_stream));
     k_m2m<<<g.n33, 512, 0, st>>>(g, g.n3, c->O4, c->ne4, c->M43, c->O3, c->ne3);
@@ -4560,10 +4594,10 @@ static int launch_fmm(sesmd_ctx *c)
     k_m2l_top_pair<<<g.n2 * g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->top_part);
     k_m2l_top_sum<<<g.n2, 64, 0, st>>>(g, c->top_part, c->L2);
     k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3);
-    launch_m2l(c, g.n3, c->O3, c->ne3, c->T3, c->L3);
+    launch_m2l(c, g.n3, c->O3, c->ne3, c->T3, c->L3, occ);
     k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4);
     HC(hipStreamWaitEvent(st, c->ev_m2l_join, 0));
-    launch_m2l_accum(c, g.nb, c->O4, c->ne4, c->T4, c->L4);
+    launch_m2l_accum(c, g.nb, c->O4, c->ne4, c->T4, c->L4, occ);
     HC(hipGetLastError());
     return 0;
 }
@@ -4572,7 +4606,7 @@ extern "C" int sesmd_fmm(sesmd_ctx *c)
 {
     if (!c) return fail("null context");
     if (!c->built) return fail("sesmd_fmm before sesmd_rebuild");
-    CK(launch_fmm(c));
+    CK(launch_fmm(c, false));
     HC(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -4808,7 +4842,7 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
     if (rebuild) {
         hipStream_t own = c->stream;
         c->stream = c->far_stream;
-        int r = launch_fmm(c);
+        int r = launch_fmm(c, true);
         c->stream = own;
         CK(r);
         HC(hipEventRecord(c->ev_far_join, c->far_stream));
@@ -4855,7 +4889,7 @@ static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered, bool far_asid
     HC(hipGetLastError());
     if (rebuild) {
         CK(launch_rebuild(c));
-        if (!far_aside) CK(launch_fmm(c));  // else the caller runs it, on far_stream
+        if (!far_aside) CK(launch_fmm(c, true));  // else the caller runs it, on far_stream
     }
     return 0;
 }
@@ -5606,9 +5640,9 @@ static int enqueue_fmm_stage(sesmd_ctx *c, int stage)
         k_m2l_top_sum<<<g.n2, 64, 0, st>>>(g, c->top_part, c->L2);
         break;
     case SESMD_L2L_23: k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3); break;
-    case SESMD_M2L_3: launch_m2l(c, g.n3, c->O3, c->ne3, c->T3, c->L3); break;
+    case SESMD_M2L_3: launch_m2l(c, g.n3, c->O3, c->ne3, c->T3, c->L3, false); break;
     case SESMD_L2L_34: k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4); break;
-    case SESMD_M2L_4: launch_m2l(c, g.nb, c->O4, c->ne4, c->T4, c->L4); break;
+    case SESMD_M2L_4: launch_m2l(c, g.nb, c->O4, c->ne4, c->T4, c->L4, false); break;
     default: return fail("unknown far-field stage %d", stage);
     }
     HC(hipGetLastError());
