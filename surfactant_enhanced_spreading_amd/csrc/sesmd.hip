@@ -4464,7 +4464,16 @@ extern "C" int sesmd_set_graphite(sesmd_ctx *c, int n, const double *xyz)
     return 0;
 }
 
-static int launch_rebuild(sesmd_ctx *c)
+// the sorted metadata (smeta, bflag) the near field and the site terms read
+static void launch_gather_meta(sesmd_ctx *c)
+{
+    const Geo &g = c->g;
+    k_gather_meta<<<nblk(g.n, 256), 256, 0, c->stream>>>(g, c->perm, c->type, c->mol, c->first, c->cls, c->anc,
+                                                         c->cbox, c->rkind, c->smeta, c->bflag);
+}
+
+// meta: also the sorted metadata (the step pipeline issues it itself, after P2M)
+static int launch_rebuild(sesmd_ctx *c, bool meta = true)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
@@ -4480,8 +4489,7 @@ static int launch_rebuild(sesmd_ctx *c)
     k_key_scatter<<<nblk(g.n, 256), 256, 0, st>>>(g, c->key_in, c->bs4, c->kcnt + K, c->val_in);
     k_key_order<<<K, 64, 0, st>>>(g, c->bs4, c->val_in, c->perm, c->key_out, c->iperm);
     c->boxlist = false;  // built on the first masked evaluation after this rebuild
-    k_gather_meta<<<nblk(g.n, 256), 256, 0, st>>>(g, c->perm, c->type, c->mol, c->first, c->cls, c->anc,
-                                                  c->cbox, c->rkind, c->smeta, c->bflag);
+    if (meta) launch_gather_meta(c);
     HC(hipGetLastError());
     c->built = true;
     return 0;
@@ -4507,17 +4515,26 @@ extern "C" int sesmd_rebuild(sesmd_ctx *c)
 // expansion is the L2P of the box's own sites (at L3, the L2L into such boxes).
 // Without it every box gets its full local expansion, as computeLjk3 /
 // computeLjk4 write it (the sesmd_fmm and sesmd_fmm_stage form).
-static void launch_m2l_products(sesmd_ctx *c, int nd, const double2 *O, const int *ne, bool occ)
+static void launch_m2l_lists(sesmd_ctx *c, int nd, const int *ne, bool occ)
+{
+    const Geo &g = c->g;
+    if (g.p != 4 && g.p != 8) return;
+    const sesmd_ctx::M2LBuf &B = c->m2l[nd == g.nb ? 0 : 1];
+    hipStream_t st = c->stream;
+    const int *tne = occ ? ne : nullptr;
+    k_m2l_count<<<M2L_LISTS, 256, 0, st>>>(nd, ne, tne, B.cnt);
+    k_m2l_scan<<<1, 512, 0, st>>>(B.cnt, B.base, B.gbase, B.bbase);
+    k_m2l_fill<<<M2L_LISTS, 256, 0, st>>>(nd, ne, tne, B.base, B.list, B.idx);
+}
+
+// the products of one level over the lists launch_m2l_lists wrote
+static void launch_m2l_gemms(sesmd_ctx *c, int nd, const double2 *O)
 {
     const Geo &g = c->g;
     if (g.p != 4 && g.p != 8) return;
     const sesmd_ctx::M2LBuf &B = c->m2l[nd == g.nb ? 0 : 1];
     hipStream_t st = c->stream;
     const double *Od = (const double *)O;
-    const int *tne = occ ? ne : nullptr;
-    k_m2l_count<<<M2L_LISTS, 256, 0, st>>>(nd, ne, tne, B.cnt);
-    k_m2l_scan<<<1, 512, 0, st>>>(B.cnt, B.base, B.gbase, B.bbase);
-    k_m2l_fill<<<M2L_LISTS, 256, 0, st>>>(nd, ne, tne, B.base, B.list, B.idx);
     if (c->precision == SESMD_MIXED) {
         const m2l_u4 *AFB = (const m2l_u4 *)(nd == g.nb ? c->AFB4 : c->AFB3);
         const m2l_u4 *OB = (const m2l_u4 *)B.ob;
@@ -4538,6 +4555,12 @@ static void launch_m2l_products(sesmd_ctx *c, int nd, const double2 *O, const in
     else  // six slabs x two tile halves: twelve waves (12 waves per CU, against 6
           // for a six-wave form at 214 VGPRs)
         k_m2l_gemm<6, 23, M2L_TPB, 2><<<B.max_blocks, 768, 0, st>>>(g, B.gbase, B.bbase, B.list, Od, AF, B.res);
+}
+
+static void launch_m2l_products(sesmd_ctx *c, int nd, const double2 *O, const int *ne, bool occ)
+{
+    launch_m2l_lists(c, nd, ne, occ);
+    launch_m2l_gemms(c, nd, O);
 }
 
 // The level's results added into its locals L (after its L2L wrote them).
@@ -4572,7 +4595,13 @@ static void launch_m2l(sesmd_ctx *c, int nd, const double2 *O, const int *ne, co
     launch_m2l_accum(c, nd, O, ne, T, L, occ);
 }
 
-static int launch_fmm(sesmd_ctx *c, bool occ)
+// The far field in two parts: the head, P2M, runs alone in a rebuild step
+// (beside the near field it took ~4x as long, and it heads the longer branch);
+// the tail runs beside the near field.  Moving M2M to L3 and the M2L lists
+// into the head as well left the L4 products sharing the CUs with the near
+// field and starved the chain of short kernels at L3 instead (rebuild step
+// ~565 -> ~590 us).
+static void launch_fmm_head(sesmd_ctx *c)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
@@ -4580,6 +4609,12 @@ static int launch_fmm(sesmd_ctx *c, bool occ)
         k_p2m<4><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
     else
         k_p2m<8><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
+}
+
+static int launch_fmm_tail(sesmd_ctx *c, bool occ)
+{
+    const Geo &g = c->g;
+    hipStream_t st = c->stream;
     // the L4 products need only O4: on m2l_stream, beside the L3 chain
     // (M2M, top level, L2L, L3 M2L, L2L), joined before their sums go into
     // L4.  The two levels' lists and results live in separate buffers.
@@ -4600,6 +4635,12 @@ static int launch_fmm(sesmd_ctx *c, bool occ)
     launch_m2l_accum(c, g.nb, c->O4, c->ne4, c->T4, c->L4, occ);
     HC(hipGetLastError());
     return 0;
+}
+
+static int launch_fmm(sesmd_ctx *c, bool occ)
+{
+    launch_fmm_head(c);
+    return launch_fmm_tail(c, occ);
 }
 
 extern "C" int sesmd_fmm(sesmd_ctx *c)
@@ -4820,18 +4861,21 @@ static unsigned step_what(const sesmd_ctx *c) { return c->surf_fast ? SESMD_ALL 
 static int launch_step(sesmd_ctx *c, bool rebuild)
 {
     bool gathered = false;
-    // a rebuild step's far field goes to far_stream, forked after the sort,
-    // and runs beside the near field: the two touch disjoint buffers (P2M
-    // reads the positions through the permutation, the near field the sorted
-    // copies, and writes only its partials); the site terms need both, so the
-    // join comes before them.  Regular steps stay on one stream: a fork and
-    // join between queues costs more than the ~10 us they could hide.
-    // The near-field launches are issued before the far field's, so a
-    // captured graph lists them first.
+    // A rebuild step runs the head of the far field (P2M) alone right after
+    // the sort, then forks the tail to far_stream, beside the sorted
+    // metadata, the position gathers and the near field: the branches touch
+    // disjoint buffers (the far field reads the multipoles, the near field
+    // the sorted copies and writes only its partials); the site terms need
+    // both, so the join comes before them.  Regular steps stay on one
+    // stream: a fork and join between queues costs more than the ~10 us they
+    // could hide.  The near-field launches are issued before the far field's,
+    // so a captured graph lists them first.
     CK(step_part_a(c, rebuild, gathered, rebuild));
     if (rebuild) {
+        launch_fmm_head(c);
         HC(hipEventRecord(c->ev_far_fork, c->stream));
         HC(hipStreamWaitEvent(c->far_stream, c->ev_far_fork, 0));
+        launch_gather_meta(c);
     }
     // with the reference topology the intramolecular terms were evaluated
     // with the eta update and part B adds them per site (with the
@@ -4842,7 +4886,7 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
     if (rebuild) {
         hipStream_t own = c->stream;
         c->stream = c->far_stream;
-        int r = launch_fmm(c, true);
+        int r = launch_fmm_tail(c, true);
         c->stream = own;
         CK(r);
         HC(hipEventRecord(c->ev_far_join, c->far_stream));
@@ -4888,8 +4932,9 @@ static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered, bool far_asid
     }
     HC(hipGetLastError());
     if (rebuild) {
-        CK(launch_rebuild(c));
-        if (!far_aside) CK(launch_fmm(c, true));  // else the caller runs it, on far_stream
+        // far_aside: the caller issues the sorted metadata and the far field
+        CK(launch_rebuild(c, !far_aside));
+        if (!far_aside) CK(launch_fmm(c, true));
     }
     return 0;
 }
