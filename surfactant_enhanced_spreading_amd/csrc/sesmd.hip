@@ -4611,19 +4611,25 @@ static void launch_fmm_head(sesmd_ctx *c)
         k_p2m<8><<<g.nb3, 64, 0, st>>>(g, c->r, c->perm, c->bstart, c->rkind, c->type, c->dff, c->O4, c->ne4);
 }
 
-static int launch_fmm_tail(sesmd_ctx *c, bool occ)
+// fork: the L4 products on m2l_stream (the step pipeline of one context);
+// otherwise everything on the context's stream (an ensemble's members already
+// run side by side on the graph's branches, and one more fork and join per
+// member cost more than they hid)
+static int launch_fmm_tail(sesmd_ctx *c, bool occ, bool fork)
 {
     const Geo &g = c->g;
     hipStream_t st = c->stream;
     // the L4 products need only O4: on m2l_stream, beside the L3 chain
     // (M2M, top level, L2L, L3 M2L, L2L), joined before their sums go into
     // L4.  The two levels' lists and results live in separate buffers.
-    HC(hipEventRecord(c->ev_m2l_fork, st));
-    HC(hipStreamWaitEvent(c->m2l_stream, c->ev_m2l_fork, 0));
-    c->stream = c->m2l_stream;
+    if (fork) {
+        HC(hipEventRecord(c->ev_m2l_fork, st));
+        HC(hipStreamWaitEvent(c->m2l_stream, c->ev_m2l_fork, 0));
+        c->stream = c->m2l_stream;
+    }
     launch_m2l_products(c, g.nb, c->O4, c->ne4, occ);
     c->stream = st;
-    HC(hipEventRecord(c->ev_m2l_join, c->m2l_stream));
+    if (fork) HC(hipEventRecord(c->ev_m2l_join, c->m2l_stream));
     k_m2m<<<g.n33, 512, 0, st>>>(g, g.n3, c->O4, c->ne4, c->M43, c->O3, c->ne3);
     k_m2m<<<g.n2, 512, 0, st>>>(g, g.nb2, c->O3, c->ne3, c->M32, c->O2, c->ne2);
     k_m2l_top_pair<<<g.n2 * g.n2, 64, 0, st>>>(g, c->O2, c->ne2, c->T2, c->top_part);
@@ -4631,23 +4637,23 @@ static int launch_fmm_tail(sesmd_ctx *c, bool occ)
     k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3);
     launch_m2l(c, g.n3, c->O3, c->ne3, c->T3, c->L3, occ);
     k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4);
-    HC(hipStreamWaitEvent(st, c->ev_m2l_join, 0));
+    if (fork) HC(hipStreamWaitEvent(st, c->ev_m2l_join, 0));
     launch_m2l_accum(c, g.nb, c->O4, c->ne4, c->T4, c->L4, occ);
     HC(hipGetLastError());
     return 0;
 }
 
-static int launch_fmm(sesmd_ctx *c, bool occ)
+static int launch_fmm(sesmd_ctx *c, bool occ, bool fork)
 {
     launch_fmm_head(c);
-    return launch_fmm_tail(c, occ);
+    return launch_fmm_tail(c, occ, fork);
 }
 
 extern "C" int sesmd_fmm(sesmd_ctx *c)
 {
     if (!c) return fail("null context");
     if (!c->built) return fail("sesmd_fmm before sesmd_rebuild");
-    CK(launch_fmm(c, false));
+    CK(launch_fmm(c, false, true));
     HC(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -4886,7 +4892,7 @@ static int launch_step(sesmd_ctx *c, bool rebuild)
     if (rebuild) {
         hipStream_t own = c->stream;
         c->stream = c->far_stream;
-        int r = launch_fmm_tail(c, true);
+        int r = launch_fmm_tail(c, true, true);
         c->stream = own;
         CK(r);
         HC(hipEventRecord(c->ev_far_join, c->far_stream));
@@ -4934,7 +4940,7 @@ static int step_part_a(sesmd_ctx *c, bool rebuild, bool &gathered, bool far_asid
     if (rebuild) {
         // far_aside: the caller issues the sorted metadata and the far field
         CK(launch_rebuild(c, !far_aside));
-        if (!far_aside) CK(launch_fmm(c, true));
+        if (!far_aside) CK(launch_fmm(c, true, false));
     }
     return 0;
 }
