@@ -136,6 +136,7 @@ struct sesmd_ctx {
     bool boxlist;                    // `work` matches the last rebuild
     int2 *rwork;                     // row-centred near-field items (the full near field)
     int *nrwork;
+    int4 *obox;                      // boxes holding sites, in box order: (box, first, end); then box -1
     int max_rwork;
     int2 *work_raw;                  // box-centred items in box order, before the cost sort
     unsigned *wkey_in, *wkey_out;    // 1 + estimated cost per item (0: padding)
@@ -1414,14 +1415,45 @@ __device__ __forceinline__ void row_list_block(const Geo &g, const int *__restri
 }
 
 // The box sort's scan and, from its box starts, the row list: one block.
+// The boxes holding sites, in box order, as (box, first, end) sorted
+// positions; every later entry is box -1.  One block (after its bstart).
+__device__ __forceinline__ void occupied_block(const Geo &g, const int *__restrict__ bstart, int4 *__restrict__ obox)
+{
+    __shared__ int s_w[16];
+    int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = (int)blockDim.x >> 6, run = 0;
+    for (int b0 = 0; b0 < g.nb3; b0 += blockDim.x) {
+        int b = b0 + t, k0 = 0, k1 = 0;
+        if (b < g.nb3) {
+            k0 = bstart[b];
+            k1 = bstart[b + 1];
+        }
+        bool f = k1 > k0;
+        unsigned long long m = __builtin_amdgcn_ballot_w64(f);
+        int before = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (lane == 0) s_w[w] = __builtin_popcountll(m);
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int j = 0; j < nw; j++) {
+            off += j < w ? s_w[j] : 0;
+            tot += s_w[j];
+        }
+        if (f) obox[run + off + before] = make_int4(b, k0, k1, 0);
+        run += tot;
+        __syncthreads();  // s_w is rewritten by the next round
+    }
+    for (int q = run + t; q < g.nb3; q += blockDim.x) obox[q] = make_int4(-1, 0, 0, 0);
+}
+
 __global__ __launch_bounds__(1024) void k_key_scan_rows(Geo g, const int *__restrict__ cnt, int *__restrict__ bs4,
                                                         int *__restrict__ bstart, int2 *__restrict__ work,
-                                                        int *__restrict__ nwork, int max_work)
+                                                        int *__restrict__ nwork, int max_work,
+                                                        int4 *__restrict__ obox)
 {
     key_scan_block(g, cnt, bs4, bstart);
     __threadfence_block();  // this block's bstart stores before its loads below
     __syncthreads();
     row_list_block(g, bstart, work, nwork, max_work);
+    occupied_block(g, bstart, obox);
 }
 
 // One row-centred item: positions c0 + lane (+ 64 for NR = 2) of the row's
@@ -2553,16 +2585,20 @@ __device__ __forceinline__ void legendre_table(double rp1, double (&P)[MAXP + 2]
 // walls (src/eval_wall.f90:149-188) + graphite.  The pair kernels before it
 // leave their results in `part`; the kernels after it add to f.
 //
-// One block per L4 box, one thread per site of the box (sorted order, so the
-// reads of the sorted copies are coalesced).  The box's local expansion is
+// One block per box holding sites (the rebuild's list obox; block q takes
+// entries q, q + grid, ...), one thread per site of the box (sorted order, so
+// the reads of the sorted copies are coalesced).  The box's local expansion is
 // staged in LDS by one round of loads, so every site's L2P reads its (p + 1)^2
 // coefficients at LDS latency; a site-indexed thread layout, whose lanes may
 // belong to different boxes, has to fetch them from memory in the middle of the
 // arithmetic (21 dependent rounds at p = 8, which dominated it).  The rfctr
-// factors go to LDS beside it.
+// factors go to LDS beside it.  Launching a block for every box instead spent
+// the launch on empty ones (most of a droplet's domain; up to 32^3 boxes for
+// an ensemble's largest members).
 constexpr int SITE_TPB = 192;
+constexpr int SITE_GRID = 2048;  // blocks at most (the headline droplet: 716 boxes hold sites)
 template <int PP>
-__global__ __launch_bounds__(SITE_TPB) void k_site_terms(Geo g, unsigned what, const int *__restrict__ bstart,
+__global__ __launch_bounds__(SITE_TPB) void k_site_terms(Geo g, unsigned what, const int4 *__restrict__ obox,
                              const double2 *__restrict__ L4,
                              const double *__restrict__ lfac, const sesmd_forcefield *__restrict__ ff,
                              const double *__restrict__ part, const double4 *__restrict__ spos,
@@ -2574,182 +2610,186 @@ __global__ __launch_bounds__(SITE_TPB) void k_site_terms(Geo g, unsigned what, c
     constexpr int p = PP, lp = (PP + 1) * (PP + 1);
     __shared__ double2 sL[lp];
     __shared__ double sF[lp];
-    const int b = blockIdx.x, nb = g.nb;
-    const int k0 = bstart[b], k1 = bstart[b + 1];
-    if (k0 >= k1) return;  // empty box (uniform over the block)
-    const bool far = (what & SESMD_FAR) != 0;
-    if (far) {
-        for (int t = threadIdx.x; t < lp; t += SITE_TPB) {
-            sL[t] = L4[(size_t)b * lp + t];
-            sF[t] = lfac[t];
-        }
-        __syncthreads();
-    }
-    const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
-    const double cx = d_cen4(g, bx), cy = d_cen4(g, by), cz = d_cen4(g, bz);
-    for (int k = k0 + threadIdx.x; k < k1; k += SITE_TPB) {
-        // the LDS coefficients are read inside each site's sum, not hoisted out of
-        // this loop into hundreds of live registers per lane
-        asm volatile("" ::: "memory");
-        int4 me = smeta[k];
-        int i = me.x, rk = ((me.y >> 11) & 7) - 1;
-        double4 pk = spos[k];
-        double x = pk.x, y = pk.y, z = pk.z;
-        double nx = 0.0, ny = 0.0, nz = 0.0;
-        if (what & SESMD_NEAR) {
-            // the slots of rows outside the grid hold stale values, and only the
-            // rows (dz, dy) of the receiver's box inside the grid are added
-            double pr[NEAR_ROWS][3];
-#pragma unroll
-            for (int q = 0; q < NEAR_ROWS; q++)
-#pragma unroll
-                for (int d = 0; d < 3; d++) pr[q][d] = part[(size_t)(3 * q + d) * g.n + k];
-            if (rk >= 0) {
-#pragma unroll
-                for (int q = 0; q < NEAR_ROWS; q++) {
-                    int yy = by + q % 3 - 1, zz = bz + q / 3 - 1;
-                    if (yy < 0 || yy >= nb || zz < 0 || zz >= nb) continue;
-                    nx += pr[q][0];
-                    ny += pr[q][1];
-                    nz += pr[q][2];
-                }
+    const int nb = g.nb;
+    for (int q = blockIdx.x; q < g.nb3; q += gridDim.x) {
+        const int4 e = obox[q];
+        if (e.x < 0) break;  // past the boxes holding sites (uniform over the block)
+        const int b = e.x, k0 = e.y, k1 = e.z;
+        const bool far = (what & SESMD_FAR) != 0;
+        if (far) {
+            for (int t = threadIdx.x; t < lp; t += SITE_TPB) {
+                sL[t] = L4[(size_t)b * lp + t];
+                sF[t] = lfac[t];
             }
+            __syncthreads();
         }
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-        if (far && rk >= 0) {
-            double rxpb = x - cx, rypb = y - cy, rzpb = z - cz;
-            double rqq = pk.w * g.xpi;
-            double rho2 = rxpb * rxpb + rypb * rypb;
-            double rr = sqrt(rho2 + rzpb * rzpb);
-            double rp1 = rzpb / rr;
-            double sr = sqrt(rho2), rsr = 1.0 / sr;
-            double2 rp2 = make_double2(rxpb * rsr, rypb * rsr);
-            double rp3 = 1.0 / rho2;
-            // the inverse of e^{i phi}: its conjugate over |e|^2
-            double rd = 1.0 / (rp2.x * rp2.x + rp2.y * rp2.y);
-            double2 rp2i = make_double2(rp2.x * rd, -rp2.y * rd);
-            double rrt[MAXP + 1];  // rr^(j-1)
-            rrt[0] = 1.0 / rr;
+        const int bx = b % nb, by = (b / nb) % nb, bz = b / (nb * nb);
+        const double cx = d_cen4(g, bx), cy = d_cen4(g, by), cz = d_cen4(g, bz);
+        for (int k = k0 + threadIdx.x; k < k1; k += SITE_TPB) {
+            // the LDS coefficients are read inside each site's sum, not hoisted out of
+            // this loop into hundreds of live registers per lane
+            asm volatile("" ::: "memory");
+            int4 me = smeta[k];
+            int i = me.x, rk = ((me.y >> 11) & 7) - 1;
+            double4 pk = spos[k];
+            double x = pk.x, y = pk.y, z = pk.z;
+            double nx = 0.0, ny = 0.0, nz = 0.0;
+            if (what & SESMD_NEAR) {
+                // the slots of rows outside the grid hold stale values, and only the
+                // rows (dz, dy) of the receiver's box inside the grid are added
+                double pr[NEAR_ROWS][3];
 #pragma unroll
-            for (int j = 1; j <= p; j++) rrt[j] = rrt[j - 1] * rr;
-            // The sum runs over m = |k| outermost: column m of the associated
-            // Legendre functions comes from the recurrence of rLegendreP
-            // (src/rLegendreP.f90:3-48) as j rises, in the same operations as
-            // legendre_table, and e^{+-i m phi} by one more product per column,
-            // so no table of either is held.
-            //
-            // Each term of src/evalLjk4.f90 adds rqq rp3 Re(rq0 (rq3 + rq4)) to
-            // fx, with rq0 = sc L e^{ik phi}, sc = rfctr(j, |k|) rr^(j-1),
-            // rq3 = x z rq1, rq4 = (-j x rr rq2, k y rr rq2); expanded, that is
-            // x U - y V with U = Re(rq0) (z rq1 - j rr rq2), V = k rr rq2 Im(rq0);
-            // fy gets y U + x V and fz -rqq Re(rq0) rq1.  So three sums carry the
-            // whole expansion, and the terms k = +m and -m, which share rq1, rq2
-            // and sc, enter them together: U and the fz sum through
-            // Re(L+ e+) + Re(L- e-), V through m (Im(L+ e+) - Im(L- e-)).
-            double2 ep = make_double2(1.0, 0.0), em = ep;  // e^{i m phi}, e^{-i m phi}
-            double somx2 = sqrt((1.0 - rp1) * (1.0 + rp1));
-            double pmm = 1.0;
-            double su = 0.0, sv = 0.0, sw = 0.0;
+                for (int q = 0; q < NEAR_ROWS; q++)
 #pragma unroll
-            for (int m = 0; m <= p; m++) {
-                if (m > 0) {
-                    pmm = -pmm * (double)(2 * m - 1) * somx2;
-                    ep = cmul(ep, rp2);
-                    em = cmul(em, rp2i);
-                }
-                const double mrr = m * rr;
-                // P(m-1, m) = 0; at m = 0 the l = -1 entry, 1, which meets j + m = 0
-                double plm1 = m == 0 ? 1.0 : 0.0, pl = pmm;
+                    for (int d = 0; d < 3; d++) pr[q][d] = part[(size_t)(3 * q + d) * g.n + k];
+                if (rk >= 0) {
 #pragma unroll
-                for (int j = m; j <= p; j++) {
-                    double rq1 = (j + m) * plm1;  // (j + |k|) P(j-1, |k|)
-                    double rq2 = pl;              // P(j, |k|)
-                    double sc = sF[j * (j + 1) + m] * rrt[j];
-                    double t = rzpb * rq1 - (j * rr) * rq2;
-                    double a;
-                    if (m == 0) {
-                        a = sc * sL[j * (j + 1)].x;
-                    } else {
-                        double2 lp = sL[j * (j + 1) + m], lm = sL[j * (j + 1) - m];
-                        double re = lp.x * ep.x - lp.y * ep.y + lm.x * em.x - lm.y * em.y;
-                        double im = lp.x * ep.y + lp.y * ep.x - lm.x * em.y - lm.y * em.x;
-                        a = sc * re;
-                        sv = fma(sc * im, mrr * rq2, sv);
-                    }
-                    su = fma(a, t, su);
-                    sw = fma(a, rq1, sw);
-                    if (j < p) {  // P(j + 1, m)
-                        double pn = j == m ? rp1 * (2 * m + 1) * pmm
-                                           : (rp1 * (2 * (j + 1) - 1) * pl - (j + m) * plm1) *
-                                                 (1.0 / (j + 1 - m));
-                        plm1 = pl;
-                        pl = pn;
+                    for (int q = 0; q < NEAR_ROWS; q++) {
+                        int yy = by + q % 3 - 1, zz = bz + q / 3 - 1;
+                        if (yy < 0 || yy >= nb || zz < 0 || zz >= nb) continue;
+                        nx += pr[q][0];
+                        ny += pr[q][1];
+                        nz += pr[q][2];
                     }
                 }
             }
-            double ax = rqq * rp3 * (rxpb * su - rypb * sv);
-            double ay = rqq * rp3 * (rypb * su + rxpb * sv);
-            double az = -rqq * sw;
-            fx += ax;
-            fy += ay;
-            fz += az;
-        }
-        if (what & SESMD_WALL) {
-            double h = g.rhs;
-            auto p13 = [](double u) {
-                double u2 = u * u, u4 = u2 * u2, u8 = u4 * u4;
-                return u8 * u4 * u;
-            };
-            fx += 48.0 / p13(x - h) + 48.0 / p13(h + x);
-            fy += 48.0 / p13(y - h) + 48.0 / p13(h + y);
-            fz += 48.0 / p13(z - h) + 48.0 / p13(h + z);
-        }
-        // graphite (src/eval_graphite.f90:13-132): receivers filed in their own box
-        // of the bottom layer, against that column's carbons
-        if (ngr > 0 && (what & SESMD_GRAPHITE)) {
-            if (b < g.nb * g.nb && (rk == 0 || rk == 1 || rk == 3)) {
-                int t = me.y & 255;
-                double c = ff->gr_c[t], s6 = ff->gr_s6[t];
-                if (c != 0.0) {
-                    // the column's carbons eight at a time: their loads are issued
-                    // together (a column holds ~800; one at a time, each a pair of
-                    // dependent loads, the walk is latency-bound), the terms are
-                    // added in list order
-                    double gx = 0.0, gy = 0.0, gz = 0.0;
-                    constexpr int U = 8;
-                    int q0 = gstart[b], q1 = gstart[b + 1];
-                    for (int q = q0; q < q1; q += U) {
-                        int a[U];
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            if (far && rk >= 0) {
+                double rxpb = x - cx, rypb = y - cy, rzpb = z - cz;
+                double rqq = pk.w * g.xpi;
+                double rho2 = rxpb * rxpb + rypb * rypb;
+                double rr = sqrt(rho2 + rzpb * rzpb);
+                double rp1 = rzpb / rr;
+                double sr = sqrt(rho2), rsr = 1.0 / sr;
+                double2 rp2 = make_double2(rxpb * rsr, rypb * rsr);
+                double rp3 = 1.0 / rho2;
+                // the inverse of e^{i phi}: its conjugate over |e|^2
+                double rd = 1.0 / (rp2.x * rp2.x + rp2.y * rp2.y);
+                double2 rp2i = make_double2(rp2.x * rd, -rp2.y * rd);
+                double rrt[MAXP + 1];  // rr^(j-1)
+                rrt[0] = 1.0 / rr;
 #pragma unroll
-                        for (int u = 0; u < U; u++) a[u] = gidx[min(q + u, q1 - 1)];
-                        double cxyz[U][3];
+                for (int j = 1; j <= p; j++) rrt[j] = rrt[j - 1] * rr;
+                // The sum runs over m = |k| outermost: column m of the associated
+                // Legendre functions comes from the recurrence of rLegendreP
+                // (src/rLegendreP.f90:3-48) as j rises, in the same operations as
+                // legendre_table, and e^{+-i m phi} by one more product per column,
+                // so no table of either is held.
+                //
+                // Each term of src/evalLjk4.f90 adds rqq rp3 Re(rq0 (rq3 + rq4)) to
+                // fx, with rq0 = sc L e^{ik phi}, sc = rfctr(j, |k|) rr^(j-1),
+                // rq3 = x z rq1, rq4 = (-j x rr rq2, k y rr rq2); expanded, that is
+                // x U - y V with U = Re(rq0) (z rq1 - j rr rq2), V = k rr rq2 Im(rq0);
+                // fy gets y U + x V and fz -rqq Re(rq0) rq1.  So three sums carry the
+                // whole expansion, and the terms k = +m and -m, which share rq1, rq2
+                // and sc, enter them together: U and the fz sum through
+                // Re(L+ e+) + Re(L- e-), V through m (Im(L+ e+) - Im(L- e-)).
+                double2 ep = make_double2(1.0, 0.0), em = ep;  // e^{i m phi}, e^{-i m phi}
+                double somx2 = sqrt((1.0 - rp1) * (1.0 + rp1));
+                double pmm = 1.0;
+                double su = 0.0, sv = 0.0, sw = 0.0;
 #pragma unroll
-                        for (int u = 0; u < U; u++)
+                for (int m = 0; m <= p; m++) {
+                    if (m > 0) {
+                        pmm = -pmm * (double)(2 * m - 1) * somx2;
+                        ep = cmul(ep, rp2);
+                        em = cmul(em, rp2i);
+                    }
+                    const double mrr = m * rr;
+                    // P(m-1, m) = 0; at m = 0 the l = -1 entry, 1, which meets j + m = 0
+                    double plm1 = m == 0 ? 1.0 : 0.0, pl = pmm;
 #pragma unroll
-                            for (int d = 0; d < 3; d++) cxyz[u][d] = gr[3 * a[u] + d];
-#pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            double xij = x - cxyz[u][0], yij = y - cxyz[u][1], zij = z - cxyz[u][2];
-                            double rijsq = xij * xij + yij * yij + zij * zij;
-                            if (q + u < q1 && sqrt(rijsq) <= g.rc) {
-                                double rsi = 1.0 / rijsq, r6 = rsi * rsi * rsi;
-                                double flj = c * r6 * (s6 * r6 - 0.5) * rsi;
-                                gx += xij * flj;
-                                gy += yij * flj;
-                                gz += zij * flj;
-                            }
+                    for (int j = m; j <= p; j++) {
+                        double rq1 = (j + m) * plm1;  // (j + |k|) P(j-1, |k|)
+                        double rq2 = pl;              // P(j, |k|)
+                        double sc = sF[j * (j + 1) + m] * rrt[j];
+                        double t = rzpb * rq1 - (j * rr) * rq2;
+                        double a;
+                        if (m == 0) {
+                            a = sc * sL[j * (j + 1)].x;
+                        } else {
+                            double2 lp = sL[j * (j + 1) + m], lm = sL[j * (j + 1) - m];
+                            double re = lp.x * ep.x - lp.y * ep.y + lm.x * em.x - lm.y * em.y;
+                            double im = lp.x * ep.y + lp.y * ep.x - lm.x * em.y - lm.y * em.x;
+                            a = sc * re;
+                            sv = fma(sc * im, mrr * rq2, sv);
+                        }
+                        su = fma(a, t, su);
+                        sw = fma(a, rq1, sw);
+                        if (j < p) {  // P(j + 1, m)
+                            double pn = j == m ? rp1 * (2 * m + 1) * pmm
+                                               : (rp1 * (2 * (j + 1) - 1) * pl - (j + m) * plm1) *
+                                                     (1.0 / (j + 1 - m));
+                            plm1 = pl;
+                            pl = pn;
                         }
                     }
-                    fx += gx;
-                    fy += gy;
-                    fz += gz;
+                }
+                double ax = rqq * rp3 * (rxpb * su - rypb * sv);
+                double ay = rqq * rp3 * (rypb * su + rxpb * sv);
+                double az = -rqq * sw;
+                fx += ax;
+                fy += ay;
+                fz += az;
+            }
+            if (what & SESMD_WALL) {
+                double h = g.rhs;
+                auto p13 = [](double u) {
+                    double u2 = u * u, u4 = u2 * u2, u8 = u4 * u4;
+                    return u8 * u4 * u;
+                };
+                fx += 48.0 / p13(x - h) + 48.0 / p13(h + x);
+                fy += 48.0 / p13(y - h) + 48.0 / p13(h + y);
+                fz += 48.0 / p13(z - h) + 48.0 / p13(h + z);
+            }
+            // graphite (src/eval_graphite.f90:13-132): receivers filed in their own box
+            // of the bottom layer, against that column's carbons
+            if (ngr > 0 && (what & SESMD_GRAPHITE)) {
+                if (b < g.nb * g.nb && (rk == 0 || rk == 1 || rk == 3)) {
+                    int t = me.y & 255;
+                    double c = ff->gr_c[t], s6 = ff->gr_s6[t];
+                    if (c != 0.0) {
+                        // the column's carbons eight at a time: their loads are issued
+                        // together (a column holds ~800; one at a time, each a pair of
+                        // dependent loads, the walk is latency-bound), the terms are
+                        // added in list order
+                        double gx = 0.0, gy = 0.0, gz = 0.0;
+                        constexpr int U = 8;
+                        int q0 = gstart[b], q1 = gstart[b + 1];
+                        for (int q = q0; q < q1; q += U) {
+                            int a[U];
+#pragma unroll
+                            for (int u = 0; u < U; u++) a[u] = gidx[min(q + u, q1 - 1)];
+                            double cxyz[U][3];
+#pragma unroll
+                            for (int u = 0; u < U; u++)
+#pragma unroll
+                                for (int d = 0; d < 3; d++) cxyz[u][d] = gr[3 * a[u] + d];
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                double xij = x - cxyz[u][0], yij = y - cxyz[u][1], zij = z - cxyz[u][2];
+                                double rijsq = xij * xij + yij * yij + zij * zij;
+                                if (q + u < q1 && sqrt(rijsq) <= g.rc) {
+                                    double rsi = 1.0 / rijsq, r6 = rsi * rsi * rsi;
+                                    double flj = c * r6 * (s6 * r6 - 0.5) * rsi;
+                                    gx += xij * flj;
+                                    gy += yij * flj;
+                                    gz += zij * flj;
+                                }
+                            }
+                        }
+                        fx += gx;
+                        fy += gy;
+                        fz += gz;
+                    }
                 }
             }
-        }
 
-        f[i] = nx + fx;
-        f[g.n + i] = ny + fy;
-        f[2 * g.n + i] = nz + fz;
+            f[i] = nx + fx;
+            f[g.n + i] = ny + fy;
+            f[2 * g.n + i] = nz + fz;
+        }
+        __syncthreads();  // before the next box's expansion replaces this one
     }
 }
 
@@ -4189,6 +4229,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->widx_out, c->max_work));
     A(dalloc(&c->nwork, 1));
     A(dalloc(&c->nrwork, 1));
+    A(dalloc(&c->obox, g.nb3));
     A(dalloc(&c->spos, n + SRC_PAD));
     A(dalloc(&c->npart, 27 * n));
     A(dalloc(&c->iperm, n));
@@ -4336,7 +4377,7 @@ extern "C" int sesmd_destroy(sesmd_ctx *c)
     if (c->graph_cycle_len) hipGraphExecDestroy(c->graph_cycle);
     void *ptrs[] = {c->r, c->v, c->a, c->f, c->type, c->mol, c->first, c->cls, c->anc, c->cbox,
                     c->rkind, c->isub, c->key_in, c->key_out, c->val_in, c->perm, c->cub_tmp,
-                    c->bstart, c->bs4, c->kcnt, c->nvalid, c->work, c->nwork, c->rwork, c->nrwork, c->work_raw, c->npart, c->iperm, c->wkey_in,
+                    c->bstart, c->bs4, c->kcnt, c->nvalid, c->work, c->nwork, c->rwork, c->nrwork, c->obox, c->work_raw, c->npart, c->iperm, c->wkey_in,
                     c->wkey_out, c->widx_in, c->widx_out, c->spos, c->smeta, c->O4, c->L4,
                     c->O3, c->L3, c->O2, c->L2, c->ne4, c->ne3, c->ne2, c->T4, c->T3, c->T2, c->top_part,
                     c->M43, c->M32, c->S23, c->S34, c->lfac, c->gstart, c->gidx, c->gr, c->dff,
@@ -4485,7 +4526,8 @@ static int launch_rebuild(sesmd_ctx *c, bool meta = true)
     k_site_box<<<nblk(g.n, 256), 256, 0, st>>>(g, c->r, c->mol, c->type, c->cls, c->anc, c->cbox,
                                                c->rkind, c->key_in, c->kcnt);
     if (g.ns) k_isub<<<g.ns, 64, 0, st>>>(g, c->r, c->anc, c->isub);
-    k_key_scan_rows<<<1, 1024, 0, st>>>(g, c->kcnt, c->bs4, c->bstart, c->rwork, c->nrwork, c->max_rwork);
+    k_key_scan_rows<<<1, 1024, 0, st>>>(g, c->kcnt, c->bs4, c->bstart, c->rwork, c->nrwork, c->max_rwork,
+                                        c->obox);
     k_key_scatter<<<nblk(g.n, 256), 256, 0, st>>>(g, c->key_in, c->bs4, c->kcnt + K, c->val_in);
     k_key_order<<<K, 64, 0, st>>>(g, c->bs4, c->val_in, c->perm, c->key_out, c->iperm);
     c->boxlist = false;  // built on the first masked evaluation after this rebuild
@@ -4733,8 +4775,8 @@ static int forces_sites(sesmd_ctx *c, unsigned what, bool fold)
     const Geo &g = c->g;
     hipStream_t st = c->stream;
     const unsigned intra = SESMD_INTERNAL | SESMD_SUBTRACT | SESMD_VALENCE | SESMD_DIHEDRAL | SESMD_DIHEDRAL_BE;
-    (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<g.nb3, SITE_TPB, 0, st>>>(
-        g, what, c->bstart, c->L4, c->lfac, c->dff, c->npart, c->spos, c->smeta, c->f, c->gr,
+    (g.p == 4 ? k_site_terms<4> : k_site_terms<8>)<<<std::min(g.nb3, SITE_GRID), SITE_TPB, 0, st>>>(
+        g, what, c->obox, c->L4, c->lfac, c->dff, c->npart, c->spos, c->smeta, c->f, c->gr,
         c->gstart, c->gidx, fold ? c->ngr : 0);
     if (g.ns && (what & intra))
         k_intra<<<nblk(29 * g.ns, 64), 64, 0, st>>>(g, what, c->r, c->isub, c->dff, c->dtopo, c->f,
