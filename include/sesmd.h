@@ -48,8 +48,9 @@ typedef struct sesmd_forcefield {
 } sesmd_forcefield;
 
 /* precision of the pair forces: SESMD_MIXED evaluates the near-field pairs in
- * FP32 and accumulates each pair force component in 64-bit fixed point
- * (resolution 2^-32 force units, range +-2^19 per pair; see sesmd_get_flags),
+ * FP32 and accumulates each pair force component in fixed point (resolution
+ * 2^-32 force units, held in a biased FP64 register: range +-2^19 per sum;
+ * see sesmd_get_flags),
  * and runs the M2L contractions as split-BF16x3 products with FP32
  * accumulation; expansions, integration, constraints and the thermostat stay
  * FP64 */
@@ -153,9 +154,9 @@ int sesmd_get_counters(sesmd_ctx *ctx, int *shake_fail);
  * some velocity); flags[2] molecule filings whose filing site lay outside the
  * (4 nb2)^3-box cube (filed in the nearest edge box instead); flags[3]
  * near-field work items of a SESMD_MIXED context where a pair force component
- * could have reached 2^19 force units, the range of its 64-bit fixed-point
- * accumulation (2^-32 resolution).  n <= 4 entries are written (more are
- * zero). */
+ * or a receiver's sum reached 2^18 force units, half the range of its
+ * fixed-point accumulation (2^-32 resolution, +-2^19).  n <= 4 entries are
+ * written (more are zero). */
 int sesmd_get_flags(sesmd_ctx *ctx, int *flags, int n);
 /* device pointers of the resident SoA arrays (x,y,z blocks of N doubles) */
 int sesmd_device_arrays(sesmd_ctx *ctx, void **r, void **v, void **a, void **f);

@@ -980,39 +980,36 @@ __device__ __forceinline__ void near_row(const Geo &g, const double4 *__restrict
 
 // Mixed precision (SESMD_MIXED): the same pair set and order, pair arithmetic
 // in FP32 on coordinates relative to the source box's centre, and every pair
-// force component accumulated in 64-bit fixed point with a resolution of 2^-32
-// force units.  The conversion is one FP64 fused multiply-add per component:
-// fma(d, f, 1.5 2^20) rounds the exact product d f (two FP32 factors) once, to
-// the 2^-32 spacing of the doubles in [2^20, 2^21), and while |d f| < 2^19 the
-// bit pattern of the result is bits(1.5 2^20) + round(d f 2^32).  Adding the
-// bit patterns as 64-bit integers therefore adds the fixed-point values; the n
-// copies of bits(1.5 2^20) come off once at the end (n is the same for every
-// lane).  Integer sums do not depend on their order.  Headroom: each lane
-// keeps the largest |f| (pair force over distance) it has seen; an item where
-// that times the longest distance its rows allow reaches 2^19 counts in
-// counters[3].
-constexpr double FIX_MAGIC = 1572864.0;                        // 1.5 2^20
-constexpr unsigned long long FIX_MAGIC_BITS = 0x4138000000000000ull;  // its bit pattern
-constexpr float FIX_LIMIT = 524288.0f;                         // 2^19
+// force component accumulated in fixed point with a resolution of 2^-32 force
+// units, held in an FP64 register.  The accumulator starts at B = 1.5 2^20:
+// the doubles of [2^20, 2^21) are exactly the multiples of 2^-32 there, so
+// each pair's acc = fma(d, f, acc) rounds acc + d f (d f being the exact
+// product of two FP32 factors) once, to that grid, and while every partial
+// sum stays within 2^19 of B, acc - B is the sum of the terms rounded to
+// multiples of 2^-32 -- a 52-bit fixed-point accumulator at the price of one
+// conversion and one FMA per component (a 64-bit integer accumulator costs
+// two more integer adds).  The bias comes off once at the end.  The order of
+// the terms is fixed (each receiver's sources in sorted order), so results
+// are deterministic; a tie (a product exactly half-way between two grid
+// points) rounds to the even neighbour of the sum.  Headroom: each lane keeps
+// the largest |f| (pair force over distance) it has seen; an item where that
+// times the longest distance its rows allow, or a receiver's final sum,
+// reaches 2^18 counts in counters[3] (a factor of two below the range, for
+// partial sums that exceed the final one).
+constexpr double FIX_BIAS = 1572864.0;  // 1.5 2^20
+constexpr float FIX_LIMIT = 262144.0f;  // 2^18
 
 struct Recv32 {
     float q, qO, qH, cO, s6O;
 };
 
-struct Fix3 {
-    unsigned long long x, y, z;
+struct Acc3 {
+    double x, y, z;  // biased fixed-point sums (see FIX_BIAS)
 };
 
-// the fixed-point sum of n converted products, as a double (exact: |sum| < 2^21)
-__device__ __forceinline__ double fix_value(unsigned long long acc, int n)
+__device__ __forceinline__ void fix_add(double &acc, float d, float f)
 {
-    long long v = (long long)(acc - (unsigned long long)n * FIX_MAGIC_BITS);
-    return (double)v * 0x1p-32;
-}
-
-__device__ __forceinline__ unsigned long long fix_term(float d, float f)
-{
-    return (unsigned long long)__double_as_longlong(fma((double)d, (double)f, FIX_MAGIC));
+    acc = fma((double)d, (double)f, acc);
 }
 
 template <int CAT, bool CHECK, bool CEN, bool HQ, int NR>
@@ -1020,7 +1017,7 @@ __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, in
                                             const float (&xr)[NR], const float (&yr)[NR],
                                             const float (&zr)[NR], const Recv (&R)[NR],
                                             const Recv32 (&Q)[NR], const float *__restrict__ lj32,
-                                            float rc2, Fix3 (&acc)[NR], float &mf, int &nadd)
+                                            float rc2, Acc3 (&acc)[NR], float &mf)
 {
     bool okc[NR];
     if (CHECK) {
@@ -1082,12 +1079,11 @@ __device__ __forceinline__ void near_pair32(const float4 &pj, const int4 &mj, in
     }
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-        acc[r].x += fix_term(dx[r], ft[r]);
-        acc[r].y += fix_term(dy[r], ft[r]);
-        acc[r].z += fix_term(dz[r], ft[r]);
+        fix_add(acc[r].x, dx[r], ft[r]);
+        fix_add(acc[r].y, dy[r], ft[r]);
+        fix_add(acc[r].z, dz[r], ft[r]);
         mf = fmaxf(mf, fabsf(ft[r]));
     }
-    nadd++;
 }
 
 template <int CAT, bool CHECK, bool CEN, bool HQ, int NR>
@@ -1097,7 +1093,7 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
                                                const float (&yr)[NR], const float (&zr)[NR],
                                                const Recv (&R)[NR], const Recv32 (&Q)[NR],
                                                const float *__restrict__ lj32, float rc2,
-                                               Fix3 (&acc)[NR], float &mf, int &nadd)
+                                               Acc3 (&acc)[NR], float &mf)
 {
     if (s0 >= s1) return;
     // No branch between the loop body's two sources, so their pair arithmetic
@@ -1117,10 +1113,10 @@ __device__ __forceinline__ void near_segment32(const float4 *__restrict__ spos32
         pA = ps[0];
         pB = ps[1];
         __asm__ volatile("" ::: "memory");  // keeps these loads ahead of the pair arithmetic
-        near_pair32<CAT, CHECK, CEN, HQ, NR>(pa, ma, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
-        near_pair32<CAT, CHECK, CEN, HQ, NR>(pb, mb, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
+        near_pair32<CAT, CHECK, CEN, HQ, NR>(pa, ma, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf);
+        near_pair32<CAT, CHECK, CEN, HQ, NR>(pb, mb, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf);
     }
-    if (t == 1) near_pair32<CAT, CHECK, CEN, HQ, NR>(pA, mA, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
+    if (t == 1) near_pair32<CAT, CHECK, CEN, HQ, NR>(pA, mA, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf);
 }
 
 template <bool CHECK, bool HQ, int NR>
@@ -1128,8 +1124,7 @@ __device__ __forceinline__ void near_row32(const Geo &g, const float4 *__restric
                                            const int4 *__restrict__ smeta, const int *__restrict__ bs4,
                                            int r0, int r1, int bc, int bx,
                                            const Recv (&R)[NR], const Recv32 (&Q)[NR],
-                                           const float *__restrict__ lj32, Fix3 (&acc)[NR], float &mf,
-                                           int &nadd)
+                                           const float *__restrict__ lj32, Acc3 (&acc)[NR], float &mf)
 {
     int nb = g.nb;
     float rc2 = (float)g.rc2;
@@ -1149,13 +1144,13 @@ __device__ __forceinline__ void near_row32(const Geo &g, const float4 *__restric
         int k2 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 2]);
         int k3 = __builtin_amdgcn_readfirstlane(bs4[4 * bb + 3]);
         if (bb == bc) {
-            near_segment32<0, CHECK, true, HQ, NR>(spos32, smeta, k0, k1, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
-            near_segment32<1, CHECK, true, HQ, NR>(spos32, smeta, k1, k2, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
-            near_segment32<2, CHECK, true, HQ, NR>(spos32, smeta, k2, k3, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
+            near_segment32<0, CHECK, true, HQ, NR>(spos32, smeta, k0, k1, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf);
+            near_segment32<1, CHECK, true, HQ, NR>(spos32, smeta, k1, k2, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf);
+            near_segment32<2, CHECK, true, HQ, NR>(spos32, smeta, k2, k3, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf);
         } else {
-            near_segment32<0, CHECK, false, HQ, NR>(spos32, smeta, k0, k1, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
-            near_segment32<1, CHECK, false, HQ, NR>(spos32, smeta, k1, k2, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
-            near_segment32<2, CHECK, false, HQ, NR>(spos32, smeta, k2, k3, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf, nadd);
+            near_segment32<0, CHECK, false, HQ, NR>(spos32, smeta, k0, k1, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf);
+            near_segment32<1, CHECK, false, HQ, NR>(spos32, smeta, k1, k2, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf);
+            near_segment32<2, CHECK, false, HQ, NR>(spos32, smeta, k2, k3, bx, xr, yr, zr, R, Q, lj32, rc2, acc, mf);
         }
     }
 }
@@ -1205,7 +1200,7 @@ __device__ __forceinline__ void near_sources(const Geo &g, const double4 *__rest
 {
     if (MIX) {
         Recv32 Q[NR];
-        Fix3 acc[NR];
+        Acc3 acc[NR];
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             Q[r].q = (float)R[r].q;
@@ -1213,25 +1208,26 @@ __device__ __forceinline__ void near_sources(const Geo &g, const double4 *__rest
             Q[r].qH = (float)R[r].qH;
             Q[r].cO = lj32[ti[r] * 32];
             Q[r].s6O = lj32[1024 + ti[r] * 32];
-            acc[r].x = acc[r].y = acc[r].z = 0ull;
+            acc[r].x = acc[r].y = acc[r].z = FIX_BIAS;
         }
         float mf = 0.0f;
-        int nadd = 0;
         if (check)
-            near_row32<true, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, acc, mf, nadd);
+            near_row32<true, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, acc, mf);
         else
-            near_row32<false, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, acc, mf, nadd);
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            ax[r] += fix_value(acc[r].x, nadd);
-            ay[r] += fix_value(acc[r].y, nadd);
-            az[r] += fix_value(acc[r].z, nadd);
-        }
+            near_row32<false, HQ, NR>(g, spos32, smeta, bs4, r0, r1, bc, bx, R, Q, lj32, acc, mf);
         // a receiver in the 3 x 3 boxes around the row's centre box and a
         // source in the row's three boxes are at most sqrt(17) box sides apart
         float dmax = 4.1231056f * (float)g.rs;
-        if (__builtin_amdgcn_ballot_w64(mf * dmax >= FIX_LIMIT) && (threadIdx.x & 63) == 0)
-            atomicAdd(&counters[3], 1);
+        bool out = mf * dmax >= FIX_LIMIT;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            double sx = acc[r].x - FIX_BIAS, sy = acc[r].y - FIX_BIAS, sz = acc[r].z - FIX_BIAS;
+            out = out || fmax(fabs(sx), fmax(fabs(sy), fabs(sz))) >= (double)FIX_LIMIT;
+            ax[r] += sx;
+            ay[r] += sy;
+            az[r] += sz;
+        }
+        if (__builtin_amdgcn_ballot_w64(out) && (threadIdx.x & 63) == 0) atomicAdd(&counters[3], 1);
     } else if (MASKED)
         near_row<true, true, HQ, NR>(g, spos, smeta, bs4, r0, r1, bc, bx, R, ff->lj_c, ff->lj_s6, mask, ax, ay,
                                      az);

@@ -1,5 +1,5 @@
-"""Mixed precision (SESMD_MIXED: FP32 pair arithmetic with 64-bit fixed-point
-accumulation, M2L as split-BF16x3 MFMA products accumulated in FP32, and
+"""Mixed precision (SESMD_MIXED: FP32 pair arithmetic with fixed-point
+accumulation at 2^-32, M2L as split-BF16x3 MFMA products accumulated in FP32, and
 everything else FP64) against the FP64 engine, which is itself checked
 against the oracle to 1e-10.
 
