@@ -250,7 +250,10 @@ int sesmd_energies(sesmd_ctx *ctx, double *e);
  * and device, equal hydrogen charges) advanced together -- the reference's
  * step loop (src/mybox.f90:130-241) for every member in one captured graph,
  * with the near field of all members in one persistent launch over their
- * flattened row-item lists.  Each member's state after sesmd_ensemble_steps
+ * flattened row-item lists, and -- when every member has the reference
+ * surfactant topology, one expansion order and the same constraint
+ * constants -- the members' integrator and site-term launches of a step
+ * batched into one launch each.  Each member's state after sesmd_ensemble_steps
  * is bit-for-bit what sesmd_steps on that context alone gives.  Members stay
  * owned by the caller and must outlive the ensemble; while an ensemble steps
  * them, their own calls must not run.  No counterpart in the reference (it
@@ -270,6 +273,9 @@ int sesmd_ensemble_tune(sesmd_ensemble *ens, int reps, int *shape3, float *best_
 int sesmd_ensemble_near_only(sesmd_ensemble *ens);
 /* the ensemble's stream (a hipStream_t) */
 void *sesmd_ensemble_stream(sesmd_ensemble *ens);
+/* 1: the members' integrator and site-term launches are batched (one launch
+ * each per step); 0: they run per member on the branch streams; -1: null */
+int sesmd_ensemble_batched(sesmd_ensemble *ens);
 /* nsteps steps of every member; kb is the shared step counter (sesmd_steps) */
 int sesmd_ensemble_steps(sesmd_ensemble *ens, int nsteps, int ifar, int *kb);
 

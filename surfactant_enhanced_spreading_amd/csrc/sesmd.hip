@@ -2594,7 +2594,7 @@ __device__ __forceinline__ void legendre_table(double rp1, double (&P)[MAXP + 2]
 constexpr int SITE_TPB = 192;
 constexpr int SITE_GRID = 2048;  // blocks at most (the headline droplet: 716 boxes hold sites)
 template <int PP>
-__global__ __launch_bounds__(SITE_TPB) void k_site_terms(Geo g, unsigned what, const int4 *__restrict__ obox,
+__device__ __forceinline__ void site_terms_boxes(int blk0, int nblk, const Geo &g, unsigned what, const int4 *__restrict__ obox,
                              const double2 *__restrict__ L4,
                              const double *__restrict__ lfac, const sesmd_forcefield *__restrict__ ff,
                              const double *__restrict__ part, const double4 *__restrict__ spos,
@@ -2607,7 +2607,7 @@ __global__ __launch_bounds__(SITE_TPB) void k_site_terms(Geo g, unsigned what, c
     __shared__ double2 sL[lp];
     __shared__ double sF[lp];
     const int nb = g.nb;
-    for (int q = blockIdx.x; q < g.nb3; q += gridDim.x) {
+    for (int q = blk0; q < g.nb3; q += nblk) {
         const int4 e = obox[q];
         if (e.x < 0) break;  // past the boxes holding sites (uniform over the block)
         const int b = e.x, k0 = e.y, k1 = e.z;
@@ -2787,6 +2787,23 @@ __global__ __launch_bounds__(SITE_TPB) void k_site_terms(Geo g, unsigned what, c
         }
         __syncthreads();  // before the next box's expansion replaces this one
     }
+}
+
+// the boxes blockIdx.x, + gridDim.x, ... of the list
+template <int PP>
+__global__ __launch_bounds__(SITE_TPB) void k_site_terms(Geo g, unsigned what, const int4 *__restrict__ obox,
+                                                         const double2 *__restrict__ L4,
+                                                         const double *__restrict__ lfac,
+                                                         const sesmd_forcefield *__restrict__ ff,
+                                                         const double *__restrict__ part,
+                                                         const double4 *__restrict__ spos,
+                                                         const int4 *__restrict__ smeta, double *__restrict__ f,
+                                                         const double *__restrict__ gr,
+                                                         const int *__restrict__ gstart,
+                                                         const int *__restrict__ gidx, int ngr)
+{
+    site_terms_boxes<PP>(blockIdx.x, gridDim.x, g, what, obox, L4, lfac, ff, part, spos, smeta, f, gr, gstart,
+                         gidx, ngr);
 }
 
 // ------------------------------------------------------------ intramolecular
@@ -3185,7 +3202,7 @@ __device__ __forceinline__ void put_sorted(const Geo &g, const SortedOut &so, in
     }
 }
 
-__device__ __forceinline__ void water_a(int blk, Geo g, double *__restrict__ r,
+__device__ __forceinline__ void water_a(int blk, const Geo &g, double *__restrict__ r,
                                                       double *__restrict__ v,
                                                       const double *__restrict__ a,
                                                       double *__restrict__ f,
@@ -3416,7 +3433,7 @@ __device__ __forceinline__ bool pipelined_sweeps(bool bond, int off, int smax, U
 // every correction to p and v alike, so v = p - r_old at the end (equal up to
 // the rounding of |r| ~ 20 sigma, ~1e-12 of v dt).
 __device__ __forceinline__ void surf_a(
-    int blk, Geo g, double *__restrict__ r, double *__restrict__ v, const double *__restrict__ a,
+    int blk, const Geo &g, double *__restrict__ r, double *__restrict__ v, const double *__restrict__ a,
     double *__restrict__ f, SurfConst sc, const double *__restrict__ nh,
     double *__restrict__ partials, int *__restrict__ counters, SortedOut so = SortedOut())
 {
@@ -3511,7 +3528,7 @@ struct IntraIn {
 };
 
 __device__ __forceinline__ void surf_b(
-    int blk, Geo g, const double *__restrict__ r, double *__restrict__ v, double *__restrict__ a,
+    int blk, const Geo &g, const double *__restrict__ r, double *__restrict__ v, double *__restrict__ a,
     const double *__restrict__ f, IntraIn in, SurfConst sc,
     const double *__restrict__ nh, int *__restrict__ counters)
 {
@@ -3638,19 +3655,19 @@ __global__ __launch_bounds__(256) void k_nh_update(Geo g, const double *__restri
 // into its table in LDS, then adds each site's contributions in table order
 // (internal pairs, angles, dihedrals) into fs[d][29 ns]; part B adds the
 // eval_subtract_fma terms with the lists of the step's own rebuild (surf_b).
-__global__ __launch_bounds__(256) void k_nh_intra(Geo g, const double *__restrict__ partials, int nblocks,
-                                                  const sesmd_forcefield *__restrict__ ff,
-                                                  double *__restrict__ nh, const double *__restrict__ r,
-                                                  const Topo *__restrict__ topo, double *__restrict__ fs,
-                                                  int *__restrict__ counters)
+__device__ __forceinline__ void nh_intra_block(int blk, const Geo &g, const double *__restrict__ partials,
+                                               int nblocks, const sesmd_forcefield *__restrict__ ff,
+                                               double *__restrict__ nh, const double *__restrict__ r,
+                                               const Topo *__restrict__ topo, double *__restrict__ fs,
+                                               int *__restrict__ counters)
 {
-    if (blockIdx.x == 0) {
+    if (blk == 0) {
         nh_update_block(g, partials, nblocks, ff, nh, 1, counters);
         return;
     }
     __shared__ double tab[29 * INTRA_MAXC * 3];
     __shared__ double pm[3 * 29];
-    int s = blockIdx.x - 1, nterm = ff->n_internal + 66;
+    int s = blk - 1, nterm = ff->n_internal + 66;
     if (threadIdx.x < 3 * 29) {
         int d = threadIdx.x / 29, j = threadIdx.x - 29 * d;
         pm[threadIdx.x] = r[(size_t)d * g.n + 3 * g.nw + 29 * s + j];
@@ -3670,9 +3687,18 @@ __global__ __launch_bounds__(256) void k_nh_intra(Geo g, const double *__restric
     }
 }
 
+__global__ __launch_bounds__(256) void k_nh_intra(Geo g, const double *__restrict__ partials, int nblocks,
+                                                  const sesmd_forcefield *__restrict__ ff,
+                                                  double *__restrict__ nh, const double *__restrict__ r,
+                                                  const Topo *__restrict__ topo, double *__restrict__ fs,
+                                                  int *__restrict__ counters)
+{
+    nh_intra_block(blockIdx.x, g, partials, nblocks, ff, nh, r, topo, fs, counters);
+}
+
 // Part B for waters (src/move_w1_b.f90:23-123): a = F dt^2/2m, velocity with
 // the new eta, RATTLE.
-__device__ __forceinline__ void water_b(int blk, Geo g, const double *__restrict__ r,
+__device__ __forceinline__ void water_b(int blk, const Geo &g, const double *__restrict__ r,
                                                       double *__restrict__ v, double *__restrict__ a,
                                                       const double *__restrict__ f,
                                                       const sesmd_forcefield *__restrict__ ff,
@@ -5140,6 +5166,13 @@ struct sesmd_ensemble {
     int *prefix;                   // first flattened item of each member; [n] = total
     bool mix;
     int waves, chunk, grid;        // launch shape: wavefronts per workgroup, receivers per wavefront, workgroups
+    // batched integrator and site-term launches (see EnsStepArgs)
+    bool batched;
+    int p;                         // the members' expansion order
+    SurfConst sc;                  // the members' shared constraint constants
+    struct EnsStepArgs *da;        // device copy of the members' arguments
+    int *pre;                      // block prefixes per launch kind: ENS_KINDS x (n + 1)
+    int tot[3];                    // blocks per launch kind
     hipGraphExec_t g_cycle, g_step;
     int cycle_len;
     bool step_ok;
@@ -5228,6 +5261,106 @@ struct EnsStreams {
     }
 };
 
+// ------------------------------------------------ batched ensemble launches
+// A step's integrator and site-term launches of all members as one launch
+// each.  Block B of such a launch belongs to member m, the last whose first
+// block pre[m] (a prefix over the members' block counts, fixed when the
+// ensemble is created) is at or below B, and runs block B - pre[m] of that
+// member's own launch with the member's arguments -- the same code on the
+// same data, so every member's results are those it has stepped alone.
+// Used when every member runs the reference topology's fused integrator, at
+// one expansion order, with one set of constraint constants.
+struct EnsStepArgs {
+    Geo g;
+    double *r, *v, *a, *f;
+    const int *cls;
+    const sesmd_forcefield *ff;
+    double *nh, *partials;
+    int *counters;
+    int nblk_s, nblk_w;
+    SortedOut so;  // the sorted copies part A writes on a regular step
+    const Topo *topo;
+    double *fs;
+    const unsigned *isub;
+    const int4 *obox;
+    const double2 *L4;
+    const double *lfac, *npart;
+    const double4 *spos;
+    const int4 *smeta;
+    const double *gr;
+    const int *gstart, *gidx;
+    int ngr;
+};
+
+enum { ENS_AB = 0, ENS_NH = 1, ENS_ST = 2, ENS_KINDS = 3 };  // parts A and B, eta + intra, site terms
+
+__device__ __forceinline__ int ens_owner(const int *__restrict__ pre, int n, int B)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= B) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// part A (k_move_a_fused); rebuild: no sorted copies (the rebuild re-sorts)
+__global__ __launch_bounds__(256) void k_ens_move_a(const EnsStepArgs *__restrict__ da, const int *__restrict__ pre,
+                                                    int n, SurfConst sc, int rebuild)
+{
+    int m = ens_owner(pre, n, blockIdx.x), blk = blockIdx.x - pre[m];
+    const EnsStepArgs &A = da[m];
+    SortedOut so = A.so;
+    if (rebuild) {
+        so.spos = nullptr;
+        so.spos32 = nullptr;
+    }
+    if (blk < A.nblk_s)
+        surf_a(blk, A.g, A.r, A.v, A.a, A.f, sc, A.nh, A.partials, A.counters, so);
+    else
+        water_a(blk - A.nblk_s, A.g, A.r, A.v, A.a, A.f, A.cls, A.ff, A.nh, A.partials, A.counters, 7u, so);
+}
+
+// the eta update and the intramolecular terms (k_nh_intra; a member without
+// surfactants has block 0 only, k_nh_update's work)
+__global__ __launch_bounds__(256) void k_ens_nh_intra(const EnsStepArgs *__restrict__ da,
+                                                      const int *__restrict__ pre, int n)
+{
+    int m = ens_owner(pre, n, blockIdx.x), blk = blockIdx.x - pre[m];
+    const EnsStepArgs &A = da[m];
+    nh_intra_block(blk, A.g, A.partials, A.nblk_w + A.nblk_s, A.ff, A.nh, A.r, A.topo, A.fs, A.counters);
+}
+
+// the site terms (k_site_terms): member m's pre[m + 1] - pre[m] blocks take
+// its occupied boxes in turn
+template <int PP>
+__global__ __launch_bounds__(SITE_TPB) void k_ens_site_terms(const EnsStepArgs *__restrict__ da,
+                                                             const int *__restrict__ pre, int n, unsigned what)
+{
+    int m = ens_owner(pre, n, blockIdx.x);
+    const EnsStepArgs &A = da[m];
+    site_terms_boxes<PP>(blockIdx.x - pre[m], pre[m + 1] - pre[m], A.g, what, A.obox, A.L4, A.lfac, A.ff,
+                         A.npart, A.spos, A.smeta, A.f, A.gr, A.gstart, A.gidx, A.ngr);
+}
+
+// part B (k_move_b_fused)
+__global__ __launch_bounds__(256) void k_ens_move_b(const EnsStepArgs *__restrict__ da, const int *__restrict__ pre,
+                                                    int n, SurfConst sc)
+{
+    int m = ens_owner(pre, n, blockIdx.x), blk = blockIdx.x - pre[m];
+    const EnsStepArgs &A = da[m];
+    if (blk < A.nblk_s) {
+        IntraIn in;
+        in.fs = A.fs;
+        in.isub = A.isub;
+        in.ff = A.ff;
+        surf_b(blk, A.g, A.r, A.v, A.a, A.f, in, sc, A.nh, A.counters);
+    } else {
+        water_b(blk - A.nblk_s, A.g, A.r, A.v, A.a, A.f, A.ff, A.nh, A.counters, A.cls, 7u);
+    }
+}
+
 // the branch streams start after the work queued on the ensemble's stream ...
 static int ens_fork(sesmd_ensemble *e)
 {
@@ -5255,12 +5388,46 @@ static void ens_near(sesmd_ensemble *e)
         k_near_rows_ens<false><<<e->grid, 64 * e->waves, 0, e->stream>>>(e->dm, e->prefix, e->n, e->chunk);
 }
 
-// One step: the members' parts A spread over the branch streams, joined;
-// the near field of all members on the ensemble's stream; the members' site
-// terms and parts B on the branches, joined again.  Captured, the branches
-// are independent paths of the graph.
+// One step.  Batched (ens_step_batched): parts A, the eta updates with the
+// intramolecular terms, the near field, the site terms and parts B of all
+// members as one launch each on the ensemble's stream, and on a rebuild step
+// each member's sort, lists and far field on the branch streams between the
+// eta updates and the near field.  Otherwise the members' parts A spread over
+// the branch streams, joined; the near field of all members on the ensemble's
+// stream; the members' site terms and parts B on the branches, joined again.
+// Captured, the branches are independent paths of the graph.
+static int ens_step_batched(sesmd_ensemble *e, bool rebuild)
+{
+    int n = e->n;
+    hipStream_t st = e->stream;
+    const int *pre = e->pre;
+    k_ens_move_a<<<e->tot[ENS_AB], 256, 0, st>>>(e->da, pre + ENS_AB * (n + 1), n, e->sc, rebuild ? 1 : 0);
+    k_ens_nh_intra<<<e->tot[ENS_NH], 256, 0, st>>>(e->da, pre + ENS_NH * (n + 1), n);
+    if (rebuild) {  // each member's sort, lists and far field on the branches
+        EnsStreams on(e);
+        CK(ens_fork(e));
+        for (int i = 0; i < n; i++) {
+            sesmd_ctx *c = e->m[i];
+            CK(launch_rebuild(c, true));
+            CK(launch_fmm(c, true, false));
+            forces_gather(c, step_what(c), ~0ull, false);
+        }
+        CK(ens_join(e));
+    }
+    ens_near(e);
+    unsigned what = step_what(e->m[0]);
+    if (e->p == 4)
+        k_ens_site_terms<4><<<e->tot[ENS_ST], SITE_TPB, 0, st>>>(e->da, pre + ENS_ST * (n + 1), n, what);
+    else
+        k_ens_site_terms<8><<<e->tot[ENS_ST], SITE_TPB, 0, st>>>(e->da, pre + ENS_ST * (n + 1), n, what);
+    k_ens_move_b<<<e->tot[ENS_AB], 256, 0, st>>>(e->da, pre + ENS_AB * (n + 1), n, e->sc);
+    HC(hipGetLastError());
+    return 0;
+}
+
 static int ens_step(sesmd_ensemble *e, bool rebuild)
 {
+    if (e->batched) return ens_step_batched(e, rebuild);
     EnsStreams on(e);
     CK(ens_fork(e));
     for (int i = 0; i < e->n; i++) {
@@ -5351,6 +5518,63 @@ extern "C" int sesmd_ensemble_create(sesmd_ensemble **out, sesmd_ctx *const *mem
         M.part = c->npart;
         M.counters = c->counters;
     }
+    // batched launches: every member on the fused integrator path, at one
+    // expansion order, with the same constraint constants
+    e->p = members[0]->g.p;
+    e->sc = members[0]->sconst;
+    e->batched = true;
+    for (int i = 0; i < n; i++) {
+        const sesmd_ctx *c = members[i];
+        if (!c->surf_fast || c->g.p != e->p || memcmp(&c->sconst, &e->sc, sizeof(SurfConst)) != 0)
+            e->batched = false;
+    }
+    std::vector<EnsStepArgs> ha(n);
+    std::vector<int> hpre((size_t)ENS_KINDS * (n + 1));
+    for (int k = 0; k < ENS_KINDS; k++) e->tot[k] = 0;
+    for (int i = 0; i < n; i++) {
+        sesmd_ctx *c = members[i];
+        EnsStepArgs &A = ha[i];
+        A.g = c->g;
+        A.r = c->r;
+        A.v = c->v;
+        A.a = c->a;
+        A.f = c->f;
+        A.cls = c->cls;
+        A.ff = c->dff;
+        A.nh = c->nh;
+        A.partials = c->partials;
+        A.counters = c->counters;
+        A.nblk_s = c->nblk_s;
+        A.nblk_w = c->nblk_w;
+        A.so.iperm = c->iperm;
+        A.so.key = c->key_out;
+        A.so.q = reinterpret_cast<const double *>(reinterpret_cast<const char *>(c->dff) +
+                                                  offsetof(sesmd_forcefield, q));
+        A.so.spos = c->spos;
+        A.so.spos32 = c->spos32;
+        A.topo = c->dtopo;
+        A.fs = c->fs;
+        A.isub = c->isub;
+        A.obox = c->obox;
+        A.L4 = c->L4;
+        A.lfac = c->lfac;
+        A.npart = c->npart;
+        A.spos = c->spos;
+        A.smeta = c->smeta;
+        A.gr = c->gr;
+        A.gstart = c->gstart;
+        A.gidx = c->gidx;
+        A.ngr = c->ngr;
+        // site terms: about one block per 64 sites (a droplet's boxes hold
+        // ~100 sites each), at most the single launch's grid
+        int nst = std::max(1, std::min(std::min(c->g.nb3, SITE_GRID), (c->g.n + 63) / 64));
+        int cnt[ENS_KINDS] = {c->nblk_s + c->nblk_w, 1 + c->g.ns, nst};
+        for (int k = 0; k < ENS_KINDS; k++) {
+            hpre[(size_t)k * (n + 1) + i] = e->tot[k];
+            e->tot[k] += cnt[k];
+        }
+    }
+    for (int k = 0; k < ENS_KINDS; k++) hpre[(size_t)k * (n + 1) + n] = e->tot[k];
     int rc = 0;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) rc = fail("stream create failed");
     for (int b = 0; b < ENS_BRANCHES && !rc; b++)
@@ -5363,11 +5587,18 @@ extern "C" int sesmd_ensemble_create(sesmd_ensemble **out, sesmd_ctx *const *mem
     if (!rc && dalloc(&e->prefix, (size_t)n + 1)) rc = 1;
     if (!rc && hipMemcpy(e->dm, hm.data(), sizeof(EnsMember) * n, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail("member table upload failed");
+    if (!rc && dalloc(&e->da, (size_t)n)) rc = 1;
+    if (!rc && dalloc(&e->pre, hpre.size())) rc = 1;
+    if (!rc && (hipMemcpy(e->da, ha.data(), sizeof(EnsStepArgs) * n, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(e->pre, hpre.data(), sizeof(int) * hpre.size(), hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail("member argument upload failed");
     for (int i = 0; i < n && !rc; i++)  // the members' queued work before the ensemble's
         if (hipStreamSynchronize(e->own[i]) != hipSuccess) rc = fail("member %d stream failed", i);
     if (rc) {
         if (e->dm) hipFree(e->dm);
         if (e->prefix) hipFree(e->prefix);
+        if (e->da) hipFree(e->da);
+        if (e->pre) hipFree(e->pre);
         if (e->stream) hipStreamDestroy(e->stream);
         for (int b = 0; b < ENS_BRANCHES; b++) {
             if (e->branch[b]) hipStreamDestroy(e->branch[b]);
@@ -5388,6 +5619,8 @@ extern "C" int sesmd_ensemble_destroy(sesmd_ensemble *e)
     ens_drop_graphs(e);
     hipFree(e->dm);
     hipFree(e->prefix);
+    hipFree(e->da);
+    hipFree(e->pre);
     hipStreamDestroy(e->stream);
     for (int b = 0; b < ENS_BRANCHES; b++) {
         hipStreamDestroy(e->branch[b]);
@@ -5411,6 +5644,12 @@ extern "C" int sesmd_ensemble_set_shape(sesmd_ensemble *e, int waves, int chunk,
     e->chunk = chunk;
     e->grid = grid;
     return 0;
+}
+
+extern "C" int sesmd_ensemble_batched(sesmd_ensemble *e)
+{
+    if (!e) return -1;
+    return e->batched ? 1 : 0;
 }
 
 extern "C" int sesmd_ensemble_get_shape(sesmd_ensemble *e, int *shape3)

@@ -130,6 +130,7 @@ def lib():
         L.sesmd_ensemble_near_only.argtypes = [vp]
         L.sesmd_ensemble_stream.argtypes = [vp]
         L.sesmd_ensemble_stream.restype = vp
+        L.sesmd_ensemble_batched.argtypes = [vp]
         L.sesmd_ensemble_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip]
         _lib = L
     return _lib
@@ -413,6 +414,11 @@ class Ensemble:
         sh = (ctypes.c_int * 3)()
         _ck(self.L.sesmd_ensemble_get_shape(self.h, sh))
         return tuple(sh)
+
+    def batched(self):
+        """True when the members' integrator and site-term launches run as
+        one launch each per step."""
+        return self.L.sesmd_ensemble_batched(self.h) == 1
 
     def near_only(self):
         _ck(self.L.sesmd_ensemble_near_only(self.h))

@@ -2,7 +2,9 @@
 
 A step of the ensemble runs every member's part A, one persistent near-field
 launch over all members' row items and every member's site terms and part B,
-captured as one graph.  Bar: bit-for-bit.  Each member's positions,
+captured as one graph; with every member at one expansion order the parts
+A, B and the site terms of all members are one launch each (batched),
+otherwise they run per member.  Bar: bit-for-bit.  Each member's positions,
 velocities, accelerations and thermostat variable after fourteen steps (a
 one-step graph, a whole ten-step cycle graph with its rebuild, an eager
 rebuild step, two more one-step graphs) equal those of an identical context
@@ -24,21 +26,25 @@ def _same(a, b):
             and ea == eb)
 
 
-@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
-def test_ensemble_bitwise_vs_single(precision):
+@pytest.mark.parametrize('precision,orders', [('fp64', (8,)), ('mixed', (8,)), ('fp64', (8, 4))])
+def test_ensemble_bitwise_vs_single(precision, orders):
+    """orders (8,): every member at p = 8, so the integrator and site-term
+    launches are batched; (8, 4): members alternate between p = 8 and 4, so
+    those launches run per member on the branch streams."""
     from surfactant_enhanced_spreading_amd import builder, engine
     prec = {'fp64': engine.FP64, 'mixed': engine.MIXED}[precision]
     systems = builder.ensemble_systems(n=5, n_min=200, n_max=2500, seed=11)
-    print('ensemble', [(s.n_water, s.n_surf, nb2) for s, nb2 in systems], flush=True)
+    print('ensemble', [(s.n_water, s.n_surf, nb2) for s, nb2 in systems], 'orders', orders, flush=True)
     solo, team = [], []
     ens = None
     try:
-        for s, nb2 in systems:
+        for k, (s, nb2) in enumerate(systems):
             for lst in (solo, team):
-                e = engine.Engine(s, nb2=nb2, p=8, precision=prec)
+                e = engine.Engine(s, nb2=nb2, p=orders[k % len(orders)], precision=prec)
                 e.bootstrap()
                 lst.append(e)
         ens = engine.Ensemble(team)
+        assert ens.batched() == (len(orders) == 1)
         shape, ms = ens.tune(reps=1)
         print(precision, 'tuned shape', shape, 'ms', ms, flush=True)
         assert ens.shape() == shape
