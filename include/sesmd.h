@@ -253,7 +253,10 @@ int sesmd_energies(sesmd_ctx *ctx, double *e);
  * flattened row-item lists, and -- when every member has the reference
  * surfactant topology, one expansion order and the same constraint
  * constants -- the members' integrator and site-term launches of a step
- * batched into one launch each.  Each member's state after sesmd_ensemble_steps
+ * batched into one launch each (a batched rebuild evaluates multipoles and
+ * L4 local expansions only for the boxes holding sites, the only ones a step
+ * reads: sesmd_get_expansions on a member then gives unspecified values for
+ * the other boxes).  Each member's state after sesmd_ensemble_steps
  * is bit-for-bit what sesmd_steps on that context alone gives.  Members stay
  * owned by the caller and must outlive the ensemble; while an ensemble steps
  * them, their own calls must not run.  No counterpart in the reference (it

@@ -399,10 +399,18 @@ __global__ void k_fill_d(double *p, size_t n, double v)
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+__device__ __forceinline__ void fill_i_blk(int blk,
+                                             int *p,
+                                             size_t n,
+                                             int v)
+{
+    size_t i = (size_t)blk * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void k_fill_i(int *p, size_t n, int v)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    fill_i_blk(blockIdx.x, p, n, v);
 }
 
 // ------------------------------------------------------------ rebuild
@@ -413,10 +421,15 @@ __global__ void k_fill_i(int *p, size_t n, int v)
 // in the nearest edge box, as the clamp below does, and counted in
 // counters[2]: the reference's walls keep every site inside, so this flags a
 // blown-up system.
-__global__ void k_classify(Geo g, const double *__restrict__ r, const int *__restrict__ first,
-                           int *__restrict__ cls, int *__restrict__ anc, int *__restrict__ counters)
+__device__ __forceinline__ void classify_blk(int blk,
+                                               const Geo &g,
+                                               const double *__restrict__ r,
+                                               const int *__restrict__ first,
+                                               int *__restrict__ cls,
+                                               int *__restrict__ anc,
+                                               int *__restrict__ counters)
 {
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    int m = blk * blockDim.x + threadIdx.x;
     if (m >= g.nmol) return;
     const double *rx = r, *ry = r + g.n, *rz = r + 2 * g.n;
     int nb2 = g.nb2;
@@ -464,6 +477,12 @@ __global__ void k_classify(Geo g, const double *__restrict__ r, const int *__res
     }
 }
 
+__global__ void k_classify(Geo g, const double *__restrict__ r, const int *__restrict__ first,
+                           int *__restrict__ cls, int *__restrict__ anc, int *__restrict__ counters)
+{
+    classify_blk(blockIdx.x, g, r, first, cls, anc, counters);
+}
+
 // Reserves one slot per active lane in counter[key] with one atomic per
 // distinct key in the wavefront (keys of neighbouring lanes mostly agree, and
 // contended global atomics serialise): the lanes are grouped by key first
@@ -494,12 +513,19 @@ __device__ __forceinline__ int wave_reserve(int *counter, int key)
 // Site boxes: W1 sites live in their molecule's box; W2 and surfactant sites in
 // the box whose half-open cube [c - rs/2, c + rs/2) holds them (isCentral,
 // src/ibox.f90:101-124, as the list builders use it).  Receiver kinds follow.
-__global__ void k_site_box(Geo g, const double *__restrict__ r, const int *__restrict__ mol,
-                           const int *__restrict__ type, const int *__restrict__ cls,
-                           const int *__restrict__ anc, int *__restrict__ cbox,
-                           int *__restrict__ rkind, int *__restrict__ key, int *__restrict__ kcount)
+__device__ __forceinline__ void site_box_blk(int blk,
+                                               const Geo &g,
+                                               const double *__restrict__ r,
+                                               const int *__restrict__ mol,
+                                               const int *__restrict__ type,
+                                               const int *__restrict__ cls,
+                                               const int *__restrict__ anc,
+                                               int *__restrict__ cbox,
+                                               int *__restrict__ rkind,
+                                               int *__restrict__ key,
+                                               int *__restrict__ kcount)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
     int m = mol[i];
     int c = cls[m];
@@ -536,16 +562,27 @@ __global__ void k_site_box(Geo g, const double *__restrict__ r, const int *__res
     wave_reserve(kcount, k);  // the box sort's key counts (cleared before this launch)
 }
 
+__global__ void k_site_box(Geo g, const double *__restrict__ r, const int *__restrict__ mol,
+                           const int *__restrict__ type, const int *__restrict__ cls,
+                           const int *__restrict__ anc, int *__restrict__ cbox,
+                           int *__restrict__ rkind, int *__restrict__ key, int *__restrict__ kcount)
+{
+    site_box_blk(blockIdx.x, g, r, mol, type, cls, anc, cbox, rkind, key, kcount);
+}
+
 // buildinternal, src/buildinternal.f90:11-63: the same-molecule partners of a
 // surfactant site that lie outside the 3^3 block around the site's box (box by
 // ibox1000 of the molecule's rank, centre by rcen1000, test by isnearest).
 // One 64-lane block per molecule: lane j < 29 tests site j against the
 // molecule's other sites, read from LDS.
-__global__ __launch_bounds__(64) void k_isub(Geo g, const double *__restrict__ r, const int *__restrict__ anc,
-                                             unsigned *__restrict__ isub)
+__device__ __forceinline__ void isub_blk(int blk,
+                                           const Geo &g,
+                                           const double *__restrict__ r,
+                                           const int *__restrict__ anc,
+                                           unsigned *__restrict__ isub)
 {
     __shared__ double xs[3][29];
-    int s = blockIdx.x, j = threadIdx.x;
+    int s = blk, j = threadIdx.x;
     int m = g.nw + s;
     int base = 3 * g.nw + 29 * s;
     if (j < 29)
@@ -573,6 +610,12 @@ __global__ __launch_bounds__(64) void k_isub(Geo g, const double *__restrict__ r
         if (k != j && !near) mask |= 1u << k;
     }
     isub[base + j] = mask;
+}
+
+__global__ __launch_bounds__(64) void k_isub(Geo g, const double *__restrict__ r, const int *__restrict__ anc,
+                                             unsigned *__restrict__ isub)
+{
+    isub_blk(blockIdx.x, g, r, anc, isub);
 }
 
 
@@ -619,23 +662,37 @@ __device__ __forceinline__ void key_scan_block(const Geo &g, const int *__restri
 }
 
 // Positions within a key in arrival order (k_key_order orders them).
-__global__ void k_key_scatter(Geo g, const int *__restrict__ key, const int *__restrict__ bs4,
-                              int *__restrict__ fill, int *__restrict__ tmp)
+__device__ __forceinline__ void key_scatter_blk(int blk,
+                                                  const Geo &g,
+                                                  const int *__restrict__ key,
+                                                  const int *__restrict__ bs4,
+                                                  int *__restrict__ fill,
+                                                  int *__restrict__ tmp)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
     int k = key[i];
     tmp[bs4[k] + wave_reserve(fill, k)] = i;
 }
 
+__global__ void k_key_scatter(Geo g, const int *__restrict__ key, const int *__restrict__ bs4,
+                              int *__restrict__ fill, int *__restrict__ tmp)
+{
+    key_scatter_blk(blockIdx.x, g, key, bs4, fill, tmp);
+}
+
 // Each key's sites in index order, by a rank sort (distinct indices; a key
 // holds at most a few hundred sites), one wavefront per key 4 box + class; also
 // the inverse map site -> sorted position.
-__global__ __launch_bounds__(64) void k_key_order(Geo g, const int *__restrict__ bs4, const int *__restrict__ tmp,
-                                                  int *__restrict__ perm, int *__restrict__ key_out,
-                                                  int *__restrict__ iperm)
+__device__ __forceinline__ void key_order_blk(int blk,
+                                                const Geo &g,
+                                                const int *__restrict__ bs4,
+                                                const int *__restrict__ tmp,
+                                                int *__restrict__ perm,
+                                                int *__restrict__ key_out,
+                                                int *__restrict__ iperm)
 {
-    int kk = blockIdx.x, s0 = bs4[kk], len = bs4[kk + 1] - s0;
+    int kk = blk, s0 = bs4[kk], len = bs4[kk + 1] - s0;
     const int *seg = tmp + s0;
     for (int j = threadIdx.x; j < len; j += 64) {
         int v = seg[j], rank = 0, t = 0;
@@ -651,6 +708,13 @@ __global__ __launch_bounds__(64) void k_key_order(Geo g, const int *__restrict__
         key_out[s0 + rank] = kk;
         iperm[v] = s0 + rank;
     }
+}
+
+__global__ __launch_bounds__(64) void k_key_order(Geo g, const int *__restrict__ bs4, const int *__restrict__ tmp,
+                                                  int *__restrict__ perm, int *__restrict__ key_out,
+                                                  int *__restrict__ iperm)
+{
+    key_order_blk(blockIdx.x, g, bs4, tmp, perm, key_out, iperm);
 }
 
 // Near-field work items (one block): per box, chunks of NEAR_CHUNK receivers, in box order,
@@ -720,13 +784,20 @@ __global__ void k_work_gather(const int2 *__restrict__ raw, const int *__restric
 // the site is listed by the reference from every box of the 3^3 block around
 // its own box (its storage ring stays within the class's list rings: W1 <= 1,
 // W2 <= 2, S <= 3), which spares the near-field loop the per-box ring test.
-__global__ void k_gather_meta(Geo g, const int *__restrict__ perm, const int *__restrict__ type,
-                              const int *__restrict__ mol, const int *__restrict__ first,
-                              const int *__restrict__ cls, const int *__restrict__ anc,
-                              const int *__restrict__ cbox, const int *__restrict__ rkind,
-                              int4 *__restrict__ smeta, int *__restrict__ bflag)
+__device__ __forceinline__ void gather_meta_blk(int blk,
+                                                  const Geo &g,
+                                                  const int *__restrict__ perm,
+                                                  const int *__restrict__ type,
+                                                  const int *__restrict__ mol,
+                                                  const int *__restrict__ first,
+                                                  const int *__restrict__ cls,
+                                                  const int *__restrict__ anc,
+                                                  const int *__restrict__ cbox,
+                                                  const int *__restrict__ rkind,
+                                                  int4 *__restrict__ smeta,
+                                                  int *__restrict__ bflag)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blk * blockDim.x + threadIdx.x;
     if (k >= g.n) return;
     int i = perm[k];
     int m = mol[i];
@@ -745,29 +816,61 @@ __global__ void k_gather_meta(Geo g, const int *__restrict__ perm, const int *__
     if (!always) atomicOr(&bflag[cbox[i]], 1);
 }
 
-// sorted positions + charge, every step
-__global__ void k_gather_pos(Geo g, const int *__restrict__ perm, const double *__restrict__ r,
-                             const int *__restrict__ type, const sesmd_forcefield *__restrict__ ff,
-                             double4 *__restrict__ spos)
+__global__ void k_gather_meta(Geo g, const int *__restrict__ perm, const int *__restrict__ type,
+                              const int *__restrict__ mol, const int *__restrict__ first,
+                              const int *__restrict__ cls, const int *__restrict__ anc,
+                              const int *__restrict__ cbox, const int *__restrict__ rkind,
+                              int4 *__restrict__ smeta, int *__restrict__ bflag)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    gather_meta_blk(blockIdx.x, g, perm, type, mol, first, cls, anc, cbox, rkind, smeta, bflag);
+}
+
+// sorted positions + charge, every step
+__device__ __forceinline__ void gather_pos_blk(int blk,
+                                                 const Geo &g,
+                                                 const int *__restrict__ perm,
+                                                 const double *__restrict__ r,
+                                                 const int *__restrict__ type,
+                                                 const sesmd_forcefield *__restrict__ ff,
+                                                 double4 *__restrict__ spos)
+{
+    int k = blk * blockDim.x + threadIdx.x;
     if (k >= g.n) return;
     int i = perm[k];
     spos[k] = make_double4(r[i], r[g.n + i], r[2 * g.n + i], ff->q[type[i]]);
 }
 
+__global__ void k_gather_pos(Geo g, const int *__restrict__ perm, const double *__restrict__ r,
+                             const int *__restrict__ type, const sesmd_forcefield *__restrict__ ff,
+                             double4 *__restrict__ spos)
+{
+    gather_pos_blk(blockIdx.x, g, perm, r, type, ff, spos);
+}
+
 // Mixed precision: each sorted site's position relative to the centre of
 // its own L4 box, in FP32 (|x - c| < rs: an absolute rounding of ~1e-7 sigma),
 // and its charge.
-__global__ void k_gather_pos32(Geo g, const int *__restrict__ perm, const int *__restrict__ key,
-                               const double *__restrict__ r, const int *__restrict__ type,
-                               const sesmd_forcefield *__restrict__ ff, float4 *__restrict__ spos32)
+__device__ __forceinline__ void gather_pos32_blk(int blk,
+                                                   const Geo &g,
+                                                   const int *__restrict__ perm,
+                                                   const int *__restrict__ key,
+                                                   const double *__restrict__ r,
+                                                   const int *__restrict__ type,
+                                                   const sesmd_forcefield *__restrict__ ff,
+                                                   float4 *__restrict__ spos32)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blk * blockDim.x + threadIdx.x;
     if (k >= g.n) return;
     int i = perm[k], b = key[k] >> 2, nb = g.nb;
     spos32[k] = make_float4((float)(r[i] - d_cen4(g, b % nb)), (float)(r[g.n + i] - d_cen4(g, (b / nb) % nb)),
                             (float)(r[2 * g.n + i] - d_cen4(g, b / (nb * nb))), (float)ff->q[type[i]]);
+}
+
+__global__ void k_gather_pos32(Geo g, const int *__restrict__ perm, const int *__restrict__ key,
+                               const double *__restrict__ r, const int *__restrict__ type,
+                               const sesmd_forcefield *__restrict__ ff, float4 *__restrict__ spos32)
+{
+    gather_pos32_blk(blockIdx.x, g, perm, key, r, type, ff, spos32);
 }
 
 // ------------------------------------------------------------ near field
@@ -1440,16 +1543,28 @@ __device__ __forceinline__ void occupied_block(const Geo &g, const int *__restri
     for (int q = run + t; q < g.nb3; q += blockDim.x) obox[q] = make_int4(-1, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(1024) void k_key_scan_rows(Geo g, const int *__restrict__ cnt, int *__restrict__ bs4,
-                                                        int *__restrict__ bstart, int2 *__restrict__ work,
-                                                        int *__restrict__ nwork, int max_work,
-                                                        int4 *__restrict__ obox)
+__device__ __forceinline__ void key_scan_rows_blk(const Geo &g,
+                                                    const int *__restrict__ cnt,
+                                                    int *__restrict__ bs4,
+                                                    int *__restrict__ bstart,
+                                                    int2 *__restrict__ work,
+                                                    int *__restrict__ nwork,
+                                                    int max_work,
+                                                    int4 *__restrict__ obox)
 {
     key_scan_block(g, cnt, bs4, bstart);
     __threadfence_block();  // this block's bstart stores before its loads below
     __syncthreads();
     row_list_block(g, bstart, work, nwork, max_work);
     occupied_block(g, bstart, obox);
+}
+
+__global__ __launch_bounds__(1024) void k_key_scan_rows(Geo g, const int *__restrict__ cnt, int *__restrict__ bs4,
+                                                        int *__restrict__ bstart, int2 *__restrict__ work,
+                                                        int *__restrict__ nwork, int max_work,
+                                                        int4 *__restrict__ obox)
+{
+    key_scan_rows_blk(g, cnt, bs4, bstart, work, nwork, max_work, obox);
 }
 
 // One row-centred item: positions c0 + lane (+ 64 for NR = 2) of the row's
@@ -1683,19 +1798,22 @@ __global__ __launch_bounds__(256) void k_pair_count_total(const unsigned long lo
 // about the box centre over the box's receivers; one block per box, one
 // thread per site, deterministic tree reduction in LDS.
 template <int PP>
-__global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
-                                            const int *__restrict__ perm,
-                                            const int *__restrict__ bstart,
-                                            const int *__restrict__ rkind,
-                                            const int *__restrict__ type,
-                                            const sesmd_forcefield *__restrict__ ff,
-                                            double2 *__restrict__ O4, int *__restrict__ ne4)
+__device__ __forceinline__ void p2m_blk(int blk,
+                                          const Geo &g,
+                                          const double *__restrict__ r,
+                                          const int *__restrict__ perm,
+                                          const int *__restrict__ bstart,
+                                          const int *__restrict__ rkind,
+                                          const int *__restrict__ type,
+                                          const sesmd_forcefield *__restrict__ ff,
+                                          double2 *__restrict__ O4,
+                                          int *__restrict__ ne4)
 {
     constexpr int p = PP, lk = (PP + 1) * (PP + 2) / 2, lp = (PP + 1) * (PP + 1);
     // sized by the order: 15 KB at p = 4, so every box's block is resident at once
     __shared__ double s_re[64][lk], s_im[64][lk];
     __shared__ double s_fac[lk];  // sqrt((n - m)! / (n + m)!) per row n (n + 1) / 2 + m
-    int b = blockIdx.x;
+    int b = blk;
     int nb = g.nb;
     int s0 = bstart[b], s1 = bstart[b + 1];
     if (s0 == s1) {  // an empty box: zero expansion (block-uniform exit)
@@ -1802,6 +1920,18 @@ __global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
     if (threadIdx.x == 0) ne4[b] = any;
 }
 
+template <int PP>
+__global__ __launch_bounds__(64) void k_p2m(Geo g, const double *__restrict__ r,
+                                            const int *__restrict__ perm,
+                                            const int *__restrict__ bstart,
+                                            const int *__restrict__ rkind,
+                                            const int *__restrict__ type,
+                                            const sesmd_forcefield *__restrict__ ff,
+                                            double2 *__restrict__ O4, int *__restrict__ ne4)
+{
+    p2m_blk<PP>(blockIdx.x, g, r, perm, bstart, rkind, type, ff, O4, ne4);
+}
+
 // complex (re, im) helpers
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
@@ -1850,12 +1980,17 @@ __device__ __forceinline__ void store_rows(const Geo &g, double2 *out, int row, 
 // M2M (computeOnm3 / computeOnm2): parent = sum over 8 children of M[child] O_child
 // M2M: the eight children of a parent contracted side by side (64 lanes
 // each), then summed in child order -- deterministic.
-__global__ __launch_bounds__(512) void k_m2m(Geo g, int nd_parent, const double2 *__restrict__ Oc,
-                                             const int *__restrict__ nec, const double2 *__restrict__ M,
-                                             double2 *__restrict__ Op, int *__restrict__ nep)
+__device__ __forceinline__ void m2m_blk(int blk,
+                                          const Geo &g,
+                                          int nd_parent,
+                                          const double2 *__restrict__ Oc,
+                                          const int *__restrict__ nec,
+                                          const double2 *__restrict__ M,
+                                          double2 *__restrict__ Op,
+                                          int *__restrict__ nep)
 {
     __shared__ double2 s_part[8][64];
-    int P = blockIdx.x;
+    int P = blk;
     int kc = threadIdx.x >> 6, row = threadIdx.x & 63;
     int px = P % nd_parent, py = (P / nd_parent) % nd_parent, pz = P / (nd_parent * nd_parent);
     int ndc = 2 * nd_parent;
@@ -1881,6 +2016,13 @@ __global__ __launch_bounds__(512) void k_m2m(Geo g, int nd_parent, const double2
         }
         nep[P] = any;
     }
+}
+
+__global__ __launch_bounds__(512) void k_m2m(Geo g, int nd_parent, const double2 *__restrict__ Oc,
+                                             const int *__restrict__ nec, const double2 *__restrict__ M,
+                                             double2 *__restrict__ Op, int *__restrict__ nep)
+{
+    m2m_blk(blockIdx.x, g, nd_parent, Oc, nec, M, Op, nep);
 }
 
 // M2L over the interaction list (computeLjk4 / computeLjk3): sources are the
@@ -2061,10 +2203,13 @@ __device__ __forceinline__ int m2l_src(int nd, const int *__restrict__ ne, const
 
 // One block per list (class, candidate): its target count.  tne (or null):
 // translate only into targets flagged there.
-__global__ __launch_bounds__(256) void k_m2l_count(int nd, const int *__restrict__ ne, const int *__restrict__ tne,
-                                                   int *__restrict__ cnt)
+__device__ __forceinline__ void m2l_count_blk(int blk,
+                                                int nd,
+                                                const int *__restrict__ ne,
+                                                const int *__restrict__ tne,
+                                                int *__restrict__ cnt)
 {
-    int L = blockIdx.x, par = L / 216, ci = L % 216, h = nd / 2, h3 = h * h * h;
+    int L = blk, par = L / 216, ci = L % 216, h = nd / 2, h3 = h * h * h;
     int dx, dy, dz, tb;
     bool far = m2l_offset(par, ci, dx, dy, dz);
     __shared__ int s_c;
@@ -2078,12 +2223,20 @@ __global__ __launch_bounds__(256) void k_m2l_count(int nd, const int *__restrict
     if (threadIdx.x == 0) cnt[L] = s_c;
 }
 
+__global__ __launch_bounds__(256) void k_m2l_count(int nd, const int *__restrict__ ne, const int *__restrict__ tne,
+                                                   int *__restrict__ cnt)
+{
+    m2l_count_blk(blockIdx.x, nd, ne, tne, cnt);
+}
+
 // One block: the lists grouped by operator offset, classes in order within a
 // group.  base[list] = its first position; gbase[grp] = the group's first
 // position, bbase[grp] its first block (gbase / bbase[M2L_GROUPS] = entries /
 // blocks of the level).
-__global__ __launch_bounds__(512) void k_m2l_scan(const int *__restrict__ cnt, int *__restrict__ base,
-                                                  int *__restrict__ gbase, int *__restrict__ bbase)
+__device__ __forceinline__ void m2l_scan_blk(const int *__restrict__ cnt,
+                                               int *__restrict__ base,
+                                               int *__restrict__ gbase,
+                                               int *__restrict__ bbase)
 {
     __shared__ int s_a[512], s_b[512];
     int t = threadIdx.x, tot = 0;
@@ -2120,6 +2273,12 @@ __global__ __launch_bounds__(512) void k_m2l_scan(const int *__restrict__ cnt, i
     }
 }
 
+__global__ __launch_bounds__(512) void k_m2l_scan(const int *__restrict__ cnt, int *__restrict__ base,
+                                                  int *__restrict__ gbase, int *__restrict__ bbase)
+{
+    m2l_scan_blk(cnt, base, gbase, bbase);
+}
+
 // the group of GEMM block B (the last group whose first block is <= B), its
 // first entry and entry count; false past the level's last block
 __device__ __forceinline__ bool m2l_block(const int *__restrict__ gbase, const int *__restrict__ bbase, int B,
@@ -2142,11 +2301,15 @@ __device__ __forceinline__ bool m2l_block(const int *__restrict__ gbase, const i
 // One block per list: its entries (target q, source box) in q order, and for
 // every target of the class the position of its result for this candidate
 // (idx[target box][ci], -1 none).
-__global__ __launch_bounds__(256) void k_m2l_fill(int nd, const int *__restrict__ ne, const int *__restrict__ tne,
-                                                  const int *__restrict__ base, int2 *__restrict__ list,
-                                                  int *__restrict__ idx)
+__device__ __forceinline__ void m2l_fill_blk(int blk,
+                                               int nd,
+                                               const int *__restrict__ ne,
+                                               const int *__restrict__ tne,
+                                               const int *__restrict__ base,
+                                               int2 *__restrict__ list,
+                                               int *__restrict__ idx)
 {
-    int L = blockIdx.x, par = L / 216, ci = L % 216, h = nd / 2, h3 = h * h * h;
+    int L = blk, par = L / 216, ci = L % 216, h = nd / 2, h3 = h * h * h;
     int dx, dy, dz;
     bool far = m2l_offset(par, ci, dx, dy, dz);
     __shared__ int s_w[4];
@@ -2177,6 +2340,13 @@ __global__ __launch_bounds__(256) void k_m2l_fill(int nd, const int *__restrict_
     }
 }
 
+__global__ __launch_bounds__(256) void k_m2l_fill(int nd, const int *__restrict__ ne, const int *__restrict__ tne,
+                                                  const int *__restrict__ base, int2 *__restrict__ list,
+                                                  int *__restrict__ idx)
+{
+    m2l_fill_blk(blockIdx.x, nd, ne, tne, base, list, idx);
+}
+
 // One block per M2L_TPB x 16 consecutive entries of one list.  MT x WG
 // wavefronts: wave w computes the 16-row slab w % MT of the operator times the
 // expansions of tiles (w / MT) TPB / WG .. + TPB / WG - 1, so the block
@@ -2187,12 +2357,14 @@ __global__ __launch_bounds__(256) void k_m2l_fill(int nd, const int *__restrict_
 // double-buffered, loaded two chunks ahead: while chunk s's products run, chunk
 // s + 1's data goes into the other buffer and chunk s + 2's loads are in flight.
 template <int MT, int KS, int TPB, int WG>
-__global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__restrict__ gbase,
-                                                           const int *__restrict__ bbase,
-                                                           const int2 *__restrict__ list,
-                                                           const double *__restrict__ O,
-                                                           const double *__restrict__ AF,
-                                                           double *__restrict__ res)
+__device__ __forceinline__ void m2l_gemm_blk(int blk,
+                                               const Geo &g,
+                                               const int *__restrict__ gbase,
+                                               const int *__restrict__ bbase,
+                                               const int2 *__restrict__ list,
+                                               const double *__restrict__ O,
+                                               const double *__restrict__ AF,
+                                               double *__restrict__ res)
 {
     constexpr int NT = 64 * MT * WG;            // threads
     constexpr int TW = TPB / WG;                // tiles per wave
@@ -2204,7 +2376,7 @@ __global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__r
     __shared__ double s_b[2][BCH];
     __shared__ int s_src[TPB * 16];
     int off, first, ne_;  // off: the operator index in [-3, 3]^3
-    if (!m2l_block(gbase, bbase, blockIdx.x, off, first, ne_)) return;
+    if (!m2l_block(gbase, bbase, blk, off, first, ne_)) return;
     for (int j = threadIdx.x; j < TPB * 16; j += NT) s_src[j] = j < ne_ ? list[first + j].y : -1;
     __syncthreads();
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -2309,14 +2481,29 @@ __global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__r
     }
 }
 
+template <int MT, int KS, int TPB, int WG>
+__global__ __launch_bounds__(64 * MT * WG) void k_m2l_gemm(Geo g, const int *__restrict__ gbase,
+                                                           const int *__restrict__ bbase,
+                                                           const int2 *__restrict__ list,
+                                                           const double *__restrict__ O,
+                                                           const double *__restrict__ AF,
+                                                           double *__restrict__ res)
+{
+    m2l_gemm_blk<MT, KS, TPB, WG>(blockIdx.x, g, gbase, bbase, list, O, AF, res);
+}
+
 // Mixed precision: each box's half-space inputs (2 lk values, m2l_half_off)
 // as the B-operand parts of k_m2l_gemm_bf3: OB[box][ks][part][32], part 0..2 =
 // hi, mid, lo of column 32 ks + c (zero past 2 lk).
 template <int KS>
-__global__ __launch_bounds__(256) void k_m2l_split(int nbox, int lp2, int kh, const double *__restrict__ O,
-                                                   unsigned short *__restrict__ OB)
+__device__ __forceinline__ void m2l_split_blk(int blk,
+                                                int nbox,
+                                                int lp2,
+                                                int kh,
+                                                const double *__restrict__ O,
+                                                unsigned short *__restrict__ OB)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blk * blockDim.x + threadIdx.x;
     if (t >= nbox * 32 * KS) return;
     int box = t / (32 * KS), k = t % (32 * KS);
     double x = k < kh ? O[(size_t)box * lp2 + m2l_half_off(k)] : 0.0;
@@ -2326,6 +2513,13 @@ __global__ __launch_bounds__(256) void k_m2l_split(int nbox, int lp2, int kh, co
     dst[0] = h;
     dst[32] = m;
     dst[64] = l;
+}
+
+template <int KS>
+__global__ __launch_bounds__(256) void k_m2l_split(int nbox, int lp2, int kh, const double *__restrict__ O,
+                                                   unsigned short *__restrict__ OB)
+{
+    m2l_split_blk<KS>(blockIdx.x, nbox, lp2, kh, O, OB);
 }
 
 // Mixed-precision M2L GEMM: one block per M2L_TPB x 16 consecutive entries of
@@ -2340,17 +2534,18 @@ __global__ __launch_bounds__(256) void k_m2l_split(int nbox, int lp2, int kh, co
 // first) per (slab, tile).  Output lane l holds rows 4 (l / 16) .. + 3 of
 // entry l % 16.
 template <int MTW, int WM, int TWW, int WT, int KS>
-__global__ __launch_bounds__(64 * WM * WT) void k_m2l_gemm_bf3(const int *__restrict__ gbase,
-                                                               const int *__restrict__ bbase,
-                                                               const int2 *__restrict__ list,
-                                                               const m2l_u4 *__restrict__ OB,
-                                                               const m2l_u4 *__restrict__ AFB,
-                                                               float *__restrict__ res)
+__device__ __forceinline__ void m2l_gemm_bf3_blk(int blk,
+                                                   const int *__restrict__ gbase,
+                                                   const int *__restrict__ bbase,
+                                                   const int2 *__restrict__ list,
+                                                   const m2l_u4 *__restrict__ OB,
+                                                   const m2l_u4 *__restrict__ AFB,
+                                                   float *__restrict__ res)
 {
     constexpr int MT = MTW * WM, TPB = TWW * WT;
     static_assert(TPB == M2L_TPB, "block tiles");
     int grp, first, ne_;
-    if (!m2l_block(gbase, bbase, blockIdx.x, grp, first, ne_)) return;
+    if (!m2l_block(gbase, bbase, blk, grp, first, ne_)) return;
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w % WM, wt = w / WM;
     int n = lane & 15, kq = lane >> 4;
     int si[TWW];
@@ -2408,17 +2603,32 @@ __global__ __launch_bounds__(64 * WM * WT) void k_m2l_gemm_bf3(const int *__rest
     }
 }
 
+template <int MTW, int WM, int TWW, int WT, int KS>
+__global__ __launch_bounds__(64 * WM * WT) void k_m2l_gemm_bf3(const int *__restrict__ gbase,
+                                                               const int *__restrict__ bbase,
+                                                               const int2 *__restrict__ list,
+                                                               const m2l_u4 *__restrict__ OB,
+                                                               const m2l_u4 *__restrict__ AFB,
+                                                               float *__restrict__ res)
+{
+    m2l_gemm_bf3_blk<MTW, WM, TWW, WT, KS>(blockIdx.x, gbase, bbase, list, OB, AFB, res);
+}
+
 // Adds each target's results in candidate order onto L (the L2L part already
 // there): one wavefront per target box, lane r its k >= 0 row r.  The 216
 // result positions of the target are loaded at once (four per lane); the
 // candidates that have a result are taken eight at a time, their rows loaded
 // together and then added in candidate order.
 template <typename T>
-__global__ __launch_bounds__(64) void k_m2l_accum(Geo g, int nd, const int *__restrict__ tne,
-                                                  const int *__restrict__ idx, const T *__restrict__ res,
-                                                  double2 *__restrict__ L)
+__device__ __forceinline__ void m2l_accum_blk(int blk,
+                                                const Geo &g,
+                                                int nd,
+                                                const int *__restrict__ tne,
+                                                const int *__restrict__ idx,
+                                                const T *__restrict__ res,
+                                                double2 *__restrict__ L)
 {
-    int b = blockIdx.x, row = threadIdx.x, lk = g.lk;
+    int b = blk, row = threadIdx.x, lk = g.lk;
     if (tne && !tne[b]) return;  // no translation was listed into it
     const int *ix = idx + (size_t)b * 216;
     int pv[4];
@@ -2467,15 +2677,25 @@ __global__ __launch_bounds__(64) void k_m2l_accum(Geo g, int nd, const int *__re
     if (k > 0) Lt[j * (j + 1) - k] = make_double2(cur.x, -cur.y);
 }
 
+template <typename T>
+__global__ __launch_bounds__(64) void k_m2l_accum(Geo g, int nd, const int *__restrict__ tne,
+                                                  const int *__restrict__ idx, const T *__restrict__ res,
+                                                  double2 *__restrict__ L)
+{
+    m2l_accum_blk<T>(blockIdx.x, g, nd, tne, idx, res, L);
+}
+
 // top level (computeLjk2): every L2 box that is not adjacent to the target
 // M2L at the top level (computeLjk2): one block per (target, source) pair of
 // L2 boxes, then a per-target sum over sources in index order.
-__global__ __launch_bounds__(64) void k_m2l_top_pair(Geo g, const double2 *__restrict__ O2,
-                                                     const int *__restrict__ ne2,
-                                                     const double2 *__restrict__ T2,
-                                                     double2 *__restrict__ part)
+__device__ __forceinline__ void m2l_top_pair_blk(int blk,
+                                                   const Geo &g,
+                                                   const double2 *__restrict__ O2,
+                                                   const int *__restrict__ ne2,
+                                                   const double2 *__restrict__ T2,
+                                                   double2 *__restrict__ part)
 {
-    int Tb = blockIdx.x / g.n2, S = blockIdx.x % g.n2, row = threadIdx.x;
+    int Tb = blk / g.n2, S = blk % g.n2, row = threadIdx.x;
     int nd = g.nb2;
     if (row >= g.lk) return;
     int tx = Tb % nd, ty = (Tb / nd) % nd, tz = Tb / (nd * nd);
@@ -2491,10 +2711,20 @@ __global__ __launch_bounds__(64) void k_m2l_top_pair(Geo g, const double2 *__res
     part[((size_t)Tb * g.n2 + S) * g.lk + row] = acc;
 }
 
-__global__ __launch_bounds__(64) void k_m2l_top_sum(Geo g, const double2 *__restrict__ part,
-                                                    double2 *__restrict__ L2)
+__global__ __launch_bounds__(64) void k_m2l_top_pair(Geo g, const double2 *__restrict__ O2,
+                                                     const int *__restrict__ ne2,
+                                                     const double2 *__restrict__ T2,
+                                                     double2 *__restrict__ part)
 {
-    int Tb = blockIdx.x, row = threadIdx.x;
+    m2l_top_pair_blk(blockIdx.x, g, O2, ne2, T2, part);
+}
+
+__device__ __forceinline__ void m2l_top_sum_blk(int blk,
+                                                  const Geo &g,
+                                                  const double2 *__restrict__ part,
+                                                  double2 *__restrict__ L2)
+{
+    int Tb = blk, row = threadIdx.x;
     if (row >= g.lk) return;
     // the sources in index order, the loads of a chunk of 9 issued together
     double2 acc = make_double2(0.0, 0.0);
@@ -2513,11 +2743,21 @@ __global__ __launch_bounds__(64) void k_m2l_top_sum(Geo g, const double2 *__rest
     store_rows(g, L2 + (size_t)Tb * g.lp, row, acc);
 }
 
-// L2L (shiftLjk2 / shiftLjk3): child table = S[child position] parent
-__global__ __launch_bounds__(64) void k_l2l(Geo g, int nd_child, const double2 *__restrict__ Lp,
-                                            const double2 *__restrict__ S, double2 *__restrict__ Lc)
+__global__ __launch_bounds__(64) void k_m2l_top_sum(Geo g, const double2 *__restrict__ part,
+                                                    double2 *__restrict__ L2)
 {
-    int C = blockIdx.x, row = threadIdx.x;
+    m2l_top_sum_blk(blockIdx.x, g, part, L2);
+}
+
+// L2L (shiftLjk2 / shiftLjk3): child table = S[child position] parent
+__device__ __forceinline__ void l2l_blk(int blk,
+                                          const Geo &g,
+                                          int nd_child,
+                                          const double2 *__restrict__ Lp,
+                                          const double2 *__restrict__ S,
+                                          double2 *__restrict__ Lc)
+{
+    int C = blk, row = threadIdx.x;
     int cx = C % nd_child, cy = (C / nd_child) % nd_child, cz = C / (nd_child * nd_child);
     int ndp = nd_child / 2;
     int P = cx / 2 + ndp * (cy / 2 + ndp * (cz / 2));
@@ -2526,6 +2766,12 @@ __global__ __launch_bounds__(64) void k_l2l(Geo g, int nd_child, const double2 *
     double2 acc = make_double2(0.0, 0.0);
     contract_rows(g, S + (size_t)kc * g.lk * g.lp, Lp + (size_t)P * g.lp, acc, row);
     store_rows(g, Lc + (size_t)C * g.lp, row, acc);
+}
+
+__global__ __launch_bounds__(64) void k_l2l(Geo g, int nd_child, const double2 *__restrict__ Lp,
+                                            const double2 *__restrict__ S, double2 *__restrict__ Lc)
+{
+    l2l_blk(blockIdx.x, g, nd_child, Lp, S, Lc);
 }
 
 // ------------------------------------------------------------ per-site terms
@@ -5172,7 +5418,7 @@ struct sesmd_ensemble {
     SurfConst sc;                  // the members' shared constraint constants
     struct EnsStepArgs *da;        // device copy of the members' arguments
     int *pre;                      // block prefixes per launch kind: ENS_KINDS x (n + 1)
-    int tot[3];                    // blocks per launch kind
+    int tot[19];                   // blocks per launch kind (ENS_KINDS)
     hipGraphExec_t g_cycle, g_step;
     int cycle_len;
     bool step_ok;
@@ -5290,9 +5536,39 @@ struct EnsStepArgs {
     const double *gr;
     const int *gstart, *gidx;
     int ngr;
+    // the rebuild and far field
+    int classes_fixed, max_rwork;
+    const int *first, *mol, *type;
+    int *cls_w, *anc, *cbox, *rkind, *key_in, *key_out, *val_in, *perm, *iperm, *kcnt, *bs4, *bstart, *bflag,
+        *nrwork;
+    unsigned *isub_w;
+    int2 *rwork;
+    int4 *obox_w, *smeta_w;
+    double4 *spos_w;
+    float4 *spos32_w;
+    double2 *O4, *O3, *O2, *L2, *L3, *L4_w, *top_part;
+    int *ne4, *ne3, *ne2;
+    const double2 *M43, *M32, *T2, *S23, *S34;
+    const double *AF4, *AF3;
+    const m2l_u4 *AFB4, *AFB3;
+    struct Level {
+        unsigned short *ob;
+        double *res;
+        int *cnt, *base, *gbase, *bbase;
+        int2 *list;
+        int *idx;
+    } m2l[2];  // L4, L3
 };
 
-enum { ENS_AB = 0, ENS_NH = 1, ENS_ST = 2, ENS_KINDS = 3 };  // parts A and B, eta + intra, site terms
+// launch kinds, each with its block-count prefix over the members: parts A
+// and B, eta + intra, site terms; then the rebuild's and far field's grids
+enum {
+    ENS_AB, ENS_NH, ENS_ST,
+    ENS_CLS, ENS_FILL, ENS_SITE, ENS_ISUB, ENS_ONE, ENS_KEY, ENS_NB3, ENS_N33, ENS_N2, ENS_N2SQ, ENS_LISTS,
+    ENS_SPLIT4, ENS_SPLIT3, ENS_GEMM4, ENS_GEMM3, ENS_NEF,
+    ENS_KINDS
+};
+static_assert(ENS_KINDS == 19, "sesmd_ensemble::tot holds one count per launch kind");
 
 __device__ __forceinline__ int ens_owner(const int *__restrict__ pre, int n, int B)
 {
@@ -5361,6 +5637,225 @@ __global__ __launch_bounds__(256) void k_ens_move_b(const EnsStepArgs *__restric
     }
 }
 
+
+// ---- the rebuild and far field of every member, one launch per kernel
+// kind (the per-member sequence of launch_rebuild, launch_fmm and the
+// position gathers; each batched kernel runs the per-block body of the
+// member's own kernel)
+struct EnsBlock {
+    int m, blk;
+};
+__device__ __forceinline__ EnsBlock ens_block(const int *__restrict__ pre, int n)
+{
+    int m = ens_owner(pre, n, blockIdx.x);
+    return EnsBlock{m, (int)blockIdx.x - pre[m]};
+}
+#define ENS_KERNEL(NAME, TPB) \
+    __global__ __launch_bounds__(TPB) void NAME(const EnsStepArgs *__restrict__ da, const int *__restrict__ pre, int n)
+#define ENS_MEMBER                     \
+    const EnsBlock b = ens_block(pre, n); \
+    const EnsStepArgs &A = da[b.m];
+
+ENS_KERNEL(k_ens_classify, 256)
+{
+    ENS_MEMBER
+    classify_blk(b.blk, A.g, A.r, A.first, A.cls_w, A.anc, A.counters);
+}
+ENS_KERNEL(k_ens_fill_keys, 256)
+{
+    ENS_MEMBER
+    fill_i_blk(b.blk, A.kcnt, (size_t)9 * A.g.nb3, 0);  // counts, arrival counters (4 nb3 each), bflag (nb3)
+}
+ENS_KERNEL(k_ens_site_box, 256)
+{
+    ENS_MEMBER
+    site_box_blk(b.blk, A.g, A.r, A.mol, A.type, A.cls, A.anc, A.cbox, A.rkind, A.key_in, A.kcnt);
+}
+ENS_KERNEL(k_ens_isub, 64)
+{
+    ENS_MEMBER
+    isub_blk(b.blk, A.g, A.r, A.anc, A.isub_w);
+}
+ENS_KERNEL(k_ens_key_scan_rows, 1024)
+{
+    ENS_MEMBER
+    key_scan_rows_blk(A.g, A.kcnt, A.bs4, A.bstart, A.rwork, A.nrwork, A.max_rwork, A.obox_w);
+}
+ENS_KERNEL(k_ens_key_scatter, 256)
+{
+    ENS_MEMBER
+    key_scatter_blk(b.blk, A.g, A.key_in, A.bs4, A.kcnt + 4 * A.g.nb3, A.val_in);
+}
+// the keys blk, + stride, ... (most keys of a droplet's domain are empty)
+ENS_KERNEL(k_ens_key_order, 64)
+{
+    ENS_MEMBER
+    int stride = pre[b.m + 1] - pre[b.m];
+    for (int kk = b.blk; kk < 4 * A.g.nb3; kk += stride)
+        key_order_blk(kk, A.g, A.bs4, A.val_in, A.perm, A.key_out, A.iperm);
+}
+ENS_KERNEL(k_ens_gather_meta, 256)
+{
+    ENS_MEMBER
+    gather_meta_blk(b.blk, A.g, A.perm, A.type, A.mol, A.first, A.cls, A.anc, A.cbox, A.rkind, A.smeta_w,
+                    A.bflag);
+}
+ENS_KERNEL(k_ens_gather_pos, 256)
+{
+    ENS_MEMBER
+    gather_pos_blk(b.blk, A.g, A.perm, A.r, A.type, A.ff, A.spos_w);
+}
+ENS_KERNEL(k_ens_gather_pos32, 256)
+{
+    ENS_MEMBER
+    gather_pos32_blk(b.blk, A.g, A.perm, A.key_out, A.r, A.type, A.ff, A.spos32_w);
+}
+// P2M over the boxes holding sites (the occupied list of this rebuild, taken
+// blk, + stride, ...), after k_ens_ne4_clear has cleared every box's flag:
+// the multipoles of the other boxes are left as they were, and no reader
+// takes a multipole whose flag is clear
+ENS_KERNEL(k_ens_ne4_clear, 256)
+{
+    ENS_MEMBER
+    fill_i_blk(b.blk, A.ne4, (size_t)A.g.nb3, 0);
+}
+template <int PP>
+ENS_KERNEL(k_ens_p2m, 64)
+{
+    ENS_MEMBER
+    int stride = pre[b.m + 1] - pre[b.m];
+    for (int q = b.blk; q < A.g.nb3; q += stride) {
+        int box = A.obox[q].x;
+        if (box < 0) break;  // past the occupied boxes (block-uniform)
+        p2m_blk<PP>(box, A.g, A.r, A.perm, A.bstart, A.rkind, A.type, A.ff, A.O4, A.ne4);
+        __syncthreads();  // the block's LDS serves the next box
+    }
+}
+// lv 0: L4 -> L3; lv 1: L3 -> L2
+ENS_KERNEL(k_ens_m2m_4, 512)
+{
+    ENS_MEMBER
+    m2m_blk(b.blk, A.g, A.g.n3, A.O4, A.ne4, A.M43, A.O3, A.ne3);
+}
+ENS_KERNEL(k_ens_m2m_3, 512)
+{
+    ENS_MEMBER
+    m2m_blk(b.blk, A.g, A.g.nb2, A.O3, A.ne3, A.M32, A.O2, A.ne2);
+}
+// the top level for target L2 box blk: k_m2l_top_pair's products, each
+// added as k_m2l_top_sum adds them (sources in index order); the pairs with
+// an adjacent or empty source, whose products are +0.0, are skipped (adding
+// +0.0 to a sum that started at +0.0 changes nothing)
+ENS_KERNEL(k_ens_top, 64)
+{
+    ENS_MEMBER
+    const Geo &g = A.g;
+    int Tb = b.blk, row = threadIdx.x, nd = g.nb2;
+    if (row >= g.lk) return;
+    int tx = Tb % nd, ty = (Tb / nd) % nd, tz = Tb / (nd * nd);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int S = 0; S < g.n2; S++) {
+        int sx = S % nd, sy = (S / nd) % nd, sz = S / (nd * nd);
+        int dx = sx - tx, dy = sy - ty, dz = sz - tz;
+        bool adjacent = iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1;
+        if (adjacent || !A.ne2[S]) continue;
+        int w = 2 * nd - 1;
+        int off = (dx + nd - 1) + w * ((dy + nd - 1) + w * (dz + nd - 1));
+        double2 v = make_double2(0.0, 0.0);
+        contract_rows(g, A.T2 + (size_t)off * g.lk * g.lp, A.O2 + (size_t)S * g.lp, v, row);
+        acc.x += v.x;
+        acc.y += v.y;
+    }
+    store_rows(g, A.L2 + (size_t)Tb * g.lp, row, acc);
+}
+ENS_KERNEL(k_ens_l2l_3, 64)
+{
+    ENS_MEMBER
+    l2l_blk(b.blk, A.g, A.g.n3, A.L2, A.S23, A.L3);
+}
+// L2L into the boxes holding sites (the only readers of L4 here are their
+// sites' far-field terms), taken blk, + stride, ...
+ENS_KERNEL(k_ens_l2l_4, 64)
+{
+    ENS_MEMBER
+    int stride = pre[b.m + 1] - pre[b.m];
+    for (int q = b.blk; q < A.g.nb3; q += stride) {
+        int box = A.obox[q].x;
+        if (box < 0) break;
+        l2l_blk(box, A.g, A.g.nb, A.L3, A.S34, A.L4_w);
+    }
+}
+// the M2L lists of level lv (0: L4, nd = nb; 1: L3, nd = n3), occupied targets only
+__global__ __launch_bounds__(256) void k_ens_m2l_count(const EnsStepArgs *__restrict__ da, const int *__restrict__ pre,
+                                                       int n, int lv)
+{
+    ENS_MEMBER
+    const int *ne = lv == 0 ? A.ne4 : A.ne3;
+    m2l_count_blk(b.blk, lv == 0 ? A.g.nb : A.g.n3, ne, ne, A.m2l[lv].cnt);
+}
+__global__ __launch_bounds__(512) void k_ens_m2l_scan(const EnsStepArgs *__restrict__ da, const int *__restrict__ pre,
+                                                      int n, int lv)
+{
+    ENS_MEMBER
+    const EnsStepArgs::Level &B = A.m2l[lv];
+    m2l_scan_blk(B.cnt, B.base, B.gbase, B.bbase);
+}
+__global__ __launch_bounds__(256) void k_ens_m2l_fill(const EnsStepArgs *__restrict__ da, const int *__restrict__ pre,
+                                                      int n, int lv)
+{
+    ENS_MEMBER
+    const int *ne = lv == 0 ? A.ne4 : A.ne3;
+    const EnsStepArgs::Level &B = A.m2l[lv];
+    m2l_fill_blk(b.blk, lv == 0 ? A.g.nb : A.g.n3, ne, ne, B.base, B.list, B.idx);
+}
+template <int KS>
+__global__ __launch_bounds__(256) void k_ens_m2l_split(const EnsStepArgs *__restrict__ da,
+                                                       const int *__restrict__ pre, int n, int lv)
+{
+    ENS_MEMBER
+    int nd = lv == 0 ? A.g.nb : A.g.n3;
+    m2l_split_blk<KS>(b.blk, nd * nd * nd, 2 * A.g.lp, 2 * A.g.lk, (const double *)(lv == 0 ? A.O4 : A.O3),
+                      A.m2l[lv].ob);
+}
+// the GEMMs: member m's pre[m + 1] - pre[m] blocks take its list blocks in turn
+template <int MTW, int WM, int TWW, int WT, int KS>
+__global__ __launch_bounds__(64 * WM * WT) void k_ens_m2l_gemm_bf3(const EnsStepArgs *__restrict__ da,
+                                                                   const int *__restrict__ pre, int n, int lv)
+{
+    ENS_MEMBER
+    const EnsStepArgs::Level &B = A.m2l[lv];
+    int nblocks = B.bbase[M2L_GROUPS], stride = pre[b.m + 1] - pre[b.m];
+    for (int bk = b.blk; bk < nblocks; bk += stride)
+        m2l_gemm_bf3_blk<MTW, WM, TWW, WT, KS>(bk, B.gbase, B.bbase, B.list, (const m2l_u4 *)B.ob,
+                                               lv == 0 ? A.AFB4 : A.AFB3, (float *)B.res);
+}
+template <int MT, int KS, int TPB, int WG>
+__global__ __launch_bounds__(64 * MT * WG) void k_ens_m2l_gemm(const EnsStepArgs *__restrict__ da,
+                                                               const int *__restrict__ pre, int n, int lv)
+{
+    ENS_MEMBER
+    const EnsStepArgs::Level &B = A.m2l[lv];
+    int nblocks = B.bbase[M2L_GROUPS], stride = pre[b.m + 1] - pre[b.m];
+    for (int bk = b.blk; bk < nblocks; bk += stride) {
+        m2l_gemm_blk<MT, KS, TPB, WG>(bk, A.g, B.gbase, B.bbase, B.list, (const double *)(lv == 0 ? A.O4 : A.O3),
+                                      lv == 0 ? A.AF4 : A.AF3, B.res);
+        __syncthreads();  // the block's LDS is reused by its next list block
+    }
+}
+template <typename T>
+__global__ __launch_bounds__(64) void k_ens_m2l_accum(const EnsStepArgs *__restrict__ da, const int *__restrict__ pre,
+                                                      int n, int lv)
+{
+    ENS_MEMBER
+    const EnsStepArgs::Level &B = A.m2l[lv];
+    if (lv == 0)
+        m2l_accum_blk<T>(b.blk, A.g, A.g.nb, A.ne4, B.idx, (const T *)B.res, A.L4_w);
+    else
+        m2l_accum_blk<T>(b.blk, A.g, A.g.n3, A.ne3, B.idx, (const T *)B.res, A.L3);
+}
+#undef ENS_MEMBER
+#undef ENS_KERNEL
+
 // the branch streams start after the work queued on the ensemble's stream ...
 static int ens_fork(sesmd_ensemble *e)
 {
@@ -5396,6 +5891,68 @@ static void ens_near(sesmd_ensemble *e)
 // the branch streams, joined; the near field of all members on the ensemble's
 // stream; the members' site terms and parts B on the branches, joined again.
 // Captured, the branches are independent paths of the graph.
+// every member's rebuild, far field and position gathers (launch_rebuild,
+// launch_fmm with occupied targets, forces_gather), one launch per kernel
+template <int PP>
+static int ens_rebuild_batched(sesmd_ensemble *e)
+{
+    int n = e->n;
+    hipStream_t st = e->stream;
+    const EnsStepArgs *da = e->da;
+    auto P = [&](int k) { return e->pre + (size_t)k * (n + 1); };
+    auto T = [&](int k) { return e->tot[k]; };
+    if (T(ENS_CLS)) k_ens_classify<<<T(ENS_CLS), 256, 0, st>>>(da, P(ENS_CLS), n);
+    k_ens_fill_keys<<<T(ENS_FILL), 256, 0, st>>>(da, P(ENS_FILL), n);
+    k_ens_site_box<<<T(ENS_SITE), 256, 0, st>>>(da, P(ENS_SITE), n);
+    if (T(ENS_ISUB)) k_ens_isub<<<T(ENS_ISUB), 64, 0, st>>>(da, P(ENS_ISUB), n);
+    k_ens_key_scan_rows<<<T(ENS_ONE), 1024, 0, st>>>(da, P(ENS_ONE), n);
+    k_ens_key_scatter<<<T(ENS_SITE), 256, 0, st>>>(da, P(ENS_SITE), n);
+    k_ens_key_order<<<T(ENS_ST), 64, 0, st>>>(da, P(ENS_ST), n);
+    k_ens_gather_meta<<<T(ENS_SITE), 256, 0, st>>>(da, P(ENS_SITE), n);
+    k_ens_ne4_clear<<<T(ENS_NEF), 256, 0, st>>>(da, P(ENS_NEF), n);
+    k_ens_p2m<PP><<<T(ENS_ST), 64, 0, st>>>(da, P(ENS_ST), n);
+    // one level's M2L products (lists, then the GEMMs into its result rows)
+    auto products = [&](int lv) {
+        k_ens_m2l_count<<<T(ENS_LISTS), 256, 0, st>>>(da, P(ENS_LISTS), n, lv);
+        k_ens_m2l_scan<<<T(ENS_ONE), 512, 0, st>>>(da, P(ENS_ONE), n, lv);
+        k_ens_m2l_fill<<<T(ENS_LISTS), 256, 0, st>>>(da, P(ENS_LISTS), n, lv);
+        int kg = lv == 0 ? ENS_GEMM4 : ENS_GEMM3, ksp = lv == 0 ? ENS_SPLIT4 : ENS_SPLIT3;
+        if (e->mix && PP == 4) {
+            k_ens_m2l_split<1><<<T(ksp), 256, 0, st>>>(da, P(ksp), n, lv);
+            k_ens_m2l_gemm_bf3<2, 1, 2, 4, 1><<<T(kg), 256, 0, st>>>(da, P(kg), n, lv);
+        } else if (e->mix) {
+            k_ens_m2l_split<3><<<T(ksp), 256, 0, st>>>(da, P(ksp), n, lv);
+            k_ens_m2l_gemm_bf3<3, 2, 4, 2, 3><<<T(kg), 256, 0, st>>>(da, P(kg), n, lv);
+        } else if (PP == 4) {
+            k_ens_m2l_gemm<2, 8, M2L_TPB, 2><<<T(kg), 256, 0, st>>>(da, P(kg), n, lv);
+        } else {
+            k_ens_m2l_gemm<6, 23, M2L_TPB, 2><<<T(kg), 768, 0, st>>>(da, P(kg), n, lv);
+        }
+    };
+    auto accum = [&](int lv) {
+        int k = lv == 0 ? ENS_NB3 : ENS_N33;
+        if (e->mix) k_ens_m2l_accum<float><<<T(k), 64, 0, st>>>(da, P(k), n, lv);
+        else k_ens_m2l_accum<double><<<T(k), 64, 0, st>>>(da, P(k), n, lv);
+    };
+    products(0);
+    k_ens_m2m_4<<<T(ENS_N33), 512, 0, st>>>(da, P(ENS_N33), n);
+    k_ens_m2m_3<<<T(ENS_N2), 512, 0, st>>>(da, P(ENS_N2), n);
+    k_ens_top<<<T(ENS_N2), 64, 0, st>>>(da, P(ENS_N2), n);
+    k_ens_l2l_3<<<T(ENS_N33), 64, 0, st>>>(da, P(ENS_N33), n);
+    products(1);
+    accum(1);
+    k_ens_l2l_4<<<T(ENS_ST), 64, 0, st>>>(da, P(ENS_ST), n);
+    accum(0);
+    k_ens_gather_pos<<<T(ENS_SITE), 256, 0, st>>>(da, P(ENS_SITE), n);
+    if (e->mix) k_ens_gather_pos32<<<T(ENS_SITE), 256, 0, st>>>(da, P(ENS_SITE), n);
+    for (int i = 0; i < n; i++) {
+        e->m[i]->boxlist = false;  // the masked evaluations' box list, rebuilt on demand
+        e->m[i]->built = true;
+    }
+    HC(hipGetLastError());
+    return 0;
+}
+
 static int ens_step_batched(sesmd_ensemble *e, bool rebuild)
 {
     int n = e->n;
@@ -5403,17 +5960,7 @@ static int ens_step_batched(sesmd_ensemble *e, bool rebuild)
     const int *pre = e->pre;
     k_ens_move_a<<<e->tot[ENS_AB], 256, 0, st>>>(e->da, pre + ENS_AB * (n + 1), n, e->sc, rebuild ? 1 : 0);
     k_ens_nh_intra<<<e->tot[ENS_NH], 256, 0, st>>>(e->da, pre + ENS_NH * (n + 1), n);
-    if (rebuild) {  // each member's sort, lists and far field on the branches
-        EnsStreams on(e);
-        CK(ens_fork(e));
-        for (int i = 0; i < n; i++) {
-            sesmd_ctx *c = e->m[i];
-            CK(launch_rebuild(c, true));
-            CK(launch_fmm(c, true, false));
-            forces_gather(c, step_what(c), ~0ull, false);
-        }
-        CK(ens_join(e));
-    }
+    if (rebuild) CK(e->p == 4 ? ens_rebuild_batched<4>(e) : ens_rebuild_batched<8>(e));
     ens_near(e);
     unsigned what = step_what(e->m[0]);
     if (e->p == 4)
@@ -5525,7 +6072,8 @@ extern "C" int sesmd_ensemble_create(sesmd_ensemble **out, sesmd_ctx *const *mem
     e->batched = true;
     for (int i = 0; i < n; i++) {
         const sesmd_ctx *c = members[i];
-        if (!c->surf_fast || c->g.p != e->p || memcmp(&c->sconst, &e->sc, sizeof(SurfConst)) != 0)
+        if (!c->surf_fast || c->g.p != e->p || (e->p != 4 && e->p != 8) ||
+            memcmp(&c->sconst, &e->sc, sizeof(SurfConst)) != 0)
             e->batched = false;
     }
     std::vector<EnsStepArgs> ha(n);
@@ -5565,10 +6113,69 @@ extern "C" int sesmd_ensemble_create(sesmd_ensemble **out, sesmd_ctx *const *mem
         A.gstart = c->gstart;
         A.gidx = c->gidx;
         A.ngr = c->ngr;
+        A.classes_fixed = c->classes_fixed ? 1 : 0;
+        A.max_rwork = c->max_rwork;
+        A.first = c->first;
+        A.mol = c->mol;
+        A.type = c->type;
+        A.cls_w = c->cls;
+        A.anc = c->anc;
+        A.cbox = c->cbox;
+        A.rkind = c->rkind;
+        A.key_in = c->key_in;
+        A.key_out = c->key_out;
+        A.val_in = c->val_in;
+        A.perm = c->perm;
+        A.iperm = c->iperm;
+        A.kcnt = c->kcnt;
+        A.bs4 = c->bs4;
+        A.bstart = c->bstart;
+        A.bflag = c->bflag;
+        A.nrwork = c->nrwork;
+        A.isub_w = c->isub;
+        A.rwork = c->rwork;
+        A.obox_w = c->obox;
+        A.smeta_w = c->smeta;
+        A.spos_w = c->spos;
+        A.spos32_w = c->spos32;
+        A.O4 = c->O4;
+        A.O3 = c->O3;
+        A.O2 = c->O2;
+        A.L2 = c->L2;
+        A.L3 = c->L3;
+        A.L4_w = c->L4;
+        A.top_part = c->top_part;
+        A.ne4 = c->ne4;
+        A.ne3 = c->ne3;
+        A.ne2 = c->ne2;
+        A.M43 = c->M43;
+        A.M32 = c->M32;
+        A.T2 = c->T2;
+        A.S23 = c->S23;
+        A.S34 = c->S34;
+        A.AF4 = c->AF4;
+        A.AF3 = c->AF3;
+        A.AFB4 = (const m2l_u4 *)c->AFB4;
+        A.AFB3 = (const m2l_u4 *)c->AFB3;
+        for (int lv = 0; lv < 2; lv++) {
+            const sesmd_ctx::M2LBuf &B = c->m2l[lv];
+            A.m2l[lv] = EnsStepArgs::Level{B.ob, B.res, B.cnt, B.base, B.gbase, B.bbase, B.list, B.idx};
+        }
         // site terms: about one block per 64 sites (a droplet's boxes hold
         // ~100 sites each), at most the single launch's grid
         int nst = std::max(1, std::min(std::min(c->g.nb3, SITE_GRID), (c->g.n + 63) / 64));
-        int cnt[ENS_KINDS] = {c->nblk_s + c->nblk_w, 1 + c->g.ns, nst};
+        const Geo &g = c->g;
+        int ks = (2 * g.lk + 31) / 32;  // k steps of the bf16 products
+        // the GEMMs: about one block per 256 (L4) / 1,024 (L3) sites, each
+        // taking the member's list blocks in turn
+        int gm4 = std::max(1, std::min(c->m2l[0].max_blocks, std::max(8, g.n / 256)));
+        int gm3 = std::max(1, std::min(c->m2l[1].max_blocks, std::max(4, g.n / 1024)));
+        auto nb256 = [](int m) { return (int)nblk(m, 256); };
+        int cnt[ENS_KINDS] = {
+            c->nblk_s + c->nblk_w, 1 + g.ns, nst,
+            c->classes_fixed ? 0 : nb256(g.nmol), nb256(9 * g.nb3), nb256(g.n), g.ns, 1, 4 * g.nb3,
+            g.nb3, g.n33, g.n2, g.n2 * g.n2, M2L_LISTS,
+            nb256(g.nb3 * 32 * ks), nb256(g.n33 * 32 * ks), gm4, gm3, nb256(g.nb3)};
         for (int k = 0; k < ENS_KINDS; k++) {
             hpre[(size_t)k * (n + 1) + i] = e->tot[k];
             e->tot[k] += cnt[k];
