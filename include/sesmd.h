@@ -142,7 +142,11 @@ int sesmd_steps(sesmd_ctx *ctx, int nsteps, int ifar, int *kb);
  * subtraction mask (bit k: site k+1 of the same surfactant); any may be NULL */
 int sesmd_get_classes(sesmd_ctx *ctx, int *cls, int *anc, int *cbox, int *rkind,
                       unsigned *isub);
-/* expansions as complex (re, im) pairs: O4/L4 nb^3*lp, O3/L3 (nb/2)^3*lp, O2/L2 nb2^3*lp */
+/* expansions as complex (re, im) pairs: O4/L4 nb^3*lp, O3/L3 (nb/2)^3*lp, O2/L2
+ * nb2^3*lp.  sesmd_fmm gives every box its full expansions; after
+ * sesmd_steps the local expansions hold the M2L terms only for boxes with a
+ * receiver site, and L4 is evaluated only for boxes holding sites (the only
+ * ones a step reads). */
 int sesmd_get_expansions(sesmd_ctx *ctx, double *O4, double *L4, double *O3, double *L3,
                          double *O2, double *L2);
 /* counters: SHAKE/RATTLE molecules that hit the iteration limit, cumulative */

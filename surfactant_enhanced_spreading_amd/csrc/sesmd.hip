@@ -149,6 +149,7 @@ struct sesmd_ctx {
     float *lj32;     // mixed precision: lj_c (1024) then lj_s6 (1024) as float
     // far field
     double2 *O4, *L4, *O3, *L3, *O2, *L2;
+    double2 *top_part;            // top-level M2L partials per (target, source)
     int *ne4, *ne3, *ne2;         // nonempty flags per box
     double2 *T4, *T3, *T2;        // M2L operators per offset
     double *AF4, *AF3;            // the same, real-expanded in MFMA operand order (k_m2l_gemm)
@@ -167,7 +168,6 @@ struct sesmd_ctx {
     // the L4 products run on m2l_stream beside the L3 chain of the far field
     hipStream_t m2l_stream;
     hipEvent_t ev_m2l_fork, ev_m2l_join;
-    double2 *top_part;            // top-level M2L partials per (target, source)
     double2 *M43, *M32, *S23, *S34; // M2M / L2L operators per child
     double *c4;                   // L4 centres (3 per box)
     double *lfac;                 // rfctr(j,|k|) table
@@ -2749,6 +2749,38 @@ __global__ __launch_bounds__(64) void k_m2l_top_sum(Geo g, const double2 *__rest
     m2l_top_sum_blk(blockIdx.x, g, part, L2);
 }
 
+// The same top level with one block per target L2 box (the batched ensemble's
+// form: its members' mostly empty domains would spend a (target, source)
+// block on every pair), lane r its row r: each non-empty, non-adjacent
+// source's product in a fresh accumulator, added in source index order as
+// k_m2l_top_sum adds them -- the same result.  (Summing
+// the products of the empty and adjacent sources as well -- each +0.0 -- would
+// change nothing: the running sum starts at +0.0, and x + (+0.0) == x for
+// every x but -0.0, which a sum that starts at +0.0 never reaches.)
+__device__ __forceinline__ void m2l_top_blk(int Tb, const Geo &g, const double2 *__restrict__ O2,
+                                            const int *__restrict__ ne2, const double2 *__restrict__ T2,
+                                            double2 *__restrict__ L2)
+{
+    int row = threadIdx.x, nd = g.nb2;
+    if (row >= g.lk) return;
+    int tx = Tb % nd, ty = (Tb / nd) % nd, tz = Tb / (nd * nd);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int S = 0; S < g.n2; S++) {
+        int sx = S % nd, sy = (S / nd) % nd, sz = S / (nd * nd);
+        int dx = sx - tx, dy = sy - ty, dz = sz - tz;
+        bool adjacent = iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1;
+        if (adjacent || !ne2[S]) continue;  // block-uniform
+        int w = 2 * nd - 1;
+        int off = (dx + nd - 1) + w * ((dy + nd - 1) + w * (dz + nd - 1));
+        double2 v = make_double2(0.0, 0.0);
+        contract_rows(g, T2 + (size_t)off * g.lk * g.lp, O2 + (size_t)S * g.lp, v, row);
+        acc.x += v.x;
+        acc.y += v.y;
+    }
+    store_rows(g, L2 + (size_t)Tb * g.lp, row, acc);
+}
+
+
 // L2L (shiftLjk2 / shiftLjk3): child table = S[child position] parent
 __device__ __forceinline__ void l2l_blk(int blk,
                                           const Geo &g,
@@ -2772,6 +2804,20 @@ __global__ __launch_bounds__(64) void k_l2l(Geo g, int nd_child, const double2 *
                                             const double2 *__restrict__ S, double2 *__restrict__ Lc)
 {
     l2l_blk(blockIdx.x, g, nd_child, Lp, S, Lc);
+}
+
+// L2L into the boxes holding sites only (the occupied list, taken blockIdx.x,
+// + gridDim.x, ...): in a step, their sites' far-field terms are the only
+// readers of L4
+__global__ __launch_bounds__(64) void k_l2l_occupied(Geo g, const int4 *__restrict__ obox,
+                                                     const double2 *__restrict__ Lp, const double2 *__restrict__ S,
+                                                     double2 *__restrict__ Lc)
+{
+    for (int q = blockIdx.x; q < g.nb3; q += gridDim.x) {
+        int box = obox[q].x;
+        if (box < 0) break;  // past the occupied boxes (block-uniform)
+        l2l_blk(box, g, g.nb, Lp, S, Lc);
+    }
 }
 
 // ------------------------------------------------------------ per-site terms
@@ -4511,6 +4557,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->O3, (size_t)g.n33 * g.lp));
     A(dalloc(&c->L3, (size_t)g.n33 * g.lp));
     A(dalloc(&c->O2, (size_t)g.n2 * g.lp));
+    A(dalloc(&c->top_part, (size_t)g.n2 * g.n2 * g.lk));
     A(dalloc(&c->L2, (size_t)g.n2 * g.lp));
     A(dalloc(&c->ne4, g.nb3));
     A(dalloc(&c->ne3, g.n33));
@@ -4548,7 +4595,6 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->T3, (size_t)343 * g.lk * g.lp));
     int w2 = 2 * nb2 - 1;
     A(dalloc(&c->T2, (size_t)w2 * w2 * w2 * g.lk * g.lp));
-    A(dalloc(&c->top_part, (size_t)g.n2 * g.n2 * g.lk));
     A(dalloc(&c->M43, (size_t)8 * g.lk * g.lp));
     A(dalloc(&c->M32, (size_t)8 * g.lk * g.lp));
     A(dalloc(&c->S23, (size_t)8 * g.lk * g.lp));
@@ -4946,7 +4992,10 @@ static int launch_fmm_tail(sesmd_ctx *c, bool occ, bool fork)
     k_m2l_top_sum<<<g.n2, 64, 0, st>>>(g, c->top_part, c->L2);
     k_l2l<<<g.n33, 64, 0, st>>>(g, g.n3, c->L2, c->S23, c->L3);
     launch_m2l(c, g.n3, c->O3, c->ne3, c->T3, c->L3, occ);
-    k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4);
+    if (occ)  // the step pipeline: L4 only where sites read it
+        k_l2l_occupied<<<std::min(g.nb3, SITE_GRID), 64, 0, st>>>(g, c->obox, c->L3, c->S34, c->L4);
+    else
+        k_l2l<<<g.nb3, 64, 0, st>>>(g, g.nb, c->L3, c->S34, c->L4);
     if (fork) HC(hipStreamWaitEvent(st, c->ev_m2l_join, 0));
     launch_m2l_accum(c, g.nb, c->O4, c->ne4, c->T4, c->L4, occ);
     HC(hipGetLastError());
@@ -5546,7 +5595,7 @@ struct EnsStepArgs {
     int4 *obox_w, *smeta_w;
     double4 *spos_w;
     float4 *spos32_w;
-    double2 *O4, *O3, *O2, *L2, *L3, *L4_w, *top_part;
+    double2 *O4, *O3, *O2, *L2, *L3, *L4_w;
     int *ne4, *ne3, *ne2;
     const double2 *M43, *M32, *T2, *S23, *S34;
     const double *AF4, *AF3;
@@ -5742,31 +5791,10 @@ ENS_KERNEL(k_ens_m2m_3, 512)
     ENS_MEMBER
     m2m_blk(b.blk, A.g, A.g.nb2, A.O3, A.ne3, A.M32, A.O2, A.ne2);
 }
-// the top level for target L2 box blk: k_m2l_top_pair's products, each
-// added as k_m2l_top_sum adds them (sources in index order); the pairs with
-// an adjacent or empty source, whose products are +0.0, are skipped (adding
-// +0.0 to a sum that started at +0.0 changes nothing)
 ENS_KERNEL(k_ens_top, 64)
 {
     ENS_MEMBER
-    const Geo &g = A.g;
-    int Tb = b.blk, row = threadIdx.x, nd = g.nb2;
-    if (row >= g.lk) return;
-    int tx = Tb % nd, ty = (Tb / nd) % nd, tz = Tb / (nd * nd);
-    double2 acc = make_double2(0.0, 0.0);
-    for (int S = 0; S < g.n2; S++) {
-        int sx = S % nd, sy = (S / nd) % nd, sz = S / (nd * nd);
-        int dx = sx - tx, dy = sy - ty, dz = sz - tz;
-        bool adjacent = iabs_d(dx) <= 1 && iabs_d(dy) <= 1 && iabs_d(dz) <= 1;
-        if (adjacent || !A.ne2[S]) continue;
-        int w = 2 * nd - 1;
-        int off = (dx + nd - 1) + w * ((dy + nd - 1) + w * (dz + nd - 1));
-        double2 v = make_double2(0.0, 0.0);
-        contract_rows(g, A.T2 + (size_t)off * g.lk * g.lp, A.O2 + (size_t)S * g.lp, v, row);
-        acc.x += v.x;
-        acc.y += v.y;
-    }
-    store_rows(g, A.L2 + (size_t)Tb * g.lp, row, acc);
+    m2l_top_blk(b.blk, A.g, A.O2, A.ne2, A.T2, A.L2);
 }
 ENS_KERNEL(k_ens_l2l_3, 64)
 {
@@ -6144,7 +6172,6 @@ extern "C" int sesmd_ensemble_create(sesmd_ensemble **out, sesmd_ctx *const *mem
         A.L2 = c->L2;
         A.L3 = c->L3;
         A.L4_w = c->L4;
-        A.top_part = c->top_part;
         A.ne4 = c->ne4;
         A.ne3 = c->ne3;
         A.ne2 = c->ne2;
