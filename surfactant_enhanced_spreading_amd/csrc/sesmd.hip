@@ -2186,8 +2186,8 @@ __device__ __forceinline__ bool m2l_offset(int par, int ci, int &dx, int &dy, in
 }
 
 // target q (0..h^3 - 1) of class par -> box coordinates; source box for the
-// offset, or -1 (outside the grid / without charge; or, with `tne`, a target
-// whose flag there is 0)
+// offset, or -1 (outside the grid / without charge), or -2 (with `tne`: a
+// target whose flag there is 0, which nothing translates into)
 __device__ __forceinline__ int m2l_src(int nd, const int *__restrict__ ne, const int *__restrict__ tne, int par,
                                        int q, int dx, int dy, int dz, int *tb)
 {
@@ -2196,7 +2196,7 @@ __device__ __forceinline__ int m2l_src(int nd, const int *__restrict__ ne, const
     *tb = tx + nd * (ty + nd * tz);
     int sx = tx + dx, sy = ty + dy, sz = tz + dz;
     if (sx < 0 || sx >= nd || sy < 0 || sy >= nd || sz < 0 || sz >= nd) return -1;
-    if (tne && !tne[*tb]) return -1;
+    if (tne && !tne[*tb]) return -2;
     int si = sx + nd * (sy + nd * sz);
     return ne[si] ? si : -1;
 }
@@ -2319,7 +2319,7 @@ __device__ __forceinline__ void m2l_fill_blk(int blk,
         int q = q0 + (int)threadIdx.x, tb = -1, si = -1;
         if (q < h3) {
             si = m2l_src(nd, ne, tne, par, q, dx, dy, dz, &tb);
-            if (!far) si = -1;
+            if (!far && si != -2) si = -1;
         }
         unsigned long long m = __builtin_amdgcn_ballot_w64(si >= 0);
         int before = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -2333,7 +2333,8 @@ __device__ __forceinline__ void m2l_fill_blk(int blk,
         int pos = run + off + before;
         if (q < h3) {
             if (si >= 0) list[pos] = make_int2(q, si);
-            idx[(size_t)tb * 216 + ci] = si >= 0 ? pos : -1;
+            // the sums read the result positions of flagged targets only
+            if (si != -2) idx[(size_t)tb * 216 + ci] = si >= 0 ? pos : -1;
         }
         run += tot;
         __syncthreads();  // s_w is rewritten by the next round
@@ -5791,6 +5792,8 @@ ENS_KERNEL(k_ens_m2m_3, 512)
     ENS_MEMBER
     m2m_blk(b.blk, A.g, A.g.nb2, A.O3, A.ne3, A.M32, A.O2, A.ne2);
 }
+// one block per target L2 box (m2l_top_blk); bound by the operator reads (a
+// 58 KB operator per product at p = 8), not by the per-target chain
 ENS_KERNEL(k_ens_top, 64)
 {
     ENS_MEMBER
