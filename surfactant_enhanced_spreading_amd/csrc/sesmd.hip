@@ -143,7 +143,7 @@ struct sesmd_ctx {
     int *widx_in, *widx_out;
     double4 *spos;
     int4 *smeta;
-    double *npart;   // near-field partial forces: 9 rows x (x, y, z) x N, sorted order
+    double *npart;   // near-field partial forces: 27 slots (row, source box) x (x, y, z) x N, sorted order
     int *iperm;      // site -> sorted position
     float4 *spos32;  // mixed precision: position relative to its box centre, charge
     float *lj32;     // mixed precision: lj_c (1024) then lj_s6 (1024) as float
@@ -883,6 +883,12 @@ __global__ void k_gather_pos32(Geo g, const int *__restrict__ perm, const int *_
 // keeps only the (receiver kind, source kind) pairs of `mask` (one reference
 // kernel or a subset of them).
 constexpr int NEAR_ROWS = 9;  // (dz, dy) rows of the 3^3 neighbourhood, one wave each
+// Partial-force slots per receiver: row q (0..8) and source box j (0..2: x - 1,
+// x, x + 1 of the receiver's box) -> slot 3 q + j, three components each.  The
+// row-centred near field writes one slot per (row, box) wave; the box-centred
+// (masked) form writes a row's whole partial into its j = 1 slot and zeros
+// into the other two.
+constexpr int NEAR_SLOTS = 3 * NEAR_ROWS;
 constexpr int SRC_PAD = 4;    // spare records after the sorted source arrays (read ahead, never used)
 
 struct Recv {
@@ -1376,10 +1382,13 @@ __device__ __forceinline__ void near_item(
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         int k = k0 + NEAR_TILE * r + lane;
-        if (k < kend) {
-            part[(size_t)(3 * row) * g.n + k] = ax[r];
-            part[(size_t)(3 * row + 1) * g.n + k] = ay[r];
-            part[(size_t)(3 * row + 2) * g.n + k] = az[r];
+        if (k < kend) {  // the row's partial in its j = 1 slot, zeros in j = 0 and 2
+            size_t o = (size_t)(3 * (3 * row)) * g.n + k;
+            part[o] = part[o + g.n] = part[o + 2 * g.n] = 0.0;
+            part[o + 3 * g.n] = ax[r];
+            part[o + 4 * g.n] = ay[r];
+            part[o + 5 * g.n] = az[r];
+            part[o + 6 * g.n] = part[o + 7 * g.n] = part[o + 8 * g.n] = 0.0;
         }
     }
 }
@@ -1568,17 +1577,20 @@ __global__ __launch_bounds__(1024) void k_key_scan_rows(Geo g, const int *__rest
 }
 
 // One row-centred item: positions c0 + lane (+ 64 for NR = 2) of the row's
-// receiver list, below cend, against the row's sources; each receiver's
-// partial goes to the row slot its box sees this row in.
+// receiver list, below cend, against the sources of box j (0..2) of the row
+// (x0 - 1 + j, y, z); each receiver's partial goes to the slot (row as its
+// box sees it, j).  A box outside the grid has no slot (the site terms skip
+// it) and returns at once.
 template <bool HQ, bool MIX, int NR>
 __device__ __forceinline__ void near_rows_item(
     const Geo &g, const double4 *__restrict__ spos, const int4 *__restrict__ smeta,
     const int *__restrict__ bs4, const int *__restrict__ bflag, const int *__restrict__ rkind,
     const sesmd_forcefield *__restrict__ ff, const float4 *__restrict__ spos32,
     const float *__restrict__ lj32, double *__restrict__ part, const RowRecv &L, int x0, int y, int z,
-    int c0, int cend, int lane, int *__restrict__ counters)
+    int j, int c0, int cend, int lane, int *__restrict__ counters)
 {
-    int nb = g.nb;
+    int nb = g.nb, xs = x0 - 1 + j;
+    if (xs < 0 || xs >= nb) return;  // wave-uniform
     Recv R[NR];
     int ti[NR], k[NR], slot[NR];
     bool cen = false;  // some receiver of this wave lives in box (x0, y, z)
@@ -1610,18 +1622,20 @@ __device__ __forceinline__ void near_rows_item(
     double ax[NR], ay[NR], az[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) ax[r] = ay[r] = az[r] = 0.0;
-    int r0 = max(x0 - 1, 0) + nb * (y + nb * z), r1 = min(x0 + 1, nb - 1) + nb * (y + nb * z);
-    bool check = false;
-    for (int bb = r0; bb <= r1; bb++) check = check || bflag[bb];
+    // one source box; a box whose sites every neighbour lists needs no ring
+    // test (its sources pass it), so its own flag decides
+    int bb = xs + nb * (y + nb * z);
+    bool check = bflag[bb] != 0;
     int bc = anycen ? x0 + nb * (y + nb * z) : -1;
-    near_sources<false, HQ, MIX, NR>(g, spos, smeta, bs4, check, ff, 0ull, spos32, lj32, r0, r1, bc, x0, R, ti,
+    near_sources<false, HQ, MIX, NR>(g, spos, smeta, bs4, check, ff, 0ull, spos32, lj32, bb, bb, bc, x0, R, ti,
                                      ax, ay, az, counters);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         if (c0 + NEAR_TILE * r + lane < cend) {
-            part[(size_t)(3 * slot[r]) * g.n + k[r]] = ax[r];
-            part[(size_t)(3 * slot[r] + 1) * g.n + k[r]] = ay[r];
-            part[(size_t)(3 * slot[r] + 2) * g.n + k[r]] = az[r];
+            size_t o = (size_t)(3 * (3 * slot[r] + j)) * g.n + k[r];
+            part[o] = ax[r];
+            part[o + g.n] = ay[r];
+            part[o + 2 * g.n] = az[r];
         }
     }
 }
@@ -1643,7 +1657,10 @@ __global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
     // box-centred form is evaluated once, with the same sources in the same
     // order, into the same row slot; slots of rows outside the grid are not
     // written (the site-terms kernel skips them).
-    int item = blockIdx.x;  // items come longest first (see row_list_block)
+    // block 3 i + j: item i (items come longest first, see row_list_block),
+    // source box j of its row -- a row's three boxes as three waves keep the
+    // waves short enough for the dispatcher to balance the SIMDs
+    int item = blockIdx.x / 3, j = blockIdx.x % 3;
     if (item >= *nwork) return;
     int lane = threadIdx.x, nb = g.nb;
     int2 w = work[item];
@@ -1660,10 +1677,10 @@ __global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
     }
     int cend = min(L.pre[9], c0 + NEAR_CHUNK);
     if (cend - c0 > NEAR_TILE)
-        near_rows_item<HQ, MIX, 2>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, c0,
+        near_rows_item<HQ, MIX, 2>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, j, c0,
                                    cend, lane, counters);
     else
-        near_rows_item<HQ, MIX, 1>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, c0,
+        near_rows_item<HQ, MIX, 1>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, j, c0,
                                    cend, lane, counters);
 }
 
@@ -2924,21 +2941,23 @@ __device__ __forceinline__ void site_terms_boxes(int blk0, int nblk, const Geo &
             double x = pk.x, y = pk.y, z = pk.z;
             double nx = 0.0, ny = 0.0, nz = 0.0;
             if (what & SESMD_NEAR) {
-                // the slots of rows outside the grid hold stale values, and only the
-                // rows (dz, dy) of the receiver's box inside the grid are added
-                double pr[NEAR_ROWS][3];
-#pragma unroll
-                for (int q = 0; q < NEAR_ROWS; q++)
-#pragma unroll
-                    for (int d = 0; d < 3; d++) pr[q][d] = part[(size_t)(3 * q + d) * g.n + k];
+                // the slots of boxes outside the grid hold stale values: only
+                // the (row (dz, dy), box dx) slots of neighbour boxes inside the
+                // grid are added, rows in order, each row's boxes in x order
                 if (rk >= 0) {
 #pragma unroll
                     for (int q = 0; q < NEAR_ROWS; q++) {
                         int yy = by + q % 3 - 1, zz = bz + q / 3 - 1;
                         if (yy < 0 || yy >= nb || zz < 0 || zz >= nb) continue;
-                        nx += pr[q][0];
-                        ny += pr[q][1];
-                        nz += pr[q][2];
+#pragma unroll
+                        for (int j = 0; j < 3; j++) {
+                            int xx = bx + j - 1;
+                            if (xx < 0 || xx >= nb) continue;
+                            size_t o = (size_t)(3 * (3 * q + j)) * g.n + k;
+                            nx += part[o];
+                            ny += part[o + g.n];
+                            nz += part[o + 2 * g.n];
+                        }
                     }
                 }
             }
@@ -4546,7 +4565,7 @@ extern "C" int sesmd_create(sesmd_ctx **out, int n_water, int n_surf, int nb2, i
     A(dalloc(&c->nrwork, 1));
     A(dalloc(&c->obox, g.nb3));
     A(dalloc(&c->spos, n + SRC_PAD));
-    A(dalloc(&c->npart, 27 * n));
+    A(dalloc(&c->npart, (size_t)3 * NEAR_SLOTS * n));
     A(dalloc(&c->iperm, n));
     A(dalloc(&c->smeta, n + SRC_PAD));
     if (precision == SESMD_MIXED) {
@@ -5041,7 +5060,7 @@ static void launch_near(sesmd_ctx *c, uint64_t mask)
     bool hq = c->ff.q[1] == c->ff.q[2];
     if (mask == ~0ull) {
         bool mix = c->precision == SESMD_MIXED;
-        dim3 grid(c->max_rwork), block(NEAR_TILE);
+        dim3 grid(3 * c->max_rwork), block(NEAR_TILE);
 #define ROWS_ARGS \
     g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind, c->rwork, c->nrwork, c->dff, c->spos32, \
         c->lj32, c->npart, c->counters
@@ -5515,7 +5534,9 @@ __global__ __launch_bounds__(256) void k_near_rows_ens(const EnsMember *__restri
         }
         const EnsMember &M = dm[lo];
         int it = i - prefix[lo];
-        int item = it / per, sub = it % per;
+        // unit 3 (per item + sub) + j: receiver sub-chunk `sub` of item
+        // `item`, source box j of its row (as k_near_rows' waves)
+        int item = it / (3 * per), sub = (it / 3) % per, j = it % 3;
         const Geo &g = M.g;
         int nb = g.nb;
         int2 w = M.rwork[item];
@@ -5536,10 +5557,10 @@ __global__ __launch_bounds__(256) void k_near_rows_ens(const EnsMember *__restri
         if (s0 >= s1) continue;
         if (s1 - s0 > NEAR_TILE)
             near_rows_item<true, MIX, 2>(g, M.spos, M.smeta, M.bs4, M.bflag, M.rkind, M.ff, M.spos32, M.lj32,
-                                         M.part, L, x0, y, z, s0, s1, lane, M.counters);
+                                         M.part, L, x0, y, z, j, s0, s1, lane, M.counters);
         else
             near_rows_item<true, MIX, 1>(g, M.spos, M.smeta, M.bs4, M.bflag, M.rkind, M.ff, M.spos32, M.lj32,
-                                         M.part, L, x0, y, z, s0, s1, lane, M.counters);
+                                         M.part, L, x0, y, z, j, s0, s1, lane, M.counters);
     }
 }
 
@@ -5907,7 +5928,7 @@ static int ens_join(sesmd_ensemble *e)
 
 static void ens_near(sesmd_ensemble *e)
 {
-    k_ens_prefix<<<1, ENS_MAX, 0, e->stream>>>(e->dm, e->n, NEAR_CHUNK / e->chunk, e->prefix);
+    k_ens_prefix<<<1, ENS_MAX, 0, e->stream>>>(e->dm, e->n, 3 * (NEAR_CHUNK / e->chunk), e->prefix);
     if (e->mix)
         k_near_rows_ens<true><<<e->grid, 64 * e->waves, 0, e->stream>>>(e->dm, e->prefix, e->n, e->chunk);
     else
@@ -6332,7 +6353,7 @@ extern "C" int sesmd_ensemble_tune(sesmd_ensemble *e, int reps, int *shape3, flo
     float best = -1.f;
     for (int waves : {1, 2, 4})
         for (int chunk : {64, 128})
-            for (int per_simd : {2, 4, 8, 16}) {
+            for (int per_simd : {2, 4, 8, 16, 32, 64}) {
                 if (!e->mix && waves != 1) continue;
                 e->waves = waves;
                 e->chunk = chunk;
