@@ -2945,7 +2945,7 @@ __device__ __forceinline__ void site_terms_boxes(int blk0, int nblk, const Geo &
                 // the (row (dz, dy), box dx) slots of neighbour boxes inside the
                 // grid are added, rows in order, each row's boxes in x order
                 if (rk >= 0) {
-#pragma unroll
+#pragma unroll 1
                     for (int q = 0; q < NEAR_ROWS; q++) {
                         int yy = by + q % 3 - 1, zz = bz + q / 3 - 1;
                         if (yy < 0 || yy >= nb || zz < 0 || zz >= nb) continue;
