@@ -1647,7 +1647,7 @@ __global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
     const int *__restrict__ bstart, const int *__restrict__ bs4, const int *__restrict__ bflag,
     const int *__restrict__ rkind, const int2 *__restrict__ work, const int *__restrict__ nwork,
     const sesmd_forcefield *__restrict__ ff, const float4 *__restrict__ spos32,
-    const float *__restrict__ lj32, double *__restrict__ part, int *__restrict__ counters)
+    const float *__restrict__ lj32, double *__restrict__ part, int *__restrict__ counters, int mr)
 {
     // The full near field, row-centred: one 64-lane wavefront per work item,
     // the item being up to NEAR_CHUNK receivers of the nine boxes that all see
@@ -1660,8 +1660,22 @@ __global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
     // block 3 i + j: item i (items come longest first, see row_list_block),
     // source box j of its row -- a row's three boxes as three waves keep the
     // waves short enough for the dispatcher to balance the SIMDs
-    int item = blockIdx.x / 3, j = blockIdx.x % 3;
-    if (item >= *nwork) return;
+    // The cheapest quarter of the list (items split .. nw - 1) runs as
+    // halves of 64 receivers: blocks 3 i + j take the first half, and a
+    // second segment of blocks after the 3 mr of the first, dispatched last,
+    // takes the second halves -- short waves where SIMDs run out of work.
+    int nw = *nwork, split = nw - nw / 4, item, j, sub = 0;
+    int b = blockIdx.x;
+    if (b < 3 * mr) {
+        item = b / 3;
+        j = b % 3;
+    } else {
+        item = split + (b - 3 * mr) / 3;
+        j = (b - 3 * mr) % 3;
+        sub = 1;
+    }
+    if (item >= nw) return;
+    bool half = item >= split;
     int lane = threadIdx.x, nb = g.nb;
     int2 w = work[item];
     int q = w.x, c0 = w.y;
@@ -1676,12 +1690,18 @@ __global__ __launch_bounds__(NEAR_TILE, MIX ? 1 : 5) void k_near_rows(
         __syncthreads();
     }
     int cend = min(L.pre[9], c0 + NEAR_CHUNK);
-    if (cend - c0 > NEAR_TILE)
+    if (half) {  // receivers c0 + 64 sub .. + 63
+        int s0 = c0 + NEAR_TILE * sub, s1 = min(cend, s0 + NEAR_TILE);
+        if (s0 < s1)
+            near_rows_item<HQ, MIX, 1>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, j,
+                                       s0, s1, lane, counters);
+    } else if (cend - c0 > NEAR_TILE) {
         near_rows_item<HQ, MIX, 2>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, j, c0,
                                    cend, lane, counters);
-    else
+    } else {
         near_rows_item<HQ, MIX, 1>(g, spos, smeta, bs4, bflag, rkind, ff, spos32, lj32, part, L, x0, y, z, j, c0,
                                    cend, lane, counters);
+    }
 }
 
 
@@ -5060,10 +5080,11 @@ static void launch_near(sesmd_ctx *c, uint64_t mask)
     bool hq = c->ff.q[1] == c->ff.q[2];
     if (mask == ~0ull) {
         bool mix = c->precision == SESMD_MIXED;
-        dim3 grid(3 * c->max_rwork), block(NEAR_TILE);
+        // three waves per item, and the second halves of the last quarter
+        dim3 grid(3 * c->max_rwork + 3 * (c->max_rwork / 4 + 1)), block(NEAR_TILE);
 #define ROWS_ARGS \
     g, c->spos, c->smeta, c->bstart, c->bs4, c->bflag, c->rkind, c->rwork, c->nrwork, c->dff, c->spos32, \
-        c->lj32, c->npart, c->counters
+        c->lj32, c->npart, c->counters, c->max_rwork
         if (mix && hq) k_near_rows<true, true><<<grid, block, 0, c->stream>>>(ROWS_ARGS);
         else if (mix) k_near_rows<false, true><<<grid, block, 0, c->stream>>>(ROWS_ARGS);
         else if (hq) k_near_rows<true, false><<<grid, block, 0, c->stream>>>(ROWS_ARGS);
